@@ -1,0 +1,108 @@
+"""ctypes binding of the aerofx C ABI (include/aerofx.h).
+
+Loads the in-tree library ``aerof_dev_amd/_lib/libaerofx.so``; raises
+immediately if it is missing, so no caller can fall back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libaerofx.so")
+
+AEROFX_BC_SLIP_WALL = 2
+AEROFX_BC_INLET = 4
+AEROFX_BC_OUTLET = 5
+TIMER_NAMES = ("nodalGrad", "fvTerm", "fvJac", "fluidPrecSetup", "jacApply", "precApply",
+               "gramSchmidt", "reserved")
+
+c_double_p = C.POINTER(C.c_double)
+c_int_p = C.POINTER(C.c_int)
+
+
+class BoxSpec(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("jitter", C.c_double),
+        ("seed", C.c_uint64),
+        ("face_code", C.c_int * 6),
+        ("nbumps", C.c_int),
+        ("bump_amp", c_double_p), ("bump_center", c_double_p), ("bump_width", c_double_p),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("gamma", C.c_double), ("pstiff", C.c_double), ("beta", C.c_double),
+        ("second_order", C.c_int),
+        ("rho_inf", C.c_double), ("p_inf", C.c_double), ("mach", C.c_double),
+        ("alpha", C.c_double), ("yaw", C.c_double),
+    ]
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_int), ("num_tets", C.c_int), ("num_faces", C.c_int),
+        ("X", c_double_p), ("tets", c_int_p), ("faces", c_int_p), ("face_codes", c_int_p),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/aerofx.h
+SIGNATURES = {
+    "aerofx_last_error": (C.c_char_p, []),
+    "aerofx_version": (C.c_int, []),
+    "aerofx_mesh_box_sizes": (C.c_int, [C.POINTER(BoxSpec), c_int_p, c_int_p, c_int_p]),
+    "aerofx_mesh_box": (C.c_int, [C.POINTER(BoxSpec), c_double_p, c_int_p, c_int_p, c_int_p]),
+    "aerofx_mesh_edges": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p]),
+    "aerofx_mesh_rcm": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
+    "aerofx_mesh_renumber": (C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p, c_double_p, c_int_p, c_int_p]),
+    "aerofx_mesh_color_edges": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p]),
+    "aerofx_state_init": (C.c_int, [C.POINTER(Params), C.c_int, C.c_double, C.c_uint64, C.c_int,
+                                    c_double_p, c_double_p]),
+    "aerofx_ctx_create": (C.c_int, [C.POINTER(Mesh), C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "aerofx_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "aerofx_ctx_sizes": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p]),
+    "aerofx_ctx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "aerofx_ctx_sync": (C.c_int, [C.c_void_p]),
+    "aerofx_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "aerofx_ctx_timers": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_longlong)]),
+    "aerofx_ctx_tables": (C.c_int, [C.c_void_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p]),
+    "aerofx_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_residual_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "aerofx_error_count": (C.c_int, [C.c_void_p, c_int_p]),
+    "aerofx_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_primitive": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library with typed signatures (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"aerofx native library not built: {LIB_PATH} is missing "
+                "(run __graft_entry__.build() or make -C aerof_dev_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class AerofxError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> int:
+    """Raises AerofxError on rc < 0; returns rc (>= 0) otherwise."""
+    if rc < 0:
+        msg = lib().aerofx_last_error().decode(errors="replace")
+        raise AerofxError(f"{what} failed ({rc}): {msg}")
+    return rc

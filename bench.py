@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""FP64 edge-residual throughput on one MI355X.
+
+Workload: 128^3-hex Kuhn-split unit box (2,146,689 nodes, 12,582,912 tets,
+14,827,904 edges, 196,608 boundary triangles), interior nodes jittered
++-0.15h (Philox, seed 1234), renumbered with the reference RCM rule;
+perfect gas, freestream M = 0.8, density x (1 + 0.02 sin sin sin) with
+seeded phases; second-order MUSCL (LSQ gradients, Van Albada, Roe flux),
+Steger-Warming farfield on 5 faces and a slip wall at z = 0.
+
+A step is one full residual evaluation (gradient pass + flux pass, 1/V
+scaling) with U resident in HBM.  value = edges / (ms per step) in Medges/s.
+
+roofline: the dominant kernel is the flux pass; its algorithmic bytes per
+launch are 232 B/node + 32 B/edge + 40 B/face (read U, X, grad, 1/V and
+write R per node; edge pair + normal per edge; face nodes + normal + code per
+face), divided by that kernel's mean duration from HIP events bracketing
+each launch on the context stream.  traffic (HBM bytes per launch) comes from
+the rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed under profiles/.
+
+cpu_baseline: the oracle's single-thread C restatement of the same residual
+on the same mesh and state, timed on the host over a bounded number of
+evaluations.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NX = 128
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_traffic(name):
+    """HBM bytes per launch for a kernel from profiles/traffic_<name>.json."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(mesh, params, U, budget_s):
+    import numpy as np
+    import oracle as O
+    t0 = time.time()
+    om = O.OracleMesh(mesh, params)
+    log(f"[bench] oracle set-up {time.time() - t0:.1f} s")
+    times = []
+    t_start = time.time()
+    while True:
+        t0 = time.perf_counter()
+        out = om.residual(U)
+        times.append(time.perf_counter() - t0)
+        if out["ierr"] != 0:
+            raise RuntimeError("oracle residual reported non-physical states")
+        if time.time() - t_start > budget_s or len(times) >= 10:
+            break
+    med = float(np.median(times))
+    return dict(value=om.E / med / 1e6, unit="Medges/s", cores=1, kind="port",
+                sample=f"{len(times)} full residual evaluations on the same 128^3 mesh and state, "
+                       f"single thread, median {med * 1e3:.0f} ms each")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=NX, help="hexes per axis")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle timing")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from aerof_dev_amd import FlowParams, init_state
+    from aerof_dev_amd import mesh as M
+    from aerof_dev_amd.operators import SpaceOperator
+
+    t0 = time.time()
+    mesh = M.box_mesh(args.n, jitter=0.15, seed=1234, rcm_order=True)
+    params = FlowParams(mach=0.8)
+    U = init_state(params, mesh.X, "sine", 0.02, seed=2024)
+    log(f"[bench] mesh {mesh.num_nodes} nodes, {mesh.num_tets} tets ({time.time() - t0:.1f} s)")
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    op = SpaceOperator(mesh, params)
+    N, E, F = op.num_nodes, op.num_edges, op.num_faces
+    log(f"[bench] context: E={E} F={F} ({time.time() - t0:.1f} s)")
+    Ud = torch.from_numpy(U).to(dev)
+    Rd = torch.empty_like(Ud)
+    ierr = op.compute_residual(Ud, Rd)
+    if ierr != 0:
+        raise RuntimeError(f"non-physical reconstruction count {ierr}")
+    for _ in range(args.warmup):
+        op.compute_residual_async(Ud, Rd)
+    torch.cuda.synchronize()
+    op.profile(True)
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        op.compute_residual_async(Ud, Rd)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    timers = op.timers()
+    op.profile(False)
+    ms_step = ms_total / args.steps
+    log(f"[bench] {args.steps} steps, {ms_step:.4f} ms/step; kernels {timers}")
+
+    flux_ms = timers["fvTerm"][0] / timers["fvTerm"][1]
+    grad_ms = timers["nodalGrad"][0] / timers["nodalGrad"][1]
+    flux_bytes = 232 * N + 32 * E + 40 * F
+    grad_bytes = 232 * N + 8 * E
+    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=load_traffic("fvterm_box128"),
+                    kernel="k_fluxes<true> (flux pass)", algorithmic_bytes=flux_bytes,
+                    ms_per_launch=round(flux_ms, 5))
+    grad_gbs = grad_bytes / (grad_ms * 1e-3) / 1e9
+    whole_bytes = 464 * N + 40 * E + 40 * F
+    out = {
+        "metric": "Medge-flux evals/s (FP64 residual, 128^3 Kuhn box)",
+        "value": round(E / (ms_step * 1e-3) / 1e6, 1),
+        "unit": "Medges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Kuhn-split jittered box, sine density perturbation)",
+        "config": {"workload": "FP64 residual, 128^3-hex Kuhn box, RCM order, M=0.8",
+                   "nodes": N, "edges": E, "faces": F, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "kernels": {"nodalGrad": {"ms": round(grad_ms, 5), "algorithmic_GBs": round(grad_gbs, 1)},
+                    "fvTerm": {"ms": round(flux_ms, 5), "algorithmic_GBs": round(achieved, 1)}},
+        "residual_hbm_frac": round(whole_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    if not args.no_cpu:
+        log("[bench] timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(mesh, params, U, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

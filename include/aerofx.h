@@ -1,0 +1,161 @@
+/*
+ * aerofx.h -- C ABI of the MI355X-native edge-based finite-volume operator.
+ *
+ * One opaque context per (mesh, parameters).  All compute entry points take
+ * DEVICE pointers, enqueue work on the context's HIP stream and return
+ * without synchronising unless stated.  Every entry point returns an int:
+ *   0   success
+ *   >0  count of non-physical reconstructed states (the reference's ierr)
+ *   <0  error (AEROFX_ERR_*); aerofx_last_error() describes it.
+ * No entry point throws.  Contexts are not re-entrant: calls on one context
+ * must be serialised by the caller (as the reference's SpaceOperator is).
+ *
+ * Array layouts follow the reference's SVec: row-major [n][dim] doubles.
+ * Node, edge and face numbering is the caller's; the edge table is the
+ * reference's (EdgeCore.C:55-119): unique (i,j), i<j, sorted
+ * lexicographically.
+ */
+#ifndef AEROFX_H
+#define AEROFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AEROFX_OK 0
+#define AEROFX_ERR_ARG (-1)
+#define AEROFX_ERR_HIP (-2)
+#define AEROFX_ERR_MESH (-3)
+#define AEROFX_ERR_NOMEM (-4)
+#define AEROFX_ERR_STATE (-5)
+
+/* Boundary codes (BcDef.h:4-28). */
+#define AEROFX_BC_INTERNAL 0
+#define AEROFX_BC_SLIP_WALL 2
+#define AEROFX_BC_INLET 4
+#define AEROFX_BC_OUTLET 5
+
+typedef struct aerofx_ctx aerofx_ctx;
+
+/* Mesh handed to aerofx_ctx_create: tets [T][4] (positively oriented),
+ * boundary triangles [F][3] with outward normal 0.5*(x2-x0)^(x1-x0), one
+ * code per triangle.  Host pointers; copied. */
+typedef struct {
+  int num_nodes, num_tets, num_faces;
+  const double *X;
+  const int *tets;
+  const int *faces;
+  const int *face_codes;
+} aerofx_mesh;
+
+/* Flow and scheme parameters (IoData defaults, IoDataCore.C:2418-2429,
+ * 2662; DistBcData.C:652-706 for the freestream). Angles in radians. */
+typedef struct {
+  double gamma;        /* specific heat ratio (1.4) */
+  double pstiff;       /* pressure constant (0: perfect gas) */
+  double beta;         /* MUSCL beta (1/3) */
+  int second_order;    /* 1: LSQ gradients + Van Albada; 0: first order */
+  double rho_inf, p_inf, mach, alpha, yaw;
+} aerofx_params;
+
+/* Synthetic Kuhn-split box (no reference counterpart: the reference reads
+ * pre-partitioned SOWER meshes, GeoSource.C:246-356). */
+typedef struct {
+  int nx, ny, nz;            /* hexes per axis */
+  double lo[3], hi[3];       /* extent */
+  double jitter;             /* interior nodes moved by U(-j,j)*h per axis */
+  uint64_t seed;             /* Philox4x32-10 key */
+  int face_code[6];          /* x-, x+, y-, y+, z-, z+ */
+  int nbumps;                /* Gaussian bumps on the z- wall, blended */
+  const double *bump_amp, *bump_center, *bump_width;  /* x-positions */
+} aerofx_box_spec;
+
+const char *aerofx_last_error(void);
+int aerofx_version(void);
+
+/* ---- host-side mesh preparation (no GPU needed) ------------------------ */
+int aerofx_mesh_box_sizes(const aerofx_box_spec *spec, int *num_nodes,
+                          int *num_tets, int *num_faces);
+int aerofx_mesh_box(const aerofx_box_spec *spec, double *X, int *tets,
+                    int *faces, int *face_codes);
+/* Edge count and table (EdgeSet::find/createPointers, EdgeCore.C:55-119):
+ * ptr must hold 2*num_edges ints (query with ptr == NULL). */
+int aerofx_mesh_edges(int num_nodes, int num_tets, const int *tets,
+                      int *num_edges, int *ptr);
+/* Reference RCM (Connectivity::renumByComponent with renumRCM,
+ * ConnectivityCore.C:311-635): renum[old] = new. */
+int aerofx_mesh_rcm(int num_nodes, int num_tets, const int *tets, int *renum);
+/* Relabels a mesh in place by renum[old] = new. */
+int aerofx_mesh_renumber(int num_nodes, int num_tets, int num_faces,
+                         const int *renum, double *X, int *tets, int *faces);
+/* Edge colouring rule (no reference counterpart): returns colours. */
+int aerofx_mesh_color_edges(int num_nodes, int num_edges, const int *ptr,
+                            int *color, int *num_colors);
+
+/* Synthetic states: conservative U[N][5] on host.
+ *  kind 0: uniform freestream
+ *  kind 1: freestream with multiplicative Gaussian noise of std `amp` on
+ *          density and pressure (Philox Box-Muller, key = seed)
+ *  kind 2: freestream with density * (1 + amp*sin(2pi(x+a))sin(2pi(y+b))
+ *          sin(2pi(z+c))), phases a,b,c from Philox(seed) */
+int aerofx_state_init(const aerofx_params *p, int kind, double amp,
+                      uint64_t seed, int num_nodes, const double *X,
+                      double *U);
+
+/* ---- context ----------------------------------------------------------- */
+int aerofx_ctx_create(const aerofx_mesh *mesh, const aerofx_params *params,
+                      aerofx_ctx **out);
+int aerofx_ctx_destroy(aerofx_ctx *ctx);
+int aerofx_ctx_sizes(const aerofx_ctx *ctx, int *num_nodes, int *num_edges,
+                     int *num_faces);
+/* Uses `stream` (a hipStream_t) for all later work; NULL = context stream. */
+int aerofx_ctx_set_stream(aerofx_ctx *ctx, void *stream);
+int aerofx_ctx_sync(aerofx_ctx *ctx);
+/* Host copies of the set-up tables (for parity checks): edge table [E][2],
+ * control volumes [N], edge normals [E][3], face normals [F][3] (full
+ * triangle normal, FaceTria::computeNormal), LSQ factors [N][6]. Any may be
+ * NULL. */
+int aerofx_ctx_tables(const aerofx_ctx *ctx, int *ptr, double *ctrl_vol,
+                      double *edge_normals, double *face_normals,
+                      double *lsq);
+
+/* Per-kernel timers, named after the reference's Timer buckets
+ * (Timer.h:12-15).  When enabled, each kernel launch is bracketed by HIP
+ * events on the context stream; aerofx_ctx_timers synchronises and returns
+ * accumulated milliseconds and launch counts per bucket. Enabling resets. */
+#define AEROFX_TIMER_NODAL_GRAD 0   /* LSQ gradient pass */
+#define AEROFX_TIMER_FV_TERM 1      /* edge + face flux pass */
+#define AEROFX_TIMER_FV_JAC 2       /* Jacobian assembly */
+#define AEROFX_TIMER_PREC_SETUP 3   /* block-Jacobi inversion */
+#define AEROFX_TIMER_JAC_APPLY 4    /* Jacobian-vector product */
+#define AEROFX_TIMER_PREC_APPLY 5   /* preconditioner apply */
+#define AEROFX_TIMER_GRAM_SCHMIDT 6 /* Krylov orthogonalisation */
+#define AEROFX_NUM_TIMERS 8
+int aerofx_ctx_profile(aerofx_ctx *ctx, int enable);
+int aerofx_ctx_timers(aerofx_ctx *ctx, double *ms, long long *counts);
+
+/* ---- SpaceOperator<5> -------------------------------------------------- */
+/* SpaceOperator<5>::computeResidual (SpaceOperator.C:524-634), NONDESCRIPTOR
+ * form: R = (sum of edge and face fluxes)/V.  U, R: device [N][5].
+ * Returns 0, or the count of non-physical reconstructions (>0; R is then not
+ * meaningful: the reference aborts, Domain.C:876-882), or <0.
+ * Synchronises to read the error count. */
+int aerofx_residual(aerofx_ctx *ctx, const double *U, double *R);
+/* As aerofx_residual; flags bit 0: skip the 1/V scaling (raw flux sums);
+ * bit 1: do not synchronise or read the error count (returns 0; use
+ * aerofx_error_count afterwards). */
+int aerofx_residual_ex(aerofx_ctx *ctx, const double *U, double *R, int flags);
+int aerofx_error_count(aerofx_ctx *ctx, int *count);
+/* DistNodalGrad::compute with LeastSquares (DistNodalGrad.C:790-806,
+ * SubDomain.C:297-374) on primitive V [N][5]: ddx, ddy, ddz [N][5]. */
+int aerofx_gradients(aerofx_ctx *ctx, const double *V, double *ddx,
+                     double *ddy, double *ddz);
+/* VarFcnSGEuler::conservativeToPrimitive over all nodes. */
+int aerofx_primitive(aerofx_ctx *ctx, const double *U, double *V);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

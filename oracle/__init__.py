@@ -1,0 +1,209 @@
+"""ctypes access to the oracle (CPU restatement) and to oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libaerof_ref.so")
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+class OrParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("pstiff", C.c_double), ("beta", C.c_double),
+                ("second_order", C.c_int), ("rho_inf", C.c_double), ("p_inf", C.c_double),
+                ("mach", C.c_double), ("alpha", C.c_double), ("yaw", C.c_double)]
+
+
+class OrMesh(C.Structure):
+    _fields_ = [("N", C.c_int), ("T", C.c_int), ("F", C.c_int), ("E", C.c_int),
+                ("X", dp), ("tets", ip), ("faces", ip), ("codes", ip),
+                ("ptr", ip), ("tet_edges", ip), ("cv", dp), ("normals", dp), ("normal_vel", dp),
+                ("fnorm", dp), ("R", dp),
+                ("Uin", C.c_double * 5), ("Uout", C.c_double * 5),
+                ("Ub_in", C.c_double * 5), ("Ub_out", C.c_double * 5)]
+
+
+def build(quiet: bool = True) -> None:
+    """Builds liboracle.so (and oracle/_ref when the reference tree exists)."""
+    out = subprocess.DEVNULL if quiet else None
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=out)
+    if os.path.isdir("/root/reference"):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "ref")], stdout=out)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        L.or_edges.restype = C.c_int
+        L.or_setup.restype = C.c_int
+        L.or_residual_ex.restype = C.c_int
+        L.or_residual.restype = C.c_int
+        L.or_color_edges.restype = C.c_int
+        L.or_edge_priority.restype = C.c_uint32
+        L.or_edge_priority.argtypes = [C.c_uint32]
+        for name in ("or_roeflux7", "or_boundflux5", "or_vanalbada5", "or_wallflux"):
+            getattr(L, name).restype = None
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own kernels compiled from its sources (oracle/_ref)."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            raise FileNotFoundError(REF_SO)
+        R = C.CDLL(REF_SO)
+        for name in ("refh_roeflux7", "refh_boundflux5", "refh_roejac6", "refh_vanalbada5",
+                     "refh_apply5_f32", "refh_apply5_f64"):
+            getattr(R, name).restype = None
+        R.refh_invert5_f32.restype = C.c_int
+        R.refh_invert5_f64.restype = C.c_int
+        _ref = R
+    return _ref
+
+
+def params(p) -> OrParams:
+    """OrParams from an aerof_dev_amd.FlowParams-like object."""
+    return OrParams(p.gamma, p.pstiff, p.beta, int(p.second_order), p.rho_inf, p.p_inf, p.mach,
+                    p.alpha, p.yaw)
+
+
+class OracleMesh:
+    """or_mesh with its derived tables; keeps the numpy inputs alive."""
+
+    def __init__(self, mesh, p):
+        self.X = np.ascontiguousarray(mesh.X, dtype=np.float64)
+        self.tets = np.ascontiguousarray(mesh.tets, dtype=np.int32)
+        self.faces = np.ascontiguousarray(mesh.faces, dtype=np.int32)
+        self.codes = np.ascontiguousarray(mesh.codes, dtype=np.int32)
+        self.p = params(p)
+        m = OrMesh()
+        m.N, m.T, m.F = self.X.shape[0], self.tets.shape[0], self.faces.shape[0]
+        m.X, m.tets, m.faces, m.codes = _d(self.X), _i(self.tets), _i(self.faces), _i(self.codes)
+        lib().or_setup(C.byref(m), C.byref(self.p))
+        self.m = m
+        self.N, self.E, self.F = m.N, m.E, m.F
+
+    def __del__(self):
+        try:
+            lib().or_free(C.byref(self.m))
+        except Exception:
+            pass
+
+    def tables(self):
+        N, E, F = self.N, self.E, self.F
+        m = self.m
+        return dict(
+            ptr=np.ctypeslib.as_array(m.ptr, shape=(E, 2)).copy(),
+            ctrl_vol=np.ctypeslib.as_array(m.cv, shape=(N,)).copy(),
+            edge_normals=np.ctypeslib.as_array(m.normals, shape=(E, 3)).copy(),
+            normal_vel=np.ctypeslib.as_array(m.normal_vel, shape=(E,)).copy(),
+            face_normals=np.ctypeslib.as_array(m.fnorm, shape=(F, 3)).copy() if F else np.zeros((0, 3)),
+            lsq=np.ctypeslib.as_array(m.R, shape=(N, 6)).copy(),
+        )
+
+    def residual(self, U, fluxes_only=False, want_abs=False, want_grad=False):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        N = self.N
+        R = np.empty((N, 5))
+        Rabs = np.empty((N, 5)) if want_abs else None
+        V = np.empty((N, 5))
+        g = [np.empty((N, 5)) for _ in range(3)] if want_grad else None
+        ierr = lib().or_residual_ex(C.byref(self.m), C.byref(self.p), _d(U), _d(R),
+                                    _d(Rabs) if want_abs else None, int(fluxes_only), _d(V),
+                                    *( [_d(a) for a in g] if want_grad else [None, None, None]))
+        out = dict(R=R, ierr=ierr, V=V)
+        if want_abs:
+            out["Rabs"] = Rabs
+        if want_grad:
+            out["ddx"], out["ddy"], out["ddz"] = g
+        return out
+
+    def gradients(self, V):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        g = [np.empty((self.N, 5)) for _ in range(3)]
+        lib().or_gradients(self.N, self.E, self.m.X, self.m.ptr, self.m.R, _d(V), *[_d(a) for a in g])
+        return g
+
+
+def rcm(N: int, ptr: np.ndarray) -> np.ndarray:
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    renum = np.empty(N, np.int32)
+    lib().or_rcm(N, ptr.shape[0], _i(ptr), _i(renum))
+    return renum
+
+
+def color_edges(N: int, ptr: np.ndarray):
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    color = np.empty(ptr.shape[0], np.int32)
+    nc = lib().or_color_edges(N, ptr.shape[0], _i(ptr), _i(color))
+    return color, nc
+
+
+def roeflux7(gam, pstiff, n, vn, Vl, Vr, use_ref=False):
+    out = np.empty(5)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    Vl = np.ascontiguousarray(Vl, dtype=np.float64)
+    Vr = np.ascontiguousarray(Vr, dtype=np.float64)
+    if use_ref:
+        ref().refh_roeflux7(C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn), _d(Vl), _d(Vr), _d(out))
+    else:
+        lib().or_roeflux7(C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn), _d(Vl), _d(Vr), _d(out))
+    return out
+
+
+def boundflux5(gam, n, sig, ua, ub, use_ref=False):
+    out = np.empty(5)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    ua = np.ascontiguousarray(ua, dtype=np.float64)
+    ub = np.ascontiguousarray(ub, dtype=np.float64)
+    fn = ref().refh_boundflux5 if use_ref else lib().or_boundflux5
+    fn(C.c_double(gam), _d(n), C.c_double(sig), _d(ua), _d(ub), _d(out))
+    return out
+
+
+def vanalbada5(beta, Vi, ddVij, Vj, ddVji, use_ref=False):
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (Vi, ddVij, Vj, ddVji)]
+    Vij, Vji = np.empty(5), np.empty(5)
+    fn = ref().refh_vanalbada5 if use_ref else lib().or_vanalbada5
+    fn(C.c_double(beta), *[_d(x) for x in a], _d(Vij), _d(Vji))
+    return Vij, Vji
+
+
+def wallflux(n, nvel, V):
+    out = np.empty(5)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    lib().or_wallflux(_d(n), C.c_double(nvel), _d(V), _d(out))
+    return out

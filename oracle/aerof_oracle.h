@@ -1,0 +1,148 @@
+/*
+ * aerof_oracle.h -- CPU restatement of the AERO-F edge-based finite-volume
+ * residual and its Newton-Krylov building blocks.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path in
+ * aerof_dev_amd/: only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product never links or calls it.
+ *
+ * Every routine follows the loop order and operation order of the reference
+ * function cited beside it (paths relative to the reference tree).  The file
+ * is compiled with -ffp-contract=off so that, on x86-64, each floating-point
+ * operation rounds exactly once, as in the reference's own -O3 build without
+ * FMA.
+ *
+ * Data layout mirrors the reference (SVec = row-major [n][dim]).
+ */
+#ifndef AEROF_ORACLE_H
+#define AEROF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- mesh tables --------------------------------------------------------*/
+
+/* EdgeSet::find / createPointers (EdgeCore.C:55-119), Elem::numberEdges
+ * (ElemCore.C:38-50): unique edges sorted lexicographically by
+ * (min id, max id).  ptr must hold 6*T pairs; returns E.  tet_edges[T][6]
+ * receives the edge number of each local tet edge (ElemTet::edgeEndTet). */
+int or_edges(int N, int T, const int *tets, int *ptr, int *tet_edges);
+
+/* ElemTet::computeControlVolumes (ElemTetCore.C:31-44, 102-112). */
+void or_control_volumes(int N, int T, const double *X, const int *tets,
+                        double *cv);
+
+/* ElemTet::computeEdgeNormalsGCL1 with Xn == Xnp1 and Xdot == 0
+ * (ElemTetCore.C:281-330): edge normals [E][3] and normal velocities [E]. */
+void or_edge_normals(int T, const double *X, const int *tets,
+                     const int *tet_edges, int E, double *normals,
+                     double *normal_vel);
+
+/* FaceTria::computeNormal (FaceTriaCore.C:31-37): faceNorm[F][3]. */
+void or_face_normals(int F, const double *X, const int *faces,
+                     double *fnorm);
+
+/* SubDomain::computeWeightsLeastSquaresEdgePart + NodePart
+ * (SubDomainCore.C:1334-1381, 1755-1777): R[N][6]. */
+void or_lsq_weights(int N, int E, const double *X, const int *ptr, double *R);
+
+/* ---- physics kernels ----------------------------------------------------*/
+
+/* VarFcnSGEuler::conservativeToPrimitive (VarFcnSGEuler.h:307-336). */
+void or_cons_to_prim(int N, const double *U, double *V, double gam,
+                     double pstiff);
+
+/* SubDomain::computeGradientsLeastSquares (SubDomain.C:137-159, 297-374). */
+void or_gradients(int N, int E, const double *X, const int *ptr,
+                  const double *R, const double *V, double *ddx, double *ddy,
+                  double *ddz);
+
+/* RecFcn::vanalbada per variable, RecFcnVanAlbada<5>::compute
+ * (RecFcn.h:42-50, 269-281, 433-446; RecFcnDesc.h:705-713). */
+void or_vanalbada5(double beta, const double *Vi, const double *ddVij,
+                   const double *Vj, const double *ddVji, double *Vij,
+                   double *Vji);
+
+/* ROEFLUX7 (f77src/roeflux7.f:1-202), type 0 (5 equations). */
+void or_roeflux7(double gam, double pstiff, const double *enormal,
+                 double evitno, const double *Ug, const double *Ud,
+                 double *phi);
+
+/* BOUNDFLUX5 (f77src/boundflux5.f:1-111), type 0. */
+void or_boundflux5(double gam, const double *normal, double sig,
+                   const double *ua, const double *ub, double *flux);
+
+/* FluxFcnSGWallEuler3D::compute (FluxFcnDescSG.C:779-789). */
+void or_wallflux(const double *normal, double normal_vel, const double *V,
+                 double *flux);
+
+/* ---- residual -----------------------------------------------------------*/
+
+typedef struct {
+  double gamma;      /* specific heat ratio */
+  double pstiff;     /* stiffened-gas pressure constant (0: perfect gas) */
+  double beta;       /* MUSCL beta (Schemes.NavierStokes.Beta) */
+  int second_order;  /* 1: linear reconstruction + Van Albada; 0: constant */
+  double rho_inf, p_inf, mach, alpha, yaw;  /* freestream (radians) */
+} or_params;
+
+typedef struct {
+  int N, T, F, E;
+  const double *X;      /* [N][3] */
+  const int *tets;      /* [T][4] */
+  const int *faces;     /* [F][3] */
+  const int *codes;     /* [F] */
+  /* derived tables (filled by or_setup) */
+  int *ptr;             /* [E][2] */
+  int *tet_edges;       /* [T][6] */
+  double *cv;           /* [N] */
+  double *normals;      /* [E][3] */
+  double *normal_vel;   /* [E] */
+  double *fnorm;        /* [F][3] */
+  double *R;            /* [N][6] */
+  double Uin[5], Uout[5];
+  double Ub_in[5], Ub_out[5];   /* face states (Face::assignFreeStreamValues2) */
+} or_mesh;
+
+/* Builds every derived table of or_mesh (allocates; free with or_free). */
+int or_setup(or_mesh *m, const or_params *p);
+void or_free(or_mesh *m);
+
+/* DistBcDataEuler::setBoundaryConditionsGas (DistBcData.C:652-706) and the
+ * per-node farfield state (DistBcData.C:706-735): Uin/Uout[5]. */
+void or_freestream(const or_params *p, double *Uin, double *Uout);
+
+/* SpaceOperator<5>::computeResidual (SpaceOperator.C:524-634) with the
+ * NONDESCRIPTOR 1/V scaling.  Returns the reference ierr (number of
+ * non-physical reconstructions, Edge.C:414-419).  If fluxes_only != 0 the
+ * 1/V scaling is skipped.  Optional ddx/ddy/ddz/V receive intermediates. */
+int or_residual(const or_mesh *m, const or_params *p, const double *U,
+                double *R, int fluxes_only, double *V_out, double *ddx,
+                double *ddy, double *ddz);
+
+/* As or_residual; Rabs (optional) receives the same sums taken over
+ * |contribution|, the per-node scale used by parity tolerances. */
+int or_residual_ex(const or_mesh *m, const or_params *p, const double *U,
+                   double *R, double *Rabs, int fluxes_only, double *V_out,
+                   double *ddx, double *ddy, double *ddz);
+
+/* ---- orderings ----------------------------------------------------------*/
+
+/* Connectivity::renumByComponent(2) over the edge-based node graph with
+ * self loops (ConnectivityCore.C:324-635, SubDomainCore.C:253-320):
+ * renum[old] = new. */
+void or_rcm(int N, int E, const int *ptr, int *renum);
+
+/* Jones-Plassmann edge colouring with priority = or_edge_priority(l) (no
+ * reference counterpart; project rule, see the C source).  Returns the
+ * number of colours. */
+uint32_t or_edge_priority(uint32_t l);
+int or_color_edges(int N, int E, const int *ptr, int *color);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
