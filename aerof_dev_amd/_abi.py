@@ -14,8 +14,8 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libaerofx.so")
 AEROFX_BC_SLIP_WALL = 2
 AEROFX_BC_INLET = 4
 AEROFX_BC_OUTLET = 5
-TIMER_NAMES = ("nodalGrad", "fvTerm", "fvJac", "fluidPrecSetup", "jacApply", "precApply",
-               "gramSchmidt", "reserved")
+TIMER_NAMES = ("nodalGrad", "fvTerm", "fvTermNodes", "fvJac", "fluidPrecSetup", "jacApply",
+               "precApply", "gramSchmidt")
 
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
@@ -64,7 +64,7 @@ SIGNATURES = {
     "aerofx_ctx_create": (C.c_int, [C.POINTER(Mesh), C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "aerofx_ctx_destroy": (C.c_int, [C.c_void_p]),
     "aerofx_ctx_sizes": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p]),
-    "aerofx_ctx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "aerofx_ctx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "aerofx_ctx_sync": (C.c_int, [C.c_void_p]),
     "aerofx_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_timers": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_longlong)]),

@@ -1,0 +1,104 @@
+// context.h -- the per-mesh context behind the C ABI (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "aerofx.h"
+#include "device_physics.h"
+
+namespace aerofx {
+
+constexpr int kBlock = 256;       // threads per block of the edge/entry kernels
+constexpr int kMaxRows = 64;      // node rows per block (LDS staging bound)
+constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, blocks dealt round-robin
+
+// Device-resident mesh tables.  All arrays are laid out for the kernel that
+// streams them: contiguous per node row or per edge.
+struct DevTables {
+  int N, E, F;
+  const double *X4;      // [N][4] coordinates padded to 32 B
+  const double *lsq6;    // [N][6] (1/r11, 1/r22, 1/r33, r12/r11, r23/r22, psi)
+  const double *invcv;   // [N] 1/ctrlVol
+  // gradient pass: node adjacency, rows sorted ascending (= edge-id order)
+  int ngchunks;
+  const int *gchunk;     // [ngchunks+1] row starts
+  const int *rowptr;     // [N+1]
+  const int *adj;        // [2E]
+  // flux pass A: edges grouped by their lower node i (edge order)
+  int nfchunks;
+  const int *fchunk;     // [nfchunks+1] row starts
+  const int *eup;        // [N+1] first edge whose lower node is i
+  const int *jcol;       // [E] upper node j of edge l
+  const int *lslot;      // [E] slot of edge l in the lower list of j
+  const double *normal;  // [E][3] dual-face area vector oriented i -> j
+  // flux pass B: per node, lower-edge slots and boundary faces
+  const int *lrow;       // [N+1] lower-list pointers (slots ascending in i)
+  const int *frow;       // [N+1] face-list pointers
+  const int *fidx;       // [3F] faces touching each node, ascending
+  const int *fcode;      // [F] boundary code
+  const double *fnorm3;  // [F][3] (1/3) face normal
+};
+
+struct FluxConsts {
+  Gas gas;
+  double beta;
+  double ub_in[5], ub_out[5];
+};
+
+int set_hip_error(hipError_t e, const char *what);
+
+}  // namespace aerofx
+
+struct aerofx_ctx {
+  int N = 0, T = 0, F = 0, E = 0;
+  aerofx_params p{};
+  aerofx::Gas gas{};
+  aerofx::FluxConsts fc{};
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  // host copies of the set-up tables
+  std::vector<int> ptr;
+  std::vector<double> cv, normals, fnorm, lsq;
+  // device
+  aerofx::DevTables tab{};
+  std::vector<void *> allocs;
+  double *d_V = nullptr;     // [N][5] primitive state of the last residual
+  double *d_grad = nullptr;  // [N][16] LSQ gradients, [d*5+k]
+  double *d_Fm = nullptr;    // [E][5] edge fluxes in lower-list slot order
+  double *d_P = nullptr;     // [N][5] per-node sums over upper edges
+  int *d_err = nullptr;
+  int *h_err = nullptr;
+  // per-kernel timers (the reference's Timer buckets, Timer.h:12-15)
+  bool prof = false;
+  struct Pending {
+    int bucket;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  double timer_ms[AEROFX_NUM_TIMERS] = {0};
+  long long timer_n[AEROFX_NUM_TIMERS] = {0};
+};
+
+namespace aerofx {
+
+// Brackets one kernel launch with events on the context stream when
+// profiling is on.
+struct ProfScope {
+  aerofx_ctx *c;
+  int bucket;
+  hipEvent_t a = nullptr;
+  ProfScope(aerofx_ctx *ctx, int b);
+  ~ProfScope();
+};
+
+// device allocation owned by the context
+int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out);
+
+// residual passes (residual.hip); all enqueue on c->stream
+int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale);
+int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz);
+int launch_primitive(aerofx_ctx *c, const double *U, double *V);
+
+}  // namespace aerofx

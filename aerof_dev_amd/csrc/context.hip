@@ -1,0 +1,367 @@
+// context.hip -- context set-up and the context part of the C ABI.
+//
+// aerofx_ctx_create builds, on the host, the reference's tables for the mesh
+// (edge table, dual normals, control volumes, LSQ factors: host_mesh.cpp),
+// derives the kernel-facing layouts (adjacency CSR, edge rows, lower-edge
+// slot lists, per-node face lists, block chunking) and uploads them once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "context.h"
+#include "host_internal.h"
+#include "host_mesh.h"
+
+namespace aerofx {
+
+int set_hip_error(hipError_t e, const char *what) {
+  return set_error(AEROFX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+  if (e != hipSuccess) return set_hip_error(e, "hipMalloc");
+  c->allocs.push_back(p);
+  *out = p;
+  return 0;
+}
+
+namespace {
+
+template <class T>
+int upload(aerofx_ctx *c, const std::vector<T> &v, const T **dst) {
+  void *p = nullptr;
+  int rc = ctx_alloc(c, v.size() * sizeof(T), &p);
+  if (rc) return rc;
+  if (!v.empty()) {
+    hipError_t e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_hip_error(e, "hipMemcpy");
+  }
+  *dst = static_cast<const T *>(p);
+  return 0;
+}
+
+hipEvent_t take_event(aerofx_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void collect_timers(aerofx_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->timer_ms[p.bucket] += ms;
+      c->timer_n[p.bucket] += 1;
+    }
+    c->pool.push_back(p.a);
+    c->pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+void destroy(aerofx_ctx *c) {
+  collect_timers(c);
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->h_err) (void)hipHostFree(c->h_err);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+// Whole-row chunks with at most kBlock items and kMaxRows rows.
+std::vector<int> chunk_rows(const std::vector<int> &rowptr, int N, const char **why) {
+  std::vector<int> ch(1, 0);
+  int acc = 0, rows = 0;
+  for (int n = 0; n < N; ++n) {
+    int cnt = rowptr[n + 1] - rowptr[n];
+    if (cnt > kBlock) {
+      *why = "node degree exceeds 256";
+      return {};
+    }
+    if (acc + cnt > kBlock || rows == kMaxRows) {
+      ch.push_back(n);
+      acc = 0;
+      rows = 0;
+    }
+    acc += cnt;
+    rows += 1;
+  }
+  ch.push_back(N);
+  return ch;
+}
+
+}  // namespace
+
+ProfScope::ProfScope(aerofx_ctx *ctx, int b) : c(ctx), bucket(b) {
+  if (c->prof && (a = take_event(c))) (void)hipEventRecord(a, c->stream);
+}
+
+ProfScope::~ProfScope() {
+  if (!a) return;
+  hipEvent_t b = take_event(c);
+  if (!b) return;
+  (void)hipEventRecord(b, c->stream);
+  c->pending.push_back({bucket, a, b});
+}
+
+}  // namespace aerofx
+
+using namespace aerofx;
+
+extern "C" {
+
+int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx **out) {
+  if (!m || !p || !out) return set_error(AEROFX_ERR_ARG, "ctx_create: null argument");
+  *out = nullptr;
+  const int N = m->num_nodes, T = m->num_tets, F = m->num_faces;
+  if (N <= 0 || T <= 0 || F < 0 || !m->X || !m->tets || (F > 0 && (!m->faces || !m->face_codes)))
+    return set_error(AEROFX_ERR_ARG, "ctx_create: bad mesh");
+  if (!(p->gamma > 1.0) || !(p->rho_inf > 0.0) || !(p->p_inf > 0.0))
+    return set_error(AEROFX_ERR_ARG, "ctx_create: bad parameters");
+  for (long long k = 0; k < 4LL * T; ++k)
+    if (m->tets[k] < 0 || m->tets[k] >= N) return set_error(AEROFX_ERR_MESH, "ctx_create: tet node out of range");
+  for (long long k = 0; k < 3LL * F; ++k)
+    if (m->faces[k] < 0 || m->faces[k] >= N) return set_error(AEROFX_ERR_MESH, "ctx_create: face node out of range");
+  for (int f = 0; f < F; ++f) {
+    int cd = m->face_codes[f] < 0 ? -m->face_codes[f] : m->face_codes[f];
+    if (cd != AEROFX_BC_SLIP_WALL && cd != AEROFX_BC_INLET && cd != AEROFX_BC_OUTLET)
+      return set_error(AEROFX_ERR_MESH, "ctx_create: unsupported face code (2, 4, 5 only)");
+  }
+  aerofx_ctx *c = new aerofx_ctx();
+  c->N = N;
+  c->T = T;
+  c->F = F;
+  c->p = *p;
+  c->gas.gam = p->gamma;
+  c->gas.gam1 = p->gamma - 1.0;
+  c->gas.invgam1 = 1.0 / c->gas.gam1;
+  c->gas.pstiff = p->pstiff;
+  c->fc.gas = c->gas;
+  c->fc.beta = p->beta;
+  {
+    double Uf[5];
+    freestream_state(*p, Uf);
+    face_state(Uf, c->fc.ub_in);
+    face_state(Uf, c->fc.ub_out);
+  }
+  int rc = 0;
+  try {
+    std::vector<int> tet_edges;
+    build_edges(N, T, m->tets, c->ptr, &tet_edges);
+    c->E = (int)(c->ptr.size() / 2);
+    const int E = c->E;
+    c->cv.resize(N);
+    c->normals.resize(3 * (size_t)E);
+    c->fnorm.resize(3 * (size_t)F);
+    c->lsq.resize(6 * (size_t)N);
+    control_volumes(N, T, m->X, m->tets, c->cv.data());
+    edge_normals(T, m->X, m->tets, tet_edges.data(), E, c->normals.data());
+    face_normals(F, m->X, m->faces, c->fnorm.data());
+    lsq_factors(N, E, m->X, c->ptr.data(), c->lsq.data());
+    for (int i = 0; i < N; ++i)
+      if (!(c->cv[i] > 0.0)) {
+        destroy(c);
+        return set_error(AEROFX_ERR_MESH, "ctx_create: non-positive control volume");
+      }
+    const int *ptr = c->ptr.data();
+    // gradient pass: node adjacency in edge-id order
+    std::vector<int> rowptr, adj, eid;
+    node_csr(N, E, ptr, rowptr, adj, eid);
+    // flux pass A: edge rows by lower node; pass B: lower-edge slot lists
+    std::vector<int> eup(N + 1, 0), jcol(E), lslot(E), lrow(N + 1, 0);
+    for (int l = 0; l < E; ++l) {
+      eup[ptr[2 * (size_t)l] + 1]++;
+      lrow[ptr[2 * (size_t)l + 1] + 1]++;
+    }
+    for (int n = 0; n < N; ++n) {
+      eup[n + 1] += eup[n];
+      lrow[n + 1] += lrow[n];
+    }
+    {
+      std::vector<int> fill(lrow.begin(), lrow.end() - 1);
+      for (int l = 0; l < E; ++l) {
+        jcol[l] = ptr[2 * (size_t)l + 1];
+        lslot[l] = fill[jcol[l]]++;  // ascending l = ascending lower node
+      }
+    }
+    // per-node boundary faces in face order
+    std::vector<int> frow(N + 1, 0), fidx(3 * (size_t)F), fcode(F);
+    for (long long k = 0; k < 3LL * F; ++k) frow[m->faces[k] + 1]++;
+    for (int n = 0; n < N; ++n) frow[n + 1] += frow[n];
+    {
+      std::vector<int> fill(frow.begin(), frow.end() - 1);
+      for (int f = 0; f < F; ++f) {
+        fcode[f] = m->face_codes[f];
+        for (int a = 0; a < 3; ++a) fidx[fill[m->faces[3 * f + a]]++] = f;
+      }
+    }
+    const char *why = nullptr;
+    std::vector<int> gchunk = chunk_rows(rowptr, N, &why);
+    std::vector<int> fchunk = why ? std::vector<int>() : chunk_rows(eup, N, &why);
+    if (why) {
+      destroy(c);
+      return set_error(AEROFX_ERR_MESH, std::string("ctx_create: ") + why);
+    }
+    // node-derived tables
+    std::vector<double> X4(4 * (size_t)N), lsq6(6 * (size_t)N), invcv(N), fnorm3(3 * (size_t)F);
+    for (int n = 0; n < N; ++n) {
+      for (int a = 0; a < 3; ++a) X4[4 * (size_t)n + a] = m->X[3 * (size_t)n + a];
+      X4[4 * (size_t)n + 3] = 0.0;
+      // computeLocalWeightsLeastSquares (SubDomain.C:137-159): node factors
+      const double *R = c->lsq.data() + 6 * (size_t)n;
+      double or11 = 1.0 / R[0], or22 = 1.0 / R[3], or33 = 1.0 / R[5];
+      double *L = lsq6.data() + 6 * (size_t)n;
+      L[0] = or11;
+      L[1] = or22;
+      L[2] = or33;
+      L[3] = R[1] * or11;
+      L[4] = R[4] * or22;
+      L[5] = (R[1] * R[4] - R[2] * R[3]) * or11 * or22;
+      invcv[n] = 1.0 / c->cv[n];  // SpaceOperator.C:615
+    }
+    const double third = 1.0 / 3.0;  // FaceTria::getNormal (FaceTriaCore.C:288-292)
+    for (size_t k = 0; k < 3 * (size_t)F; ++k) fnorm3[k] = third * c->fnorm[k];
+    DevTables &t = c->tab;
+    t.N = N;
+    t.E = E;
+    t.F = F;
+    if ((rc = upload(c, X4, &t.X4)) || (rc = upload(c, lsq6, &t.lsq6)) || (rc = upload(c, invcv, &t.invcv)) ||
+        (rc = upload(c, gchunk, &t.gchunk)) || (rc = upload(c, rowptr, &t.rowptr)) ||
+        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, fchunk, &t.fchunk)) || (rc = upload(c, eup, &t.eup)) ||
+        (rc = upload(c, jcol, &t.jcol)) || (rc = upload(c, lslot, &t.lslot)) ||
+        (rc = upload(c, c->normals, &t.normal)) || (rc = upload(c, lrow, &t.lrow)) ||
+        (rc = upload(c, frow, &t.frow)) || (rc = upload(c, fidx, &t.fidx)) || (rc = upload(c, fcode, &t.fcode)) ||
+        (rc = upload(c, fnorm3, &t.fnorm3))) {
+      destroy(c);
+      return rc;
+    }
+    t.ngchunks = (int)gchunk.size() - 1;
+    t.nfchunks = (int)fchunk.size() - 1;
+    void *pv = nullptr;
+    if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
+    c->d_V = static_cast<double *>(pv);
+    if ((rc = ctx_alloc(c, 16 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
+    c->d_grad = static_cast<double *>(pv);
+    if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)E, &pv))) { destroy(c); return rc; }
+    c->d_Fm = static_cast<double *>(pv);
+    if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
+    c->d_P = static_cast<double *>(pv);
+    if ((rc = ctx_alloc(c, 4 * sizeof(int), &pv))) { destroy(c); return rc; }
+    c->d_err = static_cast<int *>(pv);
+    hipError_t e = hipMemset(c->d_err, 0, 4 * sizeof(int));
+    if (e != hipSuccess) { destroy(c); return set_hip_error(e, "hipMemset"); }
+    e = hipHostMalloc((void **)&c->h_err, 4 * sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) { destroy(c); return set_hip_error(e, "hipHostMalloc"); }
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { destroy(c); return set_hip_error(e, "hipStreamCreate"); }
+    c->stream = c->own_stream;
+  } catch (std::exception &ex) {
+    destroy(c);
+    return set_error(AEROFX_ERR_MESH, ex.what());
+  }
+  *out = c;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_destroy(aerofx_ctx *c) {
+  if (!c) return AEROFX_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  destroy(c);
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_sizes(const aerofx_ctx *c, int *N, int *E, int *F) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "ctx_sizes: null ctx");
+  if (N) *N = c->N;
+  if (E) *E = c->E;
+  if (F) *F = c->F;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_set_stream(aerofx_ctx *c, void *stream, int use_context_stream) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "set_stream: null ctx");
+  c->stream = use_context_stream ? c->own_stream : static_cast<hipStream_t>(stream);
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_sync(aerofx_ctx *c) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "sync: null ctx");
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "hipStreamSynchronize");
+}
+
+int aerofx_ctx_profile(aerofx_ctx *c, int enable) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "ctx_profile: null ctx");
+  collect_timers(c);
+  for (int k = 0; k < AEROFX_NUM_TIMERS; ++k) {
+    c->timer_ms[k] = 0.0;
+    c->timer_n[k] = 0;
+  }
+  c->prof = enable != 0;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_timers(aerofx_ctx *c, double *ms, long long *counts) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "ctx_timers: null ctx");
+  collect_timers(c);
+  for (int k = 0; k < AEROFX_NUM_TIMERS; ++k) {
+    if (ms) ms[k] = c->timer_ms[k];
+    if (counts) counts[k] = c->timer_n[k];
+  }
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_tables(const aerofx_ctx *c, int *ptr, double *cv, double *en, double *fn, double *lsq) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "ctx_tables: null ctx");
+  if (ptr) std::memcpy(ptr, c->ptr.data(), c->ptr.size() * sizeof(int));
+  if (cv) std::memcpy(cv, c->cv.data(), c->cv.size() * sizeof(double));
+  if (en) std::memcpy(en, c->normals.data(), c->normals.size() * sizeof(double));
+  if (fn) std::memcpy(fn, c->fnorm.data(), c->fnorm.size() * sizeof(double));
+  if (lsq) std::memcpy(lsq, c->lsq.data(), c->lsq.size() * sizeof(double));
+  return AEROFX_OK;
+}
+
+int aerofx_error_count(aerofx_ctx *c, int *count) {
+  if (!c || !count) return set_error(AEROFX_ERR_ARG, "error_count: null argument");
+  hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "hipMemcpyAsync");
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "hipStreamSynchronize");
+  *count = c->h_err[0];
+  return AEROFX_OK;
+}
+
+int aerofx_residual_ex(aerofx_ctx *c, const double *U, double *R, int flags) {
+  if (!c || !U || !R) return set_error(AEROFX_ERR_ARG, "residual: null argument");
+  int rc = launch_residual(c, U, R, (flags & 1) == 0);
+  if (rc) return rc;
+  if (flags & 2) return AEROFX_OK;
+  int cnt = 0;
+  rc = aerofx_error_count(c, &cnt);
+  return rc ? rc : cnt;
+}
+
+int aerofx_residual(aerofx_ctx *c, const double *U, double *R) { return aerofx_residual_ex(c, U, R, 0); }
+
+int aerofx_gradients(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz) {
+  if (!c || !V || !ddx || !ddy || !ddz) return set_error(AEROFX_ERR_ARG, "gradients: null argument");
+  return launch_gradients_v(c, V, ddx, ddy, ddz);
+}
+
+int aerofx_primitive(aerofx_ctx *c, const double *U, double *V) {
+  if (!c || !U || !V) return set_error(AEROFX_ERR_ARG, "primitive: null argument");
+  return launch_primitive(c, U, V);
+}
+
+}  // extern "C"

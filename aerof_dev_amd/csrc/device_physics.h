@@ -27,13 +27,15 @@ __device__ __forceinline__ void prim_from_cons(const double U[5], const Gas &g, 
   V[4] = (g.gam - 1.0) * (U[4] - 0.5 * U[0] * vel2) - g.gam * g.pstiff;
 }
 
+// Branch-free: the numerator is selected, then one division (0/den = 0 for
+// ab < 0), so lanes of a wave never diverge.
 __device__ __forceinline__ double va_fcn(double eps, double a, double b) {
   double a2 = a * a;
   double b2 = b * b;
   double ab = a * b;
-  if (ab > 0.0) return (a * (b2 + eps) + b * (a2 + eps)) / (a2 + b2 + 2.0 * eps);
-  if (ab == 0.0) return 0.5 * eps * (a + b) / (a2 + b2 + 2.0 * eps);
-  return 0.0;
+  double den = a2 + b2 + 2.0 * eps;
+  double num = ab > 0.0 ? (a * (b2 + eps) + b * (a2 + eps)) : (ab == 0.0 ? 0.5 * eps * (a + b) : 0.0);
+  return num / den;
 }
 
 // V_ij = V_i + 1/2 VA(d_i, dV), V_ji = V_j - 1/2 VA(d_j, dV) for one variable.

@@ -68,8 +68,10 @@ class SpaceOperator:
         import torch
         if stream is None and torch.cuda.is_available():
             stream = torch.cuda.current_stream()
-        handle = stream.cuda_stream if stream is not None else None
-        check(lib().aerofx_ctx_set_stream(self._h, C.c_void_p(handle) if handle else None), "set_stream")
+        if stream is None:
+            check(lib().aerofx_ctx_set_stream(self._h, None, 1), "set_stream")
+        else:
+            check(lib().aerofx_ctx_set_stream(self._h, C.c_void_p(stream.cuda_stream), 0), "set_stream")
 
     def profile(self, enable: bool = True):
         """Turns per-kernel event timing on/off (resets the buckets)."""

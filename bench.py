@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--n", type=int, default=NX, help="hexes per axis")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -88,6 +90,19 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        # A plain copy kernel of known size: 1 GiB read + 1 GiB written per
+        # launch.  rocprofv3's FETCH_SIZE / WRITE_SIZE on it give the scale
+        # between counter units and bytes.
+        n = (1 << 30) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "launches": 3}))
+        return
 
     from aerof_dev_amd import FlowParams, init_state
     from aerof_dev_amd import mesh as M

@@ -110,8 +110,9 @@ int aerofx_ctx_create(const aerofx_mesh *mesh, const aerofx_params *params,
 int aerofx_ctx_destroy(aerofx_ctx *ctx);
 int aerofx_ctx_sizes(const aerofx_ctx *ctx, int *num_nodes, int *num_edges,
                      int *num_faces);
-/* Uses `stream` (a hipStream_t) for all later work; NULL = context stream. */
-int aerofx_ctx_set_stream(aerofx_ctx *ctx, void *stream);
+/* Enqueues all later work on `stream` (a hipStream_t; NULL is the null
+ * stream), or on the context's own stream if use_context_stream != 0. */
+int aerofx_ctx_set_stream(aerofx_ctx *ctx, void *stream, int use_context_stream);
 int aerofx_ctx_sync(aerofx_ctx *ctx);
 /* Host copies of the set-up tables (for parity checks): edge table [E][2],
  * control volumes [N], edge normals [E][3], face normals [F][3] (full
@@ -126,12 +127,13 @@ int aerofx_ctx_tables(const aerofx_ctx *ctx, int *ptr, double *ctrl_vol,
  * events on the context stream; aerofx_ctx_timers synchronises and returns
  * accumulated milliseconds and launch counts per bucket. Enabling resets. */
 #define AEROFX_TIMER_NODAL_GRAD 0   /* LSQ gradient pass */
-#define AEROFX_TIMER_FV_TERM 1      /* edge + face flux pass */
-#define AEROFX_TIMER_FV_JAC 2       /* Jacobian assembly */
-#define AEROFX_TIMER_PREC_SETUP 3   /* block-Jacobi inversion */
-#define AEROFX_TIMER_JAC_APPLY 4    /* Jacobian-vector product */
-#define AEROFX_TIMER_PREC_APPLY 5   /* preconditioner apply */
-#define AEROFX_TIMER_GRAM_SCHMIDT 6 /* Krylov orthogonalisation */
+#define AEROFX_TIMER_FV_TERM 1      /* edge flux pass */
+#define AEROFX_TIMER_FV_NODES 2     /* node assembly of fluxes + faces */
+#define AEROFX_TIMER_FV_JAC 3       /* Jacobian assembly */
+#define AEROFX_TIMER_PREC_SETUP 4   /* block-Jacobi inversion */
+#define AEROFX_TIMER_JAC_APPLY 5    /* Jacobian-vector product */
+#define AEROFX_TIMER_PREC_APPLY 6   /* preconditioner apply */
+#define AEROFX_TIMER_GRAM_SCHMIDT 7 /* Krylov orthogonalisation */
 #define AEROFX_NUM_TIMERS 8
 int aerofx_ctx_profile(aerofx_ctx *ctx, int enable);
 int aerofx_ctx_timers(aerofx_ctx *ctx, double *ms, long long *counts);
