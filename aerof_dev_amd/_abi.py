@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libaerofx.so")
+# AEROFX_LIB may point at another build of the same library (e.g. a build
+# with different compiler flags); the default is the in-tree build.
+LIB_PATH = os.environ.get("AEROFX_LIB") or os.path.join(_HERE, "_lib", "libaerofx.so")
 
 AEROFX_BC_SLIP_WALL = 2
 AEROFX_BC_INLET = 4
@@ -49,6 +51,19 @@ class Mesh(C.Structure):
     ]
 
 
+class NewtonOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("max_steps", C.c_int), ("eps", C.c_double),
+                ("cfl0", C.c_double), ("cfl_max", C.c_double), ("ser", C.c_double),
+                ("cfl_coef1", C.c_double), ("cfl_coef2", C.c_double),
+                ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("krylov_its", C.c_int),
+                ("res0", C.c_double), ("res", C.c_double), ("final_cfl", C.c_double),
+                ("seconds", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/aerofx.h
 SIGNATURES = {
     "aerofx_last_error": (C.c_char_p, []),
@@ -74,6 +89,15 @@ SIGNATURES = {
     "aerofx_error_count": (C.c_int, [C.c_void_p, c_int_p]),
     "aerofx_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_primitive": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_timestep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    "aerofx_jacobian_setup": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "aerofx_jacobian_blocks": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "aerofx_jvp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_prec_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_gmres": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                               c_int_p, c_double_p]),
+    "aerofx_steady_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats),
+                                      c_double_p, C.c_int]),
 }
 
 _lib = None

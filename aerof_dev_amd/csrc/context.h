@@ -21,24 +21,28 @@ struct DevTables {
   const double *X4;      // [N][4] coordinates padded to 32 B
   const double *lsq6;    // [N][6] (1/r11, 1/r22, 1/r33, r12/r11, r23/r22, psi)
   const double *invcv;   // [N] 1/ctrlVol
+  const double *cv;      // [N] ctrlVol
   // gradient pass: node adjacency, rows sorted ascending (= edge-id order)
   int ngchunks;
   const int *gchunk;     // [ngchunks+1] row starts
   const int *rowptr;     // [N+1]
   const int *adj;        // [2E]
+  const int *eidx;       // [2E] edge id of each adjacency entry
   // flux pass A: edges grouped by their lower node i (edge order)
   int nfchunks;
   const int *fchunk;     // [nfchunks+1] row starts
   const int *eup;        // [N+1] first edge whose lower node is i
-  const int *jcol;       // [E] upper node j of edge l
-  const int *lslot;      // [E] slot of edge l in the lower list of j
+  const int2 *eij;       // [E] edge endpoints (i, j), i < j
+
   const double *normal;  // [E][3] dual-face area vector oriented i -> j
   // flux pass B: per node, lower-edge slots and boundary faces
-  const int *lrow;       // [N+1] lower-list pointers (slots ascending in i)
+  const int *lrow;       // [N+1] lower-list pointers
+  const int *ledge;      // [E] edges (m, n), m < n, of node n, ascending m
   const int *frow;       // [N+1] face-list pointers
   const int *fidx;       // [3F] faces touching each node, ascending
   const int *fcode;      // [F] boundary code
   const double *fnorm3;  // [F][3] (1/3) face normal
+  const double *fnorm;   // [F][3] face normal
 };
 
 struct FluxConsts {
@@ -48,6 +52,7 @@ struct FluxConsts {
 };
 
 int set_hip_error(hipError_t e, const char *what);
+struct Krylov;
 
 }  // namespace aerofx
 
@@ -65,10 +70,15 @@ struct aerofx_ctx {
   std::vector<void *> allocs;
   double *d_V = nullptr;     // [N][5] primitive state of the last residual
   double *d_grad = nullptr;  // [N][16] LSQ gradients, [d*5+k]
-  double *d_Fm = nullptr;    // [E][5] edge fluxes in lower-list slot order
+  double *d_Fm = nullptr;    // [E][5] edge fluxes in edge order
   double *d_P = nullptr;     // [N][5] per-node sums over upper edges
-  int *d_err = nullptr;
+  int *d_err = nullptr;      // [0] reconstruction errors, [1] singular blocks, [2] bad states
   int *h_err = nullptr;
+  // implicit operators (jacobian.hip) and Krylov workspace (krylov.hip)
+  int jac_precision = -1;    // 0: FP64 blocks, 1: FP32 blocks
+  void *d_jacD = nullptr, *d_jacB = nullptr, *d_jacDi = nullptr;
+  double *d_dt = nullptr;
+  aerofx::Krylov *ksp = nullptr;
   // per-kernel timers (the reference's Timer buckets, Timer.h:12-15)
   bool prof = false;
   struct Pending {
@@ -96,9 +106,18 @@ struct ProfScope {
 // device allocation owned by the context
 int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out);
 
+__global__ void k_zero_int(int *p, int n);
+
 // residual passes (residual.hip); all enqueue on c->stream
 int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale);
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz);
 int launch_primitive(aerofx_ctx *c, const double *U, double *V);
+
+// implicit operators (jacobian.hip)
+int launch_timestep(aerofx_ctx *c, double cfl, double *dt);
+int launch_jacobian(aerofx_ctx *c, const double *dt, int precision);
+int launch_h1_apply(aerofx_ctx *c, const double *p, double *y);
+int launch_prec_apply(aerofx_ctx *c, const double *y, double *x);
+void free_krylov(aerofx_ctx *c);
 
 }  // namespace aerofx

@@ -72,6 +72,7 @@ void collect_timers(aerofx_ctx *c) {
 
 void destroy(aerofx_ctx *c) {
   collect_timers(c);
+  free_krylov(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -179,7 +180,8 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     std::vector<int> rowptr, adj, eid;
     node_csr(N, E, ptr, rowptr, adj, eid);
     // flux pass A: edge rows by lower node; pass B: lower-edge slot lists
-    std::vector<int> eup(N + 1, 0), jcol(E), lslot(E), lrow(N + 1, 0);
+    std::vector<int> eup(N + 1, 0), ledge(E), lrow(N + 1, 0);
+    std::vector<int2> eij(E);
     for (int l = 0; l < E; ++l) {
       eup[ptr[2 * (size_t)l] + 1]++;
       lrow[ptr[2 * (size_t)l + 1] + 1]++;
@@ -191,8 +193,8 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     {
       std::vector<int> fill(lrow.begin(), lrow.end() - 1);
       for (int l = 0; l < E; ++l) {
-        jcol[l] = ptr[2 * (size_t)l + 1];
-        lslot[l] = fill[jcol[l]]++;  // ascending l = ascending lower node
+        eij[l] = make_int2(ptr[2 * (size_t)l], ptr[2 * (size_t)l + 1]);
+        ledge[fill[eij[l].y]++] = l;  // ascending l = ascending lower node
       }
     }
     // per-node boundary faces in face order
@@ -236,13 +238,13 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     t.N = N;
     t.E = E;
     t.F = F;
-    if ((rc = upload(c, X4, &t.X4)) || (rc = upload(c, lsq6, &t.lsq6)) || (rc = upload(c, invcv, &t.invcv)) ||
+    if ((rc = upload(c, X4, &t.X4)) || (rc = upload(c, lsq6, &t.lsq6)) || (rc = upload(c, invcv, &t.invcv)) || (rc = upload(c, c->cv, &t.cv)) ||
         (rc = upload(c, gchunk, &t.gchunk)) || (rc = upload(c, rowptr, &t.rowptr)) ||
-        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, fchunk, &t.fchunk)) || (rc = upload(c, eup, &t.eup)) ||
-        (rc = upload(c, jcol, &t.jcol)) || (rc = upload(c, lslot, &t.lslot)) ||
+        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, eid, &t.eidx)) || (rc = upload(c, fchunk, &t.fchunk)) || (rc = upload(c, eup, &t.eup)) ||
+        (rc = upload(c, eij, &t.eij)) || (rc = upload(c, ledge, &t.ledge)) ||
         (rc = upload(c, c->normals, &t.normal)) || (rc = upload(c, lrow, &t.lrow)) ||
         (rc = upload(c, frow, &t.frow)) || (rc = upload(c, fidx, &t.fidx)) || (rc = upload(c, fcode, &t.fcode)) ||
-        (rc = upload(c, fnorm3, &t.fnorm3))) {
+        (rc = upload(c, fnorm3, &t.fnorm3)) || (rc = upload(c, c->fnorm, &t.fnorm))) {
       destroy(c);
       return rc;
     }
@@ -257,6 +259,8 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     c->d_Fm = static_cast<double *>(pv);
     if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
     c->d_P = static_cast<double *>(pv);
+    if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
+    c->d_dt = static_cast<double *>(pv);
     if ((rc = ctx_alloc(c, 4 * sizeof(int), &pv))) { destroy(c); return rc; }
     c->d_err = static_cast<int *>(pv);
     hipError_t e = hipMemset(c->d_err, 0, 4 * sizeof(int));

@@ -17,6 +17,44 @@ struct Gas {
   double gam, gam1, invgam1, pstiff;
 };
 
+// Division and square root for the flux kernels.  With AEROFX_FAST_DIV the
+// quotient is a * r with r = 1/b from v_rcp_f64 refined by two Newton steps
+// and one residual correction (error <= 1 ulp, no range scaling: operands
+// are O(1) flow quantities); the square root is x * rsqrt(x) refined the same
+// way.  Without it, IEEE-rounded division and sqrt are used.
+__device__ __forceinline__ double ddiv(double a, double b) {
+#ifdef AEROFX_FAST_DIV
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  double q = a * r;
+  double res = fma(-b, q, a);
+  return fma(res, r, q);
+#else
+  return a / b;
+#endif
+}
+
+__device__ __forceinline__ double dsqrt(double x) {
+#ifdef AEROFX_FAST_DIV
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * y;
+  double g = x * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  return fma(d, h, g);
+#else
+  return sqrt(x);
+#endif
+}
+
 __device__ __forceinline__ void prim_from_cons(const double U[5], const Gas &g, double V[5]) {
   V[0] = U[0];
   double invRho = 1.0 / U[0];
@@ -35,7 +73,7 @@ __device__ __forceinline__ double va_fcn(double eps, double a, double b) {
   double ab = a * b;
   double den = a2 + b2 + 2.0 * eps;
   double num = ab > 0.0 ? (a * (b2 + eps) + b * (a2 + eps)) : (ab == 0.0 ? 0.5 * eps * (a + b) : 0.0);
-  return num / den;
+  return ddiv(num, den);
 }
 
 // V_ij = V_i + 1/2 VA(d_i, dV), V_ji = V_j - 1/2 VA(d_j, dV) for one variable.
@@ -57,21 +95,21 @@ __device__ __forceinline__ void vanalbada(double beta, double Vi, double ddVij, 
 __device__ __forceinline__ void roeflux7(const Gas &g, const double en[3], double evitno,
                                          const double Ug[5], const double Ud[5], double phi[5]) {
   const double gam = g.gam, gam1 = g.gam1, invgam1 = g.invgam1, pstiff = g.pstiff;
-  double rnorm = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
-  double invnorm = 1.0 / rnorm;
+  double rnorm = dsqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+  double invnorm = ddiv(1.0, rnorm);
   double n0 = en[0] * invnorm, n1 = en[1] * invnorm, n2 = en[2] * invnorm;
   double vitno = evitno * invnorm;
   double vitg2 = Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3];
   double vitd2 = Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3];
-  double squsr1 = sqrt(Ug[0]);
-  double squsr2 = sqrt(Ud[0]);
+  double squsr1 = dsqrt(Ug[0]);
+  double squsr2 = dsqrt(Ud[0]);
   double ener1 = (Ug[4] + gam * pstiff) * invgam1 + 0.5 * Ug[0] * vitg2;
   double ener2 = (Ud[4] + gam * pstiff) * invgam1 + 0.5 * Ud[0] * vitd2;
-  double usro = 1.0 / (squsr1 + squsr2);
+  double usro = ddiv(1.0, squsr1 + squsr2);
   double uar2 = (squsr1 * Ug[1] + squsr2 * Ud[1]) * usro;
   double uar3 = (squsr1 * Ug[2] + squsr2 * Ud[2]) * usro;
   double uar4 = (squsr1 * Ug[3] + squsr2 * Ud[3]) * usro;
-  double uar5 = ((ener1 + Ug[4]) / squsr1 + (ener2 + Ud[4]) / squsr2) * usro;
+  double uar5 = (ddiv(ener1 + Ug[4], squsr1) + ddiv(ener2 + Ud[4], squsr2)) * usro;
   double VdotNt = n0 * uar2 + n1 * uar3 + n2 * uar4;
   const bool left = (VdotNt - vitno > 0);
   // centred term: one-sided, upwinded by the Roe normal velocity
@@ -79,7 +117,7 @@ __device__ __forceinline__ void roeflux7(const Gas &g, const double en[3], doubl
   double vit2 = left ? vitg2 : vitd2;
   double VdotN = S[1] * n0 + S[2] * n1 + S[3] * n2;
   double H = gam * (S[4] + pstiff) + 0.5 * gam1 * S[0] * vit2;
-  H = H / (gam1 * S[0]);
+  H = ddiv(H, gam1 * S[0]);
   phi[0] = S[0] * (VdotN - vitno);
   phi[1] = phi[0] * S[1] + S[4] * n0;
   phi[2] = phi[0] * S[2] + S[4] * n1;
@@ -89,14 +127,14 @@ __device__ __forceinline__ void roeflux7(const Gas &g, const double en[3], doubl
   VdotN = VdotNt;
   double qir = 0.5 * (uar2 * uar2 + uar3 * uar3 + uar4 * uar4);
   double cr2 = gam1 * (uar5 - qir);
-  double cr = sqrt(cr2);
-  cr2 = 1.0 / cr2;
+  double cr = dsqrt(cr2);
+  cr2 = ddiv(1.0, cr2);
   double vp4 = VdotN + cr;
   double vp5 = VdotN - cr;
   double r = cr;
   double s = -cr;
   double cr2byt = -cr;
-  double invtbeta2 = -1.0 / cr;
+  double invtbeta2 = ddiv(-1.0, cr);
   double dif1 = -Ug[0] + Ud[0];
   double dif2 = -Ug[0] * Ug[1] + Ud[0] * Ud[1];
   double dif3 = -Ug[0] * Ug[2] + Ud[0] * Ud[2];

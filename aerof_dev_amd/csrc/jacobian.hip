@@ -1,0 +1,303 @@
+// jacobian.hip -- implicit operators on gfx950: local time step, the
+// first-order approximate Jacobian H1, its product with a vector (the
+// MatVecProdH1 surface), and the 5x5 block-Jacobi preconditioner (the
+// JacobiPrec surface), with FP64 or FP32 block storage.
+//
+// H1 layout (per context, storage type T = double or float):
+//   D  [N][25]    diagonal blocks: sum of edge and face Jacobians, scaled by
+//                 1/V, plus alpha/dt on the diagonal
+//   B  [2E][25]   one off-diagonal block per adjacency entry, in the node's
+//                 entry order (= ascending edge id): entry (n, m) holds
+//                 A_nm, i.e. A_ij of the edge for n = i and A_ji for n = j
+//   Di [N][25]    inverse of D (block-Jacobi), LINPACK in precision T
+// All kernels are node-parallel with each node's sums in the reference's
+// order; arithmetic is FP64 (FP32 blocks are widened on load, as
+// DenseMatrixOp<float,5,25> does with double vectors).
+#include <hip/hip_runtime.h>
+
+#include "context.h"
+#include "device_jacobian.h"
+#include "host_internal.h"
+
+namespace aerofx {
+
+// ---------------------------------------------------------------------------
+// local time step (Edge.C:87-143, Face.C:242-290, Domain.C:32-100), beta = 1
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sound_speed(const Gas &g, const double V[5]) {
+  return sqrt(g.gam * (V[4] + g.pstiff) / V[0]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *__restrict__ V, Gas g, double cfl,
+                                                     double *__restrict__ dt) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= t.N) return;
+  double Vn[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
+  const double beta2 = 1.0;
+  double idti = 0.0;
+  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+    const int m = t.adj[e], l = t.eidx[e];
+    const bool nlow = n < m;
+    const double *nl = t.normal + 3 * (size_t)l;
+    double S = sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+    double invS = 1.0 / S;
+    double nn0 = invS * nl[0], nn1 = invS * nl[1], nn2 = invS * nl[2];
+    double ndot = invS * 0.0;
+    double Vmid[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double vm = V[5 * (size_t)m + k];
+      Vmid[k] = 0.5 * (nlow ? Vn[k] + vm : vm + Vn[k]);
+    }
+    double a = sound_speed(g, Vmid);
+    double un = Vmid[1] * nn0 + Vmid[2] * nn1 + Vmid[3] * nn2 - ndot;
+    double coeff1 = (1.0 + beta2) * un;
+    double q = (1.0 - beta2) * un, r = 2.0 * a;
+    double coeff2 = sqrt(q * q + r * r);
+    double c = nlow ? 0.5 * (coeff1 - coeff2) : 0.5 * (-coeff1 - coeff2);
+    idti += (c < 0.0 ? c : 0.0) * S;
+  }
+  for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
+    // Face::computeTimeStep: full face normal, 1/3 of the face per node
+    const double *nf = t.fnorm + 3 * (size_t)t.fidx[q];
+    double S = sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+    double invS = 1.0 / S;
+    double un = Vn[1] * (invS * nf[0]) + Vn[2] * (invS * nf[1]) + Vn[3] * (invS * nf[2]) - invS * 0.0;
+    double a = sound_speed(g, Vn);
+    double coeff1 = (1.0 + beta2) * un;
+    double qq = (1.0 - beta2) * un, r = 2.0 * a;
+    double coeff2 = sqrt(qq * qq + r * r);
+    double c = 0.5 * (coeff1 - coeff2);
+    idti += (c < 0.0 ? c : 0.0) * S / 3;
+  }
+  dt[n] = cfl * t.cv[n] / (-1.0 * idti + 0.0 * 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// H1 assembly (SpaceOperator.C:2088-2150, Edge.C:4472-4526, Face.C:816-847,
+// SubDomain.C:1744-1767, TimeState.C:731-756)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
+                                                     const double *__restrict__ dt, T *__restrict__ D,
+                                                     T *__restrict__ B) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= t.N) return;
+  const Gas &g = fc.gas;
+  double Vn[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
+  const double voln = t.invcv[n];
+  double Dn[25];
+#pragma unroll
+  for (int k = 0; k < 25; ++k) Dn[k] = 0.0;
+  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+    const int m = t.adj[e], l = t.eidx[e];
+    const double nl[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
+    T *Be = B + 25 * (size_t)e;
+    if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) {
+#pragma unroll
+      for (int k = 0; k < 25; ++k) Be[k] = (T)0;
+      continue;
+    }
+    double Vm[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vm[k] = V[5 * (size_t)m + k];
+    double blk[25];
+    if (n < m) {
+      // n = i: A_ii += dF/dU_i, A_ij = dF/dU_j / V_i
+      roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) Dn[k] += blk[k];
+      roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) Be[k] = (T)(0.0 + blk[k] * voln);
+    } else {
+      // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
+      roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) Dn[k] -= blk[k];
+      roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) Be[k] = (T)(0.0 - blk[k] * voln);
+    }
+  }
+  for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
+    const int face = t.fidx[q];
+    const int code = t.fcode[face];
+    const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
+                          t.fnorm3[3 * (size_t)face + 2]};
+    double jac[25];
+    if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+      wall_jac(g, nv, 0.0, Vn, jac);
+    } else {
+      const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
+      farfield_jac(g, nv, 0.0, Vn, ub, jac);
+    }
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Dn[k] += jac[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 25; ++k) Dn[k] *= voln;
+  if (dt) {
+    const double c_np1 = 1.0 / dt[n];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Dn[6 * k] += c_np1;
+  }
+  T *Dd = D + 25 * (size_t)n;
+#pragma unroll
+  for (int k = 0; k < 25; ++k) Dd[k] = (T)Dn[k];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_invert(int N, const T *__restrict__ D, T *__restrict__ Di,
+                                                   int *__restrict__ err) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  T a[25];
+#pragma unroll
+  for (int k = 0; k < 25; ++k) a[k] = D[25 * (size_t)n + k];
+  // DiagMat::invert (DiagMatrix.C:44-82): blocks with a zero diagonal entry
+  // are left as they are
+  bool inv = true;
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    if (a[6 * k] == (T)0) inv = false;
+  if (inv && invert5<T>(a) != 0) atomicAdd(err, 1);
+#pragma unroll
+  for (int k = 0; k < 25; ++k) Di[25 * (size_t)n + k] = a[k];
+}
+
+// c += A b (DENSEMATRIXTIMESVECTOR5, DenseMatrixOps.h:441-446)
+template <class T>
+__device__ __forceinline__ void amv5_add(const T *__restrict__ a, const double b[5], double c[5]) {
+  double r[25];
+#pragma unroll
+  for (int k = 0; k < 25; ++k) r[k] = (double)a[k];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    c[i] += r[5 * i] * b[0] + r[5 * i + 1] * b[1] + r[5 * i + 2] * b[2] + r[5 * i + 3] * b[3] + r[5 * i + 4] * b[4];
+}
+
+// y = H1 p (SubDomain::computeMatVecProdH1, SubDomain.C:3917-3953)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
+                                                     const double *__restrict__ p, double *__restrict__ y) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= t.N) return;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double pn[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) pn[k] = p[5 * (size_t)n + k];
+  amv5_add(D + 25 * (size_t)n, pn, acc);
+  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+    const int m = t.adj[e];
+    double pm[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pm[k] = p[5 * (size_t)m + k];
+    amv5_add(B + 25 * (size_t)e, pm, acc);
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) y[5 * (size_t)n + k] = acc[k];
+}
+
+// x = Di y (DiagMat::apply, DiagMatrix.C:86-100)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restrict__ Di,
+                                                         const double *__restrict__ y, double *__restrict__ x) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double yn[5], acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) yn[k] = y[5 * (size_t)n + k];
+  amv5_add(Di + 25 * (size_t)n, yn, acc);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, what);
+}
+
+int ensure_jacobian_storage(aerofx_ctx *c, int precision) {
+  const size_t esz = precision ? sizeof(float) : sizeof(double);
+  const size_t need_b = 25 * esz * 2 * (size_t)c->E, need_d = 25 * esz * (size_t)c->N;
+  if (c->jac_precision == precision && c->d_jacD) return 0;
+  void *p = nullptr;
+  int rc;
+  if ((rc = ctx_alloc(c, need_d, &p))) return rc;
+  c->d_jacD = p;
+  if ((rc = ctx_alloc(c, need_b, &p))) return rc;
+  c->d_jacB = p;
+  if ((rc = ctx_alloc(c, need_d, &p))) return rc;
+  c->d_jacDi = p;
+  c->jac_precision = precision;
+  return 0;
+}
+
+int launch_timestep(aerofx_ctx *c, double cfl, double *dt) {
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_timestep, dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->gas, cfl, dt);
+  return check_launch("timestep launch");
+}
+
+int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
+  int rc = ensure_jacobian_storage(c, precision);
+  if (rc) return rc;
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  {
+    ProfScope ps(c, AEROFX_TIMER_FV_JAC);
+    if (precision)
+      hipLaunchKernelGGL((k_jacobian<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
+                         (float *)c->d_jacD, (float *)c->d_jacB);
+    else
+      hipLaunchKernelGGL((k_jacobian<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
+                         (double *)c->d_jacD, (double *)c->d_jacB);
+  }
+  if ((rc = check_launch("jacobian launch"))) return rc;
+  {
+    ProfScope ps(c, AEROFX_TIMER_PREC_SETUP);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 1, 1);
+    if (precision)
+      hipLaunchKernelGGL((k_invert<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->N, (const float *)c->d_jacD,
+                         (float *)c->d_jacDi, c->d_err + 1);
+    else
+      hipLaunchKernelGGL((k_invert<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                         (const double *)c->d_jacD, (double *)c->d_jacDi, c->d_err + 1);
+  }
+  return check_launch("invert launch");
+}
+
+int launch_h1_apply(aerofx_ctx *c, const double *p, double *y) {
+  if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "jvp: Jacobian not set up");
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
+  if (c->jac_precision)
+    hipLaunchKernelGGL((k_h1_apply<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
+                       (const float *)c->d_jacD, (const float *)c->d_jacB, p, y);
+  else
+    hipLaunchKernelGGL((k_h1_apply<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
+                       (const double *)c->d_jacD, (const double *)c->d_jacB, p, y);
+  return check_launch("jvp launch");
+}
+
+int launch_prec_apply(aerofx_ctx *c, const double *y, double *x) {
+  if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+  if (c->jac_precision)
+    hipLaunchKernelGGL((k_jacobi_apply<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const float *)c->d_jacDi, y, x);
+  else
+    hipLaunchKernelGGL((k_jacobi_apply<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const double *)c->d_jacDi, y, x);
+  return check_launch("prec launch");
+}
+
+}  // namespace aerofx

@@ -11,13 +11,12 @@
 //     reference's edge order, EdgeCore.C:92-119): each edge flux is computed
 //     once (MUSCL + Van Albada + ROEFLUX7, Edge.C:372-439); the block sums the
 //     +f contributions of each of its rows in edge order (P_i) and writes f
-//     to the slot of the edge in the lower list of j (Fm).
-//  3. k_flux_nodes node-parallel: R_n = (sum over lower edges, ascending, of
-//     -f) + P_n + boundary-face fluxes in face order (Face.C:356-389), times
+//     contiguously in edge order (Fm).
+//  3. k_flux_nodes node-parallel: R_n = (sum over lower edges (m, n), m
+//     ascending, of -f, gathered from Fm) + P_n + boundary-face fluxes in face order (Face.C:356-389), times
 //     1/V_n (SpaceOperator.C:599-624).
 //
-// Rows of node data used by a block are staged through LDS; neighbour data
-// are gathered from L2.  blockIdx is remapped so that each XCD sweeps one
+// Node data are gathered through L2 (the RCM neighbour window is small).  blockIdx is remapped so that each XCD sweeps one
 // contiguous node range, keeping the RCM neighbour window in its L2.
 #include <hip/hip_runtime.h>
 
@@ -109,7 +108,11 @@ __global__ __launch_bounds__(kBlock) void k_gradients(DevTables t, const double 
 }
 
 // ---------------------------------------------------------------------------
-// 2. edge fluxes, computed once per edge
+// 2. edge fluxes, computed once per edge.  A block covers whole rows (all
+//    edges whose lower node lies in [r0, r1), <= kBlock edges); lane k of the
+//    block takes edge l0 + k.  Node i data are read directly (the ~7 lanes of
+//    a row hit the same lines); only the fluxes go through LDS, for the
+//    per-row sums P_i in edge order.
 // ---------------------------------------------------------------------------
 template <bool SECOND>
 __global__ __launch_bounds__(kBlock) void k_flux_edges(DevTables t, const double *__restrict__ V,
@@ -117,47 +120,36 @@ __global__ __launch_bounds__(kBlock) void k_flux_edges(DevTables t, const double
                                                        double *__restrict__ Fm, double *__restrict__ P,
                                                        int *__restrict__ err) {
   __shared__ double sf[5 * kPad];
-  __shared__ double sV[kMaxRows * 5];
-  __shared__ double sG[SECOND ? kMaxRows * 16 : 1];
-  __shared__ double sX[kMaxRows * 4];
-  __shared__ int srow[kBlock];
   const int c = xcd_chunk(blockIdx.x, t.nfchunks);
   const int r0 = t.fchunk[c], r1 = t.fchunk[c + 1];
-  const int nr = r1 - r0;
   const int l0 = t.eup[r0], l1 = t.eup[r1];
   const int tid = threadIdx.x;
-  for (int rr = tid; rr < nr; rr += kBlock) {
-    const int r = r0 + rr;
-    for (int l = t.eup[r]; l < t.eup[r + 1]; ++l) srow[l - l0] = rr;
-  }
-  for (int idx = tid; idx < nr * 5; idx += kBlock) sV[idx] = V[5 * (size_t)r0 + idx];
-  if (SECOND)
-    for (int idx = tid; idx < nr * 16; idx += kBlock) sG[idx] = grad[16 * (size_t)r0 + idx];
-  for (int idx = tid; idx < nr * 4; idx += kBlock) sX[idx] = t.X4[4 * (size_t)r0 + idx];
-  __syncthreads();
   const Gas &g = fc.gas;
   if (tid < l1 - l0) {
     const int l = l0 + tid;
-    const int rr = srow[tid];
-    const int j = t.jcol[l];
+    const int2 ij = t.eij[l];
+    const int i = ij.x, j = ij.y;
     double Vi[5], Vj[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      Vi[k] = sV[5 * rr + k];
+      Vi[k] = V[5 * (size_t)i + k];
       Vj[k] = V[5 * (size_t)j + k];
     }
     double Vij[5], Vji[5];
     if (SECOND) {
+      const double4 xi = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)i);
       const double4 xj = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)j);
-      const double dx0 = xj.x - sX[4 * rr], dx1 = xj.y - sX[4 * rr + 1], dx2 = xj.z - sX[4 * rr + 2];
-      const double *gi = sG + 16 * rr;
+      const double dx0 = xj.x - xi.x, dx1 = xj.y - xi.y, dx2 = xj.z - xi.z;
+      const double2 *gi2 = reinterpret_cast<const double2 *>(grad + 16 * (size_t)i);
       const double2 *gj2 = reinterpret_cast<const double2 *>(grad + 16 * (size_t)j);
-      double gj[16];
+      double gi[16], gj[16];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const double2 v = gj2[q];
-        gj[2 * q] = v.x;
-        gj[2 * q + 1] = v.y;
+        const double2 a = gi2[q], b = gj2[q];
+        gi[2 * q] = a.x;
+        gi[2 * q + 1] = a.y;
+        gj[2 * q] = b.x;
+        gj[2 * q + 1] = b.y;
       }
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -183,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_edges(DevTables t, const double
     } else {
       roeflux7(g, nv, 0.0, Vij, Vji, f);
     }
-    double *dst = Fm + 5 * (size_t)t.lslot[l];
+    double *dst = Fm + 5 * (size_t)l;  // contiguous in edge order
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       sf[k * kPad + tid] = f[k];
@@ -191,6 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_edges(DevTables t, const double
     }
   }
   __syncthreads();
+  const int nr = r1 - r0;
   for (int idx = tid; idx < nr * 5; idx += kBlock) {
     const int rr = idx / 5, q = idx - 5 * rr;
     const int r = r0 + rr;
@@ -211,9 +204,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double
   const int n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= t.N) return;
   double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int q = t.lrow[n]; q < t.lrow[n + 1]; ++q) {
+  const int q0 = t.lrow[n], q1 = t.lrow[n + 1];
+#pragma unroll 4
+  for (int q = q0; q < q1; ++q) {
+    const size_t base = 5 * (size_t)t.ledge[q];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) s[k] -= Fm[5 * (size_t)q + k];
+    for (int k = 0; k < 5; ++k) s[k] -= Fm[base + k];
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) s[k] += P[5 * (size_t)n + k];

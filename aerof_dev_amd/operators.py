@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from ._abi import Mesh as _CMesh
-from ._abi import c_double_p, c_int_p, check, lib
+from ._abi import NewtonOpts, NewtonStats, c_double_p, c_int_p, check, lib
 from .mesh import FlowParams, Mesh
 
 
@@ -117,3 +117,67 @@ class SpaceOperator:
 
     def conservative_to_primitive(self, U, V):
         check(lib().aerofx_primitive(self._h, _ptr(U), _ptr(V)), "primitive")
+
+
+class MatVecProdH1:
+    """MatVecProdH1<5,Scalar,5> (MatVecProd.h:37-259) and JacobiPrec<Scalar,5>
+    (KspPrec.h:69-93) over one SpaceOperator context.
+
+    evaluate() assembles the first-order Roe Jacobian at U plus alpha/dt on
+    the diagonal and sets up the block-Jacobi inverse; precision 0 stores the
+    blocks in FP64, 1 in FP32 (the mixed-precision JVP and preconditioner).
+    """
+
+    def __init__(self, op: SpaceOperator):
+        self.op = op
+
+    def evaluate(self, U, dt=None, precision: int = 0):
+        check(lib().aerofx_jacobian_setup(self.op._h, _ptr(U), _ptr(dt) if dt is not None else None,
+                                          int(precision)), "jacobian_setup")
+
+    def apply(self, p, prod):
+        check(lib().aerofx_jvp(self.op._h, _ptr(p), _ptr(prod)), "jvp")
+
+    def prec_apply(self, y, x):
+        check(lib().aerofx_prec_apply(self.op._h, _ptr(y), _ptr(x)), "prec_apply")
+
+    def blocks(self):
+        """Host copies: D [N,5,5] and per edge (A_ij, A_ji) [E,2,5,5]."""
+        N, E = self.op.num_nodes, self.op.num_edges
+        D = np.empty((N, 5, 5))
+        A = np.empty((E, 2, 5, 5))
+        check(lib().aerofx_jacobian_blocks(self.op._h, D.ctypes.data_as(c_double_p),
+                                           A.ctypes.data_as(c_double_p)), "jacobian_blocks")
+        return D, A
+
+    def timestep(self, U, cfl, dt):
+        check(lib().aerofx_timestep(self.op._h, _ptr(U), float(cfl), _ptr(dt)), "timestep")
+
+
+class GmresSolver:
+    """GmresSolver (KspSolver.h:23-143): right-preconditioned GMRES(m)."""
+
+    def __init__(self, op: SpaceOperator, m: int = 30, eps: float = 1e-2, maxits: int = 30):
+        self.op, self.m, self.eps, self.maxits = op, m, eps, maxits
+
+    def solve(self, b, x):
+        its = C.c_int()
+        res = C.c_double()
+        check(lib().aerofx_gmres(self.op._h, _ptr(b), _ptr(x), self.m, self.eps, self.maxits, C.byref(its),
+                                 C.byref(res)), "gmres")
+        return its.value, res.value
+
+
+def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200, eps: float = 1e-8,
+                 cfl0: float = 10.0, cfl_max: float = 1e4, ser: float = 0.7, gmres_m: int = 30,
+                 gmres_maxits: int = 30, gmres_eps: float = 1e-2):
+    """Pseudo-transient Newton-GMRES to ||R|| <= eps ||R0|| (U updated in place)."""
+    o = NewtonOpts(int(precision), int(max_steps), float(eps), float(cfl0), float(cfl_max), float(ser), 0.0, 0.0,
+                   int(gmres_m), int(gmres_maxits), float(gmres_eps))
+    st = NewtonStats()
+    hist = np.zeros(max_steps + 1)
+    check(lib().aerofx_steady_solve(op._h, _ptr(U), C.byref(o), C.byref(st), hist.ctypes.data_as(c_double_p),
+                                    max_steps + 1), "steady_solve")
+    n = min(st.steps + 1, max_steps + 1)
+    return dict(steps=st.steps, converged=bool(st.converged), krylov_its=st.krylov_its, res0=st.res0,
+                res=st.res, final_cfl=st.final_cfl, seconds=st.seconds, history=hist[:n].copy())

@@ -157,6 +157,52 @@ int aerofx_gradients(aerofx_ctx *ctx, const double *V, double *ddx,
 /* VarFcnSGEuler::conservativeToPrimitive over all nodes. */
 int aerofx_primitive(aerofx_ctx *ctx, const double *U, double *V);
 
+/* ---- MatVecProd / KspPrec / KspSolver ---------------------------------- */
+/* Local pseudo-time step dt_i = cfl V_i / sum(-min(lambda, 0) S)
+ * (Edge.C:87-143, Face.C:242-290, Domain.C:32-100) at state U: dt [N]. */
+int aerofx_timestep(aerofx_ctx *ctx, const double *U, double cfl, double *dt);
+/* MatVecProdH1::evaluate + JacobiPrec::setup (MatVecProd.C:1075-1100,
+ * KspPrec.C:9-124): assembles the first-order Roe Jacobian H1 at U
+ * (SpaceOperator.C:2088-2150) plus alpha/dt_i on the diagonal (dt may be
+ * NULL: no time term), then inverts its 5x5 diagonal blocks (LINPACK,
+ * DiagMatrix.C:44-82).  precision 0: FP64 blocks; 1: FP32 blocks (the
+ * mixed-precision JVP and preconditioner). */
+int aerofx_jacobian_setup(aerofx_ctx *ctx, const double *U, const double *dt,
+                          int precision);
+/* Host copies of the assembled H1: D [N][25] and, per edge, (A_ij, A_ji)
+ * [E][2][25] (the reference's block order, SubDomain.C:3929-3947). */
+int aerofx_jacobian_blocks(aerofx_ctx *ctx, double *D, double *Aedge);
+/* MatVecProdH1::apply (MatVecProd.C:1169-1187): Jp = H1 p. */
+int aerofx_jvp(aerofx_ctx *ctx, const double *p, double *Jp);
+/* JacobiPrec::apply (KspPrec.C:87-111): x = D^-1 y. */
+int aerofx_prec_apply(aerofx_ctx *ctx, const double *y, double *x);
+/* GmresSolver::solve (KspSolver.C:353-492): right-preconditioned GMRES(m)
+ * on the operators of the last aerofx_jacobian_setup, x0 = 0, stop at
+ * ||r|| <= eps ||b|| or maxits.  Orthogonalisation is CGS2 in FP64. */
+int aerofx_gmres(aerofx_ctx *ctx, const double *b, double *x, int m,
+                 double eps, int maxits, int *its, double *final_res);
+
+/* ---- steady pseudo-transient Newton-GMRES ------------------------------ */
+typedef struct {
+  int precision;        /* 0: all FP64; 1: FP32 Jacobian/JVP/block-Jacobi */
+  int max_steps;        /* pseudo-time steps */
+  double eps;           /* stop at ||R||_2 <= eps ||R_0||_2 */
+  double cfl0, cfl_max, ser, cfl_coef1, cfl_coef2;  /* CflLaw Old */
+  int gmres_m, gmres_maxits;
+  double gmres_eps;     /* relative Krylov tolerance */
+} aerofx_newton_opts;
+typedef struct {
+  int steps, converged, krylov_its;
+  double res0, res, final_cfl, seconds;
+} aerofx_newton_stats;
+/* TsSolver::resolve + NewtonSolver::solve with one Newton iteration per
+ * pseudo-time step (NewtonSolver.h:70-231, TsDesc.C:697-730, 1178-1278):
+ * U (device [N][5]) is advanced in place; history (optional, hist_cap
+ * entries) receives ||R||_2 per step. */
+int aerofx_steady_solve(aerofx_ctx *ctx, double *U, const aerofx_newton_opts *opts,
+                        aerofx_newton_stats *stats, double *history,
+                        int hist_cap);
+
 #ifdef __cplusplus
 }
 #endif

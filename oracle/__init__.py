@@ -150,6 +150,53 @@ class OracleMesh:
             out["ddx"], out["ddy"], out["ddz"] = g
         return out
 
+    def timestep(self, V, cfl):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        dt = np.empty(self.N)
+        lib().or_timestep(C.byref(self.m), C.byref(self.p), _d(V), C.c_double(cfl), _d(dt))
+        return dt
+
+    def jacobian(self, V, dt=None):
+        """H1 blocks: D [N,5,5], per edge (A_ij, A_ji) [E,2,5,5]."""
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        D = np.empty((self.N, 5, 5))
+        A = np.empty((self.E, 2, 5, 5))
+        dtp = _d(np.ascontiguousarray(dt, dtype=np.float64)) if dt is not None else None
+        lib().or_jacobian_h1(C.byref(self.m), C.byref(self.p), _d(V), dtp, _d(D), _d(A))
+        return D, A
+
+    def h1_apply(self, D, A, p):
+        y = np.empty((self.N, 5))
+        D = np.ascontiguousarray(D); A = np.ascontiguousarray(A); p = np.ascontiguousarray(p, dtype=np.float64)
+        lib().or_h1_apply(C.byref(self.m), _d(D), _d(A), _d(p), _d(y))
+        return y
+
+    def gmres(self, V, dt, b, precision=0, m=30, eps=1e-2, maxits=30):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(b)
+        fr = C.c_double()
+        L = lib()
+        L.or_gmres_h1.restype = C.c_int
+        its = L.or_gmres_h1(C.byref(self.m), C.byref(self.p), _d(V), _d(dt), int(precision), _d(b), _d(x),
+                            int(m), C.c_double(eps), int(maxits), C.byref(fr))
+        return x, its, fr.value
+
+    def steady_solve(self, U, precision=0, max_steps=50, eps=1e-8, cfl0=10.0, cfl_max=1e4, ser=0.7, m=30,
+                     maxits=30, geps=1e-2):
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        hist = np.zeros(max_steps + 1)
+        conv, kits = C.c_int(), C.c_int()
+        L = lib()
+        L.or_steady_solve.restype = C.c_int
+        steps = L.or_steady_solve(C.byref(self.m), C.byref(self.p), _d(U), int(precision), int(max_steps),
+                                  C.c_double(eps), C.c_double(cfl0), C.c_double(cfl_max), C.c_double(ser),
+                                  int(m), int(maxits), C.c_double(geps), _d(hist), C.byref(conv),
+                                  C.byref(kits))
+        return dict(U=U, steps=steps, converged=bool(conv.value), krylov_its=kits.value,
+                    history=hist[:steps + 1].copy())
+
     def gradients(self, V):
         V = np.ascontiguousarray(V, dtype=np.float64)
         g = [np.empty((self.N, 5)) for _ in range(3)]
@@ -207,3 +254,45 @@ def wallflux(n, nvel, V):
     V = np.ascontiguousarray(V, dtype=np.float64)
     lib().or_wallflux(_d(n), C.c_double(nvel), _d(V), _d(out))
     return out
+
+
+def roejac6(gam, pstiff, n, vn, Vl, Vr, sw, gamma_diss=1.0, use_ref=False):
+    """ROEJAC6 5x5 block (row-major)."""
+    jac = np.empty(25)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    Vl = np.ascontiguousarray(Vl, dtype=np.float64)
+    Vr = np.ascontiguousarray(Vr, dtype=np.float64)
+    if use_ref:
+        ref().refh_roejac6(C.c_double(gamma_diss), C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn),
+                           _d(Vl), _d(Vr), C.c_int(sw), _d(jac))
+    else:
+        lib().or_roejac6(C.c_double(gamma_diss), C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn),
+                         _d(Vl), _d(Vr), C.c_int(sw), _d(jac))
+    return jac.reshape(5, 5)
+
+
+def invert5(a, single=False, use_ref=False):
+    """LINPACK dgefa/dgedi inverse of a row-major 5x5 block; returns (inv, info)."""
+    if single:
+        b = np.ascontiguousarray(a, dtype=np.float32).reshape(25).copy()
+        fn = ref().refh_invert5_f32 if use_ref else lib().or_invert5f
+        info = fn(b.ctypes.data_as(C.POINTER(C.c_float)))
+    else:
+        b = np.ascontiguousarray(a, dtype=np.float64).reshape(25).copy()
+        fn = ref().refh_invert5_f64 if use_ref else lib().or_invert5
+        info = fn(_d(b))
+    return b.reshape(5, 5), info
+
+
+def apply5(a, y, single=False, use_ref=False):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.empty(5)
+    if single:
+        b = np.ascontiguousarray(a, dtype=np.float32).reshape(25)
+        fn = ref().refh_apply5_f32 if use_ref else lib().or_apply5f
+        fn(b.ctypes.data_as(C.POINTER(C.c_float)), _d(y), _d(x))
+    else:
+        b = np.ascontiguousarray(a, dtype=np.float64).reshape(25)
+        fn = ref().refh_apply5_f64 if use_ref else lib().or_apply5
+        fn(_d(b), _d(y), _d(x))
+    return x

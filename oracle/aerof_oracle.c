@@ -775,3 +775,576 @@ int or_color_edges(int N, int E, const int *ptr, int *color) {
   free(used);
   return ncol;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Implicit operators                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* ROEJAC6 (f77src/roejac6.f:1-316) with type 0 and prec 0 (beta = 1):
+ * 0.5*|n|*(A(Ug) +/- |A_roe|), row-major 5x5 into jac. gamma_diss is the
+ * Schemes.NavierStokes.Gamma dissipation factor, gam the heat ratio. */
+void or_roejac6(double gamma_diss, double gam, double pstiff, const double *enormal,
+                double evitno, const double *Ug, const double *Ud, int sw, double *jac) {
+  double H[5][5], tmw[5][5], dmat[5][5];
+  double gam1 = gam - 1.0;
+  double rnorm = sqrt(enormal[0] * enormal[0] + enormal[1] * enormal[1] + enormal[2] * enormal[2]);
+  double invnorm = 1.0 / rnorm;
+  double n1 = enormal[0] * invnorm, n2 = enormal[1] * invnorm, n3 = enormal[2] * invnorm;
+  double vitno = evitno * invnorm;
+  double VdotN = Ug[1] * n1 + Ug[2] * n2 + Ug[3] * n3;
+  double qir = 0.5 * (Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3]);
+  double enth = gam * (Ug[4] + pstiff) + gam1 * Ug[0] * qir;
+  enth = enth / (gam1 * Ug[0]);
+  H[0][0] = -vitno; H[0][1] = n1; H[0][2] = n2; H[0][3] = n3; H[0][4] = 0.0;
+  H[1][0] = gam1 * n1 * qir - Ug[1] * VdotN;
+  H[1][1] = (1.0 - gam1) * n1 * Ug[1] + VdotN - vitno;
+  H[1][2] = -gam1 * n1 * Ug[2] + Ug[1] * n2;
+  H[1][3] = -gam1 * n1 * Ug[3] + Ug[1] * n3;
+  H[1][4] = gam1 * n1;
+  H[2][0] = gam1 * n2 * qir - Ug[2] * VdotN;
+  H[2][1] = -gam1 * n2 * Ug[1] + Ug[2] * n1;
+  H[2][2] = (1.0 - gam1) * n2 * Ug[2] + VdotN - vitno;
+  H[2][3] = -gam1 * n2 * Ug[3] + Ug[2] * n3;
+  H[2][4] = gam1 * n2;
+  H[3][0] = gam1 * n3 * qir - Ug[3] * VdotN;
+  H[3][1] = -gam1 * n3 * Ug[1] + Ug[3] * n1;
+  H[3][2] = -gam1 * n3 * Ug[2] + Ug[3] * n2;
+  H[3][3] = (1.0 - gam1) * n3 * Ug[3] + VdotN - vitno;
+  H[3][4] = gam1 * n3;
+  H[4][0] = (gam1 * qir - enth) * VdotN;
+  H[4][1] = enth * n1 - gam1 * Ug[1] * VdotN;
+  H[4][2] = enth * n2 - gam1 * Ug[2] * VdotN;
+  H[4][3] = enth * n3 - gam1 * Ug[3] * VdotN;
+  H[4][4] = (1.0 + gam1) * VdotN - vitno;
+  double squsr1 = sqrt(Ug[0]);
+  double squsr2 = sqrt(Ud[0]);
+  double ener1 = (Ug[4] + gam * pstiff) / gam1 + 0.5 * Ug[0] * (Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3]);
+  double ener2 = (Ud[4] + gam * pstiff) / gam1 + 0.5 * Ud[0] * (Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3]);
+  double usro = 1.0 / (squsr1 + squsr2);
+  double uar2 = (squsr1 * Ug[1] + squsr2 * Ud[1]) * usro;
+  double uar3 = (squsr1 * Ug[2] + squsr2 * Ud[2]) * usro;
+  double uar4 = (squsr1 * Ug[3] + squsr2 * Ud[3]) * usro;
+  double uar5 = ((ener1 + Ug[4]) / squsr1 + (ener2 + Ud[4]) / squsr2) * usro;
+  enth = uar5;
+  VdotN = n1 * uar2 + n2 * uar3 + n3 * uar4;
+  qir = 0.5 * (uar2 * uar2 + uar3 * uar3 + uar4 * uar4);
+  double cr2 = gam1 * (uar5 - qir);
+  double cr = sqrt(cr2);
+  cr2 = 1.0 / cr2;
+  double gc = gam1 / (cr * cr);
+  double usc2 = 0.5 / (cr * cr);
+  double beta = 1.0;
+  double betaRT2 = beta * beta;
+  double vp1 = VdotN;
+  double t = (1.0 - betaRT2) * VdotN;
+  double root = sqrt(t * t + 4.0 * betaRT2 * (cr * cr));
+  double vp4 = 0.5 * ((1.0 + betaRT2) * VdotN + root);
+  double vp5 = 0.5 * ((1.0 + betaRT2) * VdotN - root);
+  double rRT = vp4 - vp1 * betaRT2;
+  double sRT = vp5 - vp1 * betaRT2;
+  double tRT = 0.5 * (vp5 - vp4);
+  vp1 = fabs(vp1 - vitno) * gamma_diss;
+  vp4 = fabs(vp4 - vitno) * gamma_diss;
+  vp5 = fabs(vp5 - vitno) * gamma_diss;
+  double bt = betaRT2 * tRT;
+  double cc = cr * cr;
+  tmw[0][0] = n1 + n2 * uar4 - uar3 * n3 - gc * n1 * qir;
+  tmw[0][1] = gc * n1 * uar2;
+  tmw[0][2] = gc * n1 * uar3 + n3;
+  tmw[0][3] = gc * n1 * uar4 - n2;
+  tmw[0][4] = -gc * n1;
+  tmw[1][0] = n2 + n3 * uar2 - uar4 * n1 - gc * n2 * qir;
+  tmw[1][1] = gc * n2 * uar2 - n3;
+  tmw[1][2] = gc * n2 * uar3;
+  tmw[1][3] = gc * n2 * uar4 + n1;
+  tmw[1][4] = -gc * n2;
+  tmw[2][0] = n3 + n1 * uar3 - uar2 * n2 - gc * n3 * qir;
+  tmw[2][1] = gc * n3 * uar2 + n2;
+  tmw[2][2] = gc * n3 * uar3 - n1;
+  tmw[2][3] = gc * n3 * uar4;
+  tmw[2][4] = -gc * n3;
+  tmw[3][0] = gam1 * qir * sRT / bt + VdotN * cc / tRT;
+  tmw[3][1] = -gam1 * uar2 * sRT / bt - cc * n1 / tRT;
+  tmw[3][2] = -gam1 * uar3 * sRT / bt - cc * n2 / tRT;
+  tmw[3][3] = -gam1 * uar4 * sRT / bt - cc * n3 / tRT;
+  tmw[3][4] = gam1 * sRT / bt;
+  tmw[4][0] = -gam1 * qir * rRT / bt - VdotN * cc / tRT;
+  tmw[4][1] = gam1 * uar2 * rRT / bt + cc * n1 / tRT;
+  tmw[4][2] = gam1 * uar3 * rRT / bt + cc * n2 / tRT;
+  tmw[4][3] = gam1 * uar4 * rRT / bt + cc * n3 / tRT;
+  tmw[4][4] = -gam1 * rRT / bt;
+  for (int j = 0; j < 5; ++j) {
+    dmat[0][j] = vp1 * (n1 * tmw[0][j] + n2 * tmw[1][j] + n3 * tmw[2][j]) + vp4 * usc2 * tmw[3][j] +
+                 vp5 * usc2 * tmw[4][j];
+    dmat[1][j] = vp1 * (n1 * uar2 * tmw[0][j] + (n2 * uar2 - n3) * tmw[1][j] + (n3 * uar2 + n2) * tmw[2][j]) +
+                 vp4 * (usc2 * (uar2 + rRT * n1)) * tmw[3][j] + vp5 * (usc2 * (uar2 + sRT * n1)) * tmw[4][j];
+    dmat[2][j] = vp1 * (n2 * uar3 * tmw[1][j] + (n1 * uar3 + n3) * tmw[0][j] + (n3 * uar3 - n1) * tmw[2][j]) +
+                 vp4 * (usc2 * (uar3 + rRT * n2)) * tmw[3][j] + vp5 * (usc2 * (uar3 + sRT * n2)) * tmw[4][j];
+    dmat[3][j] = vp1 * (n3 * uar4 * tmw[2][j] + (n1 * uar4 - n2) * tmw[0][j] + (n2 * uar4 + n1) * tmw[1][j]) +
+                 vp4 * (usc2 * (uar4 + rRT * n3)) * tmw[3][j] + vp5 * (usc2 * (uar4 + sRT * n3)) * tmw[4][j];
+    dmat[4][j] = vp1 * ((n1 * qir + (n3 * uar3 - n2 * uar4)) * tmw[0][j] +
+                        (n2 * qir + (n1 * uar4 - n3 * uar2)) * tmw[1][j] +
+                        (n3 * qir + (n2 * uar2 - n1 * uar3)) * tmw[2][j]) +
+                 vp4 * (usc2 * (enth + VdotN * rRT)) * tmw[3][j] + vp5 * (usc2 * (enth + VdotN * sRT)) * tmw[4][j];
+  }
+  for (int i = 0; i < 5; ++i)
+    for (int k = 0; k < 5; ++k) {
+      if (sw == 1) H[i][k] = H[i][k] + dmat[i][k];
+      else if (sw == 2) H[i][k] = H[i][k] - dmat[i][k];
+    }
+  for (int i = 0; i < 5; ++i)
+    for (int k = 0; k < 5; ++k) jac[5 * i + k] = 0.5 * H[i][k] * rnorm;
+}
+
+/* VarFcnSGEuler::computedVdU (VarFcnSGEuler.h:588-603) with
+ * VarFcnBase::postMultiplyBydVdU (VarFcnBase.h:383-411): res = mat * dV/dU. */
+void or_post_dVdU(double gam, const double *V, const double *mat, double *res) {
+  double gam1 = gam - 1.0;
+  double d[25];
+  double invrho = 1.0 / V[0];
+  d[0] = 1.0;
+  d[5] = -invrho * V[1];
+  d[6] = invrho;
+  d[10] = -invrho * V[2];
+  d[12] = invrho;
+  d[15] = -invrho * V[3];
+  d[18] = invrho;
+  d[20] = gam1 * 0.5 * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]);
+  d[21] = -gam1 * V[1];
+  d[22] = -gam1 * V[2];
+  d[23] = -gam1 * V[3];
+  d[24] = gam1;
+  for (int r = 0; r < 5; ++r) {
+    const double *m = mat + 5 * r;
+    double *o = res + 5 * r;
+    o[0] = m[0] * d[0] + m[1] * d[5] + m[2] * d[10] + m[3] * d[15] + m[4] * d[20];
+    o[1] = m[1] * d[6] + m[4] * d[21];
+    o[2] = m[2] * d[12] + m[4] * d[22];
+    o[3] = m[3] * d[18] + m[4] * d[23];
+    o[4] = m[4] * d[24];
+  }
+}
+
+/* FluxFcnSGWallEuler3D::computeJacobian (FluxFcnDescSG.C:838-886). */
+void or_wall_jac(double gam, const double *n, double nvel, const double *V, double *jac) {
+  double dfdV[25];
+  for (int k = 0; k < 25; ++k) dfdV[k] = 0.0;
+  dfdV[9] = n[0];
+  dfdV[14] = n[1];
+  dfdV[19] = n[2];
+  dfdV[24] = nvel;
+  or_post_dVdU(gam, V, dfdV, jac);
+}
+
+/* FluxFcnFD<5>::computeJacobian (FluxFcnBase.h:112-159) over BOUNDFLUX5
+ * (FluxFcnSGInflow/OutflowEuler3D), then postMultiplyBydVdU. */
+void or_farfield_jac(double gam, const double *n, double sig, const double *VL, const double *Ub,
+                     double *jac) {
+  const double eps0 = 1.e-6;
+  double Veps[5], flux[5], fluxeps[5], dfdVL[25];
+  or_boundflux5(gam, n, sig, VL, Ub, flux);
+  for (int k = 0; k < 5; ++k) Veps[k] = VL[k];
+  for (int k = 0; k < 5; ++k) {
+    double eps;
+    if (fabs(Veps[k]) < 1.e-10) eps = eps0;
+    else eps = eps0 * Veps[k];
+    double inveps = 1.0 / eps;
+    Veps[k] += eps;
+    if (k != 0) Veps[k - 1] = VL[k - 1];
+    or_boundflux5(gam, n, sig, Veps, Ub, fluxeps);
+    for (int j = 0; j < 5; ++j) dfdVL[5 * j + k] = (fluxeps[j] - flux[j]) * inveps;
+  }
+  or_post_dVdU(gam, VL, dfdVL, jac);
+}
+
+/* Local time step (Edge.C:87-143, Face.C:242-290, Domain.C:32-100) with
+ * the default time low-Mach constants (beta = 1, no viscous term):
+ * idti_i = sum of min(0.5(+-coeff1 - coeff2), 0)*S, dt_i = cfl*V_i/(-idti_i). */
+void or_timestep(const or_mesh *m, const or_params *p, const double *V, double cfl, double *dt) {
+  int N = m->N, E = m->E, F = m->F;
+  double gam = p->gamma, pstiff = p->pstiff;
+  double *idti = (double *)malloc(sizeof(double) * (size_t)N);
+  for (int i = 0; i < N; ++i) idti[i] = 0.0;
+  const double locbeta = 1.0, beta2 = locbeta * locbeta;
+  for (int l = 0; l < E; ++l) {
+    int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
+    const double *nl = m->normals + 3 * l;
+    double S = sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+    double invS = 1.0 / S;
+    double n[3] = {invS * nl[0], invS * nl[1], invS * nl[2]};
+    double ndot = invS * m->normal_vel[l];
+    double Vmid[5];
+    for (int k = 0; k < 5; ++k) Vmid[k] = 0.5 * (V[5 * i + k] + V[5 * j + k]);
+    double a = sqrt(gam * (Vmid[4] + pstiff) / Vmid[0]);
+    double un = Vmid[1] * n[0] + Vmid[2] * n[1] + Vmid[3] * n[2] - ndot;
+    double coeff1 = (1.0 + beta2) * un;
+    double coeff2 = pow(pow((1.0 - beta2) * un, 2.0) + pow(2.0 * locbeta * a, 2.0), 0.5);
+    double ci = 0.5 * (coeff1 - coeff2), cj = 0.5 * (-coeff1 - coeff2);
+    idti[i] += (ci < 0.0 ? ci : 0.0) * S;
+    idti[j] += (cj < 0.0 ? cj : 0.0) * S;
+  }
+  for (int f = 0; f < F; ++f) {
+    const double *nf = m->fnorm + 3 * f;
+    double S = sqrt(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]);
+    double invS = 1.0 / S;
+    double n[3] = {invS * nf[0], invS * nf[1], invS * nf[2]};
+    double ndot = invS * 0.0;
+    for (int q = 0; q < 3; ++q) {
+      int node = m->faces[3 * f + q];
+      const double *v = V + 5 * node;
+      double a = sqrt(gam * (v[4] + pstiff) / v[0]);
+      double un = v[1] * n[0] + v[2] * n[1] + v[3] * n[2] - ndot;
+      double coeff1 = (1.0 + beta2) * un;
+      double coeff2 = pow(pow((1.0 - beta2) * un, 2.0) + pow(2.0 * locbeta * a, 2.0), 0.5);
+      double ci = 0.5 * (coeff1 - coeff2);
+      idti[node] += (ci < 0.0 ? ci : 0.0) * S / 3;
+    }
+  }
+  for (int i = 0; i < N; ++i) dt[i] = cfl * m->cv[i] / (-1.0 * idti[i] + 0.0 * 0.0);
+  free(idti);
+}
+
+/* First-order approximate Jacobian H1 (SpaceOperator.C:2088-2150,
+ * Edge.C:4472-4526, Face.C:816-847, SubDomain.C:1744-1767) plus the
+ * backward-Euler term alpha_np1/dt_i (TimeState.C:731-756).
+ * Adiag[N][25]; Aedge[E][2][25] = (A_ij, A_ji) per edge. */
+void or_jacobian_h1(const or_mesh *m, const or_params *p, const double *V, const double *dt,
+                    double *Adiag, double *Aedge) {
+  int N = m->N, E = m->E, F = m->F;
+  double gam = p->gamma, pstiff = p->pstiff;
+  for (size_t k = 0; k < 25 * (size_t)N; ++k) Adiag[k] = 0.0;
+  for (size_t k = 0; k < 50 * (size_t)E; ++k) Aedge[k] = 0.0;
+  double dfdUi[25], dfdUj[25];
+  for (int l = 0; l < E; ++l) {
+    int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
+    const double *nl = m->normals + 3 * l;
+    if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) continue;
+    /* roejacappr3Dgas (FluxFcnDescSG.C:157-175), conservative Jacobians */
+    or_roejac6(1.0, gam, pstiff, nl, m->normal_vel[l], V + 5 * i, V + 5 * j, 1, dfdUi);
+    or_roejac6(1.0, gam, pstiff, nl, m->normal_vel[l], V + 5 * j, V + 5 * i, 2, dfdUj);
+    double *Aii = Adiag + 25 * (size_t)i, *Ajj = Adiag + 25 * (size_t)j;
+    for (int k = 0; k < 25; ++k) { Aii[k] += dfdUi[k]; Ajj[k] -= dfdUj[k]; }
+    double *Aij = Aedge + 50 * (size_t)l, *Aji = Aij + 25;
+    double voli = 1.0 / m->cv[i], volj = 1.0 / m->cv[j];
+    for (int k = 0; k < 25; ++k) { Aij[k] += dfdUj[k] * voli; Aji[k] -= dfdUi[k] * volj; }
+  }
+  const double third = 1.0 / 3.0;
+  for (int f = 0; f < F; ++f) {
+    int code = m->codes[f];
+    double n[3] = {third * m->fnorm[3 * f], third * m->fnorm[3 * f + 1], third * m->fnorm[3 * f + 2]};
+    double nvel = third * 0.0;
+    for (int a = 0; a < 3; ++a) {
+      int node = m->faces[3 * f + a];
+      double jac[25];
+      if (code == 2 || code == -2) or_wall_jac(gam, n, nvel, V + 5 * node, jac);
+      else if (code == 4 || code == -4) or_farfield_jac(gam, n, nvel, V + 5 * node, m->Ub_in, jac);
+      else if (code == 5 || code == -5) or_farfield_jac(gam, n, nvel, V + 5 * node, m->Ub_out, jac);
+      else continue;
+      double *Aii = Adiag + 25 * (size_t)node;
+      for (int k = 0; k < 25; ++k) Aii[k] += jac[k];
+    }
+  }
+  for (int i = 0; i < N; ++i) {
+    double voli = 1.0 / m->cv[i];
+    double *Aii = Adiag + 25 * (size_t)i;
+    for (int k = 0; k < 25; ++k) Aii[k] *= voli;
+    if (dt) {
+      double c_np1 = 1.0 / dt[i];
+      for (int k = 0; k < 5; ++k) Aii[k + 5 * k] += c_np1;
+    }
+  }
+}
+
+/* SubDomain::computeMatVecProdH1 (SubDomain.C:3917-3953) with
+ * DENSEMATRIXTIMESVECTOR5 (DenseMatrixOps.h:441-446). */
+static void amv5_add(const double *a, const double *b, double *c) {
+  c[0] += a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4];
+  c[1] += a[5] * b[0] + a[6] * b[1] + a[7] * b[2] + a[8] * b[3] + a[9] * b[4];
+  c[2] += a[10] * b[0] + a[11] * b[1] + a[12] * b[2] + a[13] * b[3] + a[14] * b[4];
+  c[3] += a[15] * b[0] + a[16] * b[1] + a[17] * b[2] + a[18] * b[3] + a[19] * b[4];
+  c[4] += a[20] * b[0] + a[21] * b[1] + a[22] * b[2] + a[23] * b[3] + a[24] * b[4];
+}
+
+void or_h1_apply(const or_mesh *m, const double *Adiag, const double *Aedge, const double *p, double *y) {
+  int N = m->N, E = m->E;
+  for (size_t k = 0; k < 5 * (size_t)N; ++k) y[k] = 0.0;
+  for (int i = 0; i < N; ++i) amv5_add(Adiag + 25 * (size_t)i, p + 5 * (size_t)i, y + 5 * (size_t)i);
+  for (int l = 0; l < E; ++l) {
+    int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
+    amv5_add(Aedge + 50 * (size_t)l, p + 5 * (size_t)j, y + 5 * (size_t)i);
+    amv5_add(Aedge + 50 * (size_t)l + 25, p + 5 * (size_t)i, y + 5 * (size_t)j);
+  }
+}
+
+/* LINPACKdgefa / LINPACKdgedi on the transpose (DenseMatrixOps.h:1006-1181)
+ * for one row-major 5x5 block, in place; returns the dgefa info (0 = ok).
+ * The column-major, 1-based LINPACK indexing a[i + j*n] is kept by storing
+ * the transposed block at buf[6..30].  The pivot search compares |a|, which
+ * is what sqrt(sqNorm(.)) comparisons select (Vector.h:56). */
+#define OR_DEFINE_INVERT(NAME, T)                                                     \
+  int NAME(T *blk) {                                                                  \
+    enum { n = 5 };                                                                   \
+    T buf[n * n + n + 1];                                                             \
+    for (int r = 0; r < 5; ++r)                                                       \
+      for (int c = 0; c < 5; ++c) buf[n + 1 + 5 * c + r] = blk[5 * r + c];            \
+    T *a = buf;                                                                       \
+    int ipvt[n + 1];                                                                  \
+    T work[n + 1];                                                                    \
+    for (int k = 1; k <= n - 1; ++k) {                                                \
+      int kp1 = k + 1, kn = k * n, knp1 = k * n + k;                                  \
+      T *aa = &a[knp1];                                                               \
+      double mx = fabs((double)aa[0]);                                                \
+      int l = 1;                                                                      \
+      for (int ll = 1; ll < n - k + 1; ll++) {                                        \
+        double t2 = fabs((double)aa[ll]);                                             \
+        if (t2 > mx) { mx = t2; l = ll + 1; }                                         \
+      }                                                                               \
+      l += k - 1;                                                                     \
+      ipvt[k] = l;                                                                    \
+      if (a[l + kn] == 0.) return k;                                                  \
+      if (l != k) { T t = a[l + kn]; a[l + kn] = a[knp1]; a[knp1] = t; }              \
+      T t = (T)(-1. / a[knp1]);                                                       \
+      aa = &a[1 + knp1];                                                              \
+      for (int ll = 0; ll < n - k; ll++) aa[ll] *= t;                                 \
+      T *ax = aa;                                                                     \
+      for (int j = kp1; j <= n; ++j) {                                                \
+        int jn1 = j * n;                                                              \
+        t = a[l + jn1];                                                               \
+        if (l != k) { a[l + jn1] = a[k + jn1]; a[k + jn1] = t; }                      \
+        T *ay = &a[1 + k + jn1];                                                      \
+        for (int ll = 0; ll < n - k; ll++) ay[ll] += t * ax[ll];                      \
+      }                                                                               \
+    }                                                                                 \
+    ipvt[n] = n;                                                                      \
+    if (a[n + n * n] == 0.) return n;                                                 \
+    for (int k = 1; k <= n; ++k) {                                                    \
+      int kn = k * n, knp1 = kn + k;                                                  \
+      a[knp1] = (T)(1.0 / a[knp1]);                                                   \
+      T t = -a[knp1];                                                                 \
+      T *aa = &a[1 + kn];                                                             \
+      for (int ll = 0; ll < k - 1; ll++) aa[ll] *= t;                                 \
+      int kp1 = k + 1;                                                                \
+      if (n < kp1) continue;                                                          \
+      T *ax = aa;                                                                     \
+      for (int j = kp1; j <= n; ++j) {                                                \
+        int jn1 = j * n;                                                              \
+        t = a[k + jn1];                                                               \
+        a[k + jn1] = 0.;                                                              \
+        T *ay = &a[1 + jn1];                                                          \
+        for (int ll = 0; ll < k; ll++) ay[ll] += t * ax[ll];                          \
+      }                                                                               \
+    }                                                                                 \
+    for (int kb = 1; kb <= n - 1; ++kb) {                                             \
+      int k = n - kb, kn = k * n, kp1 = k + 1;                                        \
+      T *aa = a + kn;                                                                 \
+      for (int i = kp1; i <= n; ++i) { work[i] = aa[i]; aa[i] = 0.; }                 \
+      for (int j = kp1; j <= n; ++j) {                                                \
+        T t = work[j];                                                                \
+        T *ax = &a[j * n + 1];                                                        \
+        T *ay = &a[kn + 1];                                                           \
+        for (int ll = 0; ll < n; ll++) ay[ll] += t * ax[ll];                          \
+      }                                                                               \
+      int l = ipvt[k];                                                                \
+      if (l != k) {                                                                   \
+        T *ax = &a[kn + 1], *ay = &a[l * n + 1];                                      \
+        for (int ll = 0; ll < n; ll++) { T t3 = ax[ll]; ax[ll] = ay[ll]; ay[ll] = t3; } \
+      }                                                                               \
+    }                                                                                 \
+    for (int r = 0; r < 5; ++r)                                                       \
+      for (int c = 0; c < 5; ++c) blk[5 * r + c] = buf[n + 1 + 5 * c + r];            \
+    return 0;                                                                         \
+  }
+OR_DEFINE_INVERT(or_invert5, double)
+OR_DEFINE_INVERT(or_invert5f, float)
+
+/* DiagMat::apply (DiagMatrix.C:86-100): x_i = A_i y_i (float or double
+ * blocks, double arithmetic). */
+void or_apply5(const double *a, const double *y, double *x) {
+  x[0] = a[0] * y[0] + a[1] * y[1] + a[2] * y[2] + a[3] * y[3] + a[4] * y[4];
+  x[1] = a[5] * y[0] + a[6] * y[1] + a[7] * y[2] + a[8] * y[3] + a[9] * y[4];
+  x[2] = a[10] * y[0] + a[11] * y[1] + a[12] * y[2] + a[13] * y[3] + a[14] * y[4];
+  x[3] = a[15] * y[0] + a[16] * y[1] + a[17] * y[2] + a[18] * y[3] + a[19] * y[4];
+  x[4] = a[20] * y[0] + a[21] * y[1] + a[22] * y[2] + a[23] * y[3] + a[24] * y[4];
+}
+void or_apply5f(const float *a, const double *y, double *x) {
+  x[0] = a[0] * y[0] + a[1] * y[1] + a[2] * y[2] + a[3] * y[3] + a[4] * y[4];
+  x[1] = a[5] * y[0] + a[6] * y[1] + a[7] * y[2] + a[8] * y[3] + a[9] * y[4];
+  x[2] = a[10] * y[0] + a[11] * y[1] + a[12] * y[2] + a[13] * y[3] + a[14] * y[4];
+  x[3] = a[15] * y[0] + a[16] * y[1] + a[17] * y[2] + a[18] * y[3] + a[19] * y[4];
+  x[4] = a[20] * y[0] + a[21] * y[1] + a[22] * y[2] + a[23] * y[3] + a[24] * y[4];
+}
+
+/* ------------------------------------------------------------------------ */
+/* GMRES and the steady Newton loop                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  const or_mesh *m;
+  const double *Adiag, *Aedge;  /* H1 */
+  double *Dinv;                 /* [N][25] inverse diagonal blocks (FP64) */
+  float *Dinvf;                 /* or FP32 blocks (mixed precision) */
+} or_ops;
+
+static void ops_matvec(const or_ops *o, const double *p, double *y) { or_h1_apply(o->m, o->Adiag, o->Aedge, p, y); }
+
+static void ops_prec(const or_ops *o, const double *y, double *x) {
+  int N = o->m->N;
+  for (int i = 0; i < N; ++i) {
+    if (o->Dinvf) or_apply5f(o->Dinvf + 25 * (size_t)i, y + 5 * (size_t)i, x + 5 * (size_t)i);
+    else or_apply5(o->Dinv + 25 * (size_t)i, y + 5 * (size_t)i, x + 5 * (size_t)i);
+  }
+}
+
+static double vdot(size_t n, const double *a, const double *b) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* GmresSolver::solve (KspSolver.C:353-492), typePrec = 2 (right), modified
+ * Gram-Schmidt (:404-411), rotations and back-substitution (:919-982).
+ * x is set to 0 first (ImplicitCoupledTsDesc.C:355).  Returns iterations. */
+int or_gmres(const or_ops *o, const double *b, double *x, int m, double eps, int maxits, double *final_res) {
+  size_t n = 5 * (size_t)o->m->N;
+  double *V = (double *)malloc(sizeof(double) * n * (size_t)(m + 1));
+  double *w = (double *)malloc(sizeof(double) * n), *r = (double *)malloc(sizeof(double) * n);
+  double *H = (double *)calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = (double *)calloc(2 * (size_t)m, sizeof(double));
+  double *g = (double *)calloc((size_t)m + 1, sizeof(double)), *y = (double *)calloc((size_t)m, sizeof(double));
+#define HH(i, j) H[(size_t)(i) * m + (j)]
+  for (size_t i = 0; i < n; ++i) x[i] = 0.0;
+  int iter = 0, exitLoop = 0;
+  double target = 0.0, l2res = 0.0;
+  do {
+    ops_matvec(o, x, w);
+    for (size_t i = 0; i < n; ++i) r[i] = b[i] - w[i];
+    double beta = sqrt(vdot(n, r, r));
+    if (iter == 0) target = eps * sqrt(vdot(n, b, b));
+    if (beta == 0.0) break;
+    for (size_t i = 0; i < n; ++i) V[i] = (1.0 / beta) * r[i];
+    for (int k = 0; k <= m; ++k) g[k] = 0.0;
+    g[0] = beta;
+    int j;
+    for (j = 0; j < m; ++j) {
+      ops_prec(o, V + (size_t)j * n, r);
+      ops_matvec(o, r, w);
+      for (int i = 0; i <= j; ++i) {
+        HH(i, j) = vdot(n, w, V + (size_t)i * n);
+        const double *vi = V + (size_t)i * n;
+        for (size_t q = 0; q < n; ++q) w[q] -= HH(i, j) * vi[q];
+      }
+      HH(j + 1, j) = sqrt(vdot(n, w, w));
+      double hn = HH(j + 1, j);
+      for (int i = 0; i < j; ++i) {
+        double hj = cs[2 * i + 1] * HH(i, j) + cs[2 * i] * HH(i + 1, j);
+        double hjp1 = -cs[2 * i] * HH(i, j) + cs[2 * i + 1] * HH(i + 1, j);
+        HH(i, j) = hj;
+        HH(i + 1, j) = hjp1;
+      }
+      if (HH(j, j) * HH(j, j) == 0.0) {
+        cs[2 * j + 1] = 0.0;
+        cs[2 * j] = 1.0;
+      } else {
+        double coef = 1.0 / sqrt(HH(j, j) * HH(j, j) + HH(j + 1, j) * HH(j + 1, j));
+        cs[2 * j + 1] = sqrt(HH(j, j) * HH(j, j)) * coef;
+        cs[2 * j] = cs[2 * j + 1] * HH(j + 1, j) / HH(j, j);
+      }
+      HH(j, j) = cs[2 * j + 1] * HH(j, j) + cs[2 * j] * HH(j + 1, j);
+      HH(j + 1, j) = 0.0;
+      g[j + 1] = -cs[2 * j] * g[j];
+      g[j] = cs[2 * j + 1] * g[j];
+      if (hn == 0.0) { ++iter; exitLoop = 1; break; }
+      double *vn = V + (size_t)(j + 1) * n;
+      for (size_t q = 0; q < n; ++q) vn[q] = (1. / hn) * w[q];
+      l2res = fabs(g[j + 1]);
+      ++iter;
+      if (l2res <= target || iter >= maxits) { exitLoop = 1; break; }
+    }
+    if (j == m) --j;
+    for (int i = j; i >= 0; --i) {
+      y[i] = g[i];
+      for (int k = j; k >= i + 1; --k) y[i] -= HH(i, k) * y[k];
+      y[i] /= HH(i, i);
+    }
+    for (size_t q = 0; q < n; ++q) w[q] = 0.0;
+    for (int k = 0; k <= j; ++k) {
+      const double *vk = V + (size_t)k * n;
+      for (size_t q = 0; q < n; ++q) w[q] += y[k] * vk[q];
+    }
+    ops_prec(o, w, r);
+    for (size_t q = 0; q < n; ++q) x[q] += r[q];
+  } while (!exitLoop);
+#undef HH
+  if (final_res) *final_res = l2res;
+  free(V); free(w); free(r); free(H); free(cs); free(g); free(y);
+  return iter;
+}
+
+/* Builds the operators of one Newton step: H1 at V with the dt term and the
+ * inverted diagonal blocks (FP64, or FP32 copies for precision 1, as
+ * JacobiPrec<float,5>::getData + setup). */
+int or_gmres_h1(const or_mesh *m, const or_params *p, const double *V, const double *dt, int precision,
+                const double *b, double *x, int gm, double eps, int maxits, double *final_res) {
+  int N = m->N, E = m->E;
+  double *Ad = (double *)malloc(sizeof(double) * 25 * (size_t)N);
+  double *Ae = (double *)malloc(sizeof(double) * 50 * (size_t)E);
+  or_jacobian_h1(m, p, V, dt, Ad, Ae);
+  or_ops o = {m, Ad, Ae, 0, 0};
+  if (precision) {
+    o.Dinvf = (float *)malloc(sizeof(float) * 25 * (size_t)N);
+    for (size_t k = 0; k < 25 * (size_t)N; ++k) o.Dinvf[k] = (float)Ad[k];
+    for (int i = 0; i < N; ++i) or_invert5f(o.Dinvf + 25 * (size_t)i);
+  } else {
+    o.Dinv = (double *)malloc(sizeof(double) * 25 * (size_t)N);
+    for (size_t k = 0; k < 25 * (size_t)N; ++k) o.Dinv[k] = Ad[k];
+    for (int i = 0; i < N; ++i) or_invert5(o.Dinv + 25 * (size_t)i);
+  }
+  int its = or_gmres(&o, b, x, gm, eps, maxits, final_res);
+  free(Ad); free(Ae); free(o.Dinv); free(o.Dinvf);
+  return its;
+}
+
+/* TsParameters::computeCflNumber, CflLaw Old (TsParameters.C:198-200):
+ * cfl = min(max(max(c1, min(its*c2, cfl + c2)), cfl0/res^ser), cflMax). */
+double or_cfl_old(int its, double res, double cfl, double cfl0, double cflMax, double ser, double c1, double c2) {
+  double a = its * c2 < cfl + c2 ? its * c2 : cfl + c2;
+  double b = c1 > a ? c1 : a;
+  double c = cfl0 / pow(res, ser);
+  double d = b > c ? b : c;
+  return d < cflMax ? d : cflMax;
+}
+
+/* Steady pseudo-transient Newton (one Newton iteration per step) with the
+ * CflLaw Old (TsParameters.C:198-200); returns steps taken; history[k] =
+ * ||R||_2 at step k; *converged set when ||R|| <= eps ||R0||. */
+int or_steady_solve(const or_mesh *m, const or_params *p, double *U, int precision, int max_steps,
+                    double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
+                    double geps, double *history, int *converged, int *krylov_its) {
+  int N = m->N;
+  size_t n = 5 * (size_t)N;
+  double *R = (double *)malloc(sizeof(double) * n), *V = (double *)malloc(sizeof(double) * n);
+  double *dt = (double *)malloc(sizeof(double) * N), *b = (double *)malloc(sizeof(double) * n);
+  double *dQ = (double *)malloc(sizeof(double) * n);
+  double res0 = 0.0, cfl = cfl0;
+  int step;
+  *converged = 0;
+  *krylov_its = 0;
+  for (step = 0; step <= max_steps; ++step) {
+    int ierr = or_residual(m, p, U, R, 0, V, 0, 0, 0);
+    if (ierr) break;
+    double res = sqrt(vdot(n, R, R));
+    if (step == 0) res0 = res;
+    if (history) history[step] = res;
+    if (res == 0.0 || res <= eps * res0) { *converged = 1; break; }
+    if (step == max_steps) break;
+    double rr = res / res0;
+    cfl = or_cfl_old(step, rr, cfl, cfl0, cfl_max, ser, 0.0, 0.0);
+    or_timestep(m, p, V, cfl, dt);
+    for (size_t k = 0; k < n; ++k) b[k] = -1.0 * R[k];
+    double fr;
+    *krylov_its += or_gmres_h1(m, p, V, dt, precision, b, dQ, gm, geps, gmaxits, &fr);
+    for (size_t k = 0; k < n; ++k) U[k] += dQ[k];
+  }
+  free(R); free(V); free(dt); free(b); free(dQ);
+  return step;
+}
