@@ -72,6 +72,46 @@ def cpu_baseline(mesh, params, U, budget_s):
                        f"single thread, median {med * 1e3:.0f} ms each")
 
 
+def newton_bench(dev, max_steps):
+    """Steady Newton-GMRES on the 96x32x32 bump channel, FP64 vs mixed."""
+    import numpy as np
+    import torch
+
+    from aerof_dev_amd import FlowParams, init_state
+    from aerof_dev_amd import mesh as M
+    from aerof_dev_amd.operators import SpaceOperator, steady_solve
+
+    mesh = M.channel_mesh(96, 32, 32, bump_height=0.05, bump_width=0.2)
+    params = FlowParams(mach=0.675)
+    U0 = init_state(params, mesh.X, "freestream")
+    op = SpaceOperator(mesh, params)
+    # warm-up: allocations and first launches
+    Uw = torch.from_numpy(U0).to(dev)
+    steady_solve(op, Uw, precision=0, max_steps=2)
+    steady_solve(op, Uw, precision=1, max_steps=2)
+    out = {"workload": f"steady Euler, 96x32x32 Kuhn channel with Gaussian bump ({op.num_nodes} nodes, "
+                       f"{op.num_edges} edges), M=0.675, Newton-GMRES(30) + block-Jacobi, CFL 10->1e4, "
+                       "stop at ||R||/||R0|| <= 1e-8"}
+    states = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        Ud = torch.from_numpy(U0).to(dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = steady_solve(op, Ud, precision=prec, max_steps=max_steps, eps=1e-8, cfl0=10.0, cfl_max=1e4)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        states[name] = Ud.cpu().numpy()
+        out[name] = {"seconds": round(wall, 4), "steps": r["steps"], "krylov_its": r["krylov_its"],
+                     "converged": r["converged"], "rel_res": float(r["res"] / r["res0"])}
+        log(f"[bench] newton {name}: {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its, "
+            f"converged={r['converged']} rel_res={r['res'] / r['res0']:.2e}")
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    out["state_rel_diff_mixed_vs_fp64"] = float(np.linalg.norm(states["mixed"] - states["fp64"]) /
+                                                np.linalg.norm(states["fp64"]))
+    op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +120,8 @@ def main():
     ap.add_argument("--n", type=int, default=NX, help="hexes per axis")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle timing")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--newton-steps", type=int, default=600)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -173,6 +215,9 @@ def main():
                     "fvTerm": {"ms": round(flux_ms, 5), "algorithmic_GBs": round(achieved, 1)}},
         "residual_hbm_frac": round(whole_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+    if not args.no_newton:
+        op.close()
+        out["newton"] = newton_bench(dev, args.newton_steps)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mesh, params, U, args.cpu_budget)
