@@ -118,6 +118,9 @@ int launch_timestep(aerofx_ctx *c, double cfl, double *dt);
 int launch_jacobian(aerofx_ctx *c, const double *dt, int precision);
 int launch_h1_apply(aerofx_ctx *c, const double *p, double *y);
 int launch_prec_apply(aerofx_ctx *c, const double *y, double *x);
+// variants that return at once when *done != 0 (device-resident Krylov loops)
+int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done);
+int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done);
 void free_krylov(aerofx_ctx *c);
 
 }  // namespace aerofx

@@ -184,7 +184,9 @@ __device__ __forceinline__ void amv5_add(const T *__restrict__ a, const double b
 // y = H1 p (SubDomain::computeMatVecProdH1, SubDomain.C:3917-3953)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
-                                                     const double *__restrict__ p, double *__restrict__ y) {
+                                                     const double *__restrict__ p, double *__restrict__ y,
+                                                     const int *__restrict__ done) {
+  if (done && *done) return;
   const int n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= t.N) return;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -206,7 +208,9 @@ __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__res
 // x = Di y (DiagMat::apply, DiagMatrix.C:86-100)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restrict__ Di,
-                                                         const double *__restrict__ y, double *__restrict__ x) {
+                                                         const double *__restrict__ y, double *__restrict__ x,
+                                                         const int *__restrict__ done) {
+  if (done && *done) return;
   const int n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= N) return;
   double yn[5], acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -274,30 +278,38 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   return check_launch("invert launch");
 }
 
-int launch_h1_apply(aerofx_ctx *c, const double *p, double *y) {
+int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done) {
   if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "jvp: Jacobian not set up");
   const int nb = (c->N + kBlock - 1) / kBlock;
-  ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
   if (c->jac_precision)
     hipLaunchKernelGGL((k_h1_apply<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
-                       (const float *)c->d_jacD, (const float *)c->d_jacB, p, y);
+                       (const float *)c->d_jacD, (const float *)c->d_jacB, p, y, done);
   else
     hipLaunchKernelGGL((k_h1_apply<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
-                       (const double *)c->d_jacD, (const double *)c->d_jacB, p, y);
+                       (const double *)c->d_jacD, (const double *)c->d_jacB, p, y, done);
   return check_launch("jvp launch");
 }
 
-int launch_prec_apply(aerofx_ctx *c, const double *y, double *x) {
+int launch_h1_apply(aerofx_ctx *c, const double *p, double *y) {
+  ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
+  return launch_h1_apply_flag(c, p, y, nullptr);
+}
+
+int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done) {
   if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
   const int nb = (c->N + kBlock - 1) / kBlock;
-  ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
   if (c->jac_precision)
     hipLaunchKernelGGL((k_jacobi_apply<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
-                       (const float *)c->d_jacDi, y, x);
+                       (const float *)c->d_jacDi, y, x, done);
   else
     hipLaunchKernelGGL((k_jacobi_apply<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
-                       (const double *)c->d_jacDi, y, x);
+                       (const double *)c->d_jacDi, y, x, done);
   return check_launch("prec launch");
+}
+
+int launch_prec_apply(aerofx_ctx *c, const double *y, double *x) {
+  ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+  return launch_prec_apply_flag(c, y, x, nullptr);
 }
 
 }  // namespace aerofx

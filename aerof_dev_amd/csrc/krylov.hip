@@ -1,15 +1,22 @@
 // krylov.hip -- GMRES and the pseudo-transient Newton driver.
 //
-// GmresSolver::solve (KspSolver.C:353-492) with right preconditioning,
-// Givens rotations and back-substitution (KspSolver.C:919-982) on the host;
-// every vector operation runs on the GPU in FP64.  Orthogonalisation is
-// classical Gram-Schmidt with one reorthogonalisation pass (CGS2): each pass
-// is one fused multi-dot (all basis vectors against w in one sweep) and one
-// fused update, instead of the reference's j+1 sequential dot/axpy pairs
-// (modified Gram-Schmidt, KspSolver.C:404-411).  Dot products reduce
-// deterministically: a fixed grid of kDotBlocks blocks, each summing its
-// contiguous slice with a fixed tree, then one block summing the partials in
-// order.
+// GmresSolver::solve (KspSolver.C:353-492): right-preconditioned GMRES(m)
+// with Givens rotations and back-substitution (KspSolver.C:919-982).  The
+// whole cycle runs on the GPU as a fixed sequence of kernels with no host
+// synchronisation: the Hessenberg matrix, rotations, residual estimate and
+// the stop flag live in device memory, a one-lane kernel applies each
+// rotation, and every vector kernel of a later Arnoldi step returns at once
+// when the flag is set.  The host synchronises once per cycle.
+//
+// Orthogonalisation is classical Gram-Schmidt with one reorthogonalisation
+// pass (CGS2) in FP64, fused into three sweeps over the basis per step:
+//   sweep 1  h1 = V^T w
+//   sweep 2  w -= V h1, then h2 = V^T w   (one read of V serves both)
+//   sweep 3  w -= V h2, then ||w||^2
+// instead of the reference's j+1 dependent dot/axpy pairs (modified
+// Gram-Schmidt, KspSolver.C:404-411).  Reductions are deterministic: a fixed
+// grid of kDotBlocks blocks sums contiguous slices (wave butterflies in a
+// fixed pattern), and one block sums the partials in order.
 //
 // The Newton driver follows TsSolver/NewtonSolver for a steady problem with
 // one Newton iteration per pseudo-time step (NewtonSolver.h:70-231,
@@ -31,88 +38,190 @@
 namespace aerofx {
 
 constexpr int kDotBlocks = 1024;  // fixed: the reduction order never depends on the device
-constexpr int kDotGroup = 8;      // basis vectors per multi-dot pass
+constexpr int kMaxBasis = 32;     // basis vectors handled by one sweep (m <= 31)
+constexpr int kWave = 64;
 
-// partial[g * kDotBlocks + b] = sum over block b's slice of x * Y_g
-__global__ __launch_bounds__(kBlock) void k_multidot_partial(size_t n, const double *__restrict__ x,
-                                                             const double *__restrict__ Y, size_t ldy, int ny,
-                                                             double *__restrict__ partial) {
-  __shared__ double sh[kDotGroup][kBlock];
-  const int g0 = blockIdx.y * kDotGroup;
-  const int ng = min(kDotGroup, ny - g0);
-  const size_t slice = (n + kDotBlocks - 1) / kDotBlocks;
-  const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
-  double acc[kDotGroup];
+// Device-resident GMRES state (one per right-hand side).
+struct GmresState {
+  double H[(kMaxBasis + 1) * kMaxBasis];  // row-major (i, j) -> H[i * kMaxBasis + j]
+  double cs[2 * kMaxBasis];               // (sin, cos) per rotation
+  double g[kMaxBasis + 1];
+  double y[kMaxBasis];
+  double dots[3 * kMaxBasis];             // h1 | h2 | ||w||^2
+  double beta, target, l2res, invhn;
+  int iter, jlast, done, maxits, m;
+};
+
+// ---------------------------------------------------------------------------
+// deterministic block reduction of K values per thread
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[K], int nk, double *__restrict__ partial) {
+  __shared__ double sh[K][kBlock / kWave];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
-  for (int g = 0; g < kDotGroup; ++g) acc[g] = 0.0;
-  for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
-    const double xi = x[i];
+  for (int k = 0; k < K; ++k) {
+    double v = acc[k];
 #pragma unroll
-    for (int g = 0; g < kDotGroup; ++g)
-      if (g < ng) acc[g] += xi * Y[(size_t)(g0 + g) * ldy + i];
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if (lane == 0) sh[k][wave] = v;
   }
-#pragma unroll
-  for (int g = 0; g < kDotGroup; ++g) sh[g][threadIdx.x] = acc[g];
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
+  if ((int)threadIdx.x < nk) {
+    double s = 0.0;
 #pragma unroll
-      for (int g = 0; g < kDotGroup; ++g) sh[g][threadIdx.x] += sh[g][threadIdx.x + s];
-    __syncthreads();
+    for (int w = 0; w < kBlock / kWave; ++w) s += sh[threadIdx.x][w];
+    partial[(size_t)threadIdx.x * kDotBlocks + blockIdx.x] = s;
   }
-  if (threadIdx.x < (unsigned)ng) partial[(size_t)(g0 + threadIdx.x) * kDotBlocks + blockIdx.x] = sh[threadIdx.x][0];
 }
 
-// out[g] = sum_b partial[g][b], fixed tree; one block per g
-__global__ __launch_bounds__(kBlock) void k_multidot_final(const double *__restrict__ partial,
-                                                           double *__restrict__ out) {
+// mode 0: partial_k = sum w V_k
+// mode 1: w -= sum_k h_k V_k, then partial_k = sum w V_k
+// mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__ w, const double *__restrict__ V,
+                                                  size_t ldv, int ny, const double *__restrict__ h,
+                                                  double *__restrict__ partial, const int *__restrict__ done) {
+  if (done && *done) return;
+  const size_t slice = (n + kDotBlocks - 1) / kDotBlocks;
+  const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
+  double acc[kMaxBasis];
+#pragma unroll
+  for (int k = 0; k < kMaxBasis; ++k) acc[k] = 0.0;
+  double hk[kMaxBasis];
+  if (MODE != 0) {
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) hk[k] = k < ny ? h[k] : 0.0;
+  }
+  for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
+    double v[kMaxBasis];
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) v[k] = k < ny ? V[(size_t)k * ldv + i] : 0.0;
+    double wi = w[i];
+    if (MODE != 0) {
+#pragma unroll
+      for (int k = 0; k < kMaxBasis; ++k)
+        if (k < ny) wi -= hk[k] * v[k];
+      w[i] = wi;
+    }
+    if (MODE == 2) {
+      acc[0] += wi * wi;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kMaxBasis; ++k) acc[k] += wi * v[k];
+    }
+  }
+  block_reduce_store<kMaxBasis>(acc, MODE == 2 ? 1 : ny, partial);
+}
+
+// out[k] = sum_b partial[k][b] in a fixed tree, one block per k
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
+                                                            double *__restrict__ out, const int *__restrict__ done) {
+  if (done && *done) return;
   __shared__ double sh[kBlock];
-  const int g = blockIdx.x;
+  const int k = blockIdx.x;
   double s = 0.0;
-  for (int b = threadIdx.x; b < kDotBlocks; b += kBlock) s += partial[(size_t)g * kDotBlocks + b];
+  for (int b = threadIdx.x; b < kDotBlocks; b += kBlock) s += partial[(size_t)k * kDotBlocks + b];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int st = kBlock / 2; st > 0; st >>= 1) {
     if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[g] = sh[0];
+  if (threadIdx.x == 0) out[k] = sh[0];
 }
 
-// w -= sum_k h[k] Y_k  (h on the device)
-__global__ __launch_bounds__(kBlock) void k_update(size_t n, double *__restrict__ w, const double *__restrict__ Y,
-                                                   size_t ldy, int ny, const double *__restrict__ h) {
+// y = a x + b y, a possibly scaled by a device scalar (*as)
+__global__ __launch_bounds__(kBlock) void k_axpby(size_t n, double a, const double *__restrict__ as,
+                                                  const double *__restrict__ x, double b, double *__restrict__ y,
+                                                  const int *__restrict__ done) {
+  if (done && *done) return;
   const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
   if (i >= n) return;
-  double s = w[i];
-  for (int k = 0; k < ny; ++k) s -= h[k] * Y[(size_t)k * ldy + i];
-  w[i] = s;
-}
-
-// y = a x + b y
-__global__ __launch_bounds__(kBlock) void k_axpby(size_t n, double a, const double *__restrict__ x, double b,
-                                                  double *__restrict__ y) {
-  const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
-  if (i >= n) return;
-  y[i] = a * x[i] + b * y[i];
-}
-
-// w = sum_k c[k] Y_k (c passed by value through a small struct)
-struct Coefs {
-  double c[64];
-};
-__global__ __launch_bounds__(kBlock) void k_combine(size_t n, double *__restrict__ w, const double *__restrict__ Y,
-                                                    size_t ldy, int ny, Coefs c) {
-  const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < ny; ++k) s += c.c[k] * Y[(size_t)k * ldy + i];
-  w[i] = s;
+  const double aa = as ? a * (*as) : a;
+  y[i] = aa * x[i] + b * y[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(size_t n, double v, double *__restrict__ y) {
   const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
   if (i < n) y[i] = v;
+}
+
+// w = sum_{k <= jlast} y_k V_k
+__global__ __launch_bounds__(kBlock) void k_combine(size_t n, double *__restrict__ w, const double *__restrict__ V,
+                                                    size_t ldv, const GmresState *__restrict__ s) {
+  const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int nk = s->jlast + 1;
+  double acc = 0.0;
+  for (int k = 0; k < nk; ++k) acc += s->y[k] * V[(size_t)k * ldv + i];
+  w[i] = acc;
+}
+
+// start of a cycle: beta = ||r|| (r = b - A x), target from ||b|| on the first
+__global__ void k_gmres_start(GmresState *s, const double *__restrict__ nrm2, int first, double eps) {
+  if (threadIdx.x != 0) return;
+  const double beta = sqrt(nrm2[0]);
+  s->beta = beta;
+  if (first) {
+    s->target = eps * beta;  // x0 = 0: r = b
+    s->iter = 0;
+  }
+  for (int k = 0; k <= s->m; ++k) s->g[k] = 0.0;
+  s->g[0] = beta;
+  s->done = (beta == 0.0) ? 2 : 0;
+  s->invhn = beta == 0.0 ? 0.0 : 1.0 / beta;
+  s->jlast = -1;
+}
+
+// Arnoldi step j: Hessenberg column, rotations, residual estimate, stop flag
+// (KspSolver.C:404-447, 919-962).
+__global__ void k_gmres_givens(GmresState *s, int j) {
+  if (threadIdx.x != 0 || s->done) return;
+  const int M = kMaxBasis;
+  double *H = s->H;
+  for (int i = 0; i <= j; ++i) H[i * M + j] = s->dots[i] + s->dots[kMaxBasis + i];
+  const double hn = sqrt(s->dots[2 * kMaxBasis]);
+  H[(j + 1) * M + j] = hn;
+  for (int i = 0; i < j; ++i) {
+    double hj = s->cs[2 * i + 1] * H[i * M + j] + s->cs[2 * i] * H[(i + 1) * M + j];
+    double hjp1 = -s->cs[2 * i] * H[i * M + j] + s->cs[2 * i + 1] * H[(i + 1) * M + j];
+    H[i * M + j] = hj;
+    H[(i + 1) * M + j] = hjp1;
+  }
+  const double hjj = H[j * M + j], hj1 = H[(j + 1) * M + j];
+  if (hjj * hjj == 0.0) {
+    s->cs[2 * j + 1] = 0.0;
+    s->cs[2 * j] = 1.0;
+  } else {
+    double coef = 1.0 / sqrt(hjj * hjj + hj1 * hj1);
+    s->cs[2 * j + 1] = sqrt(hjj * hjj) * coef;
+    s->cs[2 * j] = s->cs[2 * j + 1] * hj1 / hjj;
+  }
+  H[j * M + j] = s->cs[2 * j + 1] * hjj + s->cs[2 * j] * hj1;
+  H[(j + 1) * M + j] = 0.0;
+  s->g[j + 1] = -s->cs[2 * j] * s->g[j];
+  s->g[j] = s->cs[2 * j + 1] * s->g[j];
+  s->iter += 1;
+  s->jlast = j;
+  if (hn == 0.0) {
+    s->done = 1;
+    return;
+  }
+  s->invhn = 1.0 / hn;
+  s->l2res = fabs(s->g[j + 1]);
+  if (s->l2res <= s->target || s->iter >= s->maxits) s->done = 1;
+}
+
+// back-substitution (KspSolver.C:968-982)
+__global__ void k_gmres_backsolve(GmresState *s) {
+  if (threadIdx.x != 0) return;
+  const int M = kMaxBasis, j = s->jlast;
+  for (int i = j; i >= 0; --i) {
+    double yi = s->g[i];
+    for (int k = j; k >= i + 1; --k) yi -= s->H[i * M + k] * s->y[k];
+    s->y[i] = yi / s->H[i * M + i];
+  }
 }
 
 // count nodes whose density or pressure is not positive
@@ -132,20 +241,23 @@ struct Krylov {
   size_t n = 0;
   double *V = nullptr;        // [m+1][n] basis
   double *w = nullptr, *z = nullptr, *r = nullptr, *x = nullptr, *b = nullptr, *R = nullptr;
-  double *partial = nullptr;  // [64][kDotBlocks]
-  double *dots = nullptr;     // [128] device results
-  double *h_dots = nullptr;   // pinned host copy
+  double *partial = nullptr;  // [kMaxBasis][kDotBlocks]
+  double *scal = nullptr;     // device scalars
+  GmresState *st = nullptr;   // device
+  GmresState *h_st = nullptr; // pinned host mirror
+  double *h_scal = nullptr;   // pinned
 };
 
 void free_krylov(aerofx_ctx *c) {
   if (!c->ksp) return;
-  if (c->ksp->h_dots) (void)hipHostFree(c->ksp->h_dots);
+  if (c->ksp->h_st) (void)hipHostFree(c->ksp->h_st);
+  if (c->ksp->h_scal) (void)hipHostFree(c->ksp->h_scal);
   delete c->ksp;
   c->ksp = nullptr;  // device buffers belong to c->allocs
 }
 
 static int ensure_krylov(aerofx_ctx *c, int m) {
-  if (m < 1 || m > 62) return set_error(AEROFX_ERR_ARG, "gmres: Krylov dimension must be in [1, 62]");
+  if (m < 1 || m > kMaxBasis - 1) return set_error(AEROFX_ERR_ARG, "gmres: Krylov dimension must be in [1, 31]");
   if (c->ksp && c->ksp->m >= m) return 0;
   free_krylov(c);
   Krylov *k = new Krylov();
@@ -160,11 +272,14 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
     if ((rc = ctx_alloc(c, sizeof(double) * k->n, &p))) { delete k; return rc; }
     *v = (double *)p;
   }
-  if ((rc = ctx_alloc(c, sizeof(double) * 64 * kDotBlocks, &p))) { delete k; return rc; }
+  if ((rc = ctx_alloc(c, sizeof(double) * kMaxBasis * kDotBlocks, &p))) { delete k; return rc; }
   k->partial = (double *)p;
-  if ((rc = ctx_alloc(c, sizeof(double) * 128, &p))) { delete k; return rc; }
-  k->dots = (double *)p;
-  hipError_t e = hipHostMalloc((void **)&k->h_dots, sizeof(double) * 128, hipHostMallocDefault);
+  if ((rc = ctx_alloc(c, sizeof(double) * 16, &p))) { delete k; return rc; }
+  k->scal = (double *)p;
+  if ((rc = ctx_alloc(c, sizeof(GmresState), &p))) { delete k; return rc; }
+  k->st = (GmresState *)p;
+  hipError_t e = hipHostMalloc((void **)&k->h_st, sizeof(GmresState), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&k->h_scal, sizeof(double) * 16, hipHostMallocDefault);
   if (e != hipSuccess) { delete k; return set_hip_error(e, "hipHostMalloc"); }
   c->ksp = k;
   return 0;
@@ -172,137 +287,92 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
 
 static inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-// dots[0..ny) = x . Y_k on the device (no synchronisation)
-static void multidot(aerofx_ctx *c, const double *x, const double *Y, size_t ldy, int ny, double *out) {
+template <int MODE>
+static void sweep(aerofx_ctx *c, double *w, const double *V, size_t ldv, int ny, const double *h, double *out,
+                  const int *done) {
   Krylov *k = c->ksp;
-  dim3 grid(kDotBlocks, (ny + kDotGroup - 1) / kDotGroup);
-  hipLaunchKernelGGL(k_multidot_partial, grid, dim3(kBlock), 0, c->stream, k->n, x, Y, ldy, ny, k->partial);
-  hipLaunchKernelGGL(k_multidot_final, dim3(ny), dim3(kBlock), 0, c->stream, k->partial, out);
+  hipLaunchKernelGGL((k_sweep<MODE>), dim3(kDotBlocks), dim3(kBlock), 0, c->stream, k->n, w, V, ldv, ny, h,
+                     k->partial, done);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(MODE == 2 ? 1 : ny), dim3(kBlock), 0, c->stream, k->partial, out,
+                     done);
 }
 
-static int fetch(aerofx_ctx *c, const double *dev, int count) {
-  hipError_t e = hipMemcpyAsync(c->ksp->h_dots, dev, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream);
-  if (e != hipSuccess) return set_hip_error(e, "hipMemcpyAsync");
-  e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return set_hip_error(e, "hipStreamSynchronize");
-  return 0;
-}
-
-// ||x||_2 with the deterministic reduction (synchronises)
-static int norm2(aerofx_ctx *c, const double *x, double *out) {
-  multidot(c, x, x, 0, 1, c->ksp->dots);
-  int rc = fetch(c, c->ksp->dots, 1);
-  if (rc) return rc;
-  *out = std::sqrt(c->ksp->h_dots[0]);
-  return 0;
+static int fetch_state(aerofx_ctx *c) {
+  hipError_t e = hipMemcpyAsync(c->ksp->h_st, c->ksp->st, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? 0 : set_hip_error(e, "gmres state copy");
 }
 
 struct GmresOut {
   int its = 0;
-  double res0 = 0, res = 0, target = 0;
+  double res = 0, target = 0;
 };
 
-// Right-preconditioned GMRES(m) on H1 with block-Jacobi: solves A x = b,
-// x starts at 0 (ImplicitCoupledTsDesc::solveLinearSystem sets dQ = 0).
+// Right-preconditioned GMRES(m) on H1 with block-Jacobi: A x = b, x0 = 0
+// (ImplicitCoupledTsDesc::solveLinearSystem sets dQ = 0).  Enqueues one
+// cycle at a time and synchronises once per cycle.
 static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, int maxits, GmresOut *out) {
   int rc = ensure_krylov(c, m);
   if (rc) return rc;
   Krylov *k = c->ksp;
   const size_t n = k->n;
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(2 * (size_t)m), g(m + 1), y(m);
-  auto Hij = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+  *k->h_st = GmresState{};
+  k->h_st->maxits = maxits;
+  k->h_st->m = m;
+  hipError_t e = hipMemcpyAsync(k->st, k->h_st, sizeof(GmresState), hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "gmres init copy");
+  int *done = &k->st->done;
   hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, c->stream, n, 0.0, x);
-  double target = 0.0, l2res = 0.0;
-  int iter = 0;
-  bool exitLoop = false;
-  do {
+  for (int cycle = 0;; ++cycle) {
     // r = b - A x
-    if ((rc = launch_h1_apply(c, x, k->w))) return rc;
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, b, 0.0, k->r);
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, k->w, 1.0, k->r);
-    double beta;
-    if ((rc = norm2(c, k->r, &beta))) return rc;
-    if (iter == 0) {
-      double bn;
-      if ((rc = norm2(c, b, &bn))) return rc;
-      target = eps * bn;
-      out->res0 = bn;
+    if (cycle == 0) {
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, 0.0, k->r, nullptr);
+    } else {
+      if ((rc = launch_h1_apply(c, x, k->w))) return rc;
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, 0.0, k->r, nullptr);
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->w, 1.0, k->r,
+                         nullptr);
     }
-    if (beta == 0.0) break;
+    sweep<2>(c, k->r, nullptr, 0, 0, nullptr, k->scal, nullptr);
+    hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, c->stream, k->st, k->scal, cycle == 0 ? 1 : 0, eps);
     // V0 = r / beta
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0 / beta, k->r, 0.0, k->V);
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
-    int j;
-    for (j = 0; j < m; ++j) {
+    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, &k->st->invhn, k->r, 0.0, k->V,
+                       done);
+    for (int j = 0; j < m; ++j) {
       double *Vj = k->V + (size_t)j * n;
-      if ((rc = launch_prec_apply(c, Vj, k->z))) return rc;
-      if ((rc = launch_h1_apply(c, k->z, k->w))) return rc;
+      {
+        ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+        if ((rc = launch_prec_apply_flag(c, Vj, k->z, done))) return rc;
+      }
+      {
+        ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
+        if ((rc = launch_h1_apply_flag(c, k->z, k->w, done))) return rc;
+      }
       {
         ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
-        // CGS2: two passes of h = V^T w, w -= V h; then ||w||
-        multidot(c, k->w, k->V, n, j + 1, k->dots);
-        hipLaunchKernelGGL(k_update, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, n, j + 1, k->dots);
-        multidot(c, k->w, k->V, n, j + 1, k->dots + 64);
-        hipLaunchKernelGGL(k_update, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, n, j + 1,
-                           k->dots + 64);
-        multidot(c, k->w, k->w, 0, 1, k->dots + 127);
-      }
-      if ((rc = fetch(c, k->dots, 128))) return rc;
-      for (int i = 0; i <= j; ++i) Hij(i, j) = k->h_dots[i] + k->h_dots[64 + i];
-      double hn = std::sqrt(k->h_dots[127]);
-      Hij(j + 1, j) = hn;
-      // rotations (KspSolver.C:919-962)
-      for (int i = 0; i < j; ++i) {
-        double hj = cs[2 * i + 1] * Hij(i, j) + cs[2 * i] * Hij(i + 1, j);
-        double hjp1 = -cs[2 * i] * Hij(i, j) + cs[2 * i + 1] * Hij(i + 1, j);
-        Hij(i, j) = hj;
-        Hij(i + 1, j) = hjp1;
-      }
-      if (Hij(j, j) * Hij(j, j) == 0.0) {
-        cs[2 * j + 1] = 0.0;
-        cs[2 * j] = 1.0;
-      } else {
-        double coef = 1.0 / std::sqrt(Hij(j, j) * Hij(j, j) + Hij(j + 1, j) * Hij(j + 1, j));
-        cs[2 * j + 1] = std::sqrt(Hij(j, j) * Hij(j, j)) * coef;
-        cs[2 * j] = cs[2 * j + 1] * Hij(j + 1, j) / Hij(j, j);
-      }
-      Hij(j, j) = cs[2 * j + 1] * Hij(j, j) + cs[2 * j] * Hij(j + 1, j);
-      Hij(j + 1, j) = 0.0;
-      g[j + 1] = -cs[2 * j] * g[j];
-      g[j] = cs[2 * j + 1] * g[j];
-      if (hn == 0.0) {
-        ++iter;
-        exitLoop = true;
-        break;
-      }
-      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0 / hn, k->w, 0.0,
-                         k->V + (size_t)(j + 1) * n);
-      l2res = std::fabs(g[j + 1]);
-      ++iter;
-      if (l2res <= target || iter >= maxits) {
-        exitLoop = true;
-        break;
+        sweep<0>(c, k->w, k->V, n, j + 1, nullptr, k->st->dots, done);
+        sweep<1>(c, k->w, k->V, n, j + 1, k->st->dots, k->st->dots + kMaxBasis, done);
+        sweep<2>(c, k->w, k->V, n, j + 1, k->st->dots + kMaxBasis, k->st->dots + 2 * kMaxBasis, done);
+        hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(64), 0, c->stream, k->st, j);
+        hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, &k->st->invhn, k->w, 0.0,
+                           k->V + (size_t)(j + 1) * n, done);
       }
     }
-    if (j == m) --j;
-    // back-substitution (KspSolver.C:968-982)
-    for (int i = j; i >= 0; --i) {
-      y[i] = g[i];
-      for (int kk = j; kk >= i + 1; --kk) y[i] -= Hij(i, kk) * y[kk];
-      y[i] /= Hij(i, i);
+    if ((rc = fetch_state(c))) return rc;
+    const GmresState &hs = *k->h_st;
+    if (hs.jlast >= 0) {
+      hipLaunchKernelGGL(k_gmres_backsolve, dim3(1), dim3(64), 0, c->stream, k->st);
+      hipLaunchKernelGGL(k_combine, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, n, k->st);
+      if ((rc = launch_prec_apply(c, k->w, k->r))) return rc;
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->r, 1.0, x, nullptr);
     }
-    Coefs cf;
-    for (int i = 0; i <= j; ++i) cf.c[i] = y[i];
-    hipLaunchKernelGGL(k_combine, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, n, j + 1, cf);
-    if ((rc = launch_prec_apply(c, k->w, k->r))) return rc;
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, k->r, 1.0, x);
-  } while (!exitLoop);
-  hipError_t e = hipGetLastError();
+    if (hs.done) break;  // converged, maxits, zero residual or lucky breakdown
+  }
+  e = hipGetLastError();
   if (e != hipSuccess) return set_hip_error(e, "gmres launch");
-  out->its = iter;
-  out->res = l2res;
-  out->target = target;
+  out->its = k->h_st->iter;
+  out->res = k->h_st->l2res;
+  out->target = k->h_st->target;
   return 0;
 }
 
@@ -345,14 +415,14 @@ int aerofx_jacobian_blocks(aerofx_ctx *c, double *D, double *Aedge) {
   if (D)
     for (size_t k = 0; k < 25 * (size_t)c->N; ++k) D[k] = at(hd, k);
   if (Aedge) {
-    // adjacency entry e of node n -> edge layout [E][2][25] = (A_ij, A_ji)
+    // adjacency entry q of node n -> edge layout [E][2][25] = (A_ij, A_ji)
     std::vector<int> rowptr, adj, eid;
     node_csr(c->N, c->E, c->ptr.data(), rowptr, adj, eid);
     for (int n = 0; n < c->N; ++n)
       for (int q = rowptr[n]; q < rowptr[n + 1]; ++q) {
         int l = eid[q];
         int side = n < adj[q] ? 0 : 1;
-        for (int k = 0; k < 25; ++k) Aedge[50 * (size_t)l + 25 * side + k] = at(hb, 25 * (size_t)q + k);
+        for (int kk = 0; kk < 25; ++kk) Aedge[50 * (size_t)l + 25 * side + kk] = at(hb, 25 * (size_t)q + kk);
       }
   }
   return AEROFX_OK;
@@ -394,16 +464,23 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
   double res0 = 0.0, res = 0.0, cfl = o->cfl0;
   int step;
   for (step = 0; step <= o->max_steps; ++step) {
-    // spatial residual and its norm (TsDesc::computeResidualNorm)
-    int ierr = launch_residual(c, U, k->R, true);
-    if (ierr) return ierr;
-    int cnt = 0;
-    if ((rc = aerofx_error_count(c, &cnt))) return rc;
-    if (cnt) {
+    // spatial residual and its norm (TsDesc::computeResidualNorm); one
+    // synchronisation per step reads the norm and the error counters
+    if ((rc = launch_residual(c, U, k->R, true))) return rc;
+    sweep<2>(c, k->R, nullptr, 0, 0, nullptr, k->scal, nullptr);
+    hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "steady_solve sync");
+    if (step > 0 && c->h_err[2]) {
+      st->steps = step;
+      return set_error(AEROFX_ERR_STATE, "steady_solve: update produced non-physical states");
+    }
+    if (c->h_err[0]) {
       st->steps = step;
       return set_error(AEROFX_ERR_STATE, "steady_solve: non-physical reconstructed state");
     }
-    if ((rc = norm2(c, k->R, &res))) return rc;
+    res = std::sqrt(k->h_scal[0]);
     if (step == 0) res0 = res;
     if (history && step < hist_cap) history[step] = res;
     if (res == 0.0 || res <= o->eps * res0) {
@@ -419,23 +496,16 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     if ((rc = launch_timestep(c, cfl, c->d_dt))) return rc;
     if ((rc = launch_jacobian(c, c->d_dt, o->precision))) return rc;
     // b = -F
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, k->R, 0.0, k->b);
+    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 0.0, k->b,
+                       nullptr);
     GmresOut go;
-    if ((rc = gmres(c, k->b, k->x, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go)))
-      return rc;
+    if ((rc = gmres(c, k->b, k->x, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go))) return rc;
     st->krylov_its += go.its;
-    // Q += dQ
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, k->x, 1.0, U);
+    // Q += dQ; the state check is read at the next step's synchronisation
+    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->x, 1.0, U, nullptr);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 2, 1);
     hipLaunchKernelGGL(k_check_states, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N, U,
                        c->gas, c->d_err + 2);
-    hipError_t e = hipMemcpyAsync(c->h_err + 2, c->d_err + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return set_hip_error(e, "steady_solve sync");
-    if (c->h_err[2]) {
-      st->steps = step + 1;
-      return set_error(AEROFX_ERR_STATE, "steady_solve: update produced non-physical states");
-    }
     st->final_cfl = cfl;
   }
   hipError_t e = hipStreamSynchronize(c->stream);

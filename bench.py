@@ -106,6 +106,15 @@ def newton_bench(dev, max_steps):
         log(f"[bench] newton {name}: {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its, "
             f"converged={r['converged']} rel_res={r['res'] / r['res0']:.2e}")
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    # per-kernel-bucket breakdown from a separate profiled run of each precision
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        Ud = torch.from_numpy(U0).to(dev)
+        op.profile(True)
+        steady_solve(op, Ud, precision=prec, max_steps=max_steps, eps=1e-8, cfl0=10.0, cfl_max=1e4)
+        tm = op.timers()
+        op.profile(False)
+        out[name]["kernel_ms"] = {k: round(v[0], 2) for k, v in tm.items()}
+        log(f"[bench] newton {name} kernel ms: {out[name]['kernel_ms']}")
     out["state_rel_diff_mixed_vs_fp64"] = float(np.linalg.norm(states["mixed"] - states["fp64"]) /
                                                 np.linalg.norm(states["fp64"]))
     op.close()
