@@ -58,6 +58,11 @@ class NewtonOpts(C.Structure):
                 ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double)]
 
 
+class BlockStats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("inner_its", C.c_int), ("converged", C.c_int), ("res_max", C.c_double),
+                ("seconds", C.c_double), ("tune_waves_dots", C.c_int), ("tune_waves_update", C.c_int)]
+
+
 class NewtonStats(C.Structure):
     _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("krylov_its", C.c_int),
                 ("res0", C.c_double), ("res", C.c_double), ("final_cfl", C.c_double),
@@ -96,6 +101,16 @@ SIGNATURES = {
     "aerofx_prec_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_gmres": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                c_int_p, c_double_p]),
+    "aerofx_jvp_fd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "aerofx_jvp_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_prec_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_gmres_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                         c_int_p, c_double_p]),
+    "aerofx_block_gmres": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                     C.c_int, C.c_int, C.POINTER(BlockStats)]),
+    "aerofx_mesh_color_edges_gpu": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
+    "aerofx_mesh_rcm_gpu": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
+    "aerofx_adjoint_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_steady_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats),
                                       c_double_p, C.c_int]),
 }

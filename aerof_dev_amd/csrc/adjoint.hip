@@ -1,0 +1,779 @@
+// adjoint.hip -- transposed residual derivatives on gfx950:
+//   dRdU^T lambda and dRdX^T lambda for the second-order residual
+//   R_i = (1/V_i) (sum_edges +-F(V_ij, V_ji, n_ij) + sum_faces F_b(V_i, n_f/3))
+// (SpaceOperator::computeTransposeDerivativeOfResidual, SpaceOperator.C:1235-1304,
+//  and the sensitivity operators of EdgeSet, Edge.C:1452-1728, on the same
+//  residual; the reference builds per-edge sparse operator blocks, here each
+//  edge contracts its local derivatives with the adjoint weights at once).
+//
+// The MUSCL gradients are recomputed by a forward residual pass at the start
+// of each call; nothing from an earlier forward evaluation is kept.
+//
+// Stages (all deterministic: per-edge, per-tet and per-face results go to
+// their own slots and each node sums its slots in a fixed order):
+//   1. forward residual: V, LSQ gradients, R (scaled)
+//   2. k_adj_edges  one thread per edge: mu = lambda/V, g = mu_i - mu_j;
+//      g^T dF/d(V_ij, V_ji, n) by 13 forward-mode passes of the Roe flux,
+//      then reverse through the Van Albada reconstruction by hand
+//      -> per edge: Vbar_i, Vbar_j, Gbar_i, Gbar_j, dxbar, nbar
+//   3. k_adj_nodes  one thread per node: sums the edge slots, adds the
+//      boundary-face terms -> Vbar, Gbar, the dx part of Xbar
+//   4. k_adj_lsq    one thread per node: reverse of the LSQ gradients
+//      (grad_i = sum_j W_ij (V_j - V_i)), then Ubar = (dV/dU)^T Vbar; for X
+//      also the weight and Cholesky-factor terms -> S_i = d/dM_i, symmetric
+//   5. k_adj_tets / k_adj_faces: control-volume, dual-normal and face-normal
+//      geometry, reverse over each element's coordinates
+//   6. k_adj_xgather: Xbar_i = slots of 3-5 and -sum_j (S_i + S_j) dx_ij
+#include <hip/hip_runtime.h>
+
+#include "context.h"
+#include "device_dual.h"
+#include "host_internal.h"
+
+namespace aerofx {
+
+struct Adjoint {
+  double *EV = nullptr;   // [E][10] Vbar_i, Vbar_j
+  double *EG = nullptr;   // [E][30] Gbar_i[d*5+k], Gbar_j
+  double *EX = nullptr;   // [E][6]  dxbar, nbar
+  double *Vb = nullptr;   // [N][5]
+  double *Gb = nullptr;   // [N][16]
+  double *Xb = nullptr;   // [N][4]
+  double *S = nullptr;    // [N][6]
+  double *R = nullptr;    // [N][5] forward residual (scaled)
+  double *TX = nullptr;   // [T][12]
+  double *FX = nullptr;   // [F][9]
+  int *tets = nullptr;    // [T][4]
+  int *tedge = nullptr;   // [T][6]
+  int *faces = nullptr;   // [F][3]
+  int *ntrow = nullptr;   // [N+1]
+  int *ntslot = nullptr;  // [4T] 4*t + local vertex
+  int *nfslot = nullptr;  // [3F] 3*f + local vertex, in the order of tab.fidx
+};
+
+namespace {
+
+constexpr int kAdjBlock = 256;
+// ElemTet edge -> (end, end) and edge -> (face, face) tables (ElemTetCore.C:19-21)
+__constant__ int cEdgeEnd[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+__constant__ int cEdgeFace[6][2] = {{0, 1}, {2, 0}, {1, 2}, {0, 3}, {3, 1}, {2, 3}};
+
+// (1/r11, 1/r22, 1/r33, r12/r11, r23/r22, psi) from the LSQ moments
+// M = sum dx dx^T (xx, xy, xz, yy, yz, zz), in the order of the set-up
+// (SubDomainCore.C:1755-1777, SubDomain.C:137-159).
+template <class T>
+__device__ __forceinline__ void lsq_params(const T M[6], T L[6]) {
+  T r11 = gsqrt(M[0]);
+  T or11 = 1.0 / r11;
+  T r12 = M[1] * or11;
+  T r13 = M[2] * or11;
+  T r22 = gsqrt(M[3] - r12 * r12);
+  T r23 = (M[4] - r12 * r13) / r22;
+  T r33 = gsqrt(M[5] - (r13 * r13 + r23 * r23));
+  T o11 = 1.0 / r11, o22 = 1.0 / r22, o33 = 1.0 / r33;
+  L[0] = o11;
+  L[1] = o22;
+  L[2] = o33;
+  L[3] = r12 * o11;
+  L[4] = r23 * o22;
+  L[5] = (r12 * r23 - r13 * r22) * o11 * o22;
+}
+
+// LSQ weights W(dx; L) as the gradient pass evaluates them.
+__device__ __forceinline__ void lsq_weights(const double L[6], double dx0, double dx1, double dx2, double w[3]) {
+  const double or11 = L[0], or22 = L[1], or33 = L[2], r12or11 = L[3], r23or22 = L[4], psi = L[5];
+  const double alpha1 = dx0 * or11 * or11;
+  const double alpha2 = (dx1 - r12or11 * dx0) * or22 * or22;
+  const double alpha3 = (dx2 - r23or22 * dx1 + psi * dx0) * or33 * or33;
+  w[0] = alpha1 - r12or11 * alpha2 + psi * alpha3;
+  w[1] = alpha2 - r23or22 * alpha3;
+  w[2] = alpha3;
+}
+
+// Reverse of lsq_weights: given Wbar, accumulate dxbar and (optionally) Lbar.
+__device__ __forceinline__ void lsq_weights_rev(const double L[6], double dx0, double dx1, double dx2,
+                                                const double wb[3], double dxb[3], double *Lb) {
+  const double or11 = L[0], or22 = L[1], or33 = L[2], r12or11 = L[3], r23or22 = L[4], psi = L[5];
+  const double a3b = wb[2] + psi * wb[0] - r23or22 * wb[1];
+  const double a2b = wb[1] - r12or11 * wb[0];
+  const double a1b = wb[0];
+  const double s11 = or11 * or11, s22 = or22 * or22, s33 = or33 * or33;
+  const double c3 = a3b * s33, c2 = a2b * s22;
+  dxb[0] += a1b * s11 - c2 * r12or11 + c3 * psi;
+  dxb[1] += c2 - c3 * r23or22;
+  dxb[2] += c3;
+  if (Lb) {
+    const double alpha2 = (dx1 - r12or11 * dx0) * s22;
+    const double alpha3 = (dx2 - r23or22 * dx1 + psi * dx0) * s33;
+    const double t2 = dx1 - r12or11 * dx0;
+    const double t3 = dx2 - r23or22 * dx1 + psi * dx0;
+    Lb[0] += a1b * dx0 * 2.0 * or11;
+    Lb[1] += a2b * t2 * 2.0 * or22;
+    Lb[2] += a3b * t3 * 2.0 * or33;
+    Lb[3] += -alpha2 * wb[0] - c2 * dx0;
+    Lb[4] += -alpha3 * wb[1] - c3 * dx1;
+    Lb[5] += alpha3 * wb[0] + c3 * dx0;
+  }
+}
+
+// boundary flux of one face node; nv = (1/3) face normal
+template <class T>
+__device__ __forceinline__ void face_flux(const FluxConsts &fc, int code, const T nv[3], const T V[5], T fl[5]) {
+  if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+    wallflux(nv, 0.0, V, fl);
+  } else {
+    const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
+    boundflux5(fc.gas, nv, 0.0, V, ub, fl);
+  }
+}
+
+__device__ __forceinline__ double seed(int dir, int k) { return dir == k ? 1.0 : 0.0; }
+
+// d VA(a, b) / da and / db of the Van Albada function (RecFcn.h:269-281)
+__device__ __forceinline__ void va_partials(double a, double b, double &da, double &db) {
+  const double eps0 = 1.e-16;
+  da = va_fcn(eps0, Dual(a, 1.0), Dual(b, 0.0)).d;
+  db = va_fcn(eps0, Dual(a, 0.0), Dual(b, 1.0)).d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// 2. per edge
+// ---------------------------------------------------------------------------
+template <bool SECOND>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const double *__restrict__ V,
+                                                         const double *__restrict__ grad,
+                                                         const double *__restrict__ lam, FluxConsts fc,
+                                                         double *__restrict__ EV, double *__restrict__ EG,
+                                                         double *__restrict__ EX) {
+  const long long l = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
+  if (l >= t.E) return;
+  const int2 ij = t.eij[l];
+  const int i = ij.x, j = ij.y;
+  double Vi[5], Vj[5], g[5];
+  const double ici = t.invcv[i], icj = t.invcv[j];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    Vi[k] = V[5 * (size_t)i + k];
+    Vj[k] = V[5 * (size_t)j + k];
+    g[k] = lam[5 * (size_t)i + k] * ici - lam[5 * (size_t)j + k] * icj;
+  }
+  double dx[3] = {0.0, 0.0, 0.0}, gi[15], gj[15];
+  double VL[5], VR[5];
+  if (SECOND) {
+    const double4 xi = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)i);
+    const double4 xj = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)j);
+    dx[0] = xj.x - xi.x;
+    dx[1] = xj.y - xi.y;
+    dx[2] = xj.z - xi.z;
+#pragma unroll
+    for (int q = 0; q < 15; ++q) {
+      gi[q] = grad[16 * (size_t)i + q];
+      gj[q] = grad[16 * (size_t)j + q];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double ddVij = dx[0] * gi[k] + dx[1] * gi[5 + k] + dx[2] * gi[10 + k];
+      const double ddVji = dx[0] * gj[k] + dx[1] * gj[5 + k] + dx[2] * gj[10 + k];
+      vanalbada(fc.beta, Vi[k], ddVij, Vj[k], ddVji, VL[k], VR[k]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      VL[k] = Vi[k];
+      VR[k] = Vj[k];
+    }
+  }
+  const double nv[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
+  double fb[13];
+#pragma unroll
+  for (int d = 0; d < 13; ++d) fb[d] = 0.0;
+  if (!(sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) < 1e-18)) {
+#pragma unroll 1
+    for (int dir = 0; dir < 13; ++dir) {
+      Dual L[5], Rr[5], en[3], phi[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        L[k] = Dual(VL[k], seed(dir, k));
+        Rr[k] = Dual(VR[k], seed(dir, 5 + k));
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) en[a] = Dual(nv[a], seed(dir, 10 + a));
+      roeflux7(fc.gas, en, Dual(0.0), L, Rr, phi);
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s += g[k] * phi[k].d;
+      fb[dir] = s;
+    }
+  }
+  double vbi[5], vbj[5], gbi[15], gbj[15], dxb[3] = {0.0, 0.0, 0.0};
+  if (SECOND) {
+    const double beta = fc.beta, beta2 = 1.0 - 2.0 * beta;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double ddVij = dx[0] * gi[k] + dx[1] * gi[5 + k] + dx[2] * gi[10 + k];
+      const double ddVji = dx[0] * gj[k] + dx[1] * gj[5 + k] + dx[2] * gj[10 + k];
+      const double dVji = Vj[k] - Vi[k];
+      const double dV = beta2 * dVji;
+      const double dVi = dV + 2.0 * beta * ddVij;
+      const double dVj = dV + 2.0 * beta * ddVji;
+      double da_i, db_i, da_j, db_j;
+      va_partials(dVi, dVji, da_i, db_i);
+      va_partials(dVj, dVji, da_j, db_j);
+      const double Lb = fb[k], Rb = fb[5 + k];
+      const double A = 0.5 * Lb * da_i, B = 0.5 * Lb * db_i;
+      const double A2 = -0.5 * Rb * da_j, B2 = -0.5 * Rb * db_j;
+      const double dVjib = B + B2 + beta2 * (A + A2);
+      const double ddVijb = 2.0 * beta * A, ddVjib = 2.0 * beta * A2;
+      vbi[k] = Lb - dVjib;
+      vbj[k] = Rb + dVjib;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        gbi[5 * a + k] = ddVijb * dx[a];
+        gbj[5 * a + k] = ddVjib * dx[a];
+        dxb[a] += ddVijb * gi[5 * a + k] + ddVjib * gj[5 * a + k];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      vbi[k] = fb[k];
+      vbj[k] = fb[5 + k];
+    }
+  }
+  double *ev = EV + 10 * (size_t)l;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    ev[k] = vbi[k];
+    ev[5 + k] = vbj[k];
+  }
+  if (SECOND) {
+    double *eg = EG + 30 * (size_t)l;
+#pragma unroll
+    for (int q = 0; q < 15; ++q) {
+      eg[q] = gbi[q];
+      eg[15 + q] = gbj[q];
+    }
+  }
+  double *ex = EX + 6 * (size_t)l;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ex[a] = dxb[a];
+    ex[3 + a] = fb[10 + a];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. per node: edge slots in edge order, then faces
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kAdjBlock) void k_adj_nodes(DevTables t, const double *__restrict__ V,
+                                                         const double *__restrict__ lam, FluxConsts fc,
+                                                         const double *__restrict__ EV,
+                                                         const double *__restrict__ EG,
+                                                         const double *__restrict__ EX, int second,
+                                                         double *__restrict__ Vb, double *__restrict__ Gb,
+                                                         double *__restrict__ Xb) {
+  const int n = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (n >= t.N) return;
+  double vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gb[15], xb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 15; ++q) gb[q] = 0.0;
+  // edges (m, n), m < n: the j-side slots
+  for (int q = t.lrow[n]; q < t.lrow[n + 1]; ++q) {
+    const size_t l = t.ledge[q];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vb[k] += EV[10 * l + 5 + k];
+    if (second) {
+#pragma unroll
+      for (int r = 0; r < 15; ++r) gb[r] += EG[30 * l + 15 + r];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xb[a] += EX[6 * l + a];
+  }
+  // edges (n, m), n < m: the i-side slots
+  for (int l = t.eup[n]; l < t.eup[n + 1]; ++l) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vb[k] += EV[10 * (size_t)l + k];
+    if (second) {
+#pragma unroll
+      for (int r = 0; r < 15; ++r) gb[r] += EG[30 * (size_t)l + r];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xb[a] -= EX[6 * (size_t)l + a];
+  }
+  const int f0 = t.frow[n], f1 = t.frow[n + 1];
+  if (f0 < f1) {
+    const double ic = t.invcv[n];
+    double mu[5], Vn[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      mu[k] = lam[5 * (size_t)n + k] * ic;
+      Vn[k] = V[5 * (size_t)n + k];
+    }
+    for (int q = f0; q < f1; ++q) {
+      const int face = t.fidx[q];
+      const int code = t.fcode[face];
+#pragma unroll 1
+      for (int dir = 0; dir < 5; ++dir) {
+        Dual nv[3], vv[5], fl[5];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) nv[a] = Dual(t.fnorm3[3 * (size_t)face + a]);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vv[k] = Dual(Vn[k], seed(dir, k));
+        face_flux(fc, code, nv, vv, fl);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s += mu[k] * fl[k].d;
+        vb[dir] += s;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Vb[5 * (size_t)n + k] = vb[k];
+#pragma unroll
+  for (int r = 0; r < 15; ++r) Gb[16 * (size_t)n + r] = gb[r];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) Xb[4 * (size_t)n + a] = xb[a];
+}
+
+// ---------------------------------------------------------------------------
+// 4. per node: reverse of the LSQ gradients, then U
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kAdjBlock) void k_adj_lsq(DevTables t, const double *__restrict__ V,
+                                                       const double *__restrict__ U, Gas gas,
+                                                       const double *__restrict__ Vb,
+                                                       const double *__restrict__ Gb, int second, int want_x,
+                                                       double *__restrict__ Ub, double *__restrict__ Xb,
+                                                       double *__restrict__ S) {
+  const int n = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (n >= t.N) return;
+  double vb[5], Vn[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    vb[k] = Vb[5 * (size_t)n + k];
+    Vn[k] = V[5 * (size_t)n + k];
+  }
+  if (second) {
+    double Ln[6], gn[15];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Ln[q] = t.lsq6[6 * (size_t)n + q];
+#pragma unroll
+    for (int q = 0; q < 15; ++q) gn[q] = Gb[16 * (size_t)n + q];
+    const double4 xn = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)n);
+    double xacc[3] = {0.0, 0.0, 0.0}, Lb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double M[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+      const int m = t.adj[e];
+      const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
+      const double dx0 = xm.x - xn.x, dx1 = xm.y - xn.y, dx2 = xm.z - xn.z;
+      double Lm[6], gm[15], Vm[5];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Lm[q] = t.lsq6[6 * (size_t)m + q];
+#pragma unroll
+      for (int q = 0; q < 15; ++q) gm[q] = Gb[16 * (size_t)m + q];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) Vm[k] = V[5 * (size_t)m + k];
+      double wn[3], wm[3];
+      lsq_weights(Ln, dx0, dx1, dx2, wn);      // row n, neighbour m
+      lsq_weights(Lm, -dx0, -dx1, -dx2, wm);   // row m, neighbour n (dx_mn = X_n - X_m)
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        vb[k] += (wm[0] * gm[k] + wm[1] * gm[5 + k] + wm[2] * gm[10 + k]) -
+                 (wn[0] * gn[k] + wn[1] * gn[5 + k] + wn[2] * gn[10 + k]);
+      if (want_x) {
+        double wbn[3], wbm[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            s1 += gn[5 * a + k] * (Vm[k] - Vn[k]);
+            s2 += gm[5 * a + k] * (Vn[k] - Vm[k]);
+          }
+          wbn[a] = s1;
+          wbm[a] = s2;
+        }
+        double dxn[3] = {0.0, 0.0, 0.0}, dxm[3] = {0.0, 0.0, 0.0};
+        lsq_weights_rev(Ln, dx0, dx1, dx2, wbn, dxn, Lb);
+        lsq_weights_rev(Lm, -dx0, -dx1, -dx2, wbm, dxm, nullptr);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xacc[a] += dxm[a] - dxn[a];
+        // LSQ moments in edge order (rows are in ascending edge id)
+        M[0] += dx0 * dx0;
+        M[1] += dx0 * dx1;
+        M[2] += dx0 * dx2;
+        M[3] += dx1 * dx1;
+        M[4] += dx1 * dx2;
+        M[5] += dx2 * dx2;
+      }
+    }
+    if (want_x) {
+      double Mb[6];
+#pragma unroll 1
+      for (int dir = 0; dir < 6; ++dir) {
+        Dual Md[6], Ld[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Md[q] = Dual(M[q], seed(dir, q));
+        lsq_params(Md, Ld);
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s += Lb[q] * Ld[q].d;
+        Mb[dir] = s;
+      }
+      double *sn = S + 6 * (size_t)n;
+      sn[0] = 2.0 * Mb[0];
+      sn[1] = Mb[1];
+      sn[2] = Mb[2];
+      sn[3] = 2.0 * Mb[3];
+      sn[4] = Mb[4];
+      sn[5] = 2.0 * Mb[5];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) Xb[4 * (size_t)n + a] += xacc[a];
+    }
+  }
+  if (Ub) {
+    double u[5], ub[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = U[5 * (size_t)n + k];
+#pragma unroll 1
+    for (int dir = 0; dir < 5; ++dir) {
+      Dual ud[5], vd[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) ud[k] = Dual(u[k], seed(dir, k));
+      prim_from_cons(ud, gas, vd);
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s += vb[k] * vd[k].d;
+      ub[dir] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Ub[5 * (size_t)n + k] = ub[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. geometry: per tet (control volumes, dual normals), per boundary face
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kAdjBlock) void k_adj_tets(int T, const int *__restrict__ tets,
+                                                        const int *__restrict__ tedge,
+                                                        const double *__restrict__ X4,
+                                                        const double *__restrict__ invcv,
+                                                        const double *__restrict__ lam,
+                                                        const double *__restrict__ R,
+                                                        const double *__restrict__ EX, double *__restrict__ TX) {
+  const long long t = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
+  if (t >= T) return;
+  int nd[4];
+  double x[4][3], cvb = 0.0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    nd[v] = tets[4 * t + v];
+    const double4 xv = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)nd[v]);
+    x[v][0] = xv.x;
+    x[v][1] = xv.y;
+    x[v][2] = xv.z;
+    // R_i = F_i / cv_i: d/dcv_i (lambda_i . R_i) = -lambda_i . R_i / cv_i
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s += lam[5 * (size_t)nd[v] + k] * R[5 * (size_t)nd[v] + k];
+    cvb += -s * invcv[nd[v]];
+  }
+  double nb[6][3], sg[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const size_t l = tedge[6 * t + e];
+    sg[e] = nd[cEdgeEnd[e][0]] < nd[cEdgeEnd[e][1]] ? 1.0 : -1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) nb[e][a] = EX[6 * l + 3 + a];
+  }
+  const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+#pragma unroll 1
+  for (int dir = 0; dir < 12; ++dir) {
+    Dual xd[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) xd[v][a] = Dual(x[v][a], seed(dir, 3 * v + a));
+    // ElemTet::computeVolume (ElemTetCore.C:31-44)
+    Dual v1[3], v2[3], v3[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      v1[a] = xd[1][a] - xd[0][a];
+      v2[a] = xd[2][a] - xd[0][a];
+      v3[a] = xd[3][a] - xd[0][a];
+    }
+    Dual c0 = v1[1] * v2[2] - v1[2] * v2[1];
+    Dual c1 = v1[2] * v2[0] - v1[0] * v2[2];
+    Dual c2 = v1[0] * v2[1] - v1[1] * v2[0];
+    Dual vol = sixth * (v3[0] * c0 + v3[1] * c1 + v3[2] * c2);
+    double acc = 0.25 * vol.d * cvb;
+    // dual-face normals (ElemTetCore.C:281-330)
+    Dual f[4][3], gc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      f[0][a] = third * ((xd[0][a] + xd[1][a]) + xd[2][a]);
+      f[1][a] = third * ((xd[0][a] + xd[1][a]) + xd[3][a]);
+      f[2][a] = third * ((xd[0][a] + xd[2][a]) + xd[3][a]);
+      f[3][a] = third * ((xd[1][a] + xd[2][a]) + xd[3][a]);
+      gc[a] = 0.25 * (((xd[0][a] + xd[1][a]) + xd[2][a]) + xd[3][a]);
+    }
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      Dual p[3], q[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        p[a] = f[cEdgeFace[e][1]][a] - gc[a];
+        q[a] = f[cEdgeFace[e][0]][a] - gc[a];
+      }
+      Dual A[3] = {p[1] * q[2] - p[2] * q[1], p[2] * q[0] - p[0] * q[2], p[0] * q[1] - p[1] * q[0]};
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        Dual nn = 0.5 * (A[a] + A[a]) + 0.25 * (A[a] + A[a]);
+        s += nb[e][a] * nn.d;
+      }
+      acc += sg[e] * s;
+    }
+    TX[12 * t + dir] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kAdjBlock) void k_adj_faces(int F, const int *__restrict__ faces,
+                                                         const int *__restrict__ fcode,
+                                                         const double *__restrict__ fnorm3,
+                                                         const double *__restrict__ X4,
+                                                         const double *__restrict__ invcv,
+                                                         const double *__restrict__ V,
+                                                         const double *__restrict__ lam, FluxConsts fc,
+                                                         double *__restrict__ FX) {
+  const int f = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (f >= F) return;
+  const int code = fcode[f];
+  double nb[3] = {0.0, 0.0, 0.0};
+  int nd[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) nd[v] = faces[3 * (size_t)f + v];
+#pragma unroll 1
+  for (int v = 0; v < 3; ++v) {
+    const int n = nd[v];
+    const double ic = invcv[n];
+    double mu[5];
+    Dual vv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      mu[k] = lam[5 * (size_t)n + k] * ic;
+      vv[k] = Dual(V[5 * (size_t)n + k]);
+    }
+#pragma unroll 1
+    for (int dir = 0; dir < 3; ++dir) {
+      Dual nv[3], fl[5];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) nv[a] = Dual(fnorm3[3 * (size_t)f + a], seed(dir, a));
+      face_flux(fc, code, nv, vv, fl);
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s += mu[k] * fl[k].d;
+      nb[dir] += s;
+    }
+  }
+  // nv = (1/3) n_f (FaceTriaCore.C:288-292)
+  const double third = 1.0 / 3.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) nb[a] *= third;
+  double x[3][3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const double4 xv = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)nd[v]);
+    x[v][0] = xv.x;
+    x[v][1] = xv.y;
+    x[v][2] = xv.z;
+  }
+#pragma unroll 1
+  for (int dir = 0; dir < 9; ++dir) {
+    Dual xd[3][3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) xd[v][a] = Dual(x[v][a], seed(dir, 3 * v + a));
+    // FaceTria::computeNormal (FaceTriaCore.C:31-37): 0.5 (x2 - x0) x (x1 - x0)
+    Dual p[3], q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      p[a] = xd[2][a] - xd[0][a];
+      q[a] = xd[1][a] - xd[0][a];
+    }
+    Dual c0 = p[1] * q[2] - p[2] * q[1];
+    Dual c1 = p[2] * q[0] - p[0] * q[2];
+    Dual c2 = p[0] * q[1] - p[1] * q[0];
+    FX[9 * (size_t)f + dir] = nb[0] * (0.5 * c0).d + nb[1] * (0.5 * c1).d + nb[2] * (0.5 * c2).d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. per node: Xbar
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kAdjBlock) void k_adj_xgather(DevTables t, const double *__restrict__ Xb,
+                                                           const double *__restrict__ S, int second,
+                                                           const int *__restrict__ ntrow,
+                                                           const int *__restrict__ ntslot,
+                                                           const double *__restrict__ TX,
+                                                           const int *__restrict__ nfslot,
+                                                           const double *__restrict__ FX,
+                                                           double *__restrict__ out) {
+  const int n = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (n >= t.N) return;
+  double xb[3] = {Xb[4 * (size_t)n], Xb[4 * (size_t)n + 1], Xb[4 * (size_t)n + 2]};
+  if (second) {
+    const double *sn = S + 6 * (size_t)n;
+    const double4 xn = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)n);
+    for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+      const int m = t.adj[e];
+      const double *sm = S + 6 * (size_t)m;
+      const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
+      const double d0 = xm.x - xn.x, d1 = xm.y - xn.y, d2 = xm.z - xn.z;
+      const double a0 = sn[0] + sm[0], a1 = sn[1] + sm[1], a2 = sn[2] + sm[2];
+      const double a3 = sn[3] + sm[3], a4 = sn[4] + sm[4], a5 = sn[5] + sm[5];
+      xb[0] -= a0 * d0 + a1 * d1 + a2 * d2;
+      xb[1] -= a1 * d0 + a3 * d1 + a4 * d2;
+      xb[2] -= a2 * d0 + a4 * d1 + a5 * d2;
+    }
+  }
+  for (int q = ntrow[n]; q < ntrow[n + 1]; ++q) {
+    const size_t s = ntslot[q];  // 4 t + v
+    const size_t tt = s >> 2, v = s & 3;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xb[a] += TX[12 * tt + 3 * v + a];
+  }
+  for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
+    const size_t s = nfslot[q];  // 3 f + v
+    const size_t f = s / 3, v = s - 3 * f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xb[a] += FX[9 * f + 3 * v + a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[3 * (size_t)n + a] = xb[a];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+void free_adjoint(aerofx_ctx *c) {
+  delete c->adj;  // device memory is owned by the context allocation list
+  c->adj = nullptr;
+}
+
+namespace {
+
+template <class T>
+int alloc_n(aerofx_ctx *c, size_t n, T **out) {
+  void *p = nullptr;
+  int rc = ctx_alloc(c, n * sizeof(T), &p);
+  *out = static_cast<T *>(p);
+  return rc;
+}
+
+template <class T>
+int upload_vec(aerofx_ctx *c, const std::vector<T> &v, T **out) {
+  int rc = alloc_n(c, v.size() ? v.size() : 1, out);
+  if (rc) return rc;
+  if (v.empty()) return 0;
+  hipError_t e = hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : set_hip_error(e, "adjoint table upload");
+}
+
+int ensure_adjoint(aerofx_ctx *c) {
+  if (c->adj) return 0;
+  const int N = c->N, E = c->E, T = c->T, F = c->F;
+  if ((int)c->tets.size() != 4 * T || (int)c->tet_edges.size() != 6 * T || (int)c->faces.size() != 3 * F)
+    return set_error(AEROFX_ERR_STATE, "adjoint: element tables missing from the context");
+  Adjoint *a = new Adjoint();
+  int rc = 0;
+  // node -> (tet, vertex) slots in tet order; node -> (face, vertex) in the order of tab.fidx
+  std::vector<int> ntrow(N + 1, 0), ntslot(4 * (size_t)T), nfslot(3 * (size_t)F);
+  for (size_t k = 0; k < 4 * (size_t)T; ++k) ntrow[c->tets[k] + 1]++;
+  for (int n = 0; n < N; ++n) ntrow[n + 1] += ntrow[n];
+  {
+    std::vector<int> fill(ntrow.begin(), ntrow.end() - 1);
+    for (size_t k = 0; k < 4 * (size_t)T; ++k) ntslot[fill[c->tets[k]]++] = (int)k;
+  }
+  {
+    std::vector<int> frow(N + 1, 0);
+    for (size_t k = 0; k < 3 * (size_t)F; ++k) frow[c->faces[k] + 1]++;
+    for (int n = 0; n < N; ++n) frow[n + 1] += frow[n];
+    std::vector<int> fill(frow.begin(), frow.end() - 1);
+    for (size_t k = 0; k < 3 * (size_t)F; ++k) nfslot[fill[c->faces[k]]++] = (int)k;
+  }
+  if ((rc = alloc_n(c, 10 * (size_t)E, &a->EV)) || (rc = alloc_n(c, 30 * (size_t)E, &a->EG)) ||
+      (rc = alloc_n(c, 6 * (size_t)E, &a->EX)) || (rc = alloc_n(c, 5 * (size_t)N, &a->Vb)) ||
+      (rc = alloc_n(c, 16 * (size_t)N, &a->Gb)) || (rc = alloc_n(c, 4 * (size_t)N, &a->Xb)) ||
+      (rc = alloc_n(c, 6 * (size_t)N, &a->S)) || (rc = alloc_n(c, 5 * (size_t)N, &a->R)) ||
+      (rc = alloc_n(c, 12 * (size_t)std::max(T, 1), &a->TX)) ||
+      (rc = alloc_n(c, 9 * (size_t)std::max(F, 1), &a->FX)) || (rc = upload_vec(c, c->tets, &a->tets)) ||
+      (rc = upload_vec(c, c->tet_edges, &a->tedge)) || (rc = upload_vec(c, c->faces, &a->faces)) ||
+      (rc = upload_vec(c, ntrow, &a->ntrow)) || (rc = upload_vec(c, ntslot, &a->ntslot)) ||
+      (rc = upload_vec(c, nfslot, &a->nfslot))) {
+    delete a;
+    return rc;
+  }
+  c->adj = a;
+  return 0;
+}
+
+inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kAdjBlock - 1) / kAdjBlock)); }
+
+}  // namespace
+
+int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout) {
+  int rc = ensure_adjoint(c);
+  if (rc) return rc;
+  Adjoint *a = c->adj;
+  const DevTables &t = c->tab;
+  const int second = c->p.second_order ? 1 : 0;
+  const int want_x = Xout ? 1 : 0;
+  // 1. forward pass: V, gradients (recomputed), R
+  if ((rc = launch_residual(c, U, a->R, true))) return rc;
+  // 2.
+  if (t.E > 0) {
+    if (second)
+      hipLaunchKernelGGL((k_adj_edges<true>), grid_for(t.E), dim3(kAdjBlock), 0, c->stream, t, c->d_V, c->d_grad,
+                         lam, c->fc, a->EV, a->EG, a->EX);
+    else
+      hipLaunchKernelGGL((k_adj_edges<false>), grid_for(t.E), dim3(kAdjBlock), 0, c->stream, t, c->d_V, c->d_grad,
+                         lam, c->fc, a->EV, a->EG, a->EX);
+  }
+  // 3.
+  hipLaunchKernelGGL(k_adj_nodes, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, lam, c->fc, a->EV,
+                     a->EG, a->EX, second, a->Vb, a->Gb, a->Xb);
+  // 4.
+  hipLaunchKernelGGL(k_adj_lsq, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, U, c->gas, a->Vb, a->Gb,
+                     second, want_x, Ub, a->Xb, a->S);
+  if (want_x) {
+    // 5.
+    if (c->T > 0)
+      hipLaunchKernelGGL(k_adj_tets, grid_for(c->T), dim3(kAdjBlock), 0, c->stream, c->T, a->tets, a->tedge, t.X4,
+                         t.invcv, lam, a->R, a->EX, a->TX);
+    if (c->F > 0)
+      hipLaunchKernelGGL(k_adj_faces, grid_for(c->F), dim3(kAdjBlock), 0, c->stream, c->F, a->faces, t.fcode,
+                         t.fnorm3, t.X4, t.invcv, c->d_V, lam, c->fc, a->FX);
+    // 6.
+    hipLaunchKernelGGL(k_adj_xgather, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, a->Xb, a->S, second,
+                       a->ntrow, a->ntslot, a->TX, a->nfslot, a->FX, Xout);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "adjoint launch");
+}
+
+}  // namespace aerofx
+
+using namespace aerofx;
+
+extern "C" int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const double *lambda,
+                                       double *dRdU_T_lambda, double *dRdX_T_lambda) {
+  if (!ctx || !U || !lambda) return set_error(AEROFX_ERR_ARG, "adjoint_residual: null argument");
+  if (!dRdU_T_lambda && !dRdX_T_lambda) return AEROFX_OK;
+  int rc = launch_adjoint(ctx, U, lambda, dRdU_T_lambda, dRdX_T_lambda);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "adjoint_residual");
+}

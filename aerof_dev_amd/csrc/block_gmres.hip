@@ -1,0 +1,713 @@
+// block_gmres.hip -- batched right-hand sides: block GMRES(m) on H1 with
+// block-Jacobi, s = 16 columns at a time, for many sensitivity systems
+// A x_k = b_k sharing one operator.
+//
+// The reference declares a multi-RHS GmresSolver::solve(VecSet&, VecSet&)
+// (KspSolver.C:988-1135) that does not compile (it uses members the class
+// does not have) and solves sensitivities one at a time
+// (FluidShapeOptimizationHandler.C:2378-2428); this is a design of its own.
+//
+// Layout: a block vector is [n_pad][16] FP64 (dof-major, the 16 columns of a
+// dof contiguous: 128 B rows), n = 5N rounded up to a multiple of 64 with
+// zero rows.  The operator reads each H1 block once for all 16 columns.
+//
+// Per cycle (one hipGraph replay when use_graph != 0):
+//   V_0 S_0 = R                      CholQR2
+//   for j < m:
+//     Z = P V_j,  W = A Z            batched block-Jacobi and H1 apply
+//     H_j = V^T W; W -= V H_j        block CGS2 (two passes) with FP64 MFMA
+//     H_j' = V^T W; W -= V H_j'      tall-skinny GEMMs (v_mfma_f64_16x16x4f64)
+//     V_{j+1} H_{j+1,j} = W          CholQR2
+// then on the host the small least-squares problem min ||E_1 S_0 - Hbar Y||
+// (Givens on the block-Hessenberg band), and X += P (V Y).
+//
+// Precision: inner 0 = FP64 blocks; 1 = FP32 copies of H1 and of the
+// block-Jacobi inverse for the inner cycles, while every cycle starts from
+// the FP64 residual R = B - A64 X (iterative refinement) and the basis,
+// Gram-Schmidt and update stay FP64.
+//
+// Determinism: V^T W is summed over a fixed split of the rows into chunks
+// (one wave per chunk, rows in order), and the chunk partials in a fixed
+// tree, so the tuned launch shape changes which unit does the work, not the
+// arithmetic.  The autotuner times each candidate shape once per context
+// and keeps the fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "context.h"
+#include "host_internal.h"
+
+namespace aerofx {
+
+constexpr int kS = 16;          // block width
+constexpr int kTile = kS * kS;  // one 16x16 block
+constexpr int kChunks = 1024;   // fixed row split of V^T W
+constexpr int kMaxM = 16;       // block steps per cycle
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// layout conversion
+// ---------------------------------------------------------------------------
+__global__ void k_to_block(size_t n, size_t npad, int nrhs, const double *__restrict__ src, double *__restrict__ dst,
+                           unsigned seed) {
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= npad * kS) return;
+  const size_t i = idx / kS;
+  const int c = (int)(idx % kS);
+  double v = 0.0;
+  if (i < n) {
+    if (c < nrhs) {
+      v = src[(size_t)c * n + i];
+    } else {
+      // padding columns: a fixed pseudo-random vector keeps the block full rank
+      unsigned h = (unsigned)(i * 2654435761u) ^ (unsigned)(c * 40503u) ^ seed;
+      h ^= h >> 16;
+      h *= 0x7feb352du;
+      h ^= h >> 15;
+      h *= 0x846ca68bu;
+      h ^= h >> 16;
+      v = (double)(h & 0xffffff) / 16777216.0 - 0.5;
+    }
+  }
+  dst[idx] = v;
+}
+
+__global__ void k_from_block(size_t n, int nrhs, const double *__restrict__ src, double *__restrict__ dst) {
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= n * (size_t)nrhs) return;
+  const int c = (int)(idx / n);
+  const size_t i = idx % n;
+  dst[idx] = src[i * kS + c];
+}
+
+// ---------------------------------------------------------------------------
+// batched operators: one thread per (node, column)
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void bmv5_add(const T *__restrict__ a, const double *__restrict__ p, int c,
+                                         double acc[5]) {
+  double b[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) b[k] = p[k * kS + c];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    acc[i] += (double)a[5 * i] * b[0] + (double)a[5 * i + 1] * b[1] + (double)a[5 * i + 2] * b[2] +
+              (double)a[5 * i + 3] * b[3] + (double)a[5 * i + 4] * b[4];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
+                                                   const double *__restrict__ P, double *__restrict__ Y) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int n = gid / kS, c = gid % kS;
+  if (n >= t.N) return;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  bmv5_add(D + 25 * (size_t)n, P + (size_t)n * 5 * kS, c, acc);
+  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+    const int m = t.adj[e];
+    bmv5_add(B + 25 * (size_t)e, P + (size_t)m * 5 * kS, c, acc);
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = acc[k];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bjacobi_apply(int N, const T *__restrict__ Di, const double *__restrict__ P,
+                                                       double *__restrict__ Y) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int n = gid / kS, c = gid % kS;
+  if (n >= N) return;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  bmv5_add(Di + 25 * (size_t)n, P + (size_t)n * 5 * kS, c, acc);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = acc[k];
+}
+
+// Y = B - Y (refinement residual), over the padded block
+__global__ void k_bsub_from(size_t len, const double *__restrict__ B, double *__restrict__ Y) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < len) Y[i] = B[i] - Y[i];
+}
+
+__global__ void k_badd(size_t len, const double *__restrict__ Z, double *__restrict__ X) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < len) X[i] += Z[i];
+}
+
+__global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__restrict__ b) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < len) b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// tall-skinny GEMMs on the FP64 matrix cores
+// ---------------------------------------------------------------------------
+// partial[c][b] = sum over the rows of chunk c of V_b^T W (16x16 tiles).
+// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// holds D[4(l/16)+r][l%16], r < 4.  Here A = V_b^T over 4 rows, B = W over
+// the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
+__global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nb, const double *__restrict__ V, size_t ldv,
+                           const double *__restrict__ W, double *__restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (chunk >= kChunks) return;
+  const size_t r0 = (size_t)chunk * rows_per_chunk;
+  const size_t r1 = min(npad, r0 + rows_per_chunk);
+  f64x4 acc[kMaxM + 1];
+#pragma unroll
+  for (int b = 0; b <= kMaxM; ++b) acc[b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  const int rr = lane >> 4, cc = lane & 15;
+  for (size_t k0 = r0; k0 < r1; k0 += 4) {
+    const size_t off = (k0 + rr) * kS + cc;
+    const double w = W[off];
+#pragma unroll
+    for (int b = 0; b <= kMaxM; ++b)
+      if (b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[(size_t)b * ldv + off], w, acc[b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int b = 0; b <= kMaxM; ++b)
+    if (b < nb) {
+      double *dst = partial + ((size_t)chunk * (kMaxM + 1) + b) * kTile;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dst[(4 * rr + r) * kS + cc] = acc[b][r];
+    }
+}
+
+// out[b] = sum_c partial[c][b] in a fixed tree; one block per tile b
+__global__ __launch_bounds__(256) void k_tsg_reduce(const double *__restrict__ partial, double *__restrict__ out) {
+  const int b = blockIdx.x, e = threadIdx.x;  // e: element of the 16x16 tile
+  double s = 0.0;
+  for (int c = 0; c < kChunks; ++c) s += partial[((size_t)c * (kMaxM + 1) + b) * kTile + e];
+  out[(size_t)b * kTile + e] = s;
+}
+
+// Out = beta Out + alpha sum_b V_b H_b over 16-row tiles, one wave per tile.
+// A = V_b rows (16 x 4 per step), B = H_b rows (4 x 16).
+__global__ void k_tsg_update(size_t npad, int nb, const double *__restrict__ V, size_t ldv,
+                             const double *__restrict__ H, double alpha, double beta, double *__restrict__ Out) {
+  const int lane = threadIdx.x & 63;
+  const size_t tile = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (tile * 16 >= npad) return;
+  const size_t r0 = tile * 16;
+  const int ri = lane & 15, kk = lane >> 4;
+  f64x4 acc = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nb; ++b) {
+    const double *Vb = V + (size_t)b * ldv + (r0 + ri) * kS;
+    const double *Hb = H + (size_t)b * kTile;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vb[4 * q + kk], Hb[(4 * q + kk) * kS + ri], acc, 0, 0, 0);
+  }
+  // D[4 kk + r][ri] -> row r0 + 4 kk + r, column ri
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double *o = Out + (r0 + 4 * kk + r) * kS + ri;
+    *o = (beta == 0.0 ? 0.0 : beta * *o) + alpha * acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 16x16 helpers (one lane): Cholesky G = R^T R, inverse of R, products
+// ---------------------------------------------------------------------------
+// R upper triangular with G = R^T R; Rinv = R^-1.  flag set on a non-positive pivot.
+__global__ void k_chol16(const double *__restrict__ G, double *__restrict__ R, double *__restrict__ Rinv,
+                         int *__restrict__ flag) {
+  if (threadIdx.x != 0) return;
+  double a[kTile], r[kTile], ri[kTile];
+  for (int e = 0; e < kTile; ++e) {
+    a[e] = G[e];
+    r[e] = 0.0;
+    ri[e] = 0.0;
+  }
+  for (int j = 0; j < kS; ++j) {
+    double d = a[j * kS + j];
+    for (int k = 0; k < j; ++k) d -= r[k * kS + j] * r[k * kS + j];
+    if (!(d > 0.0)) {
+      *flag = 1;
+      d = 1e-300;
+    }
+    const double rjj = sqrt(d);
+    r[j * kS + j] = rjj;
+    for (int i = j + 1; i < kS; ++i) {
+      double s = a[j * kS + i];
+      for (int k = 0; k < j; ++k) s -= r[k * kS + j] * r[k * kS + i];
+      r[j * kS + i] = s / rjj;
+    }
+  }
+  // back substitution for R^-1 (upper)
+  for (int j = kS - 1; j >= 0; --j) {
+    ri[j * kS + j] = 1.0 / r[j * kS + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += r[i * kS + k] * ri[k * kS + j];
+      ri[i * kS + j] = -s / r[i * kS + i];
+    }
+  }
+  for (int e = 0; e < kTile; ++e) {
+    R[e] = r[e];
+    Rinv[e] = ri[e];
+  }
+}
+
+// C = A B (16x16), and optionally D += E
+__global__ void k_mm16(const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C) {
+  const int e = threadIdx.x;
+  if (e >= kTile) return;
+  const int i = e / kS, j = e % kS;
+  double s = 0.0;
+  for (int k = 0; k < kS; ++k) s += A[i * kS + k] * B[k * kS + j];
+  C[e] = s;
+}
+
+__global__ void k_add_tiles(int ntiles, const double *__restrict__ A, const double *__restrict__ B,
+                            double *__restrict__ C) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ntiles * kTile) C[e] = A[e] + B[e];
+}
+
+// column sums of squares of a block vector: partial over a fixed row split
+__global__ __launch_bounds__(256) void k_colnorm_partial(size_t npad, const double *__restrict__ X,
+                                                         double *__restrict__ partial) {
+  // 256 threads = 16 row lanes x 16 columns; block b covers a fixed row range
+  const int c = threadIdx.x % kS, rl = threadIdx.x / kS;
+  const size_t per = (npad + kChunks - 1) / kChunks;
+  const size_t r0 = blockIdx.x * per, r1 = min(npad, r0 + per);
+  double s = 0.0;
+  for (size_t r = r0 + rl; r < r1; r += 16) {
+    const double v = X[r * kS + c];
+    s += v * v;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < kS) {
+    double t = 0.0;
+    for (int q = 0; q < 16; ++q) t += sh[q * kS + threadIdx.x];
+    partial[(size_t)blockIdx.x * kS + threadIdx.x] = t;
+  }
+}
+
+__global__ void k_colnorm_reduce(const double *__restrict__ partial, double *__restrict__ out) {
+  const int c = threadIdx.x;
+  if (c >= kS) return;
+  double s = 0.0;
+  for (int b = 0; b < kChunks; ++b) s += partial[(size_t)b * kS + c];
+  out[c] = s;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct TsgShape {
+  int waves_dots = 4;    // waves per workgroup of k_tsg_dots
+  int waves_update = 4;  // waves per workgroup of k_tsg_update
+  bool tuned = false;
+};
+
+struct BlockSolver {
+  int m = 0;
+  size_t n = 0, npad = 0;
+  double *V = nullptr;        // [m+1][npad][16]
+  double *W = nullptr, *Z = nullptr, *R = nullptr, *X = nullptr, *B = nullptr, *T = nullptr;
+  double *partial = nullptr;  // [kChunks][kMaxM+1][256]
+  double *Hbar = nullptr;     // [(m+1)][m] tiles: block (i, j) at (i * m + j)
+  double *S0 = nullptr;       // tile
+  double *G = nullptr;        // scratch tiles [kMaxM+1]
+  double *G2 = nullptr;       // scratch tiles [kMaxM+1]
+  double *Rt = nullptr, *Ri = nullptr, *Rt2 = nullptr, *Ri2 = nullptr;  // tiles
+  double *Y = nullptr;        // [m] tiles
+  double *nrm = nullptr;      // [16]
+  int *flag = nullptr;
+  float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
+  const void *f32_src = nullptr;  // FP64 blocks the FP32 copies were made from
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_prec = -1;
+  TsgShape shape;
+  std::vector<double> hH, hS0, hY;
+  double *h_nrm = nullptr;
+  int *h_flag = nullptr;
+};
+
+void free_block(aerofx_ctx *c);
+
+namespace {
+
+inline dim3 g1(size_t n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+int alloc_d(aerofx_ctx *c, size_t count, double **p) {
+  void *v = nullptr;
+  int rc = ctx_alloc(c, count * sizeof(double), &v);
+  *p = (double *)v;
+  return rc;
+}
+
+BlockSolver *get_block(aerofx_ctx *c, int m, int *rc);
+
+void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *out) {
+  // rows per chunk: a multiple of 4 (one MFMA step), fixed by npad alone
+  const size_t rows = std::max<size_t>(4, ((s->npad + kChunks - 1) / kChunks + 3) / 4 * 4);
+  const int wpb = s->shape.waves_dots;
+  hipLaunchKernelGGL(k_tsg_dots, dim3((kChunks + wpb - 1) / wpb), dim3(64 * wpb), 0, s->stream, s->npad, rows, nb, V,
+                     s->npad * kS, W, s->partial);
+  hipLaunchKernelGGL(k_tsg_reduce, dim3(nb), dim3(256), 0, s->stream, s->partial, out);
+}
+
+void tsg_update(BlockSolver *s, int nb, const double *V, const double *H, double alpha, double beta, double *Out) {
+  const int wpb = s->shape.waves_update;
+  const size_t tiles = s->npad / 16;
+  hipLaunchKernelGGL(k_tsg_update, dim3((unsigned)((tiles + wpb - 1) / wpb)), dim3(64 * wpb), 0, s->stream, s->npad,
+                     nb, V, s->npad * kS, H, alpha, beta, Out);
+}
+
+// W = Q Rfinal with CholQR2 (two Cholesky-QR passes); Q must not alias W or
+// s->T; Rfinal = R2 R1 goes to the tile Rout.
+void cholqr2(BlockSolver *s, const double *W, double *Q, double *Rout) {
+  tsg_dots(s, 1, W, W, s->G);
+  hipLaunchKernelGGL(k_chol16, dim3(1), dim3(64), 0, s->stream, s->G, s->Rt, s->Ri, s->flag);
+  tsg_update(s, 1, W, s->Ri, 1.0, 0.0, s->T);
+  tsg_dots(s, 1, s->T, s->T, s->G);
+  hipLaunchKernelGGL(k_chol16, dim3(1), dim3(64), 0, s->stream, s->G, s->Rt2, s->Ri2, s->flag);
+  tsg_update(s, 1, s->T, s->Ri2, 1.0, 0.0, Q);
+  hipLaunchKernelGGL(k_mm16, dim3(1), dim3(256), 0, s->stream, s->Rt2, s->Rt, Rout);
+}
+
+template <class T>
+void apply_ops(aerofx_ctx *c, BlockSolver *s, const T *D, const T *B, const T *Di, const double *Vj, double *Z,
+               double *W) {
+  const int nb = (int)((c->N * (size_t)kS + 255) / 256);
+  hipLaunchKernelGGL((k_bjacobi_apply<T>), dim3(nb), dim3(256), 0, s->stream, c->N, Di, Vj, Z);
+  hipLaunchKernelGGL((k_bh1_apply<T>), dim3(nb), dim3(256), 0, s->stream, c->tab, D, B, Z, W);
+}
+
+// one inner cycle: from s->R (the residual block), builds V, Hbar, S0
+void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
+  const size_t blk = s->npad * kS;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->flag, 1);
+  cholqr2(s, s->R, s->V, s->S0);
+  for (int j = 0; j < s->m; ++j) {
+    const double *Vj = s->V + (size_t)j * blk;
+    if (prec)
+      apply_ops<float>(c, s, s->D32, s->B32, s->Di32, Vj, s->Z, s->W);
+    else
+      apply_ops<double>(c, s, (const double *)c->d_jacD, (const double *)c->d_jacB, (const double *)c->d_jacDi, Vj,
+                        s->Z, s->W);
+    // block CGS2
+    tsg_dots(s, j + 1, s->V, s->W, s->G);
+    tsg_update(s, j + 1, s->V, s->G, -1.0, 1.0, s->W);
+    tsg_dots(s, j + 1, s->V, s->W, s->G2);
+    tsg_update(s, j + 1, s->V, s->G2, -1.0, 1.0, s->W);
+    // H(0..j, j) = G + G2, stored as column j of Hbar
+    for (int i = 0; i <= j; ++i)
+      hipLaunchKernelGGL(k_add_tiles, dim3(1), dim3(256), 0, s->stream, 1, s->G + (size_t)i * kTile,
+                         s->G2 + (size_t)i * kTile, s->Hbar + ((size_t)i * s->m + j) * kTile);
+    cholqr2(s, s->W, s->V + (size_t)(j + 1) * blk, s->Hbar + ((size_t)(j + 1) * s->m + j) * kTile);
+  }
+}
+
+// host least squares: min || E1 S0 - Hbar Y ||_F, Hbar ((m+1)16 x m16) block
+// upper Hessenberg with upper-triangular subdiagonal blocks.  Givens by column.
+void host_lsq(int m, const std::vector<double> &Ht, const std::vector<double> &S0, std::vector<double> &Y,
+              std::vector<double> &resid) {
+  const int rows = (m + 1) * kS, cols = m * kS;
+  std::vector<double> H((size_t)rows * cols, 0.0), rhs((size_t)rows * kS, 0.0);
+  for (int bi = 0; bi <= m; ++bi)
+    for (int bj = 0; bj < m; ++bj) {
+      if (bi > bj + 1) continue;
+      const double *t = Ht.data() + ((size_t)bi * m + bj) * kTile;
+      for (int a = 0; a < kS; ++a)
+        for (int b = 0; b < kS; ++b) H[(size_t)(bi * kS + a) * cols + bj * kS + b] = t[a * kS + b];
+    }
+  for (int a = 0; a < kS; ++a)
+    for (int b = 0; b < kS; ++b) rhs[(size_t)a * kS + b] = S0[a * kS + b];
+  for (int col = 0; col < cols; ++col) {
+    const int last = std::min(rows - 1, col + kS);
+    for (int r = last; r > col; --r) {
+      double a = H[(size_t)(r - 1) * cols + col], b = H[(size_t)r * cols + col];
+      if (b == 0.0) continue;
+      double rr = std::hypot(a, b), cs = a / rr, sn = b / rr;
+      for (int k = col; k < cols; ++k) {
+        double x = H[(size_t)(r - 1) * cols + k], y = H[(size_t)r * cols + k];
+        H[(size_t)(r - 1) * cols + k] = cs * x + sn * y;
+        H[(size_t)r * cols + k] = -sn * x + cs * y;
+      }
+      for (int k = 0; k < kS; ++k) {
+        double x = rhs[(size_t)(r - 1) * kS + k], y = rhs[(size_t)r * kS + k];
+        rhs[(size_t)(r - 1) * kS + k] = cs * x + sn * y;
+        rhs[(size_t)r * kS + k] = -sn * x + cs * y;
+      }
+    }
+  }
+  Y.assign((size_t)cols * kS, 0.0);
+  for (int k = 0; k < kS; ++k)
+    for (int i = cols - 1; i >= 0; --i) {
+      double v = rhs[(size_t)i * kS + k];
+      for (int j = i + 1; j < cols; ++j) v -= H[(size_t)i * cols + j] * Y[(size_t)j * kS + k];
+      const double d = H[(size_t)i * cols + i];
+      Y[(size_t)i * kS + k] = d != 0.0 ? v / d : 0.0;
+    }
+  resid.assign(kS, 0.0);
+  for (int k = 0; k < kS; ++k) {
+    double s = 0.0;
+    for (int i = cols; i < rows; ++i) s += rhs[(size_t)i * kS + k] * rhs[(size_t)i * kS + k];
+    resid[k] = std::sqrt(s);
+  }
+}
+
+// times each candidate launch shape on the current buffers; keeps the fastest
+void autotune(aerofx_ctx *c, BlockSolver *s) {
+  if (s->shape.tuned) return;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  const int cand[] = {1, 2, 4, 8};
+  float best = 1e30f;
+  int nb = std::min(s->m, 8) + 1;
+  for (int w : cand) {
+    s->shape.waves_dots = w;
+    tsg_dots(s, nb, s->V, s->W, s->G);  // warm
+    (void)hipEventRecord(a, s->stream);
+    for (int r = 0; r < 3; ++r) tsg_dots(s, nb, s->V, s->W, s->G);
+    (void)hipEventRecord(b, s->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (ms < best) {
+      best = ms;
+      c->blk_tune[0] = w;
+    }
+  }
+  s->shape.waves_dots = c->blk_tune[0];
+  best = 1e30f;
+  for (int w : cand) {
+    s->shape.waves_update = w;
+    tsg_update(s, nb, s->V, s->G, 0.0, 1.0, s->W);  // W unchanged: beta 1, alpha 0
+    (void)hipEventRecord(a, s->stream);
+    for (int r = 0; r < 3; ++r) tsg_update(s, nb, s->V, s->G, 0.0, 1.0, s->W);
+    (void)hipEventRecord(b, s->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (ms < best) {
+      best = ms;
+      c->blk_tune[1] = w;
+    }
+  }
+  s->shape.waves_update = c->blk_tune[1];
+  s->shape.tuned = true;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+}
+
+BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
+  *rc = 0;
+  if (c->blk && c->blk->m == m) return c->blk;
+  free_block(c);
+  BlockSolver *s = new BlockSolver();
+  s->m = m;
+  s->n = 5 * (size_t)c->N;
+  s->npad = (s->n + 63) / 64 * 64;
+  const size_t blk = s->npad * kS;
+  int r = 0;
+  if ((r = alloc_d(c, (size_t)(m + 1) * blk, &s->V)) || (r = alloc_d(c, blk, &s->W)) || (r = alloc_d(c, blk, &s->Z)) ||
+      (r = alloc_d(c, blk, &s->R)) || (r = alloc_d(c, blk, &s->X)) || (r = alloc_d(c, blk, &s->B)) ||
+      (r = alloc_d(c, blk, &s->T)) || (r = alloc_d(c, (size_t)kChunks * (kMaxM + 1) * kTile, &s->partial)) ||
+      (r = alloc_d(c, (size_t)(m + 1) * m * kTile, &s->Hbar)) || (r = alloc_d(c, kTile, &s->S0)) ||
+      (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G)) || (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G2)) ||
+      (r = alloc_d(c, kTile, &s->Rt)) || (r = alloc_d(c, kTile, &s->Ri)) || (r = alloc_d(c, kTile, &s->Rt2)) ||
+      (r = alloc_d(c, kTile, &s->Ri2)) || (r = alloc_d(c, (size_t)m * kTile, &s->Y)) ||
+      (r = alloc_d(c, (size_t)kChunks * kS + kS, &s->nrm))) {
+    delete s;
+    *rc = r;
+    return nullptr;
+  }
+  void *v = nullptr;
+  if ((r = ctx_alloc(c, sizeof(int) * 4, &v))) {
+    delete s;
+    *rc = r;
+    return nullptr;
+  }
+  s->flag = (int *)v;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_nrm, sizeof(double) * kS, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_flag, sizeof(int) * 4, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete s;
+    *rc = set_hip_error(e, "block solver set-up");
+    return nullptr;
+  }
+  // zero padded rows once: every block vector keeps zero rows beyond n
+  (void)hipMemsetAsync(s->V, 0, (size_t)(m + 1) * blk * sizeof(double), s->stream);
+  double *vecs[6] = {s->W, s->Z, s->R, s->X, s->B, s->T};
+  for (double *p : vecs) (void)hipMemsetAsync(p, 0, blk * sizeof(double), s->stream);
+  (void)hipMemsetAsync(s->Hbar, 0, (size_t)(m + 1) * m * kTile * sizeof(double), s->stream);
+  if (c->blk_tune[0] > 0) {
+    s->shape.waves_dots = c->blk_tune[0];
+    s->shape.waves_update = c->blk_tune[1];
+    s->shape.tuned = true;
+  }
+  c->blk = s;
+  return s;
+}
+
+int ensure_f32(aerofx_ctx *c, BlockSolver *s) {
+  if (s->D32 && s->f32_src == c->d_jacD) return 0;
+  const size_t nd = 25 * (size_t)c->N, nbk = 50 * (size_t)c->E;
+  void *v = nullptr;
+  int rc;
+  if (!s->D32) {
+    if ((rc = ctx_alloc(c, nd * sizeof(float), &v))) return rc;
+    s->D32 = (float *)v;
+    if ((rc = ctx_alloc(c, nbk * sizeof(float), &v))) return rc;
+    s->B32 = (float *)v;
+    if ((rc = ctx_alloc(c, nd * sizeof(float), &v))) return rc;
+    s->Di32 = (float *)v;
+  }
+  hipLaunchKernelGGL(k_f64_to_f32, g1(nd), dim3(256), 0, s->stream, nd, (const double *)c->d_jacD, s->D32);
+  hipLaunchKernelGGL(k_f64_to_f32, g1(nbk), dim3(256), 0, s->stream, nbk, (const double *)c->d_jacB, s->B32);
+  // the FP32 block-Jacobi inverse is LINPACK in float on the FP32 blocks,
+  // as the single-RHS mixed path builds it
+  if ((rc = launch_invert_f32(c, s->stream, s->D32, s->Di32))) return rc;
+  s->f32_src = c->d_jacD;
+  return 0;
+}
+
+}  // namespace
+
+void free_block(aerofx_ctx *c) {
+  BlockSolver *s = c->blk;
+  if (!s) return;
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->h_nrm) (void)hipHostFree(s->h_nrm);
+  if (s->h_flag) (void)hipHostFree(s->h_flag);
+  delete s;
+  c->blk = nullptr;
+}
+
+}  // namespace aerofx
+
+using namespace aerofx;
+
+extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, double *Xv, int m, double eps,
+                                  int max_cycles, int precision, int use_graph, aerofx_block_stats *st) {
+  if (!c || !Bv || !Xv || !st) return set_error(AEROFX_ERR_ARG, "block_gmres: null argument");
+  if (nrhs < 1 || nrhs > kS) return set_error(AEROFX_ERR_ARG, "block_gmres: 1 <= nrhs <= 16");
+  if (m < 1 || m > kMaxM) return set_error(AEROFX_ERR_ARG, "block_gmres: 1 <= m <= 16");
+  if (precision != 0 && precision != 1) return set_error(AEROFX_ERR_ARG, "block_gmres: precision 0 or 1");
+  if (!c->d_jacD || c->jac_precision != 0)
+    return set_error(AEROFX_ERR_STATE, "block_gmres: needs FP64 operators (aerofx_jacobian_setup precision 0)");
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = 0;
+  BlockSolver *s = get_block(c, m, &rc);
+  if (!s) return rc;
+  *st = aerofx_block_stats{};
+  // order after the caller's work on the context stream
+  hipEvent_t ev;
+  hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ev, c->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s->stream, ev, 0);
+  if (e != hipSuccess) return set_hip_error(e, "block_gmres stream order");
+  if (precision == 1 && (rc = ensure_f32(c, s))) return rc;
+  const size_t blk = s->npad * kS;
+  hipLaunchKernelGGL(k_to_block, g1(blk), dim3(256), 0, s->stream, s->n, s->npad, nrhs, Bv, s->B, 0x5eedu);
+  (void)hipMemsetAsync(s->X, 0, blk * sizeof(double), s->stream);
+  autotune(c, s);
+  // ||b_k||
+  hipLaunchKernelGGL(k_colnorm_partial, dim3(kChunks), dim3(256), 0, s->stream, s->npad, s->B, s->nrm);
+  hipLaunchKernelGGL(k_colnorm_reduce, dim3(1), dim3(64), 0, s->stream, s->nrm, s->nrm + (size_t)kChunks * kS);
+  e = hipMemcpyAsync(s->h_nrm, s->nrm + (size_t)kChunks * kS, sizeof(double) * kS, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return set_hip_error(e, "block_gmres norms");
+  std::vector<double> bnorm(kS);
+  for (int k = 0; k < kS; ++k) bnorm[k] = std::sqrt(s->h_nrm[k]);
+  const int nbk = (int)((c->N * (size_t)kS + 255) / 256);
+  double worst = 0.0;
+  int cycle = 0;
+  for (;; ++cycle) {
+    // R = B - A64 X (FP64 refinement residual)
+    hipLaunchKernelGGL((k_bh1_apply<double>), dim3(nbk), dim3(256), 0, s->stream, c->tab,
+                       (const double *)c->d_jacD, (const double *)c->d_jacB, s->X, s->R);
+    hipLaunchKernelGGL(k_bsub_from, g1(blk), dim3(256), 0, s->stream, blk, s->B, s->R);
+    hipLaunchKernelGGL(k_colnorm_partial, dim3(kChunks), dim3(256), 0, s->stream, s->npad, s->R, s->nrm);
+    hipLaunchKernelGGL(k_colnorm_reduce, dim3(1), dim3(64), 0, s->stream, s->nrm, s->nrm + (size_t)kChunks * kS);
+    e = hipMemcpyAsync(s->h_nrm, s->nrm + (size_t)kChunks * kS, sizeof(double) * kS, hipMemcpyDeviceToHost,
+                       s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return set_hip_error(e, "block_gmres residual");
+    worst = 0.0;
+    for (int k = 0; k < nrhs; ++k) {
+      const double rel = bnorm[k] > 0.0 ? std::sqrt(s->h_nrm[k]) / bnorm[k] : 0.0;
+      worst = std::max(worst, rel);
+    }
+    if (worst <= eps) {
+      st->converged = 1;
+      break;
+    }
+    if (cycle >= max_cycles) break;
+    // inner cycle: graph replay or eager launches
+    if (use_graph) {
+      if (!s->exec || s->graph_prec != precision) {
+        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        if (s->graph) (void)hipGraphDestroy(s->graph);
+        s->exec = nullptr;
+        s->graph = nullptr;
+        e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return set_hip_error(e, "block_gmres capture");
+        enqueue_cycle(c, s, precision);
+        e = hipStreamEndCapture(s->stream, &s->graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return set_hip_error(e, "block_gmres graph");
+        s->graph_prec = precision;
+      }
+      e = hipGraphLaunch(s->exec, s->stream);
+      if (e != hipSuccess) return set_hip_error(e, "block_gmres graph launch");
+    } else {
+      enqueue_cycle(c, s, precision);
+    }
+    // small least-squares problem on the host
+    s->hH.resize((size_t)(m + 1) * m * kTile);
+    s->hS0.resize(kTile);
+    e = hipMemcpyAsync(s->hH.data(), s->Hbar, s->hH.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s->hS0.data(), s->S0, kTile * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->h_flag, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return set_hip_error(e, "block_gmres Hessenberg copy");
+    if (s->h_flag[0]) return set_error(AEROFX_ERR_STATE, "block_gmres: rank-deficient block (CholQR breakdown)");
+    std::vector<double> resid;
+    host_lsq(m, s->hH, s->hS0, s->hY, resid);
+    e = hipMemcpyAsync(s->Y, s->hY.data(), s->hY.size() * sizeof(double), hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) return set_hip_error(e, "block_gmres Y copy");
+    // X += P (V Y)
+    tsg_update(s, m, s->V, s->Y, 1.0, 0.0, s->W);
+    if (precision)
+      hipLaunchKernelGGL((k_bjacobi_apply<float>), dim3(nbk), dim3(256), 0, s->stream, c->N, s->Di32, s->W, s->Z);
+    else
+      hipLaunchKernelGGL((k_bjacobi_apply<double>), dim3(nbk), dim3(256), 0, s->stream, c->N,
+                         (const double *)c->d_jacDi, s->W, s->Z);
+    hipLaunchKernelGGL(k_badd, g1(blk), dim3(256), 0, s->stream, blk, s->Z, s->X);
+    st->inner_its += m;
+  }
+  hipLaunchKernelGGL(k_from_block, g1(s->n * (size_t)nrhs), dim3(256), 0, s->stream, s->n, nrhs, s->X, Xv);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(ev, s->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, ev, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipEventDestroy(ev);
+  if (e != hipSuccess) return set_hip_error(e, "block_gmres finish");
+  st->cycles = cycle;
+  st->res_max = worst;
+  st->tune_waves_dots = s->shape.waves_dots;
+  st->tune_waves_update = s->shape.waves_update;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return AEROFX_OK;
+}

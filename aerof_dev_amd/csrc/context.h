@@ -28,6 +28,7 @@ struct DevTables {
   const int *rowptr;     // [N+1]
   const int *adj;        // [2E]
   const int *eidx;       // [2E] edge id of each adjacency entry
+  const int *mirror;     // [2E] entry (m, n) of the entry (n, m)
   // flux pass A: edges grouped by their lower node i (edge order)
   int nfchunks;
   const int *fchunk;     // [nfchunks+1] row starts
@@ -53,6 +54,8 @@ struct FluxConsts {
 
 int set_hip_error(hipError_t e, const char *what);
 struct Krylov;
+struct Adjoint;
+struct BlockSolver;
 
 }  // namespace aerofx
 
@@ -65,6 +68,7 @@ struct aerofx_ctx {
   // host copies of the set-up tables
   std::vector<int> ptr;
   std::vector<double> cv, normals, fnorm, lsq;
+  std::vector<int> tets, faces, tet_edges;  // element tables for the adjoint geometry terms
   // device
   aerofx::DevTables tab{};
   std::vector<void *> allocs;
@@ -79,6 +83,9 @@ struct aerofx_ctx {
   void *d_jacD = nullptr, *d_jacB = nullptr, *d_jacDi = nullptr;
   double *d_dt = nullptr;
   aerofx::Krylov *ksp = nullptr;
+  aerofx::Adjoint *adj = nullptr;
+  aerofx::BlockSolver *blk = nullptr;
+  int blk_tune[2] = {0, 0};  // autotuned waves per workgroup of the block GEMMs
   // per-kernel timers (the reference's Timer buckets, Timer.h:12-15)
   bool prof = false;
   struct Pending {
@@ -116,11 +123,16 @@ int launch_primitive(aerofx_ctx *c, const double *U, double *V);
 // implicit operators (jacobian.hip)
 int launch_timestep(aerofx_ctx *c, double cfl, double *dt);
 int launch_jacobian(aerofx_ctx *c, const double *dt, int precision);
-int launch_h1_apply(aerofx_ctx *c, const double *p, double *y);
-int launch_prec_apply(aerofx_ctx *c, const double *y, double *x);
+// trans: apply the transposed operator (adjoint solves)
+int launch_h1_apply(aerofx_ctx *c, const double *p, double *y, bool trans = false);
+int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans = false);
 // variants that return at once when *done != 0 (device-resident Krylov loops)
-int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done);
-int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done);
+int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans = false);
+int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done, bool trans = false);
 void free_krylov(aerofx_ctx *c);
+void free_adjoint(aerofx_ctx *c);
+void free_block(aerofx_ctx *c);
+// LINPACK inverse of FP32 5x5 blocks on a given stream (error count in d_err[1])
+int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *Di);
 
 }  // namespace aerofx

@@ -73,6 +73,8 @@ void collect_timers(aerofx_ctx *c) {
 void destroy(aerofx_ctx *c) {
   collect_timers(c);
   free_krylov(c);
+  free_adjoint(c);
+  free_block(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->h_err) (void)hipHostFree(c->h_err);
@@ -158,8 +160,10 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
   }
   int rc = 0;
   try {
-    std::vector<int> tet_edges;
+    std::vector<int> &tet_edges = c->tet_edges;
     build_edges(N, T, m->tets, c->ptr, &tet_edges);
+    c->tets.assign(m->tets, m->tets + 4 * (size_t)T);
+    c->faces.assign(m->faces, m->faces + 3 * (size_t)F);
     c->E = (int)(c->ptr.size() / 2);
     const int E = c->E;
     c->cv.resize(N);
@@ -179,6 +183,14 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     // gradient pass: node adjacency in edge-id order
     std::vector<int> rowptr, adj, eid;
     node_csr(N, E, ptr, rowptr, adj, eid);
+    std::vector<int> mirror(2 * (size_t)E);
+    {
+      std::vector<int> at_i(E), at_j(E);
+      for (int n = 0; n < N; ++n)
+        for (int e = rowptr[n]; e < rowptr[n + 1]; ++e) (n < adj[e] ? at_i : at_j)[eid[e]] = e;
+      for (int n = 0; n < N; ++n)
+        for (int e = rowptr[n]; e < rowptr[n + 1]; ++e) mirror[e] = n < adj[e] ? at_j[eid[e]] : at_i[eid[e]];
+    }
     // flux pass A: edge rows by lower node; pass B: lower-edge slot lists
     std::vector<int> eup(N + 1, 0), ledge(E), lrow(N + 1, 0);
     std::vector<int2> eij(E);
@@ -240,7 +252,7 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     t.F = F;
     if ((rc = upload(c, X4, &t.X4)) || (rc = upload(c, lsq6, &t.lsq6)) || (rc = upload(c, invcv, &t.invcv)) || (rc = upload(c, c->cv, &t.cv)) ||
         (rc = upload(c, gchunk, &t.gchunk)) || (rc = upload(c, rowptr, &t.rowptr)) ||
-        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, eid, &t.eidx)) || (rc = upload(c, fchunk, &t.fchunk)) || (rc = upload(c, eup, &t.eup)) ||
+        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, eid, &t.eidx)) || (rc = upload(c, mirror, &t.mirror)) || (rc = upload(c, fchunk, &t.fchunk)) || (rc = upload(c, eup, &t.eup)) ||
         (rc = upload(c, eij, &t.eij)) || (rc = upload(c, ledge, &t.ledge)) ||
         (rc = upload(c, c->normals, &t.normal)) || (rc = upload(c, lrow, &t.lrow)) ||
         (rc = upload(c, frow, &t.frow)) || (rc = upload(c, fidx, &t.fidx)) || (rc = upload(c, fcode, &t.fcode)) ||

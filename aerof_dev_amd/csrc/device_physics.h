@@ -11,6 +11,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "device_dual.h"
+
 namespace aerofx {
 
 struct Gas {
@@ -55,24 +57,26 @@ __device__ __forceinline__ double dsqrt(double x) {
 #endif
 }
 
-__device__ __forceinline__ void prim_from_cons(const double U[5], const Gas &g, double V[5]) {
+template <class T>
+__device__ __forceinline__ void prim_from_cons(const T U[5], const Gas &g, T V[5]) {
   V[0] = U[0];
-  double invRho = 1.0 / U[0];
+  T invRho = 1.0 / U[0];
   V[1] = U[1] * invRho;
   V[2] = U[2] * invRho;
   V[3] = U[3] * invRho;
-  double vel2 = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
+  T vel2 = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
   V[4] = (g.gam - 1.0) * (U[4] - 0.5 * U[0] * vel2) - g.gam * g.pstiff;
 }
 
 // Branch-free: the numerator is selected, then one division (0/den = 0 for
 // ab < 0), so lanes of a wave never diverge.
-__device__ __forceinline__ double va_fcn(double eps, double a, double b) {
-  double a2 = a * a;
-  double b2 = b * b;
-  double ab = a * b;
-  double den = a2 + b2 + 2.0 * eps;
-  double num = ab > 0.0 ? (a * (b2 + eps) + b * (a2 + eps)) : (ab == 0.0 ? 0.5 * eps * (a + b) : 0.0);
+template <class T>
+__device__ __forceinline__ T va_fcn(double eps, T a, T b) {
+  T a2 = a * a;
+  T b2 = b * b;
+  T ab = a * b;
+  T den = a2 + b2 + 2.0 * eps;
+  T num = ab > 0.0 ? (a * (b2 + eps) + b * (a2 + eps)) : (ab == 0.0 ? 0.5 * eps * (a + b) : T(0.0));
   return ddiv(num, den);
 }
 
@@ -92,31 +96,32 @@ __device__ __forceinline__ void vanalbada(double beta, double Vi, double ddVij, 
 // Roe flux through the dual face with area vector en (not normalised) and
 // normal face velocity evitno; states Ug (left = i side), Ud (right) are
 // primitive (rho, u, v, w, p).
-__device__ __forceinline__ void roeflux7(const Gas &g, const double en[3], double evitno,
-                                         const double Ug[5], const double Ud[5], double phi[5]) {
+template <class T>
+__device__ __forceinline__ void roeflux7(const Gas &g, const T en[3], T evitno, const T Ug[5], const T Ud[5],
+                                         T phi[5]) {
   const double gam = g.gam, gam1 = g.gam1, invgam1 = g.invgam1, pstiff = g.pstiff;
-  double rnorm = dsqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
-  double invnorm = ddiv(1.0, rnorm);
-  double n0 = en[0] * invnorm, n1 = en[1] * invnorm, n2 = en[2] * invnorm;
-  double vitno = evitno * invnorm;
-  double vitg2 = Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3];
-  double vitd2 = Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3];
-  double squsr1 = dsqrt(Ug[0]);
-  double squsr2 = dsqrt(Ud[0]);
-  double ener1 = (Ug[4] + gam * pstiff) * invgam1 + 0.5 * Ug[0] * vitg2;
-  double ener2 = (Ud[4] + gam * pstiff) * invgam1 + 0.5 * Ud[0] * vitd2;
-  double usro = ddiv(1.0, squsr1 + squsr2);
-  double uar2 = (squsr1 * Ug[1] + squsr2 * Ud[1]) * usro;
-  double uar3 = (squsr1 * Ug[2] + squsr2 * Ud[2]) * usro;
-  double uar4 = (squsr1 * Ug[3] + squsr2 * Ud[3]) * usro;
-  double uar5 = (ddiv(ener1 + Ug[4], squsr1) + ddiv(ener2 + Ud[4], squsr2)) * usro;
-  double VdotNt = n0 * uar2 + n1 * uar3 + n2 * uar4;
+  T rnorm = dsqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+  T invnorm = ddiv(1.0, rnorm);
+  T n0 = en[0] * invnorm, n1 = en[1] * invnorm, n2 = en[2] * invnorm;
+  T vitno = evitno * invnorm;
+  T vitg2 = Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3];
+  T vitd2 = Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3];
+  T squsr1 = dsqrt(Ug[0]);
+  T squsr2 = dsqrt(Ud[0]);
+  T ener1 = (Ug[4] + gam * pstiff) * invgam1 + 0.5 * Ug[0] * vitg2;
+  T ener2 = (Ud[4] + gam * pstiff) * invgam1 + 0.5 * Ud[0] * vitd2;
+  T usro = ddiv(1.0, squsr1 + squsr2);
+  T uar2 = (squsr1 * Ug[1] + squsr2 * Ud[1]) * usro;
+  T uar3 = (squsr1 * Ug[2] + squsr2 * Ud[2]) * usro;
+  T uar4 = (squsr1 * Ug[3] + squsr2 * Ud[3]) * usro;
+  T uar5 = (ddiv(ener1 + Ug[4], squsr1) + ddiv(ener2 + Ud[4], squsr2)) * usro;
+  T VdotNt = n0 * uar2 + n1 * uar3 + n2 * uar4;
   const bool left = (VdotNt - vitno > 0);
   // centred term: one-sided, upwinded by the Roe normal velocity
-  const double *S = left ? Ug : Ud;
-  double vit2 = left ? vitg2 : vitd2;
-  double VdotN = S[1] * n0 + S[2] * n1 + S[3] * n2;
-  double H = gam * (S[4] + pstiff) + 0.5 * gam1 * S[0] * vit2;
+  const T *S = left ? Ug : Ud;
+  T vit2 = left ? vitg2 : vitd2;
+  T VdotN = S[1] * n0 + S[2] * n1 + S[3] * n2;
+  T H = gam * (S[4] + pstiff) + 0.5 * gam1 * S[0] * vit2;
   H = ddiv(H, gam1 * S[0]);
   phi[0] = S[0] * (VdotN - vitno);
   phi[1] = phi[0] * S[1] + S[4] * n0;
@@ -125,37 +130,37 @@ __device__ __forceinline__ void roeflux7(const Gas &g, const double en[3], doubl
   phi[4] = phi[0] * H + S[4] * vitno;
   // single acoustic wave of the Roe dissipation
   VdotN = VdotNt;
-  double qir = 0.5 * (uar2 * uar2 + uar3 * uar3 + uar4 * uar4);
-  double cr2 = gam1 * (uar5 - qir);
-  double cr = dsqrt(cr2);
+  T qir = 0.5 * (uar2 * uar2 + uar3 * uar3 + uar4 * uar4);
+  T cr2 = gam1 * (uar5 - qir);
+  T cr = dsqrt(cr2);
   cr2 = ddiv(1.0, cr2);
-  double vp4 = VdotN + cr;
-  double vp5 = VdotN - cr;
-  double r = cr;
-  double s = -cr;
-  double cr2byt = -cr;
-  double invtbeta2 = ddiv(-1.0, cr);
-  double dif1 = -Ug[0] + Ud[0];
-  double dif2 = -Ug[0] * Ug[1] + Ud[0] * Ud[1];
-  double dif3 = -Ug[0] * Ug[2] + Ud[0] * Ud[2];
-  double dif4 = -Ug[0] * Ug[3] + Ud[0] * Ud[3];
-  double dif5 = -ener1 + ener2;
+  T vp4 = VdotN + cr;
+  T vp5 = VdotN - cr;
+  T r = cr;
+  T s = -cr;
+  T cr2byt = -cr;
+  T invtbeta2 = ddiv(-1.0, cr);
+  T dif1 = -Ug[0] + Ud[0];
+  T dif2 = -Ug[0] * Ug[1] + Ud[0] * Ud[1];
+  T dif3 = -Ug[0] * Ug[2] + Ud[0] * Ud[2];
+  T dif4 = -Ug[0] * Ug[3] + Ud[0] * Ud[3];
+  T dif5 = -ener1 + ener2;
   vp4 = (vp4 - vitno);
   vp5 = (vp5 - vitno);
-  double a1 = VdotN * dif1 - n0 * dif2 - n1 * dif3 - n2 * dif4;
-  double a2 = qir * dif1 - uar2 * dif2 - uar3 * dif3 - uar4 * dif4 + dif5;
-  double flur4, flur5;
+  T a1 = VdotN * dif1 - n0 * dif2 - n1 * dif3 - n2 * dif4;
+  T a2 = qir * dif1 - uar2 * dif2 - uar3 * dif3 - uar4 * dif4 + dif5;
+  T flur4, flur5;
   if (!left) {
-    double m4 = vp4 > 0.0 ? vp4 : 0.0;
+    T m4 = vp4 > 0.0 ? vp4 : T(0.0);
     flur4 = m4 * (cr2byt * a1 + gam1 * s * invtbeta2 * a2);
-    flur5 = 0.0;
+    flur5 = T(0.0);
   } else {
-    flur4 = 0.0;
-    double m5 = vp5 < 0.0 ? vp5 : 0.0;
+    flur4 = T(0.0);
+    T m5 = vp5 < 0.0 ? vp5 : T(0.0);
     flur5 = -m5 * (-cr2byt * a1 + (-gam1 * r * invtbeta2 * a2));
   }
-  double fs = flur4 + flur5;
-  double rs = r * flur4 + s * flur5;
+  T fs = flur4 + flur5;
+  T rs = r * flur4 + s * flur5;
   phi[0] = phi[0] - 0.5 * fs * cr2;
   phi[1] = phi[1] - (n0 * rs + uar2 * fs) * 0.5 * cr2;
   phi[2] = phi[2] - (n1 * rs + uar3 * fs) * 0.5 * cr2;
@@ -169,42 +174,43 @@ __device__ __forceinline__ void roeflux7(const Gas &g, const double en[3], doubl
 }
 
 // Steger-Warming farfield flux A+(V) W(V) + A-(V) Ub through area vector nv.
-__device__ __forceinline__ void boundflux5(const Gas &g, const double nv[3], double sig,
-                                           const double ua[5], const double ub[5], double flux[5]) {
+template <class T>
+__device__ __forceinline__ void boundflux5(const Gas &g, const T nv[3], double sig, const T ua[5],
+                                           const double ub[5], T flux[5]) {
   const double gam = g.gam, gam1 = g.gam1;
-  double cap = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-  double capd1 = nv[0] / cap, capd2 = nv[1] / cap, capd3 = nv[2] / cap;
-  double ro = ua[0];
-  double usro = 1.0 / ro;
-  double u = ua[1], v = ua[2], w = ua[3];
-  double vit = u * u + v * v + w * w;
-  double pm = ua[4];
-  double rhoE = pm / gam1 + 0.5 * ro * vit;
-  double c2 = gam * pm * usro;
-  double c = sqrt(c2);
-  double usc = 1.0 / c;
-  double cc = 0.5 * vit + c2 / gam1;
-  double unx = capd1 * u + capd2 * v + capd3 * w;
-  double t1 = cap * unx - sig, t4 = cap * (unx + c) - sig, t5 = cap * (unx - c) - sig;
-  double vp1 = t1 > 0.0 ? t1 : 0.0;
-  double vp4 = t4 > 0.0 ? t4 : 0.0;
-  double vp5 = t5 > 0.0 ? t5 : 0.0;
-  double vpp = vp4 + vp5;
-  double vpm = vp4 - vp5;
-  double fgp0 = (gam1 * ro * vp1 + 0.5 * ro * vpp) / gam;
-  double fgp1 = fgp0 * u + 0.5 * pm * usc * capd1 * vpm;
-  double fgp2 = fgp0 * v + 0.5 * pm * usc * capd2 * vpm;
-  double fgp3 = fgp0 * w + 0.5 * pm * usc * capd3 * vpm;
-  double fgp4 = 0.5 * ((gam1 / gam) * ro * vit * (vp1 - 0.5 * vpp) + rhoE * vpp + ro * unx * c * vpm / gam);
-  vp1 = t1 < 0.0 ? t1 : 0.0;
-  vp4 = t4 < 0.0 ? t4 : 0.0;
-  vp5 = t5 < 0.0 ? t5 : 0.0;
+  T cap = gsqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+  T capd1 = nv[0] / cap, capd2 = nv[1] / cap, capd3 = nv[2] / cap;
+  T ro = ua[0];
+  T usro = 1.0 / ro;
+  T u = ua[1], v = ua[2], w = ua[3];
+  T vit = u * u + v * v + w * w;
+  T pm = ua[4];
+  T rhoE = pm / gam1 + 0.5 * ro * vit;
+  T c2 = gam * pm * usro;
+  T c = gsqrt(c2);
+  T usc = 1.0 / c;
+  T cc = 0.5 * vit + c2 / gam1;
+  T unx = capd1 * u + capd2 * v + capd3 * w;
+  T t1 = cap * unx - sig, t4 = cap * (unx + c) - sig, t5 = cap * (unx - c) - sig;
+  T vp1 = t1 > 0.0 ? t1 : T(0.0);
+  T vp4 = t4 > 0.0 ? t4 : T(0.0);
+  T vp5 = t5 > 0.0 ? t5 : T(0.0);
+  T vpp = vp4 + vp5;
+  T vpm = vp4 - vp5;
+  T fgp0 = (gam1 * ro * vp1 + 0.5 * ro * vpp) / gam;
+  T fgp1 = fgp0 * u + 0.5 * pm * usc * capd1 * vpm;
+  T fgp2 = fgp0 * v + 0.5 * pm * usc * capd2 * vpm;
+  T fgp3 = fgp0 * w + 0.5 * pm * usc * capd3 * vpm;
+  T fgp4 = 0.5 * ((gam1 / gam) * ro * vit * (vp1 - 0.5 * vpp) + rhoE * vpp + ro * unx * c * vpm / gam);
+  vp1 = t1 < 0.0 ? t1 : T(0.0);
+  vp4 = t4 < 0.0 ? t4 : T(0.0);
+  vp5 = t5 < 0.0 ? t5 : T(0.0);
   vpp = vp1 - 0.5 * (vp4 + vp5);
   vpm = 0.5 * (vp5 - vp4);
-  double xn1 = gam1 * usc * (-0.5 * vit * ub[0] + u * ub[1] + v * ub[2] + w * ub[3] - ub[4]);
-  double xn2 = unx * ub[0] - capd1 * ub[1] - capd2 * ub[2] - capd3 * ub[3];
-  double ff = usc * (vpp * xn1 + vpm * xn2);
-  double gg = vpm * xn1 + vpp * xn2;
+  T xn1 = gam1 * usc * (-0.5 * vit * ub[0] + u * ub[1] + v * ub[2] + w * ub[3] - ub[4]);
+  T xn2 = unx * ub[0] - capd1 * ub[1] - capd2 * ub[2] - capd3 * ub[3];
+  T ff = usc * (vpp * xn1 + vpm * xn2);
+  T gg = vpm * xn1 + vpp * xn2;
   flux[0] = fgp0 + (vp1 * ub[0] + ff);
   flux[1] = fgp1 + (vp1 * ub[1] + u * ff + capd1 * gg);
   flux[2] = fgp2 + (vp1 * ub[2] + v * ff + capd2 * gg);
@@ -212,8 +218,8 @@ __device__ __forceinline__ void boundflux5(const Gas &g, const double nv[3], dou
   flux[4] = fgp4 + (vp1 * ub[4] + cc * ff + unx * gg);
 }
 
-__device__ __forceinline__ void wallflux(const double nv[3], double nvel, const double V[5],
-                                         double flux[5]) {
+template <class T>
+__device__ __forceinline__ void wallflux(const T nv[3], double nvel, const T V[5], T flux[5]) {
   flux[0] = 0.0;
   flux[1] = V[4] * nv[0];
   flux[2] = V[4] * nv[1];

@@ -181,8 +181,21 @@ __device__ __forceinline__ void amv5_add(const T *__restrict__ a, const double b
     c[i] += r[5 * i] * b[0] + r[5 * i + 1] * b[1] + r[5 * i + 2] * b[2] + r[5 * i + 3] * b[3] + r[5 * i + 4] * b[4];
 }
 
-// y = H1 p (SubDomain::computeMatVecProdH1, SubDomain.C:3917-3953)
+// c += A^T b (DENSEMATRIXTRANSTIMESVECTOR5, DenseMatrixOps.h:448-453)
 template <class T>
+__device__ __forceinline__ void amtv5_add(const T *__restrict__ a, const double b[5], double c[5]) {
+  double r[25];
+#pragma unroll
+  for (int k = 0; k < 25; ++k) r[k] = (double)a[k];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    c[i] += r[i] * b[0] + r[5 + i] * b[1] + r[10 + i] * b[2] + r[15 + i] * b[3] + r[20 + i] * b[4];
+}
+
+// y = H1 p (SubDomain::computeMatVecProdH1, SubDomain.C:3917-3953), or with
+// TRANS y = H1^T p (SubDomain::computeMatVecProdH1transpose, SubDomain.C:3958-3990):
+// row n gathers A_mn^T p_m from the mirror entry (m, n), in edge order.
+template <class T, bool TRANS>
 __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
                                                      const double *__restrict__ p, double *__restrict__ y,
                                                      const int *__restrict__ done) {
@@ -193,20 +206,26 @@ __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__res
   double pn[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) pn[k] = p[5 * (size_t)n + k];
-  amv5_add(D + 25 * (size_t)n, pn, acc);
+  if (TRANS)
+    amtv5_add(D + 25 * (size_t)n, pn, acc);
+  else
+    amv5_add(D + 25 * (size_t)n, pn, acc);
   for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
     const int m = t.adj[e];
     double pm[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) pm[k] = p[5 * (size_t)m + k];
-    amv5_add(B + 25 * (size_t)e, pm, acc);
+    if (TRANS)
+      amtv5_add(B + 25 * (size_t)t.mirror[e], pm, acc);
+    else
+      amv5_add(B + 25 * (size_t)e, pm, acc);
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) y[5 * (size_t)n + k] = acc[k];
 }
 
-// x = Di y (DiagMat::apply, DiagMatrix.C:86-100)
-template <class T>
+// x = Di y (DiagMat::apply, DiagMatrix.C:86-100), or x = Di^T y
+template <class T, bool TRANS>
 __global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restrict__ Di,
                                                          const double *__restrict__ y, double *__restrict__ x,
                                                          const int *__restrict__ done) {
@@ -216,7 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restr
   double yn[5], acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < 5; ++k) yn[k] = y[5 * (size_t)n + k];
-  amv5_add(Di + 25 * (size_t)n, yn, acc);
+  if (TRANS)
+    amtv5_add(Di + 25 * (size_t)n, yn, acc);
+  else
+    amv5_add(Di + 25 * (size_t)n, yn, acc);
 #pragma unroll
   for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
 }
@@ -278,38 +300,60 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   return check_launch("invert launch");
 }
 
-int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done) {
-  if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "jvp: Jacobian not set up");
+int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *Di) {
   const int nb = (c->N + kBlock - 1) / kBlock;
-  if (c->jac_precision)
-    hipLaunchKernelGGL((k_h1_apply<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
-                       (const float *)c->d_jacD, (const float *)c->d_jacB, p, y, done);
+  hipLaunchKernelGGL((k_invert<float>), dim3(nb), dim3(kBlock), 0, stream, c->N, D, Di, c->d_err + 1);
+  return check_launch("invert launch");
+}
+
+template <class T>
+static void h1_launch(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  if (trans)
+    hipLaunchKernelGGL((k_h1_apply<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
+                       (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
   else
-    hipLaunchKernelGGL((k_h1_apply<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
-                       (const double *)c->d_jacD, (const double *)c->d_jacB, p, y, done);
+    hipLaunchKernelGGL((k_h1_apply<T, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
+                       (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
+}
+
+int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
+  if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "jvp: Jacobian not set up");
+  if (c->jac_precision)
+    h1_launch<float>(c, p, y, done, trans);
+  else
+    h1_launch<double>(c, p, y, done, trans);
   return check_launch("jvp launch");
 }
 
-int launch_h1_apply(aerofx_ctx *c, const double *p, double *y) {
+int launch_h1_apply(aerofx_ctx *c, const double *p, double *y, bool trans) {
   ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
-  return launch_h1_apply_flag(c, p, y, nullptr);
+  return launch_h1_apply_flag(c, p, y, nullptr, trans);
 }
 
-int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done) {
-  if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
+template <class T>
+static void jacobi_launch(aerofx_ctx *c, const double *y, double *x, const int *done, bool trans) {
   const int nb = (c->N + kBlock - 1) / kBlock;
-  if (c->jac_precision)
-    hipLaunchKernelGGL((k_jacobi_apply<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
-                       (const float *)c->d_jacDi, y, x, done);
+  if (trans)
+    hipLaunchKernelGGL((k_jacobi_apply<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const T *)c->d_jacDi, y, x, done);
   else
-    hipLaunchKernelGGL((k_jacobi_apply<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
-                       (const double *)c->d_jacDi, y, x, done);
+    hipLaunchKernelGGL((k_jacobi_apply<T, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const T *)c->d_jacDi, y, x, done);
+}
+
+int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done, bool trans) {
+  if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
+  if (c->jac_precision)
+    jacobi_launch<float>(c, y, x, done, trans);
+  else
+    jacobi_launch<double>(c, y, x, done, trans);
   return check_launch("prec launch");
 }
 
-int launch_prec_apply(aerofx_ctx *c, const double *y, double *x) {
+int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans) {
   ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
-  return launch_prec_apply_flag(c, y, x, nullptr);
+  return launch_prec_apply_flag(c, y, x, nullptr, trans);
 }
 
 }  // namespace aerofx

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
 #pragma unroll
       for (int k = 0; k < kMaxBasis; ++k)
         if (k < ny) wi -= hk[k] * v[k];
-      w[i] = wi;
+      if (ny > 0) w[i] = wi;
     }
     if (MODE == 2) {
       acc[0] += wi * wi;
@@ -236,6 +236,60 @@ __global__ __launch_bounds__(kBlock) void k_check_states(int N, const double *__
   if (!(rho > 0.0) || !(p + g.pstiff > 0.0)) atomicAdd(bad, 1);
 }
 
+// ---------------------------------------------------------------------------
+// MatVecProdFD (MatVecProd.C:262-372, 879-929)
+// ---------------------------------------------------------------------------
+// partial sums of eps0 |U| + eps0 over all entries (computeEpsilon)
+__global__ __launch_bounds__(kBlock) void k_eps_sum(size_t n, const double *__restrict__ U,
+                                                    double *__restrict__ partial) {
+  const size_t slice = (n + kDotBlocks - 1) / kDotBlocks;
+  const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
+  const double eps0 = 1.e-6;
+  double acc[1] = {0.0};
+  for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) acc[0] += eps0 * fabs(U[i]) + eps0;
+  block_reduce_store<1>(acc, 1, partial);
+}
+
+// scal[0] = p.p, scal[1] = sum(eps0 |U| + eps0) -> scal[2] = eps, [3] = 1/eps, [4] = 0.5/eps
+__global__ void k_fd_eps(double *scal, double size) {
+  if (threadIdx.x != 0) return;
+  const double norm = sqrt(scal[0]);
+  const double eps = norm < 1.0e-14 ? 1.e-6 : scal[1] / (size * norm);
+  scal[2] = eps;
+  scal[3] = 1.0 / eps;
+  scal[4] = 0.5 / eps;
+}
+
+// Q = U + s eps p (s = +1 or -1)
+__global__ __launch_bounds__(kBlock) void k_fd_shift(size_t n, const double *__restrict__ U,
+                                                     const double *__restrict__ p, const double *__restrict__ scal,
+                                                     double s, double *__restrict__ Q) {
+  const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double ep = scal[2] * p[i];
+  Q[i] = s > 0.0 ? U[i] + ep : U[i] - ep;
+}
+
+// TimeState::add_dAW_dt, backward Euler, Un = U (TimeState.C:40-73):
+// F += (1/dt) (1 Q + (-1) Un)
+__global__ __launch_bounds__(kBlock) void k_fd_time(size_t n, const double *__restrict__ Q,
+                                                    const double *__restrict__ U, const double *__restrict__ dt,
+                                                    double *__restrict__ F) {
+  const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double invDt = 1.0 / dt[i / 5];
+  F[i] += invDt * (Q[i] - U[i]);
+}
+
+// prod = scal[idx] (Fa - Fb)
+__global__ __launch_bounds__(kBlock) void k_fd_diff(size_t n, const double *__restrict__ Fa,
+                                                    const double *__restrict__ Fb, const double *__restrict__ scal,
+                                                    int idx, double *__restrict__ prod) {
+  const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
+  if (i >= n) return;
+  prod[i] = scal[idx] * (Fa[i] - Fb[i]);
+}
+
 struct Krylov {
   int m = 0;
   size_t n = 0;
@@ -311,7 +365,8 @@ struct GmresOut {
 // Right-preconditioned GMRES(m) on H1 with block-Jacobi: A x = b, x0 = 0
 // (ImplicitCoupledTsDesc::solveLinearSystem sets dQ = 0).  Enqueues one
 // cycle at a time and synchronises once per cycle.
-static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, int maxits, GmresOut *out) {
+static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, int maxits, GmresOut *out,
+                 bool trans = false) {
   int rc = ensure_krylov(c, m);
   if (rc) return rc;
   Krylov *k = c->ksp;
@@ -328,7 +383,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
     if (cycle == 0) {
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, 0.0, k->r, nullptr);
     } else {
-      if ((rc = launch_h1_apply(c, x, k->w))) return rc;
+      if ((rc = launch_h1_apply(c, x, k->w, trans))) return rc;
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, 0.0, k->r, nullptr);
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->w, 1.0, k->r,
                          nullptr);
@@ -342,11 +397,11 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
       double *Vj = k->V + (size_t)j * n;
       {
         ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
-        if ((rc = launch_prec_apply_flag(c, Vj, k->z, done))) return rc;
+        if ((rc = launch_prec_apply_flag(c, Vj, k->z, done, trans))) return rc;
       }
       {
         ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
-        if ((rc = launch_h1_apply_flag(c, k->z, k->w, done))) return rc;
+        if ((rc = launch_h1_apply_flag(c, k->z, k->w, done, trans))) return rc;
       }
       {
         ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
@@ -363,7 +418,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
     if (hs.jlast >= 0) {
       hipLaunchKernelGGL(k_gmres_backsolve, dim3(1), dim3(64), 0, c->stream, k->st);
       hipLaunchKernelGGL(k_combine, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, n, k->st);
-      if ((rc = launch_prec_apply(c, k->w, k->r))) return rc;
+      if ((rc = launch_prec_apply(c, k->w, k->r, trans))) return rc;
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->r, 1.0, x, nullptr);
     }
     if (hs.done) break;  // converged, maxits, zero residual or lucky breakdown
@@ -376,11 +431,77 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
   return 0;
 }
 
+// MatVecProdFD::evaluate + apply: prod = (F(U + eps p) - F(U)) / eps (order 1)
+// or (F(U + eps p) - F(U - eps p)) / (2 eps) (order 2), F = R + time term.
+static int jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *prod, int order, const double *dt) {
+  int rc = ensure_krylov(c, c->ksp ? c->ksp->m : 1);
+  if (rc) return rc;
+  Krylov *k = c->ksp;
+  const size_t n = k->n;
+  // computeEpsilon
+  sweep<2>(c, const_cast<double *>(p), nullptr, 0, 0, nullptr, k->scal, nullptr);
+  hipLaunchKernelGGL(k_eps_sum, dim3(kDotBlocks), dim3(kBlock), 0, c->stream, n, U, k->partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->scal + 1, nullptr);
+  hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, c->stream, k->scal, (double)n);
+  double *Fa = k->w, *Fb = k->z, *Q = k->r;
+  // F(U + eps p)
+  hipLaunchKernelGGL(k_fd_shift, grid_for(n), dim3(kBlock), 0, c->stream, n, U, p, k->scal, 1.0, Q);
+  if ((rc = launch_residual(c, Q, Fa, true))) return rc;
+  if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, Q, U, dt, Fa);
+  if (order == 1) {
+    // F(U): the time term vanishes at Q = Un
+    if ((rc = launch_residual(c, U, Fb, true))) return rc;
+    if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, U, U, dt, Fb);
+    hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, Fa, Fb, k->scal, 3, prod);
+  } else {
+    hipLaunchKernelGGL(k_fd_shift, grid_for(n), dim3(kBlock), 0, c->stream, n, U, p, k->scal, -1.0, Q);
+    if ((rc = launch_residual(c, Q, Fb, true))) return rc;
+    if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, Q, U, dt, Fb);
+    hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, Fa, Fb, k->scal, 4, prod);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "jvp_fd launch");
+}
+
 }  // namespace aerofx
 
 using namespace aerofx;
 
 extern "C" {
+
+int aerofx_jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *Jp, int order, const double *dt) {
+  if (!c || !U || !p || !Jp) return set_error(AEROFX_ERR_ARG, "jvp_fd: null argument");
+  if (order != 1 && order != 2) return set_error(AEROFX_ERR_ARG, "jvp_fd: order 1 or 2");
+  int rc = jvp_fd(c, U, p, Jp, order, dt);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "jvp_fd sync");
+  return c->h_err[0];
+}
+
+int aerofx_jvp_transpose(aerofx_ctx *c, const double *p, double *JTp) {
+  if (!c || !p || !JTp) return set_error(AEROFX_ERR_ARG, "jvp_transpose: null argument");
+  return launch_h1_apply(c, p, JTp, true);
+}
+
+int aerofx_prec_apply_transpose(aerofx_ctx *c, const double *y, double *x) {
+  if (!c || !y || !x) return set_error(AEROFX_ERR_ARG, "prec_apply_transpose: null argument");
+  return launch_prec_apply(c, y, x, true);
+}
+
+int aerofx_gmres_transpose(aerofx_ctx *c, const double *b, double *x, int m, double eps, int maxits, int *its,
+                           double *final_res) {
+  if (!c || !b || !x) return set_error(AEROFX_ERR_ARG, "gmres_transpose: null argument");
+  if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "gmres_transpose: operators not set up");
+  GmresOut o;
+  int rc = gmres(c, b, x, m, eps, maxits, &o, true);
+  if (rc) return rc;
+  if (its) *its = o.its;
+  if (final_res) *final_res = o.res;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "gmres_transpose sync");
+}
 
 int aerofx_timestep(aerofx_ctx *c, const double *U, double cfl, double *dt) {
   if (!c || !U || !dt) return set_error(AEROFX_ERR_ARG, "timestep: null argument");

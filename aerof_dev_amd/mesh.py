@@ -111,6 +111,14 @@ def rcm(mesh: Mesh) -> np.ndarray:
     return renum
 
 
+def rcm_gpu(mesh: Mesh) -> np.ndarray:
+    """The same RCM permutation with the breadth-first searches on the GPU."""
+    renum = np.empty(mesh.num_nodes, np.int32)
+    tets = np.ascontiguousarray(mesh.tets, dtype=np.int32)
+    check(lib().aerofx_mesh_rcm_gpu(mesh.num_nodes, mesh.num_tets, _ip(tets), _ip(renum)), "mesh_rcm_gpu")
+    return renum
+
+
 def renumber(mesh: Mesh, renum: np.ndarray) -> Mesh:
     X = np.ascontiguousarray(mesh.X.copy())
     tets = np.ascontiguousarray(mesh.tets.copy())
@@ -130,6 +138,15 @@ def color_edges(num_nodes: int, ptr: np.ndarray):
     check(lib().aerofx_mesh_color_edges(num_nodes, ptr.shape[0], _ip(ptr), _ip(color), C.byref(nc)),
           "color_edges")
     return color, nc.value
+
+
+def color_edges_gpu(num_nodes: int, ptr: np.ndarray):
+    """Edge colouring on the GPU (Jones-Plassmann rounds, same colours as color_edges)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    color = np.empty(ptr.shape[0], np.int32)
+    nc = check(lib().aerofx_mesh_color_edges_gpu(num_nodes, ptr.shape[0], _ip(ptr), _ip(color)),
+               "color_edges_gpu")
+    return color, nc
 
 
 def init_state(params: FlowParams, X: np.ndarray, kind: str = "freestream", amp: float = 0.0,

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from ._abi import Mesh as _CMesh
-from ._abi import NewtonOpts, NewtonStats, c_double_p, c_int_p, check, lib
+from ._abi import BlockStats, NewtonOpts, NewtonStats, c_double_p, c_int_p, check, lib
 from .mesh import FlowParams, Mesh
 
 
@@ -118,6 +118,15 @@ class SpaceOperator:
     def conservative_to_primitive(self, U, V):
         check(lib().aerofx_primitive(self._h, _ptr(U), _ptr(V)), "primitive")
 
+    def compute_transpose_derivative_of_residual(self, U, lam, dRdU_T_lam=None, dRdX_T_lam=None):
+        """(dR/dU)^T lam -> dRdU_T_lam [N,5] and (dR/dX)^T lam -> dRdX_T_lam [N,3]
+        (SpaceOperator::computeTransposeDerivativeOfResidual, SpaceOperator.C:1235-1304).
+        Either output may be None.  The MUSCL gradients are recomputed."""
+        check(lib().aerofx_adjoint_residual(self._h, _ptr(U), _ptr(lam),
+                                            _ptr(dRdU_T_lam) if dRdU_T_lam is not None else None,
+                                            _ptr(dRdX_T_lam) if dRdX_T_lam is not None else None),
+              "adjoint_residual")
+
 
 class MatVecProdH1:
     """MatVecProdH1<5,Scalar,5> (MatVecProd.h:37-259) and JacobiPrec<Scalar,5>
@@ -141,6 +150,13 @@ class MatVecProdH1:
     def prec_apply(self, y, x):
         check(lib().aerofx_prec_apply(self.op._h, _ptr(y), _ptr(x)), "prec_apply")
 
+    def apply_transpose(self, p, prod):
+        """MatVecProdH1::applyTranspose (MatVecProd.C:1190-1206)."""
+        check(lib().aerofx_jvp_transpose(self.op._h, _ptr(p), _ptr(prod)), "jvp_transpose")
+
+    def prec_apply_transpose(self, y, x):
+        check(lib().aerofx_prec_apply_transpose(self.op._h, _ptr(y), _ptr(x)), "prec_apply_transpose")
+
     def blocks(self):
         """Host copies: D [N,5,5] and per edge (A_ij, A_ji) [E,2,5,5]."""
         N, E = self.op.num_nodes, self.op.num_edges
@@ -152,6 +168,22 @@ class MatVecProdH1:
 
     def timestep(self, U, cfl, dt):
         check(lib().aerofx_timestep(self.op._h, _ptr(U), float(cfl), _ptr(dt)), "timestep")
+
+
+class MatVecProdFD:
+    """MatVecProdFD<5,5> (MatVecProd.h:83-160): finite-difference JVP of the
+    residual plus the backward-Euler time term (dt given) about U."""
+
+    def __init__(self, op: SpaceOperator, order: int = 1):
+        self.op, self.order = op, order
+        self.U = self.dt = None
+
+    def evaluate(self, U, dt=None):
+        self.U, self.dt = U, dt
+
+    def apply(self, p, prod) -> int:
+        return check(lib().aerofx_jvp_fd(self.op._h, _ptr(self.U), _ptr(p), _ptr(prod), int(self.order),
+                                         _ptr(self.dt) if self.dt is not None else None), "jvp_fd")
 
 
 class GmresSolver:
@@ -166,6 +198,35 @@ class GmresSolver:
         check(lib().aerofx_gmres(self.op._h, _ptr(b), _ptr(x), self.m, self.eps, self.maxits, C.byref(its),
                                  C.byref(res)), "gmres")
         return its.value, res.value
+
+    def solveT(self, b, x):
+        """GmresSolver::solveT (KspSolver.C:754): the transposed system."""
+        its = C.c_int()
+        res = C.c_double()
+        check(lib().aerofx_gmres_transpose(self.op._h, _ptr(b), _ptr(x), self.m, self.eps, self.maxits,
+                                           C.byref(its), C.byref(res)), "gmres_transpose")
+        return its.value, res.value
+
+
+class BlockGmresSolver:
+    """Block GMRES(m) over up to 16 right-hand sides sharing the operators of
+    the last MatVecProdH1.evaluate (FP64).  precision 1 runs the cycles with
+    FP32 copies of H1 and the Jacobi inverse and refines with the FP64
+    residual; use_graph replays each cycle as a captured hipGraph."""
+
+    def __init__(self, op: SpaceOperator, m: int = 8, eps: float = 1e-8, max_cycles: int = 200,
+                 precision: int = 1, use_graph: bool = True):
+        self.op, self.m, self.eps, self.max_cycles = op, m, eps, max_cycles
+        self.precision, self.use_graph = precision, use_graph
+
+    def solve(self, B, X) -> dict:
+        """B, X: device float64 [nrhs, N, 5], nrhs <= 16."""
+        st = BlockStats()
+        check(lib().aerofx_block_gmres(self.op._h, int(B.shape[0]), _ptr(B), _ptr(X), self.m, self.eps,
+                                       self.max_cycles, self.precision, 1 if self.use_graph else 0, C.byref(st)),
+              "block_gmres")
+        return dict(cycles=st.cycles, inner_its=st.inner_its, converged=bool(st.converged), res_max=st.res_max,
+                    seconds=st.seconds, tune=(st.tune_waves_dots, st.tune_waves_update))
 
 
 def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200, eps: float = 1e-8,

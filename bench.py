@@ -11,12 +11,16 @@ Steger-Warming farfield on 5 faces and a slip wall at z = 0.
 A step is one full residual evaluation (gradient pass + flux pass, 1/V
 scaling) with U resident in HBM.  value = edges / (ms per step) in Medges/s.
 
-roofline: the dominant kernel is the flux pass; its algorithmic bytes per
-launch are 232 B/node + 32 B/edge + 40 B/face (read U, X, grad, 1/V and
-write R per node; edge pair + normal per edge; face nodes + normal + code per
-face), divided by that kernel's mean duration from HIP events bracketing
-each launch on the context stream.  traffic (HBM bytes per launch) comes from
-the rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed under profiles/.
+roofline: one residual evaluation is three kernels (k_gradients,
+k_flux_edges, k_flux_nodes).  Its algorithmic bytes are 464 B/node +
+40 B/edge + 40 B/face: the gradient pass reads U, X and the LSQ factors and
+writes the gradients (232 B/node) plus an edge pair (8 B/edge); the flux pass
+reads U, X, gradients and 1/V and writes R (232 B/node), plus edge pair and
+normal (32 B/edge) and face nodes, normal and code (40 B/face).  They are
+divided by the sum of the three kernels' mean durations from HIP events
+bracketing each launch on the stream the kernels run on.  traffic (HBM bytes
+per evaluation) comes from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary under
+profiles/.
 
 cpu_baseline: the oracle's single-thread C restatement of the same residual
 on the same mesh and state, timed on the host over a bounded number of
@@ -193,17 +197,23 @@ def main():
     ms_step = ms_total / args.steps
     log(f"[bench] {args.steps} steps, {ms_step:.4f} ms/step; kernels {timers}")
 
-    flux_ms = timers["fvTerm"][0] / timers["fvTerm"][1]
-    grad_ms = timers["nodalGrad"][0] / timers["nodalGrad"][1]
-    flux_bytes = 232 * N + 32 * E + 40 * F
+    def per_launch(name):
+        tot, cnt = timers[name]
+        return tot / cnt
+    grad_ms = per_launch("nodalGrad")
+    edge_ms = per_launch("fvTerm")
+    node_ms = per_launch("fvTermNodes")
+    # Algorithmic bytes of one residual evaluation: gradient pass 232 B/node
+    # + 8 B/edge, flux pass 232 B/node + 32 B/edge + 40 B/face.
     grad_bytes = 232 * N + 8 * E
-    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
+    flux_bytes = 232 * N + 32 * E + 40 * F
+    whole_bytes = grad_bytes + flux_bytes
+    kern_ms = grad_ms + edge_ms + node_ms
+    achieved = whole_bytes / (kern_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=load_traffic("fvterm_box128"),
-                    kernel="k_fluxes<true> (flux pass)", algorithmic_bytes=flux_bytes,
-                    ms_per_launch=round(flux_ms, 5))
-    grad_gbs = grad_bytes / (grad_ms * 1e-3) / 1e9
-    whole_bytes = 464 * N + 40 * E + 40 * F
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=load_traffic("residual_box128"),
+                    kernel="residual = k_gradients + k_flux_edges + k_flux_nodes",
+                    algorithmic_bytes=whole_bytes, ms_per_launch=round(kern_ms, 5))
     out = {
         "metric": "Medge-flux evals/s (FP64 residual, 128^3 Kuhn box)",
         "value": round(E / (ms_step * 1e-3) / 1e6, 1),
@@ -220,9 +230,13 @@ def main():
         "config": {"workload": "FP64 residual, 128^3-hex Kuhn box, RCM order, M=0.8",
                    "nodes": N, "edges": E, "faces": F, "parallelism": "single GPU"},
         "roofline": roofline,
-        "kernels": {"nodalGrad": {"ms": round(grad_ms, 5), "algorithmic_GBs": round(grad_gbs, 1)},
-                    "fvTerm": {"ms": round(flux_ms, 5), "algorithmic_GBs": round(achieved, 1)}},
-        "residual_hbm_frac": round(whole_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": {"k_gradients": {"ms": round(grad_ms, 5),
+                                    "algorithmic_GBs": round(grad_bytes / (grad_ms * 1e-3) / 1e9, 1)},
+                    "k_flux_edges+k_flux_nodes": {
+                        "ms": round(edge_ms + node_ms, 5),
+                        "algorithmic_GBs": round(flux_bytes / ((edge_ms + node_ms) * 1e-3) / 1e9, 1)},
+                    "k_flux_edges": {"ms": round(edge_ms, 5)},
+                    "k_flux_nodes": {"ms": round(node_ms, 5)}},
     }
     if not args.no_newton:
         op.close()

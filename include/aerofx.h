@@ -93,6 +93,15 @@ int aerofx_mesh_renumber(int num_nodes, int num_tets, int num_faces,
 /* Edge colouring rule (no reference counterpart): returns colours. */
 int aerofx_mesh_color_edges(int num_nodes, int num_edges, const int *ptr,
                             int *color, int *num_colors);
+/* The same two orderings computed on the GPU, bit-identical to the host
+ * versions: aerofx_mesh_color_edges_gpu runs Jones-Plassmann rounds whose
+ * result is the greedy colouring in decreasing key order; aerofx_mesh_rcm_gpu
+ * builds each breadth-first level structure of Connectivity::rootLS /
+ * findPseudoDiam / renumRCM (ConnectivityCore.C:324-635) on the GPU, in the
+ * sequential append order.  Host arrays in and out. */
+int aerofx_mesh_color_edges_gpu(int num_nodes, int num_edges, const int *ptr,
+                                int *color);
+int aerofx_mesh_rcm_gpu(int num_nodes, int num_tets, const int *tets, int *renum);
 
 /* Synthetic states: conservative U[N][5] on host.
  *  kind 0: uniform freestream
@@ -176,11 +185,68 @@ int aerofx_jacobian_blocks(aerofx_ctx *ctx, double *D, double *Aedge);
 int aerofx_jvp(aerofx_ctx *ctx, const double *p, double *Jp);
 /* JacobiPrec::apply (KspPrec.C:87-111): x = D^-1 y. */
 int aerofx_prec_apply(aerofx_ctx *ctx, const double *y, double *x);
+/* MatVecProdH1::applyTranspose (MatVecProd.C:1189-1206,
+ * SubDomain.C:3958-3990): JTp = H1^T p, per node in the reference's order. */
+int aerofx_jvp_transpose(aerofx_ctx *ctx, const double *p, double *JTp);
+/* Transposed block-Jacobi: x = (D^-1)^T y (the preconditioner of the
+ * transposed solve, KspSolver::solveT, KspSolver.C:754). */
+int aerofx_prec_apply_transpose(aerofx_ctx *ctx, const double *y, double *x);
+/* MatVecProdFD::evaluate + apply + computeEpsilon (MatVecProd.C:88-118,
+ * 262-372, 879-929): Jp = (F(U + eps p) - F(U)) / eps (order 1) or
+ * (F(U + eps p) - F(U - eps p)) / (2 eps) (order 2), F = R + (Q - U)/dt
+ * (TimeState::add_dAW_dt, backward Euler, Un = U; dt NULL: R only),
+ * eps = sum(1e-6 |U| + 1e-6) / (5N ||p||) (1e-6 when ||p|| < 1e-14).
+ * Device U, p, Jp [N][5], dt [N].  Returns the reconstruction error count
+ * of the last residual (0 when valid) or < 0. */
+int aerofx_jvp_fd(aerofx_ctx *ctx, const double *U, const double *p, double *Jp,
+                  int order, const double *dt);
 /* GmresSolver::solve (KspSolver.C:353-492): right-preconditioned GMRES(m)
  * on the operators of the last aerofx_jacobian_setup, x0 = 0, stop at
  * ||r|| <= eps ||b|| or maxits.  Orthogonalisation is CGS2 in FP64. */
 int aerofx_gmres(aerofx_ctx *ctx, const double *b, double *x, int m,
                  double eps, int maxits, int *its, double *final_res);
+
+/* GmresSolver::solveT (KspSolver.C:754): GMRES(m) on the transposed
+ * operators H1^T and (D^-1)^T, as aerofx_gmres. */
+int aerofx_gmres_transpose(aerofx_ctx *ctx, const double *b, double *x, int m,
+                           double eps, int maxits, int *its, double *final_res);
+
+/* ---- batched right-hand sides ------------------------------------------ */
+typedef struct {
+  int cycles;            /* restart cycles (each m block steps) */
+  int inner_its;         /* block steps in total */
+  int converged;
+  double res_max;        /* max_k ||b_k - A x_k|| / ||b_k|| (FP64 operator) */
+  double seconds;
+  int tune_waves_dots, tune_waves_update;  /* autotuned launch shapes */
+} aerofx_block_stats;
+/* Block GMRES(m) for nrhs <= 16 systems A x_k = b_k with the operators of
+ * the last aerofx_jacobian_setup (which must be FP64, precision 0): H1 with
+ * right block-Jacobi.  B, X device [nrhs][N][5].  precision 0: FP64 inner
+ * operators; 1: FP32 copies of H1 and of the Jacobi inverse inside the
+ * cycles, FP64 residual B - A X at each restart (iterative refinement);
+ * the Krylov basis and its block Gram-Schmidt (FP64 MFMA tall-skinny GEMMs)
+ * are FP64 either way.  use_graph != 0 replays each cycle as one captured
+ * hipGraph.  Stops when every ||b_k - A x_k|| <= eps ||b_k|| or after
+ * max_cycles.  No counterpart in the reference: its multi-RHS
+ * GmresSolver::solve(VecSet&, VecSet&) (KspSolver.C:988-1135) is dead code. */
+int aerofx_block_gmres(aerofx_ctx *ctx, int nrhs, const double *B, double *X,
+                       int m, double eps, int max_cycles, int precision,
+                       int use_graph, aerofx_block_stats *stats);
+
+/* ---- adjoint ----------------------------------------------------------- */
+/* Transposed derivatives of the residual of aerofx_residual at U (device
+ * [N][5]) applied to lambda (device [N][5]):
+ *   dRdU_T_lambda [N][5] = (dR/dU)^T lambda   (may be NULL)
+ *   dRdX_T_lambda [N][3] = (dR/dX)^T lambda   (may be NULL; X enters through
+ *   the dual normals, control volumes, face normals, LSQ factors and the
+ *   reconstruction distances).
+ * Replaces SpaceOperator::computeTransposeDerivativeOfResidual and the
+ * transposed sensitivity operators (SpaceOperator.C:862-910, 1235-1304;
+ * Edge.C:1452-1728).  The MUSCL gradients are recomputed in the call.
+ * Synchronises before returning. */
+int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const double *lambda,
+                            double *dRdU_T_lambda, double *dRdX_T_lambda);
 
 /* ---- steady pseudo-transient Newton-GMRES ------------------------------ */
 typedef struct {

@@ -64,6 +64,7 @@ def lib():
         L.or_setup.restype = C.c_int
         L.or_residual_ex.restype = C.c_int
         L.or_residual.restype = C.c_int
+        L.or_jvp_fd.restype = C.c_int
         L.or_color_edges.restype = C.c_int
         L.or_edge_priority.restype = C.c_uint32
         L.or_edge_priority.argtypes = [C.c_uint32]
@@ -148,6 +149,17 @@ class OracleMesh:
             out["Rabs"] = Rabs
         if want_grad:
             out["ddx"], out["ddy"], out["ddz"] = g
+        return out
+
+    def jvp_fd(self, U, p, order=1, dt=None):
+        """MatVecProdFD: (F(U + eps p) - F(U)) / eps, or central (order 2)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.empty_like(U)
+        dtp = _d(np.ascontiguousarray(dt, dtype=np.float64)) if dt is not None else None
+        ierr = lib().or_jvp_fd(C.byref(self.m), C.byref(self.p), _d(U), _d(p), int(order), dtp, _d(out))
+        if ierr:
+            raise RuntimeError(f"non-physical reconstruction in jvp_fd ({ierr})")
         return out
 
     def timestep(self, V, cfl):

@@ -1348,3 +1348,58 @@ int or_steady_solve(const or_mesh *m, const or_params *p, double *U, int precisi
   free(R); free(V); free(dt); free(b); free(dQ);
   return step;
 }
+
+/* ------------------------------------------------------------------------ */
+/* MatVecProdFD                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* MatVecProdFD::computeEpsilon (MatVecProd.C:879-929), one subdomain with all
+ * nodes owned: eps = sum(eps0 |U| + eps0) / (size * ||p||), eps0 = 1e-6. */
+static double fd_epsilon(size_t n, const double *U, const double *p) {
+  const double eps0 = 1.e-6;
+  double pp = 0.0;
+  for (size_t k = 0; k < n; ++k) pp += p[k] * p[k];
+  double norm = sqrt(pp);
+  if (norm < 1.0e-14) return eps0;
+  double loceps = 0.0;
+  for (size_t k = 0; k < n; ++k) loceps += eps0 * fabs(U[k]) + eps0;
+  return loceps / ((double)n * norm);
+}
+
+/* F(Q) = R(Q) + (1/dt)(1 Q + (-1) Un) with Un = U (TimeState::add_dAW_dt,
+ * TimeState.C:40-73, backward Euler, NONDESCRIPTOR). */
+static int fd_function(const or_mesh *m, const or_params *p, const double *Q, const double *U,
+                       const double *dt, double *F) {
+  int ierr = or_residual(m, p, Q, F, 0, NULL, NULL, NULL, NULL);
+  if (dt)
+    for (int i = 0; i < m->N; ++i) {
+      double invDt = 1.0 / dt[i];
+      for (int k = 0; k < 5; ++k) F[5 * i + k] += invDt * (1.0 * Q[5 * i + k] + -1.0 * U[5 * i + k]);
+    }
+  return ierr;
+}
+
+/* MatVecProdFD::evaluate + apply (MatVecProd.C:88-118, 262-372). */
+int or_jvp_fd(const or_mesh *m, const or_params *p, const double *U, const double *pv, int order,
+              const double *dt, double *prod) {
+  size_t n = 5 * (size_t)m->N;
+  double *Q = (double *)malloc(sizeof(double) * n);
+  double *Fa = (double *)malloc(sizeof(double) * n);
+  double *Fb = (double *)malloc(sizeof(double) * n);
+  double eps = fd_epsilon(n, U, pv);
+  int ierr = 0;
+  for (size_t k = 0; k < n; ++k) Q[k] = U[k] + eps * pv[k];
+  ierr += fd_function(m, p, Q, U, dt, Fa);
+  if (order == 1) {
+    ierr += fd_function(m, p, U, U, dt, Fb);
+    double inv = 1.0 / eps;
+    for (size_t k = 0; k < n; ++k) prod[k] = inv * (Fa[k] - Fb[k]);
+  } else {
+    for (size_t k = 0; k < n; ++k) Q[k] = U[k] - eps * pv[k];
+    ierr += fd_function(m, p, Q, U, dt, Fb);
+    double half = 0.5 / eps;
+    for (size_t k = 0; k < n; ++k) prod[k] = half * (Fa[k] - Fb[k]);
+  }
+  free(Q); free(Fa); free(Fb);
+  return ierr;
+}

@@ -142,6 +142,12 @@ void or_rcm(int N, int E, const int *ptr, int *renum);
 uint32_t or_edge_priority(uint32_t l);
 int or_color_edges(int N, int E, const int *ptr, int *color);
 
+/* MatVecProdFD::evaluate + apply with computeEpsilon (MatVecProd.C:88-118,
+ * 262-372, 879-929); dt (optional) adds the backward-Euler time term with
+ * Un = U.  Returns the summed reconstruction error count. */
+int or_jvp_fd(const or_mesh *m, const or_params *p, const double *U, const double *pv, int order,
+              const double *dt, double *prod);
+
 #ifdef __cplusplus
 }
 #endif

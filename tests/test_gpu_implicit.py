@@ -110,3 +110,55 @@ def test_steady_solve_tracks_oracle(gpu):
     out2 = steady_solve(op, Ud2, precision=1, max_steps=25)
     h2 = out2["history"]
     assert np.max(np.abs(h2[:n] - hr[:n]) / hr[:n]) <= 1e-3
+
+
+@pytest.mark.parametrize("order,with_dt", [(1, False), (2, False), (1, True)])
+def test_fd_jvp_matches_oracle(gpu, order, with_dt):
+    """MatVecProdFD against the oracle's restatement.  eps comes from two
+    global sums whose order differs (fixed tree here, sequential in the
+    oracle), so eps differs in the last bits and the FD quotient by ~1e-9."""
+    import torch
+    from aerof_dev_amd.operators import MatVecProdFD
+    m, p, U, op, om = _setup(gpu, n=6)
+    dt = om.timestep(om.residual(U)["V"], 20.0) if with_dt else None
+    pv = np.random.default_rng(8).normal(size=U.shape)
+    ref = om.jvp_fd(U, pv, order, dt)
+    mv = MatVecProdFD(op, order)
+    Ud = torch.from_numpy(U).to(gpu)
+    mv.evaluate(Ud, torch.from_numpy(dt).to(gpu) if with_dt else None)
+    yd = torch.empty_like(Ud)
+    assert mv.apply(torch.from_numpy(pv).to(gpu), yd) == 0
+    y = yd.cpu().numpy()
+    assert np.linalg.norm(y - ref) <= 1e-7 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_h1_transpose_dot(gpu, precision):
+    """<q, H1 p> = <H1^T q, p>, <q, P p> = <P^T q, p>, and the transposed GMRES
+    solves H1^T x = b."""
+    import torch
+    from aerof_dev_amd.operators import GmresSolver, MatVecProdH1
+    m, p, U, op, om = _setup(gpu, channel=True)
+    dt = om.timestep(om.residual(U)["V"], 50.0)
+    Ud = torch.from_numpy(U).to(gpu)
+    h1 = MatVecProdH1(op)
+    h1.evaluate(Ud, torch.from_numpy(dt).to(gpu), precision)
+    rng = np.random.default_rng(12)
+    pv = torch.from_numpy(rng.normal(size=U.shape)).to(gpu)
+    qv = torch.from_numpy(rng.normal(size=U.shape)).to(gpu)
+    Hp, HTq, Pp, PTq = (torch.empty_like(Ud) for _ in range(4))
+    h1.apply(pv, Hp)
+    h1.apply_transpose(qv, HTq)
+    h1.prec_apply(pv, Pp)
+    h1.prec_apply_transpose(qv, PTq)
+    torch.cuda.synchronize()
+    a, b = float((qv * Hp).sum()), float((HTq * pv).sum())
+    assert abs(a - b) <= 1e-12 * float(qv.abs().mul(Hp.abs()).sum())
+    a, b = float((qv * Pp).sum()), float((PTq * pv).sum())
+    assert abs(a - b) <= 1e-12 * float(qv.abs().mul(Pp.abs()).sum())
+    x = torch.empty_like(Ud)
+    its, res = GmresSolver(op, 30, 1e-8, 300).solveT(qv, x)
+    HTx = torch.empty_like(Ud)
+    h1.apply_transpose(x, HTx)
+    torch.cuda.synchronize()
+    assert float((HTx - qv).norm()) <= 1e-7 * float(qv.norm())

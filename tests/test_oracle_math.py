@@ -254,3 +254,45 @@ def test_gmres_oracle_against_dense_solve():
     x, its, res = om.gmres(V, dt, b, precision=0, m=60, eps=1e-12, maxits=60)
     xd = np.linalg.solve(H, b.ravel())
     np.testing.assert_allclose(x.ravel(), xd, rtol=1e-8, atol=1e-10 * np.abs(xd).max())
+
+
+def test_timestep_closed_form_at_rest():
+    """At rest every spectral-radius term is -c S, so dt_i = cfl V_i / (c (sum_edges S + sum_faces S/3))."""
+    m = M.box_mesh(4, jitter=0.15, seed=8)
+    p = FlowParams(mach=0.0)
+    om = O.OracleMesh(m, p)
+    tb = om.tables()
+    rho, pr = 1.3, 0.9
+    V = np.tile([rho, 0.0, 0.0, 0.0, pr], (m.num_nodes, 1))
+    c = np.sqrt(GAM * pr / rho)
+    S = np.zeros(m.num_nodes)
+    Se = np.linalg.norm(tb["edge_normals"], axis=1)
+    np.add.at(S, tb["ptr"][:, 0], Se)
+    np.add.at(S, tb["ptr"][:, 1], Se)
+    Sf = np.linalg.norm(tb["face_normals"], axis=1)
+    for q in range(3):
+        np.add.at(S, m.faces[:, q], Sf / 3)
+    dt = om.timestep(V, 7.0)
+    np.testing.assert_allclose(dt, 7.0 * tb["ctrl_vol"] / (c * S), rtol=1e-13)
+
+
+def test_h1_equals_fd_jacobian_at_uniform_state():
+    """At equal left/right states the frozen-Roe first-order Jacobian is the
+    exact derivative of the first-order residual, so H1 p must match central
+    differences of the residual for any p (faces: wall analytic, farfield by
+    the reference's own FD, hence the 1e-6 tolerance).  A misplaced A_ij/A_ji,
+    a missing 1/V or a sign error fails by O(1)."""
+    m = M.box_mesh(4, jitter=0.15, seed=9)
+    p = FlowParams(mach=0.5, second_order=False)
+    om = O.OracleMesh(m, p)
+    Vu = np.array([1.1, 0.3, 0.1, -0.05, 0.8])
+    V = np.tile(Vu, (m.num_nodes, 1))
+    U = np.tile(cons(Vu), (m.num_nodes, 1))
+    D, A = om.jacobian(V)
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal(U.shape)
+        y = om.h1_apply(D, A, d)
+        fd = (om.residual(U + h * d)["R"] - om.residual(U - h * d)["R"]) / (2 * h)
+        assert np.linalg.norm(y - fd) <= 1e-6 * np.linalg.norm(fd)
