@@ -63,6 +63,16 @@ class BlockStats(C.Structure):
                 ("seconds", C.c_double), ("tune_waves_dots", C.c_int), ("tune_waves_update", C.c_int)]
 
 
+class AdjointOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("cfl", C.c_double), ("max_its", C.c_int), ("eps", C.c_double),
+                ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double)]
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("its", C.c_int), ("converged", C.c_int), ("krylov_its", C.c_int), ("res0", C.c_double),
+                ("res", C.c_double), ("seconds", C.c_double)]
+
+
 class NewtonStats(C.Structure):
     _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("krylov_its", C.c_int),
                 ("res0", C.c_double), ("res", C.c_double), ("final_cfl", C.c_double),
@@ -110,6 +120,9 @@ SIGNATURES = {
                                      C.c_int, C.c_int, C.POINTER(BlockStats)]),
     "aerofx_mesh_color_edges_gpu": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
     "aerofx_mesh_rcm_gpu": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
+    "aerofx_wall_force": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p, C.c_void_p, C.c_void_p]),
+    "aerofx_adjoint_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(AdjointOpts),
+                                       C.POINTER(AdjointStats)]),
     "aerofx_adjoint_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_steady_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats),
                                       c_double_p, C.c_int]),

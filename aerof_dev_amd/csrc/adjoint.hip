@@ -43,6 +43,8 @@ struct Adjoint {
   double *R = nullptr;    // [N][5] forward residual (scaled)
   double *TX = nullptr;   // [T][12]
   double *FX = nullptr;   // [F][9]
+  double *Jf = nullptr;   // [F] per-face objective values, then the sum at [F]
+  double *h_J = nullptr;  // pinned
   int *tets = nullptr;    // [T][4]
   int *tedge = nullptr;   // [T][6]
   int *faces = nullptr;   // [F][3]
@@ -659,6 +661,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_xgather(DevTables t, const do
 // host side
 // ---------------------------------------------------------------------------
 void free_adjoint(aerofx_ctx *c) {
+  if (c->adj && c->adj->h_J) (void)hipHostFree(c->adj->h_J);
   delete c->adj;  // device memory is owned by the context allocation list
   c->adj = nullptr;
 }
@@ -709,12 +712,17 @@ int ensure_adjoint(aerofx_ctx *c) {
       (rc = alloc_n(c, 16 * (size_t)N, &a->Gb)) || (rc = alloc_n(c, 4 * (size_t)N, &a->Xb)) ||
       (rc = alloc_n(c, 6 * (size_t)N, &a->S)) || (rc = alloc_n(c, 5 * (size_t)N, &a->R)) ||
       (rc = alloc_n(c, 12 * (size_t)std::max(T, 1), &a->TX)) ||
-      (rc = alloc_n(c, 9 * (size_t)std::max(F, 1), &a->FX)) || (rc = upload_vec(c, c->tets, &a->tets)) ||
+      (rc = alloc_n(c, 9 * (size_t)std::max(F, 1), &a->FX)) || (rc = alloc_n(c, (size_t)F + 1, &a->Jf)) || (rc = upload_vec(c, c->tets, &a->tets)) ||
       (rc = upload_vec(c, c->tet_edges, &a->tedge)) || (rc = upload_vec(c, c->faces, &a->faces)) ||
       (rc = upload_vec(c, ntrow, &a->ntrow)) || (rc = upload_vec(c, ntslot, &a->ntslot)) ||
       (rc = upload_vec(c, nfslot, &a->nfslot))) {
     delete a;
     return rc;
+  }
+  hipError_t e = hipHostMalloc((void **)&a->h_J, sizeof(double), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete a;
+    return set_hip_error(e, "adjoint set-up");
   }
   c->adj = a;
   return 0;
@@ -764,6 +772,115 @@ int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub
   return e == hipSuccess ? 0 : set_hip_error(e, "adjoint launch");
 }
 
+// ---------------------------------------------------------------------------
+// force objective on slip-wall faces: J = sum_f sum_{v in f} p_v (n_f / 3) . d
+// (the wall flux carries p n, FluxFcnDescSG.C:779-789, so this is the
+// pressure force on the wall projected on d)
+// ---------------------------------------------------------------------------
+// per face: J_f, and the coordinate derivative of J_f (9 slots)
+__global__ __launch_bounds__(kAdjBlock) void k_lift_faces(int F, const int *__restrict__ faces,
+                                                          const int *__restrict__ fcode,
+                                                          const double *__restrict__ fnorm3,
+                                                          const double *__restrict__ X4,
+                                                          const double *__restrict__ V, double d0, double d1,
+                                                          double d2, double *__restrict__ Jf,
+                                                          double *__restrict__ FX) {
+  const int f = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (f >= F) return;
+  const int code = fcode[f];
+  const bool wall = code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL;
+  int nd[3];
+  double psum = 0.0;
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    nd[v] = faces[3 * (size_t)f + v];
+    psum += V[5 * (size_t)nd[v] + 4];
+  }
+  if (!wall) psum = 0.0;
+  const double *nv = fnorm3 + 3 * (size_t)f;
+  Jf[f] = psum * (nv[0] * d0 + nv[1] * d1 + nv[2] * d2);
+  if (!FX) return;
+  // d J_f / d x: J_f = psum (1/3) (0.5 (x2 - x0) x (x1 - x0)) . d
+  double x[3][3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const double4 xv = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)nd[v]);
+    x[v][0] = xv.x;
+    x[v][1] = xv.y;
+    x[v][2] = xv.z;
+  }
+  const double third = 1.0 / 3.0;
+#pragma unroll 1
+  for (int dir = 0; dir < 9; ++dir) {
+    Dual p[3], q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      p[a] = Dual(x[2][a], seed(dir, 6 + a)) - Dual(x[0][a], seed(dir, a));
+      q[a] = Dual(x[1][a], seed(dir, 3 + a)) - Dual(x[0][a], seed(dir, a));
+    }
+    Dual c0 = p[1] * q[2] - p[2] * q[1];
+    Dual c1 = p[2] * q[0] - p[0] * q[2];
+    Dual c2 = p[0] * q[1] - p[1] * q[0];
+    FX[9 * (size_t)f + dir] = psum * third * 0.5 * (c0.d * d0 + c1.d * d1 + c2.d * d2);
+  }
+}
+
+// per node: dJ/dU (through p(U)) and dJ/dX (face slots)
+__global__ __launch_bounds__(kAdjBlock) void k_lift_nodes(DevTables t, const double *__restrict__ U, Gas gas,
+                                                          const int *__restrict__ nfslot,
+                                                          const double *__restrict__ FX, double d0, double d1,
+                                                          double d2, double *__restrict__ dJdU,
+                                                          double *__restrict__ dJdX) {
+  const int n = blockIdx.x * kAdjBlock + threadIdx.x;
+  if (n >= t.N) return;
+  double w = 0.0, xb[3] = {0.0, 0.0, 0.0};
+  for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
+    const int f = t.fidx[q];
+    const int code = t.fcode[f];
+    if (code != AEROFX_BC_SLIP_WALL && code != -AEROFX_BC_SLIP_WALL) continue;
+    const double *nv = t.fnorm3 + 3 * (size_t)f;
+    w += nv[0] * d0 + nv[1] * d1 + nv[2] * d2;
+    if (dJdX) {
+      const size_t sl = nfslot[q];
+      const size_t ff = sl / 3, v = sl - 3 * ff;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) xb[a] += FX[9 * ff + 3 * v + a];
+    }
+  }
+  if (dJdU) {
+    double u[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] = U[5 * (size_t)n + k];
+#pragma unroll 1
+    for (int dir = 0; dir < 5; ++dir) {
+      Dual ud[5], vd[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) ud[k] = Dual(u[k], seed(dir, k));
+      prim_from_cons(ud, gas, vd);
+      dJdU[5 * (size_t)n + dir] = w * vd[4].d;
+    }
+  }
+  if (dJdX)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dJdX[3 * (size_t)n + a] = xb[a];
+}
+
+// J = sum of the per-face values in face order (one block)
+__global__ void k_sum_ordered(int F, const double *__restrict__ Jf, double *__restrict__ out) {
+  __shared__ double sh[256];
+  const int per = (F + 255) / 256;
+  const int b0 = threadIdx.x * per, b1 = min(F, b0 + per);
+  double s = 0.0;
+  for (int f = b0; f < b1; ++f) s += Jf[f];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int k = 0; k < 256; ++k) tot += sh[k];
+    *out = tot;
+  }
+}
+
 }  // namespace aerofx
 
 using namespace aerofx;
@@ -776,4 +893,30 @@ extern "C" int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const d
   if (rc) return rc;
   hipError_t e = hipStreamSynchronize(ctx->stream);
   return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "adjoint_residual");
+}
+
+extern "C" int aerofx_wall_force(aerofx_ctx *ctx, const double *U, const double dir[3], double *J,
+                                 double *dJdU, double *dJdX) {
+  if (!ctx || !U || !dir || !J) return set_error(AEROFX_ERR_ARG, "wall_force: null argument");
+  int rc = ensure_adjoint(ctx);
+  if (rc) return rc;
+  Adjoint *a = ctx->adj;
+  const DevTables &t = ctx->tab;
+  if ((rc = launch_primitive(ctx, U, ctx->d_V))) return rc;
+  if (ctx->F > 0) {
+    hipLaunchKernelGGL(k_lift_faces, grid_for(ctx->F), dim3(kAdjBlock), 0, ctx->stream, ctx->F, a->faces, t.fcode,
+                       t.fnorm3, t.X4, ctx->d_V, dir[0], dir[1], dir[2], a->Jf, dJdX ? a->FX : nullptr);
+    hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, ctx->stream, ctx->F, a->Jf, a->Jf + ctx->F);
+  } else {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, (int *)(a->Jf), 2);
+  }
+  if (dJdU || dJdX)
+    hipLaunchKernelGGL(k_lift_nodes, grid_for(t.N), dim3(kAdjBlock), 0, ctx->stream, t, U, ctx->gas, a->nfslot,
+                       a->FX, dir[0], dir[1], dir[2], dJdU, dJdX);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(a->h_J, a->Jf + ctx->F, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return set_hip_error(e, "wall_force");
+  *J = *a->h_J;
+  return AEROFX_OK;
 }

@@ -89,16 +89,19 @@ __global__ void k_from_block(size_t n, int nrhs, const double *__restrict__ src,
 // ---------------------------------------------------------------------------
 // batched operators: one thread per (node, column)
 // ---------------------------------------------------------------------------
+// acc += A p[:, c] for one interleaved 5x5 block (element q at a[q * 64]),
+// in the single-vector operators' order
 template <class T>
 __device__ __forceinline__ void bmv5_add(const T *__restrict__ a, const double *__restrict__ p, int c,
                                          double acc[5]) {
-  double b[5];
+  double b[5], r[25];
 #pragma unroll
   for (int k = 0; k < 5; ++k) b[k] = p[k * kS + c];
 #pragma unroll
+  for (int k = 0; k < 25; ++k) r[k] = (double)a[k * kBlkStride];
+#pragma unroll
   for (int i = 0; i < 5; ++i)
-    acc[i] += (double)a[5 * i] * b[0] + (double)a[5 * i + 1] * b[1] + (double)a[5 * i + 2] * b[2] +
-              (double)a[5 * i + 3] * b[3] + (double)a[5 * i + 4] * b[4];
+    acc[i] += r[5 * i] * b[0] + r[5 * i + 1] * b[1] + r[5 * i + 2] * b[2] + r[5 * i + 3] * b[3] + r[5 * i + 4] * b[4];
 }
 
 template <class T>
@@ -108,10 +111,10 @@ __global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, const T *__restr
   const int n = gid / kS, c = gid % kS;
   if (n >= t.N) return;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  bmv5_add(D + 25 * (size_t)n, P + (size_t)n * 5 * kS, c, acc);
+  bmv5_add(D + blk64_base(n), P + (size_t)n * 5 * kS, c, acc);
   for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
     const int m = t.adj[e];
-    bmv5_add(B + 25 * (size_t)e, P + (size_t)m * 5 * kS, c, acc);
+    bmv5_add(B + blk64_base(e), P + (size_t)m * 5 * kS, c, acc);
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = acc[k];
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(256) void k_bjacobi_apply(int N, const T *__restric
   const int n = gid / kS, c = gid % kS;
   if (n >= N) return;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  bmv5_add(Di + 25 * (size_t)n, P + (size_t)n * 5 * kS, c, acc);
+  bmv5_add(Di + blk64_base(n), P + (size_t)n * 5 * kS, c, acc);
 #pragma unroll
   for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = acc[k];
 }
@@ -150,7 +153,7 @@ __global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__
 // ---------------------------------------------------------------------------
 // partial[c][b] = sum over the rows of chunk c of V_b^T W (16x16 tiles).
 // v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
-// holds D[4(l/16)+r][l%16], r < 4.  Here A = V_b^T over 4 rows, B = W over
+// holds D[l/16 + 4r][l%16], r < 4 (checked on gfx950).  Here A = V_b^T over 4 rows, B = W over
 // the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
 __global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nb, const double *__restrict__ V, size_t ldv,
                            const double *__restrict__ W, double *__restrict__ partial) {
@@ -175,7 +178,7 @@ __global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nb, const dou
     if (b < nb) {
       double *dst = partial + ((size_t)chunk * (kMaxM + 1) + b) * kTile;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) dst[(4 * rr + r) * kS + cc] = acc[b][r];
+      for (int r = 0; r < 4; ++r) dst[(rr + 4 * r) * kS + cc] = acc[b][r];
     }
 }
 
@@ -204,10 +207,10 @@ __global__ void k_tsg_update(size_t npad, int nb, const double *__restrict__ V, 
     for (int q = 0; q < 4; ++q)
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vb[4 * q + kk], Hb[(4 * q + kk) * kS + ri], acc, 0, 0, 0);
   }
-  // D[4 kk + r][ri] -> row r0 + 4 kk + r, column ri
+  // D[kk + 4 r][ri] -> row r0 + kk + 4 r, column ri
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    double *o = Out + (r0 + 4 * kk + r) * kS + ri;
+    double *o = Out + (r0 + kk + 4 * r) * kS + ri;
     *o = (beta == 0.0 ? 0.0 : beta * *o) + alpha * acc[r];
   }
 }
@@ -325,7 +328,7 @@ struct BlockSolver {
   double *nrm = nullptr;      // [16]
   int *flag = nullptr;
   float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
-  const void *f32_src = nullptr;  // FP64 blocks the FP32 copies were made from
+  long long f32_version = -1;     // assembly the FP32 copies were made from
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -558,8 +561,8 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
 }
 
 int ensure_f32(aerofx_ctx *c, BlockSolver *s) {
-  if (s->D32 && s->f32_src == c->d_jacD) return 0;
-  const size_t nd = 25 * (size_t)c->N, nbk = 50 * (size_t)c->E;
+  if (s->D32 && s->f32_version == c->jac_version) return 0;
+  const size_t nd = blk64_count(c->N), nbk = blk64_count(2 * (size_t)c->E);
   void *v = nullptr;
   int rc;
   if (!s->D32) {
@@ -575,7 +578,7 @@ int ensure_f32(aerofx_ctx *c, BlockSolver *s) {
   // the FP32 block-Jacobi inverse is LINPACK in float on the FP32 blocks,
   // as the single-RHS mixed path builds it
   if ((rc = launch_invert_f32(c, s->stream, s->D32, s->Di32))) return rc;
-  s->f32_src = c->d_jacD;
+  s->f32_version = c->jac_version;
   return 0;
 }
 

@@ -13,6 +13,22 @@ namespace aerofx {
 constexpr int kBlock = 256;       // threads per block of the edge/entry kernels
 constexpr int kMaxRows = 64;      // node rows per block (LDS staging bound)
 constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, blocks dealt round-robin
+constexpr int kPad = kBlock + 1;  // LDS row stride (odd: conflict-free)
+
+// Block b of a launch runs on XCD b % 8; remap so that each XCD sweeps one
+// contiguous range of chunks (its L2 then holds that range's neighbours).
+__device__ __forceinline__ int xcd_chunk(int b, int nchunks) {
+  const int q = nchunks / kNumXcd, r = nchunks % kNumXcd;
+  const int x = b % kNumXcd, k = b / kNumXcd;
+  return x * q + min(x, r) + k;
+}
+
+// 5x5 block storage interleaved in groups of 64 blocks: element q of block e
+// at (e / 64) * 1600 + q * 64 + e % 64, so that 64 lanes reading element q of
+// 64 consecutive blocks touch one contiguous 512 B (FP64) segment.
+__host__ __device__ inline size_t blk64_base(size_t e) { return (e >> 6) * (64 * 25) + (e & 63); }
+constexpr int kBlkStride = 64;
+inline size_t blk64_count(size_t nblocks) { return (nblocks + 63) / 64 * 64 * 25; }
 
 // Device-resident mesh tables.  All arrays are laid out for the kernel that
 // streams them: contiguous per node row or per edge.
@@ -80,7 +96,9 @@ struct aerofx_ctx {
   int *h_err = nullptr;
   // implicit operators (jacobian.hip) and Krylov workspace (krylov.hip)
   int jac_precision = -1;    // 0: FP64 blocks, 1: FP32 blocks
-  void *d_jacD = nullptr, *d_jacB = nullptr, *d_jacDi = nullptr;
+  void *d_jacD = nullptr, *d_jacB = nullptr, *d_jacDi = nullptr;  // current precision's
+  void *jac_store[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [precision][D, B, Di]
+  long long jac_version = 0;  // bumped at every assembly
   double *d_dt = nullptr;
   aerofx::Krylov *ksp = nullptr;
   aerofx::Adjoint *adj = nullptr;
@@ -131,6 +149,8 @@ int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *d
 int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done, bool trans = false);
 void free_krylov(aerofx_ctx *c);
 void free_adjoint(aerofx_ctx *c);
+// (dR/dU)^T lam -> Ub and/or (dR/dX)^T lam -> Xout (adjoint.hip), enqueued on c->stream
+int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout);
 void free_block(aerofx_ctx *c);
 // LINPACK inverse of FP32 5x5 blocks on a given stream (error count in d_err[1])
 int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *Di);

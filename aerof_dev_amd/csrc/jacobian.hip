@@ -4,12 +4,15 @@
 // JacobiPrec surface), with FP64 or FP32 block storage.
 //
 // H1 layout (per context, storage type T = double or float):
-//   D  [N][25]    diagonal blocks: sum of edge and face Jacobians, scaled by
+//   D  [N]        diagonal blocks: sum of edge and face Jacobians, scaled by
 //                 1/V, plus alpha/dt on the diagonal
-//   B  [2E][25]   one off-diagonal block per adjacency entry, in the node's
+//   B  [2E]       one off-diagonal block per adjacency entry, in the node's
 //                 entry order (= ascending edge id): entry (n, m) holds
 //                 A_nm, i.e. A_ij of the edge for n = i and A_ji for n = j
-//   Di [N][25]    inverse of D (block-Jacobi), LINPACK in precision T
+//   Di [N]        inverse of D (block-Jacobi), LINPACK in precision T
+// Each array holds 5x5 row-major blocks interleaved in groups of 64
+// (blk64_base): thread-per-node and thread-per-entry kernels then read 64
+// consecutive blocks' element q as one contiguous segment.
 // All kernels are node-parallel with each node's sums in the reference's
 // order; arithmetic is FP64 (FP32 blocks are widened on load, as
 // DenseMatrixOp<float,5,25> does with double vectors).
@@ -96,10 +99,10 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *
   for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
     const int m = t.adj[e], l = t.eidx[e];
     const double nl[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
-    T *Be = B + 25 * (size_t)e;
+    T *Be = B + blk64_base(e);
     if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) {
 #pragma unroll
-      for (int k = 0; k < 25; ++k) Be[k] = (T)0;
+      for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)0;
       continue;
     }
     double Vm[5];
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *
       for (int k = 0; k < 25; ++k) Dn[k] += blk[k];
       roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk);
 #pragma unroll
-      for (int k = 0; k < 25; ++k) Be[k] = (T)(0.0 + blk[k] * voln);
+      for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 + blk[k] * voln);
     } else {
       // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
       roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk);
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *
       for (int k = 0; k < 25; ++k) Dn[k] -= blk[k];
       roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk);
 #pragma unroll
-      for (int k = 0; k < 25; ++k) Be[k] = (T)(0.0 - blk[k] * voln);
+      for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 - blk[k] * voln);
     }
   }
   for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
@@ -146,9 +149,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *
 #pragma unroll
     for (int k = 0; k < 5; ++k) Dn[6 * k] += c_np1;
   }
-  T *Dd = D + 25 * (size_t)n;
+  T *Dd = D + blk64_base(n);
 #pragma unroll
-  for (int k = 0; k < 25; ++k) Dd[k] = (T)Dn[k];
+  for (int k = 0; k < 25; ++k) Dd[k * kBlkStride] = (T)Dn[k];
 }
 
 template <class T>
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_invert(int N, const T *__restrict__ 
   if (n >= N) return;
   T a[25];
 #pragma unroll
-  for (int k = 0; k < 25; ++k) a[k] = D[25 * (size_t)n + k];
+  for (int k = 0; k < 25; ++k) a[k] = D[blk64_base(n) + k * kBlkStride];
   // DiagMat::invert (DiagMatrix.C:44-82): blocks with a zero diagonal entry
   // are left as they are
   bool inv = true;
@@ -167,15 +170,16 @@ __global__ __launch_bounds__(kBlock) void k_invert(int N, const T *__restrict__ 
     if (a[6 * k] == (T)0) inv = false;
   if (inv && invert5<T>(a) != 0) atomicAdd(err, 1);
 #pragma unroll
-  for (int k = 0; k < 25; ++k) Di[25 * (size_t)n + k] = a[k];
+  for (int k = 0; k < 25; ++k) Di[blk64_base(n) + k * kBlkStride] = a[k];
 }
 
-// c += A b (DENSEMATRIXTIMESVECTOR5, DenseMatrixOps.h:441-446)
+// c += A b (DENSEMATRIXTIMESVECTOR5, DenseMatrixOps.h:441-446); a points at
+// element 0 of an interleaved block
 template <class T>
 __device__ __forceinline__ void amv5_add(const T *__restrict__ a, const double b[5], double c[5]) {
   double r[25];
 #pragma unroll
-  for (int k = 0; k < 25; ++k) r[k] = (double)a[k];
+  for (int k = 0; k < 25; ++k) r[k] = (double)a[k * kBlkStride];
 #pragma unroll
   for (int i = 0; i < 5; ++i)
     c[i] += r[5 * i] * b[0] + r[5 * i + 1] * b[1] + r[5 * i + 2] * b[2] + r[5 * i + 3] * b[3] + r[5 * i + 4] * b[4];
@@ -186,7 +190,7 @@ template <class T>
 __device__ __forceinline__ void amtv5_add(const T *__restrict__ a, const double b[5], double c[5]) {
   double r[25];
 #pragma unroll
-  for (int k = 0; k < 25; ++k) r[k] = (double)a[k];
+  for (int k = 0; k < 25; ++k) r[k] = (double)a[k * kBlkStride];
 #pragma unroll
   for (int i = 0; i < 5; ++i)
     c[i] += r[i] * b[0] + r[5 + i] * b[1] + r[10 + i] * b[2] + r[15 + i] * b[3] + r[20 + i] * b[4];
@@ -195,33 +199,57 @@ __device__ __forceinline__ void amtv5_add(const T *__restrict__ a, const double 
 // y = H1 p (SubDomain::computeMatVecProdH1, SubDomain.C:3917-3953), or with
 // TRANS y = H1^T p (SubDomain::computeMatVecProdH1transpose, SubDomain.C:3958-3990):
 // row n gathers A_mn^T p_m from the mirror entry (m, n), in edge order.
+// A block covers whole rows (<= 256 entries, <= 64 rows, the gradient-pass
+// chunks): lane k multiplies entry e0 + k (64 consecutive entries read as
+// contiguous segments), the products go through LDS, and each row sums its
+// diagonal product and then its entries in order -- the sums of the
+// reference's node loop followed by its edge loop.
 template <class T, bool TRANS>
 __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
                                                      const double *__restrict__ p, double *__restrict__ y,
                                                      const int *__restrict__ done) {
   if (done && *done) return;
-  const int n = blockIdx.x * kBlock + threadIdx.x;
-  if (n >= t.N) return;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  double pn[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) pn[k] = p[5 * (size_t)n + k];
-  if (TRANS)
-    amtv5_add(D + 25 * (size_t)n, pn, acc);
-  else
-    amv5_add(D + 25 * (size_t)n, pn, acc);
-  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+  __shared__ double sc[5 * kPad];
+  __shared__ double sd[5 * kMaxRows];
+  const int c = xcd_chunk(blockIdx.x, t.ngchunks);
+  const int r0 = t.gchunk[c], r1 = t.gchunk[c + 1];
+  const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
+  const int tid = threadIdx.x;
+  if (tid < e1 - e0) {
+    const int e = e0 + tid;
     const int m = t.adj[e];
-    double pm[5];
+    double pm[5], cc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 5; ++k) pm[k] = p[5 * (size_t)m + k];
     if (TRANS)
-      amtv5_add(B + 25 * (size_t)t.mirror[e], pm, acc);
+      amtv5_add(B + blk64_base(t.mirror[e]), pm, cc);
     else
-      amv5_add(B + 25 * (size_t)e, pm, acc);
-  }
+      amv5_add(B + blk64_base(e), pm, cc);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) y[5 * (size_t)n + k] = acc[k];
+    for (int k = 0; k < 5; ++k) sc[k * kPad + tid] = cc[k];
+  }
+  if (tid < r1 - r0) {
+    const int n = r0 + tid;
+    double pn[5], dd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pn[k] = p[5 * (size_t)n + k];
+    if (TRANS)
+      amtv5_add(D + blk64_base(n), pn, dd);
+    else
+      amv5_add(D + blk64_base(n), pn, dd);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sd[k * kMaxRows + tid] = dd[k];
+  }
+  __syncthreads();
+  const int nr = r1 - r0;
+  for (int idx = tid; idx < nr * 5; idx += kBlock) {
+    const int rr = idx / 5, q = idx - 5 * rr;
+    const int n = r0 + rr;
+    double s = sd[q * kMaxRows + rr];
+    const int b = t.rowptr[n] - e0, eend = t.rowptr[n + 1] - e0;
+    for (int e = b; e < eend; ++e) s += sc[q * kPad + e];
+    y[5 * (size_t)n + q] = s;
+  }
 }
 
 // x = Di y (DiagMat::apply, DiagMatrix.C:86-100), or x = Di^T y
@@ -236,9 +264,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restr
 #pragma unroll
   for (int k = 0; k < 5; ++k) yn[k] = y[5 * (size_t)n + k];
   if (TRANS)
-    amtv5_add(Di + 25 * (size_t)n, yn, acc);
+    amtv5_add(Di + blk64_base(n), yn, acc);
   else
-    amv5_add(Di + 25 * (size_t)n, yn, acc);
+    amv5_add(Di + blk64_base(n), yn, acc);
 #pragma unroll
   for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
 }
@@ -253,16 +281,17 @@ static int check_launch(const char *what) {
 
 int ensure_jacobian_storage(aerofx_ctx *c, int precision) {
   const size_t esz = precision ? sizeof(float) : sizeof(double);
-  const size_t need_b = 25 * esz * 2 * (size_t)c->E, need_d = 25 * esz * (size_t)c->N;
-  if (c->jac_precision == precision && c->d_jacD) return 0;
-  void *p = nullptr;
-  int rc;
-  if ((rc = ctx_alloc(c, need_d, &p))) return rc;
-  c->d_jacD = p;
-  if ((rc = ctx_alloc(c, need_b, &p))) return rc;
-  c->d_jacB = p;
-  if ((rc = ctx_alloc(c, need_d, &p))) return rc;
-  c->d_jacDi = p;
+  const size_t need_b = esz * blk64_count(2 * (size_t)c->E), need_d = esz * blk64_count(c->N);
+  void **st = c->jac_store[precision];
+  if (!st[0]) {
+    int rc;
+    if ((rc = ctx_alloc(c, need_d, &st[0])) || (rc = ctx_alloc(c, need_b, &st[1])) ||
+        (rc = ctx_alloc(c, need_d, &st[2])))
+      return rc;
+  }
+  c->d_jacD = st[0];
+  c->d_jacB = st[1];
+  c->d_jacDi = st[2];
   c->jac_precision = precision;
   return 0;
 }
@@ -276,6 +305,7 @@ int launch_timestep(aerofx_ctx *c, double cfl, double *dt) {
 int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   int rc = ensure_jacobian_storage(c, precision);
   if (rc) return rc;
+  c->jac_version++;
   const int nb = (c->N + kBlock - 1) / kBlock;
   {
     ProfScope ps(c, AEROFX_TIMER_FV_JAC);
@@ -308,7 +338,7 @@ int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *
 
 template <class T>
 static void h1_launch(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
-  const int nb = (c->N + kBlock - 1) / kBlock;
+  const int nb = c->tab.ngchunks;
   if (trans)
     hipLaunchKernelGGL((k_h1_apply<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
                        (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);

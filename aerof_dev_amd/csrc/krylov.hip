@@ -53,37 +53,75 @@ struct GmresState {
 };
 
 // ---------------------------------------------------------------------------
-// deterministic block reduction of K values per thread
+// deterministic block reductions
 // ---------------------------------------------------------------------------
-template <int K>
-__device__ __forceinline__ void block_reduce_store(double (&acc)[K], int nk, double *__restrict__ partial) {
-  __shared__ double sh[K][kBlock / kWave];
-  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+// K = kMaxBasis values per lane: a transposing butterfly halves the values a
+// lane carries at each of the first five exchange steps (a lane keeps the
+// half selected by its lane bit and adds the partner's copy of that half),
+// so after steps 32..2 lane l holds one index, summed over the lanes that
+// share bit 0, and the step 1 exchange completes the wave sum: 32 shuffles
+// instead of 6 per value.  The fixed pattern keeps the sums deterministic.
+template <int HALF, int OFF>
+__device__ __forceinline__ void halve_step(double (&v)[kMaxBasis], int lane) {
+  const bool up = (lane & OFF) != 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double v = acc[k];
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    if (lane == 0) sh[k][wave] = v;
+  for (int i = 0; i < HALF; ++i) {
+    const double keep = up ? v[HALF + i] : v[i];
+    const double give = up ? v[i] : v[HALF + i];
+    v[i] = keep + __shfl_xor(give, OFF, kWave);
   }
+}
+
+__device__ __forceinline__ void block_reduce_store_k(double (&v)[kMaxBasis], int nk, int nparts,
+                                                     double *__restrict__ partial) {
+  static_assert(kMaxBasis == 32 && kWave == 64, "the butterfly below is written for 32 values, 64 lanes");
+  __shared__ double sh[kBlock / kWave][kMaxBasis];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  halve_step<16, 32>(v, lane);
+  halve_step<8, 16>(v, lane);
+  halve_step<4, 8>(v, lane);
+  halve_step<2, 4>(v, lane);
+  halve_step<1, 2>(v, lane);
+  const double tot = v[0] + __shfl_xor(v[0], 1, kWave);
+  // index held by this lane: bits 5..1 of the lane select halves 16, 8, 4, 2, 1
+  const int idx = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
+                  ((lane >> 1) & 1);
+  if ((lane & 1) == 0) sh[wave][idx] = tot;
   __syncthreads();
   if ((int)threadIdx.x < nk) {
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kBlock / kWave; ++w) s += sh[threadIdx.x][w];
+    for (int w = 0; w < kBlock / kWave; ++w) s += sh[w][threadIdx.x];
     partial[(size_t)threadIdx.x * kDotBlocks + blockIdx.x] = s;
+  }
+  (void)nparts;
+}
+
+__device__ __forceinline__ void block_reduce_store_1(double v, double *__restrict__ partial) {
+  __shared__ double sh1[kBlock / kWave];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  if (lane == 0) sh1[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) s += sh1[w];
+    partial[blockIdx.x] = s;
   }
 }
 
 // mode 0: partial_k = sum w V_k
 // mode 1: w -= sum_k h_k V_k, then partial_k = sum w V_k
 // mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
+// The grid has nparts blocks (fixed by n); block b sums a contiguous slice.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__ w, const double *__restrict__ V,
                                                   size_t ldv, int ny, const double *__restrict__ h,
                                                   double *__restrict__ partial, const int *__restrict__ done) {
   if (done && *done) return;
-  const size_t slice = (n + kDotBlocks - 1) / kDotBlocks;
+  const size_t slice = (n + gridDim.x - 1) / gridDim.x;
   const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
   double acc[kMaxBasis];
 #pragma unroll
@@ -111,17 +149,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
       for (int k = 0; k < kMaxBasis; ++k) acc[k] += wi * v[k];
     }
   }
-  block_reduce_store<kMaxBasis>(acc, MODE == 2 ? 1 : ny, partial);
+  if (MODE == 2)
+    block_reduce_store_1(acc[0], partial);
+  else
+    block_reduce_store_k(acc, ny, gridDim.x, partial);
 }
 
 // out[k] = sum_b partial[k][b] in a fixed tree, one block per k
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial,
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial, int nparts,
                                                             double *__restrict__ out, const int *__restrict__ done) {
   if (done && *done) return;
   __shared__ double sh[kBlock];
   const int k = blockIdx.x;
   double s = 0.0;
-  for (int b = threadIdx.x; b < kDotBlocks; b += kBlock) s += partial[(size_t)k * kDotBlocks + b];
+  for (int b = threadIdx.x; b < nparts; b += kBlock) s += partial[(size_t)k * kDotBlocks + b];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int st = kBlock / 2; st > 0; st >>= 1) {
@@ -179,29 +220,53 @@ __global__ void k_gmres_start(GmresState *s, const double *__restrict__ nrm2, in
 __global__ void k_gmres_givens(GmresState *s, int j) {
   if (threadIdx.x != 0 || s->done) return;
   const int M = kMaxBasis;
-  double *H = s->H;
-  for (int i = 0; i <= j; ++i) H[i * M + j] = s->dots[i] + s->dots[kMaxBasis + i];
+  // column j in registers (indices are compile-time after unrolling)
+  double h[kMaxBasis + 1];
+#pragma unroll
+  for (int i = 0; i <= kMaxBasis; ++i) h[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i <= j) h[i] = s->dots[i] + s->dots[kMaxBasis + i];
   const double hn = sqrt(s->dots[2 * kMaxBasis]);
-  H[(j + 1) * M + j] = hn;
-  for (int i = 0; i < j; ++i) {
-    double hj = s->cs[2 * i + 1] * H[i * M + j] + s->cs[2 * i] * H[(i + 1) * M + j];
-    double hjp1 = -s->cs[2 * i] * H[i * M + j] + s->cs[2 * i + 1] * H[(i + 1) * M + j];
-    H[i * M + j] = hj;
-    H[(i + 1) * M + j] = hjp1;
-  }
-  const double hjj = H[j * M + j], hj1 = H[(j + 1) * M + j];
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i == j + 1) h[i] = hn;
+  if (j + 1 == kMaxBasis) h[kMaxBasis] = hn;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis - 1; ++i)
+    if (i < j) {
+      const double sn = s->cs[2 * i], cs = s->cs[2 * i + 1];
+      const double hj = cs * h[i] + sn * h[i + 1];
+      const double hjp1 = -sn * h[i] + cs * h[i + 1];
+      h[i] = hj;
+      h[i + 1] = hjp1;
+    }
+  double hjj = 0.0, hj1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i == j) {
+      hjj = h[i];
+      hj1 = h[i + 1];
+    }
+  double sn, cs;
   if (hjj * hjj == 0.0) {
-    s->cs[2 * j + 1] = 0.0;
-    s->cs[2 * j] = 1.0;
+    cs = 0.0;
+    sn = 1.0;
   } else {
     double coef = 1.0 / sqrt(hjj * hjj + hj1 * hj1);
-    s->cs[2 * j + 1] = sqrt(hjj * hjj) * coef;
-    s->cs[2 * j] = s->cs[2 * j + 1] * hj1 / hjj;
+    cs = sqrt(hjj * hjj) * coef;
+    sn = cs * hj1 / hjj;
   }
-  H[j * M + j] = s->cs[2 * j + 1] * hjj + s->cs[2 * j] * hj1;
-  H[(j + 1) * M + j] = 0.0;
-  s->g[j + 1] = -s->cs[2 * j] * s->g[j];
-  s->g[j] = s->cs[2 * j + 1] * s->g[j];
+  s->cs[2 * j + 1] = cs;
+  s->cs[2 * j] = sn;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i < j) s->H[i * M + j] = h[i];
+  s->H[j * M + j] = cs * hjj + sn * hj1;
+  s->H[(j + 1) * M + j] = 0.0;
+  const double gj = s->g[j];
+  s->g[j + 1] = -sn * gj;
+  s->g[j] = cs * gj;
   s->iter += 1;
   s->jlast = j;
   if (hn == 0.0) {
@@ -245,9 +310,9 @@ __global__ __launch_bounds__(kBlock) void k_eps_sum(size_t n, const double *__re
   const size_t slice = (n + kDotBlocks - 1) / kDotBlocks;
   const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
   const double eps0 = 1.e-6;
-  double acc[1] = {0.0};
-  for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) acc[0] += eps0 * fabs(U[i]) + eps0;
-  block_reduce_store<1>(acc, 1, partial);
+  double acc = 0.0;
+  for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) acc += eps0 * fabs(U[i]) + eps0;
+  block_reduce_store_1(acc, partial);
 }
 
 // scal[0] = p.p, scal[1] = sum(eps0 |U| + eps0) -> scal[2] = eps, [3] = 1/eps, [4] = 0.5/eps
@@ -293,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_fd_diff(size_t n, const double *__re
 struct Krylov {
   int m = 0;
   size_t n = 0;
+  int nparts = 0;             // blocks of the dot-product sweeps (fixed by n)
   double *V = nullptr;        // [m+1][n] basis
   double *w = nullptr, *z = nullptr, *r = nullptr, *x = nullptr, *b = nullptr, *R = nullptr;
   double *partial = nullptr;  // [kMaxBasis][kDotBlocks]
@@ -317,6 +383,8 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
   Krylov *k = new Krylov();
   k->m = m;
   k->n = 5 * (size_t)c->N;
+  // about 8 entries per lane per sweep, between 64 and kDotBlocks blocks
+  k->nparts = (int)std::min<size_t>(kDotBlocks, std::max<size_t>(64, (k->n + 8 * kBlock - 1) / (8 * kBlock)));
   void *p = nullptr;
   int rc;
   if ((rc = ctx_alloc(c, sizeof(double) * (size_t)(m + 1) * k->n, &p))) { delete k; return rc; }
@@ -345,10 +413,10 @@ template <int MODE>
 static void sweep(aerofx_ctx *c, double *w, const double *V, size_t ldv, int ny, const double *h, double *out,
                   const int *done) {
   Krylov *k = c->ksp;
-  hipLaunchKernelGGL((k_sweep<MODE>), dim3(kDotBlocks), dim3(kBlock), 0, c->stream, k->n, w, V, ldv, ny, h,
+  hipLaunchKernelGGL((k_sweep<MODE>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, V, ldv, ny, h,
                      k->partial, done);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(MODE == 2 ? 1 : ny), dim3(kBlock), 0, c->stream, k->partial, out,
-                     done);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(MODE == 2 ? 1 : ny), dim3(kBlock), 0, c->stream, k->partial,
+                     k->nparts, out, done);
 }
 
 static int fetch_state(aerofx_ctx *c) {
@@ -441,7 +509,8 @@ static int jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *prod,
   // computeEpsilon
   sweep<2>(c, const_cast<double *>(p), nullptr, 0, 0, nullptr, k->scal, nullptr);
   hipLaunchKernelGGL(k_eps_sum, dim3(kDotBlocks), dim3(kBlock), 0, c->stream, n, U, k->partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->scal + 1, nullptr);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, kDotBlocks, k->scal + 1,
+                     nullptr);
   hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, c->stream, k->scal, (double)n);
   double *Fa = k->w, *Fb = k->z, *Q = k->r;
   // F(U + eps p)
@@ -478,6 +547,69 @@ int aerofx_jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *Jp, i
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "jvp_fd sync");
   return c->h_err[0];
+}
+
+int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, double *psi,
+                         const aerofx_adjoint_opts *o, aerofx_adjoint_stats *st) {
+  if (!c || !U || !rhs || !psi || !o || !st) return set_error(AEROFX_ERR_ARG, "adjoint_solve: null argument");
+  if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "adjoint_solve: precision 0 or 1");
+  int rc = ensure_krylov(c, o->gmres_m);
+  if (rc) return rc;
+  Krylov *k = c->ksp;
+  const size_t n = k->n;
+  auto t0 = std::chrono::steady_clock::now();
+  *st = aerofx_adjoint_stats{};
+  // A = H1 + diag(1/dt) at U (pseudo-time step of the given CFL), its
+  // block-Jacobi inverse in the requested precision
+  if ((rc = launch_primitive(c, U, c->d_V))) return rc;
+  if (o->cfl > 0.0) {
+    if ((rc = launch_timestep(c, o->cfl, c->d_dt))) return rc;
+  }
+  if ((rc = launch_jacobian(c, o->cfl > 0.0 ? c->d_dt : nullptr, o->precision))) return rc;
+  // ||rhs||
+  sweep<2>(c, const_cast<double *>(rhs), nullptr, 0, 0, nullptr, k->scal, nullptr);
+  hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
+  const double rnorm0 = std::sqrt(k->h_scal[0]);
+  hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, c->stream, n, 0.0, psi);
+  double res = rnorm0;
+  int it;
+  for (it = 0;; ++it) {
+    // r = rhs - (dR/dU)^T psi (FP64, exact second-order adjoint)
+    if (it == 0) {
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, rhs, 0.0, k->b,
+                         nullptr);
+    } else {
+      if ((rc = launch_adjoint(c, U, psi, k->R, nullptr))) return rc;
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, rhs, 0.0, k->b,
+                         nullptr);
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 1.0, k->b,
+                         nullptr);
+    }
+    sweep<2>(c, k->b, nullptr, 0, 0, nullptr, k->scal, nullptr);
+    e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
+    res = std::sqrt(k->h_scal[0]);
+    if (res <= o->eps * rnorm0 || rnorm0 == 0.0) {
+      st->converged = 1;
+      break;
+    }
+    if (it >= o->max_its) break;
+    // A^T d = r with the transposed H1 and block-Jacobi
+    GmresOut go;
+    if ((rc = gmres(c, k->b, k->x, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go, true))) return rc;
+    st->krylov_its += go.its;
+    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->x, 1.0, psi, nullptr);
+  }
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
+  st->its = it;
+  st->res0 = rnorm0;
+  st->res = res;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return AEROFX_OK;
 }
 
 int aerofx_jvp_transpose(aerofx_ctx *c, const double *p, double *JTp) {
@@ -525,7 +657,7 @@ int aerofx_jacobian_setup(aerofx_ctx *c, const double *U, const double *dt, int 
 int aerofx_jacobian_blocks(aerofx_ctx *c, double *D, double *Aedge) {
   if (!c || !c->d_jacD) return set_error(AEROFX_ERR_STATE, "jacobian_blocks: no Jacobian");
   const size_t esz = c->jac_precision ? sizeof(float) : sizeof(double);
-  std::vector<char> hd(25 * esz * (size_t)c->N), hb(25 * esz * 2 * (size_t)c->E);
+  std::vector<char> hd(esz * blk64_count(c->N)), hb(esz * blk64_count(2 * (size_t)c->E));
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(hd.data(), c->d_jacD, hd.size(), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(hb.data(), c->d_jacB, hb.size(), hipMemcpyDeviceToHost);
@@ -534,7 +666,8 @@ int aerofx_jacobian_blocks(aerofx_ctx *c, double *D, double *Aedge) {
     return c->jac_precision ? (double)((const float *)v.data())[idx] : ((const double *)v.data())[idx];
   };
   if (D)
-    for (size_t k = 0; k < 25 * (size_t)c->N; ++k) D[k] = at(hd, k);
+    for (size_t n = 0; n < (size_t)c->N; ++n)
+      for (int kk = 0; kk < 25; ++kk) D[25 * n + kk] = at(hd, blk64_base(n) + kk * kBlkStride);
   if (Aedge) {
     // adjacency entry q of node n -> edge layout [E][2][25] = (A_ij, A_ji)
     std::vector<int> rowptr, adj, eid;
@@ -543,7 +676,8 @@ int aerofx_jacobian_blocks(aerofx_ctx *c, double *D, double *Aedge) {
       for (int q = rowptr[n]; q < rowptr[n + 1]; ++q) {
         int l = eid[q];
         int side = n < adj[q] ? 0 : 1;
-        for (int kk = 0; kk < 25; ++kk) Aedge[50 * (size_t)l + 25 * side + kk] = at(hb, 25 * (size_t)q + kk);
+        for (int kk = 0; kk < 25; ++kk)
+          Aedge[50 * (size_t)l + 25 * side + kk] = at(hb, blk64_base(q) + kk * kBlkStride);
       }
   }
   return AEROFX_OK;

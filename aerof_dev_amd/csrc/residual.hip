@@ -24,13 +24,6 @@
 
 namespace aerofx {
 
-__device__ __forceinline__ int xcd_chunk(int b, int nchunks) {
-  const int q = nchunks / kNumXcd, r = nchunks % kNumXcd;
-  const int x = b % kNumXcd, k = b / kNumXcd;
-  return x * q + min(x, r) + k;
-}
-
-constexpr int kPad = kBlock + 1;  // LDS row stride (odd: conflict-free)
 
 // ---------------------------------------------------------------------------
 // 1. gradients (+ primitive state): one lane per node, neighbours in

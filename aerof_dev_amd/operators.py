@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from ._abi import Mesh as _CMesh
-from ._abi import BlockStats, NewtonOpts, NewtonStats, c_double_p, c_int_p, check, lib
+from ._abi import (AdjointOpts, AdjointStats, BlockStats, NewtonOpts, NewtonStats, c_double_p, c_int_p, check,
+                   lib)
 from .mesh import FlowParams, Mesh
 
 
@@ -117,6 +118,15 @@ class SpaceOperator:
 
     def conservative_to_primitive(self, U, V):
         check(lib().aerofx_primitive(self._h, _ptr(U), _ptr(V)), "primitive")
+
+    def wall_force(self, U, direction=(0.0, 0.0, 1.0), dJdU=None, dJdX=None) -> float:
+        """Pressure force on the slip walls along direction; optional dJ/dU [N,5], dJ/dX [N,3]."""
+        d = (C.c_double * 3)(*direction)
+        J = C.c_double()
+        check(lib().aerofx_wall_force(self._h, _ptr(U), d, C.byref(J),
+                                      _ptr(dJdU) if dJdU is not None else None,
+                                      _ptr(dJdX) if dJdX is not None else None), "wall_force")
+        return J.value
 
     def compute_transpose_derivative_of_residual(self, U, lam, dRdU_T_lam=None, dRdX_T_lam=None):
         """(dR/dU)^T lam -> dRdU_T_lam [N,5] and (dR/dX)^T lam -> dRdX_T_lam [N,3]
@@ -227,6 +237,19 @@ class BlockGmresSolver:
               "block_gmres")
         return dict(cycles=st.cycles, inner_its=st.inner_its, converged=bool(st.converged), res_max=st.res_max,
                     seconds=st.seconds, tune=(st.tune_waves_dots, st.tune_waves_update))
+
+
+def adjoint_solve(op: SpaceOperator, U, rhs, psi, precision: int = 1, cfl: float = 1e4, max_its: int = 400,
+                  eps: float = 1e-10, gmres_m: int = 30, gmres_maxits: int = 30, gmres_eps: float = 1e-2):
+    """(dR/dU)^T psi = rhs: FP64 outer iterations on the exact adjoint, inner
+    transposed GMRES on H1 + 1/dt with block-Jacobi (FP32 when precision=1)."""
+    o = AdjointOpts(int(precision), float(cfl), int(max_its), float(eps), int(gmres_m), int(gmres_maxits),
+                    float(gmres_eps))
+    st = AdjointStats()
+    check(lib().aerofx_adjoint_solve(op._h, _ptr(U), _ptr(rhs), _ptr(psi), C.byref(o), C.byref(st)),
+          "adjoint_solve")
+    return dict(its=st.its, converged=bool(st.converged), krylov_its=st.krylov_its, res0=st.res0, res=st.res,
+                seconds=st.seconds)
 
 
 def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200, eps: float = 1e-8,

@@ -248,6 +248,35 @@ int aerofx_block_gmres(aerofx_ctx *ctx, int nrhs, const double *B, double *X,
 int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const double *lambda,
                             double *dRdU_T_lambda, double *dRdX_T_lambda);
 
+/* Pressure force on the slip-wall faces along dir (lift for dir = e_z):
+ * J = sum_{wall faces f} sum_{v in f} p_v (n_f / 3) . dir, with
+ * dJdU (device [N][5], may be NULL) and dJdX (device [N][3], may be NULL).
+ * The objective of the shape sensitivity (FluidShapeOptimizationHandler.C:880
+ * evaluates the forces through PostOperator). */
+int aerofx_wall_force(aerofx_ctx *ctx, const double *U, const double dir[3],
+                      double *J, double *dJdU, double *dJdX);
+
+typedef struct {
+  int precision;        /* 0: FP64 H1/Jacobi in the inner solves; 1: FP32 */
+  double cfl;           /* pseudo-time term 1/dt of the inner operator (0: none) */
+  int max_its;          /* outer (defect-correction) iterations */
+  double eps;           /* stop at ||rhs - (dR/dU)^T psi|| <= eps ||rhs|| */
+  int gmres_m, gmres_maxits;
+  double gmres_eps;
+} aerofx_adjoint_opts;
+typedef struct {
+  int its, converged, krylov_its;
+  double res0, res, seconds;
+} aerofx_adjoint_stats;
+/* Adjoint solve (dR/dU)^T psi = rhs at U (the sensitivity handler's
+ * ksp->solveT, FluidShapeOptimizationHandler.C:1594-1624): outer FP64
+ * iterations on the exact transposed residual derivative
+ * (aerofx_adjoint_residual), each correcting psi with GMRES on the
+ * transposed H1 + diag(1/dt) and block-Jacobi in the chosen precision. */
+int aerofx_adjoint_solve(aerofx_ctx *ctx, const double *U, const double *rhs,
+                         double *psi, const aerofx_adjoint_opts *opts,
+                         aerofx_adjoint_stats *stats);
+
 /* ---- steady pseudo-transient Newton-GMRES ------------------------------ */
 typedef struct {
   int precision;        /* 0: all FP64; 1: FP32 Jacobian/JVP/block-Jacobi */
