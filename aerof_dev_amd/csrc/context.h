@@ -147,6 +147,9 @@ int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans = fa
 // variants that return at once when *done != 0 (device-resident Krylov loops)
 int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans = false);
 int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done, bool trans = false);
+// V = src * (*scale) then x = P V (the Arnoldi normalisation fused with the preconditioner)
+int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, double *V, double *x,
+                          const int *done, bool trans = false);
 void free_krylov(aerofx_ctx *c);
 void free_adjoint(aerofx_ctx *c);
 // (dR/dU)^T lam -> Ub and/or (dR/dX)^T lam -> Xout (adjoint.hip), enqueued on c->stream

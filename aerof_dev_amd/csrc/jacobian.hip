@@ -271,6 +271,31 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restr
   for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
 }
 
+// V = src * (*scale), x = Di V (or Di^T V): the Arnoldi step's normalisation
+// of the previous vector fused with the preconditioner apply
+template <class T, bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_jacobi_norm(int N, const T *__restrict__ Di,
+                                                        const double *__restrict__ src,
+                                                        const double *__restrict__ scale, double *__restrict__ Vout,
+                                                        double *__restrict__ x, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  const double sc = *scale;
+  double yn[5], acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    yn[k] = sc * src[5 * (size_t)n + k];
+    Vout[5 * (size_t)n + k] = yn[k];
+  }
+  if (TRANS)
+    amtv5_add(Di + blk64_base(n), yn, acc);
+  else
+    amv5_add(Di + blk64_base(n), yn, acc);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -378,6 +403,28 @@ int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int 
     jacobi_launch<float>(c, y, x, done, trans);
   else
     jacobi_launch<double>(c, y, x, done, trans);
+  return check_launch("prec launch");
+}
+
+template <class T>
+static void jacobi_norm_launch(aerofx_ctx *c, const double *src, const double *scale, double *V, double *x,
+                               const int *done, bool trans) {
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  if (trans)
+    hipLaunchKernelGGL((k_jacobi_norm<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->N, (const T *)c->d_jacDi,
+                       src, scale, V, x, done);
+  else
+    hipLaunchKernelGGL((k_jacobi_norm<T, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const T *)c->d_jacDi, src, scale, V, x, done);
+}
+
+int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, double *V, double *x,
+                          const int *done, bool trans) {
+  if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
+  if (c->jac_precision)
+    jacobi_norm_launch<float>(c, src, scale, V, x, done, trans);
+  else
+    jacobi_norm_launch<double>(c, src, scale, V, x, done, trans);
   return check_launch("prec launch");
 }
 

@@ -112,14 +112,82 @@ __device__ __forceinline__ void block_reduce_store_1(double v, double *__restric
   }
 }
 
-// mode 0: partial_k = sum w V_k
-// mode 1: w -= sum_k h_k V_k, then partial_k = sum w V_k
-// mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
-// The grid has nparts blocks (fixed by n); block b sums a contiguous slice.
+// Arnoldi step j: Hessenberg column, rotations, residual estimate, stop flag
+// (KspSolver.C:404-447, 919-962).
+__device__ void givens_step(GmresState *s, int j) {
+  if (s->done) return;
+  const int M = kMaxBasis;
+  // column j in registers (indices are compile-time after unrolling)
+  double h[kMaxBasis + 1];
+#pragma unroll
+  for (int i = 0; i <= kMaxBasis; ++i) h[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i <= j) h[i] = s->dots[i] + s->dots[kMaxBasis + i];
+  const double hn = sqrt(s->dots[2 * kMaxBasis]);
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i == j + 1) h[i] = hn;
+  if (j + 1 == kMaxBasis) h[kMaxBasis] = hn;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis - 1; ++i)
+    if (i < j) {
+      const double sn = s->cs[2 * i], cs = s->cs[2 * i + 1];
+      const double hj = cs * h[i] + sn * h[i + 1];
+      const double hjp1 = -sn * h[i] + cs * h[i + 1];
+      h[i] = hj;
+      h[i + 1] = hjp1;
+    }
+  double hjj = 0.0, hj1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i == j) {
+      hjj = h[i];
+      hj1 = h[i + 1];
+    }
+  double sn, cs;
+  if (hjj * hjj == 0.0) {
+    cs = 0.0;
+    sn = 1.0;
+  } else {
+    double coef = 1.0 / sqrt(hjj * hjj + hj1 * hj1);
+    cs = sqrt(hjj * hjj) * coef;
+    sn = cs * hj1 / hjj;
+  }
+  s->cs[2 * j + 1] = cs;
+  s->cs[2 * j] = sn;
+#pragma unroll
+  for (int i = 0; i < kMaxBasis; ++i)
+    if (i < j) s->H[i * M + j] = h[i];
+  s->H[j * M + j] = cs * hjj + sn * hj1;
+  s->H[(j + 1) * M + j] = 0.0;
+  const double gj = s->g[j];
+  s->g[j + 1] = -sn * gj;
+  s->g[j] = cs * gj;
+  s->iter += 1;
+  s->jlast = j;
+  if (hn == 0.0) {
+    s->done = 1;
+    return;
+  }
+  s->invhn = 1.0 / hn;
+  s->l2res = fabs(s->g[j + 1]);
+  if (s->l2res <= s->target || s->iter >= s->maxits) s->done = 1;
+}
+
+// mode 0: out_k = sum w V_k
+// mode 1: w -= sum_k h_k V_k, then out_k = sum w V_k
+// mode 2: w -= sum_k h_k V_k, then out_0 = sum w w, and (gs != null) the
+//         Givens update of Arnoldi step j
+// The grid has nparts blocks (fixed by n); block b sums a contiguous slice
+// into partial[.][b]; the last block to finish (a ticket counter) adds the
+// partials in block order, so the result does not depend on scheduling.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__ w, const double *__restrict__ V,
                                                   size_t ldv, int ny, const double *__restrict__ h,
-                                                  double *__restrict__ partial, const int *__restrict__ done) {
+                                                  double *__restrict__ partial, double *__restrict__ out,
+                                                  int *__restrict__ ticket, GmresState *gs, int j,
+                                                  const int *__restrict__ done) {
   if (done && *done) return;
   const size_t slice = (n + gridDim.x - 1) / gridDim.x;
   const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
@@ -153,6 +221,27 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
     block_reduce_store_1(acc[0], partial);
   else
     block_reduce_store_k(acc, ny, gridDim.x, partial);
+  // ticket: the last block reduces the partials
+  __shared__ int is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const volatile double *pv = partial;
+  const int nk = MODE == 2 ? 1 : ny;
+  if ((int)threadIdx.x < nk) {
+    double t = 0.0;
+    for (int b = 0; b < (int)gridDim.x; ++b) t += pv[(size_t)threadIdx.x * kDotBlocks + b];
+    out[threadIdx.x] = t;
+  }
+  if (threadIdx.x == 0) *ticket = 0;
+  if (MODE == 2 && gs) {
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x == 0) givens_step(gs, j);
+  }
 }
 
 // out[k] = sum_b partial[k][b] in a fixed tree, one block per k
@@ -213,69 +302,6 @@ __global__ void k_gmres_start(GmresState *s, const double *__restrict__ nrm2, in
   s->done = (beta == 0.0) ? 2 : 0;
   s->invhn = beta == 0.0 ? 0.0 : 1.0 / beta;
   s->jlast = -1;
-}
-
-// Arnoldi step j: Hessenberg column, rotations, residual estimate, stop flag
-// (KspSolver.C:404-447, 919-962).
-__global__ void k_gmres_givens(GmresState *s, int j) {
-  if (threadIdx.x != 0 || s->done) return;
-  const int M = kMaxBasis;
-  // column j in registers (indices are compile-time after unrolling)
-  double h[kMaxBasis + 1];
-#pragma unroll
-  for (int i = 0; i <= kMaxBasis; ++i) h[i] = 0.0;
-#pragma unroll
-  for (int i = 0; i < kMaxBasis; ++i)
-    if (i <= j) h[i] = s->dots[i] + s->dots[kMaxBasis + i];
-  const double hn = sqrt(s->dots[2 * kMaxBasis]);
-#pragma unroll
-  for (int i = 0; i < kMaxBasis; ++i)
-    if (i == j + 1) h[i] = hn;
-  if (j + 1 == kMaxBasis) h[kMaxBasis] = hn;
-#pragma unroll
-  for (int i = 0; i < kMaxBasis - 1; ++i)
-    if (i < j) {
-      const double sn = s->cs[2 * i], cs = s->cs[2 * i + 1];
-      const double hj = cs * h[i] + sn * h[i + 1];
-      const double hjp1 = -sn * h[i] + cs * h[i + 1];
-      h[i] = hj;
-      h[i + 1] = hjp1;
-    }
-  double hjj = 0.0, hj1 = 0.0;
-#pragma unroll
-  for (int i = 0; i < kMaxBasis; ++i)
-    if (i == j) {
-      hjj = h[i];
-      hj1 = h[i + 1];
-    }
-  double sn, cs;
-  if (hjj * hjj == 0.0) {
-    cs = 0.0;
-    sn = 1.0;
-  } else {
-    double coef = 1.0 / sqrt(hjj * hjj + hj1 * hj1);
-    cs = sqrt(hjj * hjj) * coef;
-    sn = cs * hj1 / hjj;
-  }
-  s->cs[2 * j + 1] = cs;
-  s->cs[2 * j] = sn;
-#pragma unroll
-  for (int i = 0; i < kMaxBasis; ++i)
-    if (i < j) s->H[i * M + j] = h[i];
-  s->H[j * M + j] = cs * hjj + sn * hj1;
-  s->H[(j + 1) * M + j] = 0.0;
-  const double gj = s->g[j];
-  s->g[j + 1] = -sn * gj;
-  s->g[j] = cs * gj;
-  s->iter += 1;
-  s->jlast = j;
-  if (hn == 0.0) {
-    s->done = 1;
-    return;
-  }
-  s->invhn = 1.0 / hn;
-  s->l2res = fabs(s->g[j + 1]);
-  if (s->l2res <= s->target || s->iter >= s->maxits) s->done = 1;
 }
 
 // back-substitution (KspSolver.C:968-982)
@@ -363,13 +389,24 @@ struct Krylov {
   double *w = nullptr, *z = nullptr, *r = nullptr, *x = nullptr, *b = nullptr, *R = nullptr;
   double *partial = nullptr;  // [kMaxBasis][kDotBlocks]
   double *scal = nullptr;     // device scalars
+  int *ticket = nullptr;      // [3] last-block counters of the three sweep kinds
   GmresState *st = nullptr;   // device
   GmresState *h_st = nullptr; // pinned host mirror
+  // the Arnoldi sequence of one cycle, captured once per (m, precision, transpose)
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t order_ev = nullptr;
+  int graph_key = -1;
   double *h_scal = nullptr;   // pinned
 };
 
 void free_krylov(aerofx_ctx *c) {
   if (!c->ksp) return;
+  if (c->ksp->exec) (void)hipGraphExecDestroy(c->ksp->exec);
+  if (c->ksp->graph) (void)hipGraphDestroy(c->ksp->graph);
+  if (c->ksp->cap_stream) (void)hipStreamDestroy(c->ksp->cap_stream);
+  if (c->ksp->order_ev) (void)hipEventDestroy(c->ksp->order_ev);
   if (c->ksp->h_st) (void)hipHostFree(c->ksp->h_st);
   if (c->ksp->h_scal) (void)hipHostFree(c->ksp->h_scal);
   delete c->ksp;
@@ -398,6 +435,9 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
   k->partial = (double *)p;
   if ((rc = ctx_alloc(c, sizeof(double) * 16, &p))) { delete k; return rc; }
   k->scal = (double *)p;
+  if ((rc = ctx_alloc(c, sizeof(int) * 4, &p))) { delete k; return rc; }
+  k->ticket = (int *)p;
+  if (hipMemset(k->ticket, 0, sizeof(int) * 4) != hipSuccess) { delete k; return set_error(AEROFX_ERR_HIP, "ticket init"); }
   if ((rc = ctx_alloc(c, sizeof(GmresState), &p))) { delete k; return rc; }
   k->st = (GmresState *)p;
   hipError_t e = hipHostMalloc((void **)&k->h_st, sizeof(GmresState), hipHostMallocDefault);
@@ -411,18 +451,82 @@ static inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) 
 
 template <int MODE>
 static void sweep(aerofx_ctx *c, double *w, const double *V, size_t ldv, int ny, const double *h, double *out,
-                  const int *done) {
+                  const int *done, GmresState *gs = nullptr, int j = -1) {
   Krylov *k = c->ksp;
   hipLaunchKernelGGL((k_sweep<MODE>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, V, ldv, ny, h,
-                     k->partial, done);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(MODE == 2 ? 1 : ny), dim3(kBlock), 0, c->stream, k->partial,
-                     k->nparts, out, done);
+                     k->partial, out, k->ticket + MODE, gs, j, done);
 }
 
 static int fetch_state(aerofx_ctx *c) {
   hipError_t e = hipMemcpyAsync(c->ksp->h_st, c->ksp->st, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   return e == hipSuccess ? 0 : set_hip_error(e, "gmres state copy");
+}
+
+// The m Arnoldi steps of one cycle (no host decisions: a converged cycle
+// leaves the remaining kernels to return at once on the stop flag).
+static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
+  Krylov *k = c->ksp;
+  const size_t n = k->n;
+  int *done = &k->st->done;
+  int rc;
+  for (int j = 0; j < m; ++j) {
+    double *Vj = k->V + (size_t)j * n;
+    {
+      // V_j = (r or w) * invhn, z = P V_j
+      ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+      if ((rc = launch_prec_norm_flag(c, j == 0 ? k->r : k->w, &k->st->invhn, Vj, k->z, done, trans))) return rc;
+    }
+    {
+      ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
+      if ((rc = launch_h1_apply_flag(c, k->z, k->w, done, trans))) return rc;
+    }
+    {
+      ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
+      sweep<0>(c, k->w, k->V, n, j + 1, nullptr, k->st->dots, done);
+      sweep<1>(c, k->w, k->V, n, j + 1, k->st->dots, k->st->dots + kMaxBasis, done);
+      sweep<2>(c, k->w, k->V, n, j + 1, k->st->dots + kMaxBasis, k->st->dots + 2 * kMaxBasis, done, k->st, j);
+    }
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "arnoldi launch");
+}
+
+// Runs the cycle's Arnoldi steps: eagerly when profiling (per-kernel events),
+// otherwise as one hipGraph captured on a private stream, ordered after the
+// context stream by events.  The graph holds kernel nodes only.
+static int arnoldi_cycle(aerofx_ctx *c, int m, bool trans) {
+  Krylov *k = c->ksp;
+  if (c->prof) return enqueue_arnoldi(c, m, trans);
+  const int key = (m * 2 + (trans ? 1 : 0)) * 2 + c->jac_precision;
+  hipError_t e = hipSuccess;
+  if (!k->cap_stream) e = hipStreamCreateWithFlags(&k->cap_stream, hipStreamNonBlocking);
+  if (e == hipSuccess && !k->order_ev) e = hipEventCreateWithFlags(&k->order_ev, hipEventDisableTiming);
+  if (e != hipSuccess) return set_hip_error(e, "arnoldi stream");
+  if (!k->exec || k->graph_key != key) {
+    if (k->exec) (void)hipGraphExecDestroy(k->exec);
+    if (k->graph) (void)hipGraphDestroy(k->graph);
+    k->exec = nullptr;
+    k->graph = nullptr;
+    hipStream_t saved = c->stream;
+    c->stream = k->cap_stream;
+    e = hipStreamBeginCapture(k->cap_stream, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? enqueue_arnoldi(c, m, trans) : 0;
+    hipError_t e2 = hipStreamEndCapture(k->cap_stream, &k->graph);
+    c->stream = saved;
+    if (rc) return rc;
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&k->exec, k->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return set_hip_error(e, "arnoldi graph");
+    k->graph_key = key;
+  }
+  hipEvent_t ev = k->order_ev;
+  e = hipEventRecord(ev, c->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(k->cap_stream, ev, 0);
+  if (e == hipSuccess) e = hipGraphLaunch(k->exec, k->cap_stream);
+  if (e == hipSuccess) e = hipEventRecord(ev, k->cap_stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, ev, 0);
+  return e == hipSuccess ? 0 : set_hip_error(e, "arnoldi graph launch");
 }
 
 struct GmresOut {
@@ -458,29 +562,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
     }
     sweep<2>(c, k->r, nullptr, 0, 0, nullptr, k->scal, nullptr);
     hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, c->stream, k->st, k->scal, cycle == 0 ? 1 : 0, eps);
-    // V0 = r / beta
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, &k->st->invhn, k->r, 0.0, k->V,
-                       done);
-    for (int j = 0; j < m; ++j) {
-      double *Vj = k->V + (size_t)j * n;
-      {
-        ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
-        if ((rc = launch_prec_apply_flag(c, Vj, k->z, done, trans))) return rc;
-      }
-      {
-        ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
-        if ((rc = launch_h1_apply_flag(c, k->z, k->w, done, trans))) return rc;
-      }
-      {
-        ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
-        sweep<0>(c, k->w, k->V, n, j + 1, nullptr, k->st->dots, done);
-        sweep<1>(c, k->w, k->V, n, j + 1, k->st->dots, k->st->dots + kMaxBasis, done);
-        sweep<2>(c, k->w, k->V, n, j + 1, k->st->dots + kMaxBasis, k->st->dots + 2 * kMaxBasis, done);
-        hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(64), 0, c->stream, k->st, j);
-        hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, &k->st->invhn, k->w, 0.0,
-                           k->V + (size_t)(j + 1) * n, done);
-      }
-    }
+    if ((rc = arnoldi_cycle(c, m, trans))) return rc;
     if ((rc = fetch_state(c))) return rc;
     const GmresState &hs = *k->h_st;
     if (hs.jlast >= 0) {

@@ -8,8 +8,15 @@ amplitudes a_k with seeded centres on the bottom wall.
 psi comes from the mixed-precision adjoint solve (FP32 transposed H1 and
 block-Jacobi inside, exact FP64 adjoint residual outside).  Checked against
 central finite differences of J with fully re-converged states, step 1e-6 on
-the amplitudes, to 1e-6 relative.  The wall-force derivatives themselves are
-checked against central differences of J on single coordinates.
+the amplitudes:
+  first-order scheme   to 1e-6 relative;
+  second-order scheme  to 5e-5 relative.  With the Van Albada limiter J(a) is
+    only piecewise smooth (an edge's limited slope switches branch when its
+    two slopes change sign), so central differences carry an O(h) error: on
+    this case they approach the adjoint value as 8e-4, 1.9e-5, 1.5e-6 for
+    h = 1e-5, 1e-6, 1e-7, the last at the round-off floor of the steady solve.
+The wall-force derivatives themselves are checked against central
+differences of J on single coordinates.
 """
 import dataclasses
 
@@ -38,17 +45,17 @@ def _converge(op, U, eps):
     return r
 
 
-@pytest.fixture(scope="module")
-def base(gpu):
+@pytest.fixture(scope="module", params=["first", "second"])
+def base(gpu, request):
     import torch
     from aerof_dev_amd.operators import SpaceOperator
     centres = np.random.default_rng(2024).uniform(0.9, 2.1, size=3)
-    p = FlowParams(mach=0.675)
+    p = FlowParams(mach=0.675, second_order=request.param == "second")
     m = _mesh([0.0] * 3, centres)
     op = SpaceOperator(m, p)
     U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
     _converge(op, U, 1e-12)
-    return dict(p=p, m=m, op=op, U=U, centres=centres)
+    return dict(p=p, m=m, op=op, U=U, centres=centres, tol=5e-5 if request.param == "second" else 1e-6)
 
 
 def test_wall_force_derivatives(gpu, base):
@@ -105,4 +112,4 @@ def test_adjoint_shape_gradient_vs_fd(gpu, base):
             _converge(opk, Uk, 1e-9)
             Js.append(opk.wall_force(Uk, ZDIR))
         fd = (Js[0] - Js[1]) / (2 * H)
-        assert abs(grad - fd) <= 1e-6 * abs(fd), (k, grad, fd)
+        assert abs(grad - fd) <= base["tol"] * abs(fd), (k, grad, fd)
