@@ -218,30 +218,45 @@ __global__ void k_tsg_update(size_t npad, int nb, const double *__restrict__ V, 
 // ---------------------------------------------------------------------------
 // 16x16 helpers (one lane): Cholesky G = R^T R, inverse of R, products
 // ---------------------------------------------------------------------------
-// R upper triangular with G = R^T R; Rinv = R^-1.  flag set on a non-positive pivot.
+// R upper triangular with G = R^T R; Rinv = R^-1.  A non-positive pivot (a
+// numerically rank-deficient block) restarts the factorisation of G + s I
+// with s = 1e-13 max diag(G) (shifted Cholesky-QR; the second Cholesky-QR
+// pass restores orthogonality); flag = 1 only if that fails too.
 __global__ void k_chol16(const double *__restrict__ G, double *__restrict__ R, double *__restrict__ Rinv,
                          int *__restrict__ flag) {
   if (threadIdx.x != 0) return;
   double a[kTile], r[kTile], ri[kTile];
+  double dmax = 0.0;
   for (int e = 0; e < kTile; ++e) {
     a[e] = G[e];
-    r[e] = 0.0;
     ri[e] = 0.0;
   }
-  for (int j = 0; j < kS; ++j) {
-    double d = a[j * kS + j];
-    for (int k = 0; k < j; ++k) d -= r[k * kS + j] * r[k * kS + j];
-    if (!(d > 0.0)) {
-      *flag = 1;
-      d = 1e-300;
+  for (int j = 0; j < kS; ++j) dmax = fmax(dmax, a[j * kS + j]);
+  bool ok = false;
+  for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
+    const double shift = attempt == 0 ? 0.0 : 1e-13 * dmax + 1e-300;
+    for (int e = 0; e < kTile; ++e) r[e] = 0.0;
+    ok = true;
+    for (int j = 0; j < kS && ok; ++j) {
+      double d = a[j * kS + j] + shift;
+      for (int k = 0; k < j; ++k) d -= r[k * kS + j] * r[k * kS + j];
+      if (!(d > 0.0)) {
+        ok = false;
+        break;
+      }
+      const double rjj = sqrt(d);
+      r[j * kS + j] = rjj;
+      for (int i = j + 1; i < kS; ++i) {
+        double s = a[j * kS + i];
+        for (int k = 0; k < j; ++k) s -= r[k * kS + j] * r[k * kS + i];
+        r[j * kS + i] = s / rjj;
+      }
     }
-    const double rjj = sqrt(d);
-    r[j * kS + j] = rjj;
-    for (int i = j + 1; i < kS; ++i) {
-      double s = a[j * kS + i];
-      for (int k = 0; k < j; ++k) s -= r[k * kS + j] * r[k * kS + i];
-      r[j * kS + i] = s / rjj;
-    }
+  }
+  if (!ok) {
+    *flag = 1;
+    for (int j = 0; j < kS; ++j)
+      if (!(r[j * kS + j] > 0.0)) r[j * kS + j] = 1.0;
   }
   // back substitution for R^-1 (upper)
   for (int j = kS - 1; j >= 0; --j) {
@@ -272,6 +287,23 @@ __global__ void k_add_tiles(int ntiles, const double *__restrict__ A, const doub
                             double *__restrict__ C) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < ntiles * kTile) C[e] = A[e] + B[e];
+}
+
+// Y[:, c] = X[:, c] / d_c with d_c = sqrt(nrm2[c]) (1 for a zero column)
+__global__ void k_scale_cols(size_t len, const double *__restrict__ X, const double *__restrict__ nrm2,
+                             double *__restrict__ Y) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const double d = sqrt(nrm2[i % kS]);
+  Y[i] = d > 0.0 ? X[i] / d : X[i];
+}
+
+// S[:, c] *= d_c (undoes the column scaling in the triangular factor)
+__global__ void k_scale_tile_cols(double *__restrict__ S, const double *__restrict__ nrm2) {
+  const int e = threadIdx.x;
+  if (e >= kTile) return;
+  const double d = sqrt(nrm2[e % kS]);
+  if (d > 0.0) S[e] *= d;
 }
 
 // column sums of squares of a block vector: partial over a fixed row split
@@ -394,7 +426,14 @@ void apply_ops(aerofx_ctx *c, BlockSolver *s, const T *D, const T *B, const T *D
 void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
   const size_t blk = s->npad * kS;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->flag, 1);
-  cholqr2(s, s->R, s->V, s->S0);
+  // equilibrate the residual columns first: they converge at different
+  // rates, and Cholesky-QR squares the condition number of the block
+  double *cn = s->nrm + (size_t)kChunks * kS + kS;
+  hipLaunchKernelGGL(k_colnorm_partial, dim3(kChunks), dim3(256), 0, s->stream, s->npad, s->R, s->nrm);
+  hipLaunchKernelGGL(k_colnorm_reduce, dim3(1), dim3(64), 0, s->stream, s->nrm, cn);
+  hipLaunchKernelGGL(k_scale_cols, g1(blk), dim3(256), 0, s->stream, blk, s->R, cn, s->Z);
+  cholqr2(s, s->Z, s->V, s->S0);
+  hipLaunchKernelGGL(k_scale_tile_cols, dim3(1), dim3(256), 0, s->stream, s->S0, cn);
   for (int j = 0; j < s->m; ++j) {
     const double *Vj = s->V + (size_t)j * blk;
     if (prec)
@@ -526,7 +565,7 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
       (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G)) || (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G2)) ||
       (r = alloc_d(c, kTile, &s->Rt)) || (r = alloc_d(c, kTile, &s->Ri)) || (r = alloc_d(c, kTile, &s->Rt2)) ||
       (r = alloc_d(c, kTile, &s->Ri2)) || (r = alloc_d(c, (size_t)m * kTile, &s->Y)) ||
-      (r = alloc_d(c, (size_t)kChunks * kS + kS, &s->nrm))) {
+      (r = alloc_d(c, (size_t)kChunks * kS + 2 * kS, &s->nrm))) {
     delete s;
     *rc = r;
     return nullptr;

@@ -82,76 +82,115 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *
 // H1 assembly (SpaceOperator.C:2088-2150, Edge.C:4472-4526, Face.C:816-847,
 // SubDomain.C:1744-1767, TimeState.C:731-756)
 // ---------------------------------------------------------------------------
+// Entry-parallel: a block covers whole rows (<= 256 adjacency entries, <= 64
+// rows).  Lane k evaluates the two ROEJAC6 blocks of entry e0 + k, stores the
+// off-diagonal block (64 consecutive entries -> contiguous segments) and its
+// diagonal contribution in LDS; one lane per row then sums the contributions
+// in entry order, adds the faces in face order, scales by 1/V and adds
+// 1/dt -- the reference's per-node order.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
                                                      const double *__restrict__ dt, T *__restrict__ D,
                                                      T *__restrict__ B) {
-  const int n = blockIdx.x * kBlock + threadIdx.x;
-  if (n >= t.N) return;
+  __shared__ double sd[25 * kPad];
+  __shared__ int srow[kMaxRows + 1];
+  const int c = xcd_chunk(blockIdx.x, t.ngchunks);
+  const int r0 = t.gchunk[c], r1 = t.gchunk[c + 1];
+  const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
+  const int tid = threadIdx.x;
   const Gas &g = fc.gas;
-  double Vn[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
-  const double voln = t.invcv[n];
-  double Dn[25];
-#pragma unroll
-  for (int k = 0; k < 25; ++k) Dn[k] = 0.0;
-  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
+  if (tid <= r1 - r0) srow[tid] = t.rowptr[r0 + tid] - e0;
+  __syncthreads();
+  if (tid < e1 - e0) {
+    const int e = e0 + tid;
+    // row of entry tid: the last row start <= tid (binary search over <= 65 starts)
+    int lo = 0, hi = r1 - r0;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (srow[mid] <= tid) lo = mid; else hi = mid;
+    }
+    const int n = r0 + lo;
     const int m = t.adj[e], l = t.eidx[e];
     const double nl[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
     T *Be = B + blk64_base(e);
     if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) {
 #pragma unroll
-      for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)0;
-      continue;
-    }
-    double Vm[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Vm[k] = V[5 * (size_t)m + k];
-    double blk[25];
-    if (n < m) {
-      // n = i: A_ii += dF/dU_i, A_ij = dF/dU_j / V_i
-      roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk);
-#pragma unroll
-      for (int k = 0; k < 25; ++k) Dn[k] += blk[k];
-      roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk);
-#pragma unroll
-      for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 + blk[k] * voln);
+      for (int k = 0; k < 25; ++k) {
+        Be[k * kBlkStride] = (T)0;
+        sd[k * kPad + tid] = 0.0;
+      }
     } else {
-      // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
-      roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk);
+      double Vn[5], Vm[5];
 #pragma unroll
-      for (int k = 0; k < 25; ++k) Dn[k] -= blk[k];
-      roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk);
+      for (int k = 0; k < 5; ++k) {
+        Vn[k] = V[5 * (size_t)n + k];
+        Vm[k] = V[5 * (size_t)m + k];
+      }
+      const double voln = t.invcv[n];
+      double blk[25];
+      if (n < m) {
+        // n = i: A_ii += dF/dU_i, A_ij = dF/dU_j / V_i
+        roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk);
 #pragma unroll
-      for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 - blk[k] * voln);
+        for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = blk[k];
+        roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk);
+#pragma unroll
+        for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 + blk[k] * voln);
+      } else {
+        // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
+        roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk);
+#pragma unroll
+        for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = -blk[k];
+        roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk);
+#pragma unroll
+        for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 - blk[k] * voln);
+      }
     }
   }
-  for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
-    const int face = t.fidx[q];
-    const int code = t.fcode[face];
-    const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
-                          t.fnorm3[3 * (size_t)face + 2]};
-    double jac[25];
-    if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
-      wall_jac(g, nv, 0.0, Vn, jac);
-    } else {
-      const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
-      farfield_jac(g, nv, 0.0, Vn, ub, jac);
+  __syncthreads();
+  if (tid < r1 - r0) {
+    const int n = r0 + tid;
+    const int b = t.rowptr[n] - e0, bend = t.rowptr[n + 1] - e0;
+    double Dn[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+      double s = 0.0;
+      for (int q = b; q < bend; ++q) s += sd[k * kPad + q];
+      Dn[k] = s;
     }
+    const int f0 = t.frow[n], f1 = t.frow[n + 1];
+    if (f0 < f1) {
+      double Vn[5];
 #pragma unroll
-    for (int k = 0; k < 25; ++k) Dn[k] += jac[k];
+      for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
+      for (int q = f0; q < f1; ++q) {
+        const int face = t.fidx[q];
+        const int code = t.fcode[face];
+        const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
+                              t.fnorm3[3 * (size_t)face + 2]};
+        double jac[25];
+        if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+          wall_jac(g, nv, 0.0, Vn, jac);
+        } else {
+          const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
+          farfield_jac(g, nv, 0.0, Vn, ub, jac);
+        }
+#pragma unroll
+        for (int k = 0; k < 25; ++k) Dn[k] += jac[k];
+      }
+    }
+    const double voln = t.invcv[n];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Dn[k] *= voln;
+    if (dt) {
+      const double c_np1 = 1.0 / dt[n];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) Dn[6 * k] += c_np1;
+    }
+    T *Dd = D + blk64_base(n);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Dd[k * kBlkStride] = (T)Dn[k];
   }
-#pragma unroll
-  for (int k = 0; k < 25; ++k) Dn[k] *= voln;
-  if (dt) {
-    const double c_np1 = 1.0 / dt[n];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Dn[6 * k] += c_np1;
-  }
-  T *Dd = D + blk64_base(n);
-#pragma unroll
-  for (int k = 0; k < 25; ++k) Dd[k * kBlkStride] = (T)Dn[k];
 }
 
 template <class T>
@@ -332,14 +371,15 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   if (rc) return rc;
   c->jac_version++;
   const int nb = (c->N + kBlock - 1) / kBlock;
+  const int nchunks = c->tab.ngchunks;
   {
     ProfScope ps(c, AEROFX_TIMER_FV_JAC);
     if (precision)
-      hipLaunchKernelGGL((k_jacobian<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
+      hipLaunchKernelGGL((k_jacobian<float>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
                          (float *)c->d_jacD, (float *)c->d_jacB);
     else
-      hipLaunchKernelGGL((k_jacobian<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
-                         (double *)c->d_jacD, (double *)c->d_jacB);
+      hipLaunchKernelGGL((k_jacobian<double>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc,
+                         dt, (double *)c->d_jacD, (double *)c->d_jacB);
   }
   if ((rc = check_launch("jacobian launch"))) return rc;
   {

@@ -175,19 +175,15 @@ __device__ void givens_step(GmresState *s, int j) {
   if (s->l2res <= s->target || s->iter >= s->maxits) s->done = 1;
 }
 
-// mode 0: out_k = sum w V_k
-// mode 1: w -= sum_k h_k V_k, then out_k = sum w V_k
-// mode 2: w -= sum_k h_k V_k, then out_0 = sum w w, and (gs != null) the
-//         Givens update of Arnoldi step j
+// mode 0: partial_k = sum w V_k
+// mode 1: w -= sum_k h_k V_k, then partial_k = sum w V_k
+// mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
 // The grid has nparts blocks (fixed by n); block b sums a contiguous slice
-// into partial[.][b]; the last block to finish (a ticket counter) adds the
-// partials in block order, so the result does not depend on scheduling.
+// into partial[.][b].
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__ w, const double *__restrict__ V,
                                                   size_t ldv, int ny, const double *__restrict__ h,
-                                                  double *__restrict__ partial, double *__restrict__ out,
-                                                  int *__restrict__ ticket, GmresState *gs, int j,
-                                                  const int *__restrict__ done) {
+                                                  double *__restrict__ partial, const int *__restrict__ done) {
   if (done && *done) return;
   const size_t slice = (n + gridDim.x - 1) / gridDim.x;
   const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
@@ -221,32 +217,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
     block_reduce_store_1(acc[0], partial);
   else
     block_reduce_store_k(acc, ny, gridDim.x, partial);
-  // ticket: the last block reduces the partials
-  __shared__ int is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const volatile double *pv = partial;
-  const int nk = MODE == 2 ? 1 : ny;
-  if ((int)threadIdx.x < nk) {
-    double t = 0.0;
-    for (int b = 0; b < (int)gridDim.x; ++b) t += pv[(size_t)threadIdx.x * kDotBlocks + b];
-    out[threadIdx.x] = t;
-  }
-  if (threadIdx.x == 0) *ticket = 0;
-  if (MODE == 2 && gs) {
-    __threadfence_block();
-    __syncthreads();
-    if (threadIdx.x == 0) givens_step(gs, j);
-  }
 }
 
 // out[k] = sum_b partial[k][b] in a fixed tree, one block per k
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial, int nparts,
-                                                            double *__restrict__ out, const int *__restrict__ done) {
+                                                            double *__restrict__ out, const int *__restrict__ done,
+                                                            GmresState *gs, int j) {
   if (done && *done) return;
   __shared__ double sh[kBlock];
   const int k = blockIdx.x;
@@ -258,7 +234,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__rest
     if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[k] = sh[0];
+  if (threadIdx.x == 0) {
+    out[k] = sh[0];
+    // after the norm of step j: the Givens update (single block launch)
+    if (gs) givens_step(gs, j);
+  }
 }
 
 // y = a x + b y, a possibly scaled by a device scalar (*as)
@@ -389,7 +369,6 @@ struct Krylov {
   double *w = nullptr, *z = nullptr, *r = nullptr, *x = nullptr, *b = nullptr, *R = nullptr;
   double *partial = nullptr;  // [kMaxBasis][kDotBlocks]
   double *scal = nullptr;     // device scalars
-  int *ticket = nullptr;      // [3] last-block counters of the three sweep kinds
   GmresState *st = nullptr;   // device
   GmresState *h_st = nullptr; // pinned host mirror
   // the Arnoldi sequence of one cycle, captured once per (m, precision, transpose)
@@ -435,9 +414,7 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
   k->partial = (double *)p;
   if ((rc = ctx_alloc(c, sizeof(double) * 16, &p))) { delete k; return rc; }
   k->scal = (double *)p;
-  if ((rc = ctx_alloc(c, sizeof(int) * 4, &p))) { delete k; return rc; }
-  k->ticket = (int *)p;
-  if (hipMemset(k->ticket, 0, sizeof(int) * 4) != hipSuccess) { delete k; return set_error(AEROFX_ERR_HIP, "ticket init"); }
+
   if ((rc = ctx_alloc(c, sizeof(GmresState), &p))) { delete k; return rc; }
   k->st = (GmresState *)p;
   hipError_t e = hipHostMalloc((void **)&k->h_st, sizeof(GmresState), hipHostMallocDefault);
@@ -454,7 +431,9 @@ static void sweep(aerofx_ctx *c, double *w, const double *V, size_t ldv, int ny,
                   const int *done, GmresState *gs = nullptr, int j = -1) {
   Krylov *k = c->ksp;
   hipLaunchKernelGGL((k_sweep<MODE>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, V, ldv, ny, h,
-                     k->partial, out, k->ticket + MODE, gs, j, done);
+                     k->partial, done);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(MODE == 2 ? 1 : ny), dim3(kBlock), 0, c->stream, k->partial,
+                     k->nparts, out, done, MODE == 2 ? gs : nullptr, j);
 }
 
 static int fetch_state(aerofx_ctx *c) {
@@ -592,7 +571,7 @@ static int jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *prod,
   sweep<2>(c, const_cast<double *>(p), nullptr, 0, 0, nullptr, k->scal, nullptr);
   hipLaunchKernelGGL(k_eps_sum, dim3(kDotBlocks), dim3(kBlock), 0, c->stream, n, U, k->partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, kDotBlocks, k->scal + 1,
-                     nullptr);
+                     nullptr, nullptr, -1);
   hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, c->stream, k->scal, (double)n);
   double *Fa = k->w, *Fb = k->z, *Q = k->r;
   // F(U + eps p)

@@ -56,7 +56,7 @@ def test_block_matches_single_rhs(gpu, precision):
     op, h1, B = _system(gpu)
     Xs = _single(op, B)
     X = torch.empty_like(B)
-    out = BlockGmresSolver(op, m=8, eps=1e-10, max_cycles=500, precision=precision, use_graph=False).solve(B, X)
+    out = BlockGmresSolver(op, m=16, eps=1e-10, max_cycles=500, precision=precision, use_graph=False).solve(B, X)
     assert out["converged"], out
     for k in range(B.shape[0]):
         assert _rel(X[k], Xs[k]) <= 1e-7, (k, _rel(X[k], Xs[k]), out)
@@ -72,11 +72,11 @@ def test_block_graph_replays_match_eager(gpu):
     from aerof_dev_amd.operators import BlockGmresSolver
     op, h1, B = _system(gpu, nrhs=11)  # padded to 16 columns internally
     Xe = torch.empty_like(B)
-    eager = BlockGmresSolver(op, m=6, eps=1e-9, max_cycles=400, precision=1, use_graph=False).solve(B, Xe)
+    eager = BlockGmresSolver(op, m=12, eps=1e-9, max_cycles=400, precision=1, use_graph=False).solve(B, Xe)
     assert eager["converged"]
     for rep in range(3):
         Xg = torch.empty_like(B)
-        g = BlockGmresSolver(op, m=6, eps=1e-9, max_cycles=400, precision=1, use_graph=True).solve(B, Xg)
+        g = BlockGmresSolver(op, m=12, eps=1e-9, max_cycles=400, precision=1, use_graph=True).solve(B, Xg)
         torch.cuda.synchronize()
         assert g["cycles"] == eager["cycles"], (rep, g, eager)
         assert torch.equal(Xg, Xe), rep
