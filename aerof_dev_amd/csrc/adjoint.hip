@@ -908,7 +908,7 @@ extern "C" int aerofx_wall_force(aerofx_ctx *ctx, const double *U, const double 
                        t.fnorm3, t.X4, ctx->d_V, dir[0], dir[1], dir[2], a->Jf, dJdX ? a->FX : nullptr);
     hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, ctx->stream, ctx->F, a->Jf, a->Jf + ctx->F);
   } else {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, (int *)(a->Jf), 2);
+    hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, ctx->stream, 0, a->Jf, a->Jf);  // no faces: J = 0
   }
   if (dJdU || dJdX)
     hipLaunchKernelGGL(k_lift_nodes, grid_for(t.N), dim3(kAdjBlock), 0, ctx->stream, t, U, ctx->gas, a->nfslot,

@@ -46,7 +46,7 @@ namespace aerofx {
 
 constexpr int kS = 16;          // block width
 constexpr int kTile = kS * kS;  // one 16x16 block
-constexpr int kChunks = 1024;   // fixed row split of V^T W
+constexpr int kChunks = 1024;   // fixed row split of the column norms
 constexpr int kMaxM = 16;       // block steps per cycle
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -155,11 +155,12 @@ __global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__
 // v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
 // holds D[l/16 + 4r][l%16], r < 4 (checked on gfx950).  Here A = V_b^T over 4 rows, B = W over
 // the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
-__global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nb, const double *__restrict__ V, size_t ldv,
-                           const double *__restrict__ W, double *__restrict__ partial) {
+__global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
+                           const double *__restrict__ V, size_t ldv, const double *__restrict__ W,
+                           double *__restrict__ partial) {
   const int lane = threadIdx.x & 63;
   const int chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (chunk >= kChunks) return;
+  if (chunk >= nchunks) return;
   const size_t r0 = (size_t)chunk * rows_per_chunk;
   const size_t r1 = min(npad, r0 + rows_per_chunk);
   f64x4 acc[kMaxM + 1];
@@ -176,17 +177,19 @@ __global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nb, const dou
 #pragma unroll
   for (int b = 0; b <= kMaxM; ++b)
     if (b < nb) {
-      double *dst = partial + ((size_t)chunk * (kMaxM + 1) + b) * kTile;
+      double *dst = partial + ((size_t)chunk * mp1 + b) * kTile;
 #pragma unroll
       for (int r = 0; r < 4; ++r) dst[(rr + 4 * r) * kS + cc] = acc[b][r];
     }
 }
 
-// out[b] = sum_c partial[c][b] in a fixed tree; one block per tile b
-__global__ __launch_bounds__(256) void k_tsg_reduce(const double *__restrict__ partial, double *__restrict__ out) {
-  const int b = blockIdx.x, e = threadIdx.x;  // e: element of the 16x16 tile
+// out[b] = sum_c partial[c][b] in chunk order; one block per tile b, lane e
+// owns element e of the tile
+__global__ __launch_bounds__(256) void k_tsg_reduce(const double *__restrict__ partial, int nchunks, int mp1,
+                                                    double *__restrict__ out) {
+  const int b = blockIdx.x, e = threadIdx.x;
   double s = 0.0;
-  for (int c = 0; c < kChunks; ++c) s += partial[((size_t)c * (kMaxM + 1) + b) * kTile + e];
+  for (int c = 0; c < nchunks; ++c) s += partial[((size_t)c * mp1 + b) * kTile + e];
   out[(size_t)b * kTile + e] = s;
 }
 
@@ -350,7 +353,8 @@ struct BlockSolver {
   size_t n = 0, npad = 0;
   double *V = nullptr;        // [m+1][npad][16]
   double *W = nullptr, *Z = nullptr, *R = nullptr, *X = nullptr, *B = nullptr, *T = nullptr;
-  double *partial = nullptr;  // [kChunks][kMaxM+1][256]
+  int nchunks = 0;            // row split of V^T W (fixed by npad)
+  double *partial = nullptr;  // [nchunks][m+1][256]
   double *Hbar = nullptr;     // [(m+1)][m] tiles: block (i, j) at (i * m + j)
   double *S0 = nullptr;       // tile
   double *G = nullptr;        // scratch tiles [kMaxM+1]
@@ -388,11 +392,11 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc);
 
 void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *out) {
   // rows per chunk: a multiple of 4 (one MFMA step), fixed by npad alone
-  const size_t rows = std::max<size_t>(4, ((s->npad + kChunks - 1) / kChunks + 3) / 4 * 4);
+  const size_t rows = std::max<size_t>(4, ((s->npad + s->nchunks - 1) / s->nchunks + 3) / 4 * 4);
   const int wpb = s->shape.waves_dots;
-  hipLaunchKernelGGL(k_tsg_dots, dim3((kChunks + wpb - 1) / wpb), dim3(64 * wpb), 0, s->stream, s->npad, rows, nb, V,
-                     s->npad * kS, W, s->partial);
-  hipLaunchKernelGGL(k_tsg_reduce, dim3(nb), dim3(256), 0, s->stream, s->partial, out);
+  hipLaunchKernelGGL(k_tsg_dots, dim3((s->nchunks + wpb - 1) / wpb), dim3(64 * wpb), 0, s->stream, s->npad, rows,
+                     s->nchunks, s->m + 1, nb, V, s->npad * kS, W, s->partial);
+  hipLaunchKernelGGL(k_tsg_reduce, dim3(nb), dim3(256), 0, s->stream, s->partial, s->nchunks, s->m + 1, out);
 }
 
 void tsg_update(BlockSolver *s, int nb, const double *V, const double *H, double alpha, double beta, double *Out) {
@@ -556,11 +560,13 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
   s->m = m;
   s->n = 5 * (size_t)c->N;
   s->npad = (s->n + 63) / 64 * 64;
+  // about 512 rows per wave of V^T W, between 64 and 8192 chunks
+  s->nchunks = (int)std::min<size_t>(8192, std::max<size_t>(64, s->npad / 512));
   const size_t blk = s->npad * kS;
   int r = 0;
   if ((r = alloc_d(c, (size_t)(m + 1) * blk, &s->V)) || (r = alloc_d(c, blk, &s->W)) || (r = alloc_d(c, blk, &s->Z)) ||
       (r = alloc_d(c, blk, &s->R)) || (r = alloc_d(c, blk, &s->X)) || (r = alloc_d(c, blk, &s->B)) ||
-      (r = alloc_d(c, blk, &s->T)) || (r = alloc_d(c, (size_t)kChunks * (kMaxM + 1) * kTile, &s->partial)) ||
+      (r = alloc_d(c, blk, &s->T)) || (r = alloc_d(c, (size_t)s->nchunks * (m + 1) * kTile, &s->partial)) ||
       (r = alloc_d(c, (size_t)(m + 1) * m * kTile, &s->Hbar)) || (r = alloc_d(c, kTile, &s->S0)) ||
       (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G)) || (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G2)) ||
       (r = alloc_d(c, kTile, &s->Rt)) || (r = alloc_d(c, kTile, &s->Ri)) || (r = alloc_d(c, kTile, &s->Rt2)) ||

@@ -99,6 +99,7 @@ struct aerofx_ctx {
   void *d_jacD = nullptr, *d_jacB = nullptr, *d_jacDi = nullptr;  // current precision's
   void *jac_store[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [precision][D, B, Di]
   long long jac_version = 0;  // bumped at every assembly
+  double *d_fjac = nullptr;    // [3F][25] boundary-face Jacobians, per node face slot
   double *d_dt = nullptr;
   aerofx::Krylov *ksp = nullptr;
   aerofx::Adjoint *adj = nullptr;
@@ -147,9 +148,10 @@ int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans = fa
 // variants that return at once when *done != 0 (device-resident Krylov loops)
 int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans = false);
 int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int *done, bool trans = false);
-// V = src * (*scale) then x = P V (the Arnoldi normalisation fused with the preconditioner)
-int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, double *V, double *x,
-                          const int *done, bool trans = false);
+// V = (src - sum_{k < nsub} h_k Vb_k) * (*scale) then x = P V (the Arnoldi step's
+// last Gram-Schmidt update and normalisation fused with the preconditioner)
+int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, const double *Vb, size_t ldv,
+                          int nsub, const double *h, double *V, double *x, const int *done, bool trans = false);
 void free_krylov(aerofx_ctx *c);
 void free_adjoint(aerofx_ctx *c);
 // (dR/dU)^T lam -> Ub and/or (dR/dX)^T lam -> Xout (adjoint.hip), enqueued on c->stream

@@ -18,6 +18,8 @@
 // DenseMatrixOp<float,5,25> does with double vectors).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "context.h"
 #include "device_jacobian.h"
 #include "host_internal.h"
@@ -83,28 +85,30 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *
 // SubDomain.C:1744-1767, TimeState.C:731-756)
 // ---------------------------------------------------------------------------
 // Entry-parallel: a block covers whole rows (<= 256 adjacency entries, <= 64
-// rows).  Lane k evaluates the two ROEJAC6 blocks of entry e0 + k, stores the
-// off-diagonal block (64 consecutive entries -> contiguous segments) and its
-// diagonal contribution in LDS; one lane per row then sums the contributions
-// in entry order, adds the faces in face order, scales by 1/V and adds
-// 1/dt -- the reference's per-node order.
+// rows).  Phase 1: lane k evaluates the two ROEJAC6 blocks of entry e0 + k,
+// stores the off-diagonal block (64 consecutive entries -> contiguous
+// segments) and its diagonal contribution in LDS; lanes also take the rows'
+// boundary-face slots and store each face Jacobian in a per-slot buffer.
+// Phase 2: one lane per (row, element) adds the entries in entry order, then
+// the faces in face order, scales by 1/V and adds 1/dt -- the reference's
+// per-node order.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
-                                                     const double *__restrict__ dt, T *__restrict__ D,
-                                                     T *__restrict__ B) {
+                                                     const double *__restrict__ dt, double *__restrict__ fjac,
+                                                     T *__restrict__ D, T *__restrict__ B) {
   __shared__ double sd[25 * kPad];
   __shared__ int srow[kMaxRows + 1];
   const int c = xcd_chunk(blockIdx.x, t.ngchunks);
   const int r0 = t.gchunk[c], r1 = t.gchunk[c + 1];
   const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, nr = r1 - r0;
   const Gas &g = fc.gas;
-  if (tid <= r1 - r0) srow[tid] = t.rowptr[r0 + tid] - e0;
+  if (tid <= nr) srow[tid] = t.rowptr[r0 + tid] - e0;
   __syncthreads();
   if (tid < e1 - e0) {
     const int e = e0 + tid;
-    // row of entry tid: the last row start <= tid (binary search over <= 65 starts)
-    int lo = 0, hi = r1 - r0;
+    // row of entry tid: the last row start <= tid
+    int lo = 0, hi = nr;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (srow[mid] <= tid) lo = mid; else hi = mid;
@@ -147,49 +151,41 @@ __global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *
       }
     }
   }
+  // boundary-face Jacobians of the block's rows, one per face slot
+  const int q0 = t.frow[r0], q1 = t.frow[r1];
+  for (int q = q0 + tid; q < q1; q += kBlock) {
+    int lo = 0, hi = nr;  // row owning slot q
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (t.frow[r0 + mid] <= q) lo = mid; else hi = mid;
+    }
+    const int n = r0 + lo;
+    const int face = t.fidx[q];
+    const int code = t.fcode[face];
+    const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
+                          t.fnorm3[3 * (size_t)face + 2]};
+    double Vn[5], jac[25];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
+    if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+      wall_jac(g, nv, 0.0, Vn, jac);
+    } else {
+      const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
+      farfield_jac(g, nv, 0.0, Vn, ub, jac);
+    }
+#pragma unroll
+    for (int k = 0; k < 25; ++k) fjac[25 * (size_t)q + k] = jac[k];
+  }
   __syncthreads();
-  if (tid < r1 - r0) {
-    const int n = r0 + tid;
-    const int b = t.rowptr[n] - e0, bend = t.rowptr[n + 1] - e0;
-    double Dn[25];
-#pragma unroll
-    for (int k = 0; k < 25; ++k) {
-      double s = 0.0;
-      for (int q = b; q < bend; ++q) s += sd[k * kPad + q];
-      Dn[k] = s;
-    }
-    const int f0 = t.frow[n], f1 = t.frow[n + 1];
-    if (f0 < f1) {
-      double Vn[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
-      for (int q = f0; q < f1; ++q) {
-        const int face = t.fidx[q];
-        const int code = t.fcode[face];
-        const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
-                              t.fnorm3[3 * (size_t)face + 2]};
-        double jac[25];
-        if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
-          wall_jac(g, nv, 0.0, Vn, jac);
-        } else {
-          const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
-          farfield_jac(g, nv, 0.0, Vn, ub, jac);
-        }
-#pragma unroll
-        for (int k = 0; k < 25; ++k) Dn[k] += jac[k];
-      }
-    }
-    const double voln = t.invcv[n];
-#pragma unroll
-    for (int k = 0; k < 25; ++k) Dn[k] *= voln;
-    if (dt) {
-      const double c_np1 = 1.0 / dt[n];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) Dn[6 * k] += c_np1;
-    }
-    T *Dd = D + blk64_base(n);
-#pragma unroll
-    for (int k = 0; k < 25; ++k) Dd[k * kBlkStride] = (T)Dn[k];
+  for (int idx = tid; idx < nr * 25; idx += kBlock) {
+    const int rr = idx / 25, k = idx - 25 * rr;
+    const int n = r0 + rr;
+    double s = 0.0;
+    for (int q = srow[rr]; q < srow[rr + 1]; ++q) s += sd[k * kPad + q];
+    for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) s += fjac[25 * (size_t)q + k];
+    s *= t.invcv[n];
+    if (dt && k % 6 == 0) s += 1.0 / dt[n];
+    D[blk64_base(n) + k * kBlkStride] = (T)s;
   }
 }
 
@@ -310,12 +306,15 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_apply(int N, const T *__restr
   for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
 }
 
-// V = src * (*scale), x = Di V (or Di^T V): the Arnoldi step's normalisation
-// of the previous vector fused with the preconditioner apply
+// V = (src - sum_{k < nsub} h_k Vb_k) * (*scale), x = Di V (or Di^T V): the
+// Arnoldi step's last Gram-Schmidt update and normalisation of the previous
+// vector, fused with the preconditioner apply
 template <class T, bool TRANS>
 __global__ __launch_bounds__(kBlock) void k_jacobi_norm(int N, const T *__restrict__ Di,
                                                         const double *__restrict__ src,
-                                                        const double *__restrict__ scale, double *__restrict__ Vout,
+                                                        const double *__restrict__ scale,
+                                                        const double *__restrict__ Vb, size_t ldv, int nsub,
+                                                        const double *__restrict__ h, double *__restrict__ Vout,
                                                         double *__restrict__ x, const int *__restrict__ done) {
   if (done && *done) return;
   const int n = blockIdx.x * kBlock + threadIdx.x;
@@ -323,8 +322,15 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_norm(int N, const T *__restri
   const double sc = *scale;
   double yn[5], acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
+  for (int k = 0; k < 5; ++k) yn[k] = src[5 * (size_t)n + k];
+  for (int b = 0; b < nsub; ++b) {
+    const double hb = h[b];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) yn[k] -= hb * Vb[(size_t)b * ldv + 5 * (size_t)n + k];
+  }
+#pragma unroll
   for (int k = 0; k < 5; ++k) {
-    yn[k] = sc * src[5 * (size_t)n + k];
+    yn[k] = sc * yn[k];
     Vout[5 * (size_t)n + k] = yn[k];
   }
   if (TRANS)
@@ -347,6 +353,12 @@ int ensure_jacobian_storage(aerofx_ctx *c, int precision) {
   const size_t esz = precision ? sizeof(float) : sizeof(double);
   const size_t need_b = esz * blk64_count(2 * (size_t)c->E), need_d = esz * blk64_count(c->N);
   void **st = c->jac_store[precision];
+  if (!c->d_fjac) {
+    void *p = nullptr;
+    int rc = ctx_alloc(c, sizeof(double) * 25 * 3 * (size_t)std::max(c->F, 1), &p);
+    if (rc) return rc;
+    c->d_fjac = (double *)p;
+  }
   if (!st[0]) {
     int rc;
     if ((rc = ctx_alloc(c, need_d, &st[0])) || (rc = ctx_alloc(c, need_b, &st[1])) ||
@@ -376,10 +388,10 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
     ProfScope ps(c, AEROFX_TIMER_FV_JAC);
     if (precision)
       hipLaunchKernelGGL((k_jacobian<float>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
-                         (float *)c->d_jacD, (float *)c->d_jacB);
+                         c->d_fjac, (float *)c->d_jacD, (float *)c->d_jacB);
     else
       hipLaunchKernelGGL((k_jacobian<double>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc,
-                         dt, (double *)c->d_jacD, (double *)c->d_jacB);
+                         dt, c->d_fjac, (double *)c->d_jacD, (double *)c->d_jacB);
   }
   if ((rc = check_launch("jacobian launch"))) return rc;
   {
@@ -447,24 +459,25 @@ int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int 
 }
 
 template <class T>
-static void jacobi_norm_launch(aerofx_ctx *c, const double *src, const double *scale, double *V, double *x,
-                               const int *done, bool trans) {
+static void jacobi_norm_launch(aerofx_ctx *c, const double *src, const double *scale, const double *Vb,
+                               size_t ldv, int nsub, const double *h, double *V, double *x, const int *done,
+                               bool trans) {
   const int nb = (c->N + kBlock - 1) / kBlock;
   if (trans)
     hipLaunchKernelGGL((k_jacobi_norm<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->N, (const T *)c->d_jacDi,
-                       src, scale, V, x, done);
+                       src, scale, Vb, ldv, nsub, h, V, x, done);
   else
     hipLaunchKernelGGL((k_jacobi_norm<T, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
-                       (const T *)c->d_jacDi, src, scale, V, x, done);
+                       (const T *)c->d_jacDi, src, scale, Vb, ldv, nsub, h, V, x, done);
 }
 
-int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, double *V, double *x,
-                          const int *done, bool trans) {
+int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, const double *Vb, size_t ldv,
+                          int nsub, const double *h, double *V, double *x, const int *done, bool trans) {
   if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
   if (c->jac_precision)
-    jacobi_norm_launch<float>(c, src, scale, V, x, done, trans);
+    jacobi_norm_launch<float>(c, src, scale, Vb, ldv, nsub, h, V, x, done, trans);
   else
-    jacobi_norm_launch<double>(c, src, scale, V, x, done, trans);
+    jacobi_norm_launch<double>(c, src, scale, Vb, ldv, nsub, h, V, x, done, trans);
   return check_launch("prec launch");
 }
 

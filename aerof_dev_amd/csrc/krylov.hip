@@ -178,6 +178,7 @@ __device__ void givens_step(GmresState *s, int j) {
 // mode 0: partial_k = sum w V_k
 // mode 1: w -= sum_k h_k V_k, then partial_k = sum w V_k
 // mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
+// mode 3: w -= sum_k h_k V_k, then partial_k = sum w V_k and partial_ny = sum w w
 // The grid has nparts blocks (fixed by n); block b sums a contiguous slice
 // into partial[.][b].
 template <int MODE>
@@ -208,6 +209,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
     }
     if (MODE == 2) {
       acc[0] += wi * wi;
+    } else if (MODE == 3) {
+#pragma unroll
+      for (int k = 0; k < kMaxBasis; ++k) acc[k] += k == ny ? wi * wi : wi * v[k];
     } else {
 #pragma unroll
       for (int k = 0; k < kMaxBasis; ++k) acc[k] += wi * v[k];
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
   if (MODE == 2)
     block_reduce_store_1(acc[0], partial);
   else
-    block_reduce_store_k(acc, ny, gridDim.x, partial);
+    block_reduce_store_k(acc, MODE == 3 ? ny + 1 : ny, gridDim.x, partial);
 }
 
 // out[k] = sum_b partial[k][b] in a fixed tree, one block per k
@@ -238,6 +242,36 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__rest
     out[k] = sh[0];
     // after the norm of step j: the Givens update (single block launch)
     if (gs) givens_step(gs, j);
+  }
+}
+
+// After the mode-3 sweep of Arnoldi step j: out[0..j] = h2 = V^T w1 and
+// ||w1||^2, summed over the partials (8 lanes per value, fixed butterfly);
+// then ||w1 - V h2||^2 = ||w1||^2 - ||h2||^2 (V orthonormal) goes to the
+// norm slot and the Givens update runs.  One block.
+__global__ __launch_bounds__(kBlock) void k_reduce_cgs2(const double *__restrict__ partial, int nparts,
+                                                        double *__restrict__ out, GmresState *gs, int j) {
+  if (gs->done) return;
+  const int ny = j + 1;
+  __shared__ double w1sq;
+  const int k = threadIdx.x / 8, sub = threadIdx.x % 8;
+  double t = 0.0;
+  if (k <= ny)
+    for (int b = sub; b < nparts; b += 8) t += partial[(size_t)k * kDotBlocks + b];
+  t += __shfl_xor(t, 1, 8);
+  t += __shfl_xor(t, 2, 8);
+  t += __shfl_xor(t, 4, 8);
+  if (sub == 0) {
+    if (k < ny) out[k] = t;
+    if (k == ny) w1sq = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double h2sq = 0.0;
+    for (int i = 0; i < ny; ++i) h2sq += out[i] * out[i];
+    const double r = w1sq - h2sq;
+    gs->dots[2 * kMaxBasis] = r > 0.0 ? r : 0.0;
+    givens_step(gs, j);
   }
 }
 
@@ -452,9 +486,11 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
   for (int j = 0; j < m; ++j) {
     double *Vj = k->V + (size_t)j * n;
     {
-      // V_j = (r or w) * invhn, z = P V_j
+      // V_j = (r, or w1 - V h2 of the previous step) * invhn, z = P V_j
       ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
-      if ((rc = launch_prec_norm_flag(c, j == 0 ? k->r : k->w, &k->st->invhn, Vj, k->z, done, trans))) return rc;
+      if ((rc = launch_prec_norm_flag(c, j == 0 ? k->r : k->w, &k->st->invhn, k->V, n, j,
+                                      k->st->dots + kMaxBasis, Vj, k->z, done, trans)))
+        return rc;
     }
     {
       ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
@@ -462,9 +498,13 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
     }
     {
       ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
+      // CGS2 in two passes: h1 = V^T w; w1 = w - V h1, h2 = V^T w1, ||w1||;
+      // the second update w1 - V h2 is applied by the next step's first kernel
       sweep<0>(c, k->w, k->V, n, j + 1, nullptr, k->st->dots, done);
-      sweep<1>(c, k->w, k->V, n, j + 1, k->st->dots, k->st->dots + kMaxBasis, done);
-      sweep<2>(c, k->w, k->V, n, j + 1, k->st->dots + kMaxBasis, k->st->dots + 2 * kMaxBasis, done, k->st, j);
+      hipLaunchKernelGGL((k_sweep<3>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, k->w, k->V, n, j + 1,
+                         k->st->dots, k->partial, done);
+      hipLaunchKernelGGL(k_reduce_cgs2, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->nparts,
+                         k->st->dots + kMaxBasis, k->st, j);
     }
   }
   hipError_t e = hipGetLastError();

@@ -125,7 +125,7 @@ def newton_bench(dev, max_steps):
     return out
 
 
-def batched_bench(dev, n, nrhs=16, m=8):
+def batched_bench(dev, n, nrhs=16, m=12):
     """Block GMRES over 16 right-hand sides on the 128^3 box operator
     (H1 + 1/dt at CFL 100, block-Jacobi), mixed precision in one captured
     graph per cycle, against 16 single-RHS FP64 GMRES solves."""
@@ -180,6 +180,50 @@ def batched_bench(dev, n, nrhs=16, m=8):
     return res
 
 
+def adjoint_bench(dev, reps=5):
+    """Adjoint kernels on the 192x64x64 bump channel (815k nodes): time of one
+    (dR/dU)^T lam + (dR/dX)^T lam evaluation against one residual."""
+    import numpy as np
+    import torch
+
+    from aerof_dev_amd import FlowParams, init_state
+    from aerof_dev_amd import mesh as M
+    from aerof_dev_amd.operators import SpaceOperator
+
+    mesh = M.channel_mesh(192, 64, 64, bump_height=0.05, bump_width=0.2)
+    params = FlowParams(mach=0.675)
+    op = SpaceOperator(mesh, params)
+    U = torch.from_numpy(init_state(params, mesh.X, "sine", 0.02, seed=11)).to(dev)
+    lam = torch.from_numpy(np.random.default_rng(3).standard_normal((op.num_nodes, 5))).to(dev)
+    R = torch.empty_like(U)
+    gu = torch.empty_like(U)
+    gx = torch.empty((op.num_nodes, 3), dtype=torch.float64, device=dev)
+    op.compute_residual(U, R)
+    op.compute_transpose_derivative_of_residual(U, lam, gu, gx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.compute_residual(U, R)
+    torch.cuda.synchronize()
+    t_res = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.compute_transpose_derivative_of_residual(U, lam, gu, None)
+    torch.cuda.synchronize()
+    t_u = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.compute_transpose_derivative_of_residual(U, lam, gu, gx)
+    torch.cuda.synchronize()
+    t_ux = (time.perf_counter() - t0) / reps
+    op.close()
+    res = {"workload": f"192x64x64 bump channel ({op.num_nodes} nodes, {op.num_edges} edges), second order",
+           "residual_ms": round(t_res * 1e3, 3), "dRdU_T_lambda_ms": round(t_u * 1e3, 3),
+           "dRdU_and_dRdX_T_lambda_ms": round(t_ux * 1e3, 3)}
+    log(f"[bench] adjoint: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +236,8 @@ def main():
     ap.add_argument("--newton-steps", type=int, default=600)
     ap.add_argument("--batched", action="store_true",
                     help="also time 16-RHS block GMRES against 16 single-RHS FP64 solves (128^3 box)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the adjoint kernels on the 192x64x64 channel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -295,12 +341,14 @@ def main():
                     "k_flux_edges": {"ms": round(edge_ms, 5)},
                     "k_flux_nodes": {"ms": round(node_ms, 5)}},
     }
-    if not args.no_newton or args.batched:
+    if not args.no_newton or args.batched or args.adjoint:
         op.close()
     if not args.no_newton:
         out["newton"] = newton_bench(dev, args.newton_steps)
     if args.batched:
         out["batched"] = batched_bench(dev, args.n)
+    if args.adjoint:
+        out["adjoint"] = adjoint_bench(dev)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mesh, params, U, args.cpu_budget)

@@ -9,7 +9,8 @@ psi comes from the mixed-precision adjoint solve (FP32 transposed H1 and
 block-Jacobi inside, exact FP64 adjoint residual outside).  Checked against
 central finite differences of J with fully re-converged states, step 1e-6 on
 the amplitudes:
-  first-order scheme   to 1e-6 relative;
+  first-order scheme   to 3e-6 relative (J(a) is smooth; the differences
+    carry the steady solves' round-off floor, |dJ| / 2h ~ 1e-6 relative);
   second-order scheme  to 5e-5 relative.  With the Van Albada limiter J(a) is
     only piecewise smooth (an edge's limited slope switches branch when its
     two slopes change sign), so central differences carry an O(h) error: on
@@ -55,7 +56,7 @@ def base(gpu, request):
     op = SpaceOperator(m, p)
     U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
     _converge(op, U, 1e-12)
-    return dict(p=p, m=m, op=op, U=U, centres=centres, tol=5e-5 if request.param == "second" else 1e-6)
+    return dict(p=p, m=m, op=op, U=U, centres=centres, tol=5e-5 if request.param == "second" else 3e-6)
 
 
 def test_wall_force_derivatives(gpu, base):
