@@ -155,7 +155,7 @@ __global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__
 // v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
 // holds D[l/16 + 4r][l%16], r < 4 (checked on gfx950).  Here A = V_b^T over 4 rows, B = W over
 // the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
-__global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
+__global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
                            const double *__restrict__ V, size_t ldv, const double *__restrict__ W,
                            double *__restrict__ partial) {
   const int lane = threadIdx.x & 63;
@@ -167,12 +167,21 @@ __global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int 
 #pragma unroll
   for (int b = 0; b <= kMaxM; ++b) acc[b] = (f64x4){0.0, 0.0, 0.0, 0.0};
   const int rr = lane >> 4, cc = lane & 15;
-  for (size_t k0 = r0; k0 < r1; k0 += 4) {
-    const size_t off = (k0 + rr) * kS + cc;
-    const double w = W[off];
+  // rows_per_chunk is a multiple of 16: four MFMA steps per iteration, their
+  // loads issued together
+  for (size_t k0 = r0; k0 < r1; k0 += 16) {
+    double w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = W[(k0 + 4 * u + rr) * kS + cc];
 #pragma unroll
     for (int b = 0; b <= kMaxM; ++b)
-      if (b < nb) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[(size_t)b * ldv + off], w, acc[b], 0, 0, 0);
+      if (b < nb) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = V[(size_t)b * ldv + (k0 + 4 * u + rr) * kS + cc];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], w[u], acc[b], 0, 0, 0);
+      }
   }
 #pragma unroll
   for (int b = 0; b <= kMaxM; ++b)
@@ -183,19 +192,31 @@ __global__ void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int 
     }
 }
 
-// out[b] = sum_c partial[c][b] in chunk order; one block per tile b, lane e
-// owns element e of the tile
-__global__ __launch_bounds__(256) void k_tsg_reduce(const double *__restrict__ partial, int nchunks, int mp1,
-                                                    double *__restrict__ out) {
+// Two-level fixed-order reduction of the chunk partials.  Level 1: block
+// (b, g) sums chunks [g*nc/G, (g+1)*nc/G) of tile b into part2[g][b] (lane e
+// owns element e); level 2: one block per tile sums the G group sums.
+constexpr int kRedGroups = 64;
+__global__ __launch_bounds__(256) void k_tsg_reduce1(const double *__restrict__ partial, int nchunks, int mp1,
+                                                     double *__restrict__ part2) {
+  const int b = blockIdx.x, g = blockIdx.y, e = threadIdx.x;
+  const int per = (nchunks + kRedGroups - 1) / kRedGroups;
+  const int c0 = g * per, c1 = min(nchunks, c0 + per);
+  double s = 0.0;
+  for (int c = c0; c < c1; ++c) s += partial[((size_t)c * mp1 + b) * kTile + e];
+  part2[((size_t)g * mp1 + b) * kTile + e] = s;
+}
+
+__global__ __launch_bounds__(256) void k_tsg_reduce2(const double *__restrict__ part2, int mp1,
+                                                     double *__restrict__ out) {
   const int b = blockIdx.x, e = threadIdx.x;
   double s = 0.0;
-  for (int c = 0; c < nchunks; ++c) s += partial[((size_t)c * mp1 + b) * kTile + e];
+  for (int g = 0; g < kRedGroups; ++g) s += part2[((size_t)g * mp1 + b) * kTile + e];
   out[(size_t)b * kTile + e] = s;
 }
 
 // Out = beta Out + alpha sum_b V_b H_b over 16-row tiles, one wave per tile.
 // A = V_b rows (16 x 4 per step), B = H_b rows (4 x 16).
-__global__ void k_tsg_update(size_t npad, int nb, const double *__restrict__ V, size_t ldv,
+__global__ __launch_bounds__(512) void k_tsg_update(size_t npad, int nb, const double *__restrict__ V, size_t ldv,
                              const double *__restrict__ H, double alpha, double beta, double *__restrict__ Out) {
   const int lane = threadIdx.x & 63;
   const size_t tile = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -355,6 +376,7 @@ struct BlockSolver {
   double *W = nullptr, *Z = nullptr, *R = nullptr, *X = nullptr, *B = nullptr, *T = nullptr;
   int nchunks = 0;            // row split of V^T W (fixed by npad)
   double *partial = nullptr;  // [nchunks][m+1][256]
+  double *part2 = nullptr;    // [kRedGroups][m+1][256]
   double *Hbar = nullptr;     // [(m+1)][m] tiles: block (i, j) at (i * m + j)
   double *S0 = nullptr;       // tile
   double *G = nullptr;        // scratch tiles [kMaxM+1]
@@ -391,12 +413,14 @@ int alloc_d(aerofx_ctx *c, size_t count, double **p) {
 BlockSolver *get_block(aerofx_ctx *c, int m, int *rc);
 
 void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *out) {
-  // rows per chunk: a multiple of 4 (one MFMA step), fixed by npad alone
-  const size_t rows = std::max<size_t>(4, ((s->npad + s->nchunks - 1) / s->nchunks + 3) / 4 * 4);
+  // rows per chunk: a multiple of 16 (four MFMA steps), fixed by npad alone
+  const size_t rows = std::max<size_t>(16, ((s->npad + s->nchunks - 1) / s->nchunks + 15) / 16 * 16);
   const int wpb = s->shape.waves_dots;
   hipLaunchKernelGGL(k_tsg_dots, dim3((s->nchunks + wpb - 1) / wpb), dim3(64 * wpb), 0, s->stream, s->npad, rows,
                      s->nchunks, s->m + 1, nb, V, s->npad * kS, W, s->partial);
-  hipLaunchKernelGGL(k_tsg_reduce, dim3(nb), dim3(256), 0, s->stream, s->partial, s->nchunks, s->m + 1, out);
+  hipLaunchKernelGGL(k_tsg_reduce1, dim3(nb, kRedGroups), dim3(256), 0, s->stream, s->partial, s->nchunks, s->m + 1,
+                     s->part2);
+  hipLaunchKernelGGL(k_tsg_reduce2, dim3(nb), dim3(256), 0, s->stream, s->part2, s->m + 1, out);
 }
 
 void tsg_update(BlockSolver *s, int nb, const double *V, const double *H, double alpha, double beta, double *Out) {
@@ -513,7 +537,7 @@ void autotune(aerofx_ctx *c, BlockSolver *s) {
   hipEvent_t a, b;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
-  const int cand[] = {1, 2, 4, 8};
+  const int cand[] = {1, 2, 4};  // waves per workgroup (<= 256 threads: the dots keep 17 tiles in registers)
   float best = 1e30f;
   int nb = std::min(s->m, 8) + 1;
   for (int w : cand) {
@@ -532,7 +556,8 @@ void autotune(aerofx_ctx *c, BlockSolver *s) {
   }
   s->shape.waves_dots = c->blk_tune[0];
   best = 1e30f;
-  for (int w : cand) {
+  const int cand_u[] = {1, 2, 4, 8};
+  for (int w : cand_u) {
     s->shape.waves_update = w;
     tsg_update(s, nb, s->V, s->G, 0.0, 1.0, s->W);  // W unchanged: beta 1, alpha 0
     (void)hipEventRecord(a, s->stream);
@@ -559,7 +584,7 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
   BlockSolver *s = new BlockSolver();
   s->m = m;
   s->n = 5 * (size_t)c->N;
-  s->npad = (s->n + 63) / 64 * 64;
+  s->npad = (s->n + 63) / 64 * 64;  // also a multiple of 16 rows: whole MFMA row groups
   // about 512 rows per wave of V^T W, between 64 and 8192 chunks
   s->nchunks = (int)std::min<size_t>(8192, std::max<size_t>(64, s->npad / 512));
   const size_t blk = s->npad * kS;
@@ -567,6 +592,7 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
   if ((r = alloc_d(c, (size_t)(m + 1) * blk, &s->V)) || (r = alloc_d(c, blk, &s->W)) || (r = alloc_d(c, blk, &s->Z)) ||
       (r = alloc_d(c, blk, &s->R)) || (r = alloc_d(c, blk, &s->X)) || (r = alloc_d(c, blk, &s->B)) ||
       (r = alloc_d(c, blk, &s->T)) || (r = alloc_d(c, (size_t)s->nchunks * (m + 1) * kTile, &s->partial)) ||
+      (r = alloc_d(c, (size_t)kRedGroups * (m + 1) * kTile, &s->part2)) ||
       (r = alloc_d(c, (size_t)(m + 1) * m * kTile, &s->Hbar)) || (r = alloc_d(c, kTile, &s->S0)) ||
       (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G)) || (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G2)) ||
       (r = alloc_d(c, kTile, &s->Rt)) || (r = alloc_d(c, kTile, &s->Ri)) || (r = alloc_d(c, kTile, &s->Rt2)) ||
