@@ -24,6 +24,7 @@
 
 namespace aerofx {
 
+constexpr int kGradChunk = 4;  // neighbours per load batch of the gradient pass
 
 // ---------------------------------------------------------------------------
 // 1. gradients (+ primitive state): one lane per node, neighbours in
@@ -64,25 +65,49 @@ __global__ __launch_bounds__(kBlock) void k_gradients(DevTables t, const double 
   double acc[15];
 #pragma unroll
   for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+  // Neighbours in chunks of kGradChunk: the chunk's states and coordinates
+  // are requested together and the next chunk's indices are prefetched, so a
+  // row costs about one memory latency per chunk.  The sums are still formed
+  // one neighbour at a time in ascending order.
   const int e0 = t.rowptr[n], e1 = t.rowptr[n + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int m = t.adj[e];
-    double Vm[5];
-    load_prim<FROM_U>(W, m, gas, Vm);
-    const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
-    const double dx0 = xm.x - xn.x, dx1 = xm.y - xn.y, dx2 = xm.z - xn.z;
-    const double alpha1 = dx0 * or11 * or11;
-    const double alpha2 = (dx1 - r12or11 * dx0) * or22 * or22;
-    const double alpha3 = (dx2 - r23or22 * dx1 + psi * dx0) * or33 * or33;
-    const double w0 = alpha1 - r12or11 * alpha2 + psi * alpha3;
-    const double w1 = alpha2 - r23or22 * alpha3;
-    const double w2 = alpha3;
+  int mi[kGradChunk];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const double d = Vm[k] - Vn[k];
-      acc[k] += w0 * d;
-      acc[5 + k] += w1 * d;
-      acc[10 + k] += w2 * d;
+  for (int k = 0; k < kGradChunk; ++k) mi[k] = e0 + k < e1 ? t.adj[e0 + k] : n;
+  for (int e = e0; e < e1; e += kGradChunk) {
+    double Wm[kGradChunk][5];
+    double4 xm[kGradChunk];
+#pragma unroll
+    for (int k = 0; k < kGradChunk; ++k) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) Wm[k][q] = W[5 * (size_t)mi[k] + q];
+      xm[k] = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)mi[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kGradChunk; ++k) mi[k] = e + kGradChunk + k < e1 ? t.adj[e + kGradChunk + k] : n;
+#pragma unroll
+    for (int k = 0; k < kGradChunk; ++k) {
+      if (e + k >= e1) break;
+      double Vm[5];
+      if (FROM_U) {
+        prim_from_cons(Wm[k], gas, Vm);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) Vm[q] = Wm[k][q];
+      }
+      const double dx0 = xm[k].x - xn.x, dx1 = xm[k].y - xn.y, dx2 = xm[k].z - xn.z;
+      const double alpha1 = dx0 * or11 * or11;
+      const double alpha2 = (dx1 - r12or11 * dx0) * or22 * or22;
+      const double alpha3 = (dx2 - r23or22 * dx1 + psi * dx0) * or33 * or33;
+      const double w0 = alpha1 - r12or11 * alpha2 + psi * alpha3;
+      const double w1 = alpha2 - r23or22 * alpha3;
+      const double w2 = alpha3;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const double d = Vm[q] - Vn[q];
+        acc[q] += w0 * d;
+        acc[5 + q] += w1 * d;
+        acc[10 + q] += w2 * d;
+      }
     }
   }
   if (!OUT3) {
