@@ -114,30 +114,48 @@ __device__ __forceinline__ void block_reduce_store_1(double v, double *__restric
 
 // Arnoldi step j: Hessenberg column, rotations, residual estimate, stop flag
 // (KspSolver.C:404-447, 919-962).
-__device__ void givens_step(GmresState *s, int j) {
-  if (s->done) return;
+// Givens update of Arnoldi step j.  The block's threads stage what it reads
+// (the column h1 + h2 of H, ||w||, the stored rotations, g[j]) in LDS in one
+// parallel round; thread 0 then runs the rotation recurrence from LDS with
+// unconditional loads, so the update costs one global latency.  Every thread
+// of the block must call it (it synchronises).
+struct GivensStage {
+  double h[kMaxBasis + 1];
+  double rot[2 * kMaxBasis];
+  double gj;
+  int done;
+};
+
+__device__ void givens_step(GmresState *s, int j, const double *h2, double wsq, GivensStage &g) {
+  const int t = threadIdx.x;
+  const double hn = sqrt(wsq);
+  if (t <= kMaxBasis) {
+    double v = 0.0;
+    if (t <= j) v = s->dots[t] + h2[t];
+    if (t == j + 1) v = hn;
+    g.h[t] = v;
+  }
+  if (t < 2 * kMaxBasis) g.rot[t] = s->cs[t];
+  if (t == 0) {
+    g.gj = s->g[j];
+    g.done = s->done;
+  }
+  __syncthreads();
+  if (t != 0 || g.done) return;
   const int M = kMaxBasis;
-  // column j in registers (indices are compile-time after unrolling)
   double h[kMaxBasis + 1];
 #pragma unroll
-  for (int i = 0; i <= kMaxBasis; ++i) h[i] = 0.0;
+  for (int i = 0; i <= kMaxBasis; ++i) h[i] = g.h[i];
 #pragma unroll
-  for (int i = 0; i < kMaxBasis; ++i)
-    if (i <= j) h[i] = s->dots[i] + s->dots[kMaxBasis + i];
-  const double hn = sqrt(s->dots[2 * kMaxBasis]);
-#pragma unroll
-  for (int i = 0; i < kMaxBasis; ++i)
-    if (i == j + 1) h[i] = hn;
-  if (j + 1 == kMaxBasis) h[kMaxBasis] = hn;
-#pragma unroll
-  for (int i = 0; i < kMaxBasis - 1; ++i)
+  for (int i = 0; i < kMaxBasis - 1; ++i) {
+    const double sn = g.rot[2 * i], cs = g.rot[2 * i + 1];
+    const double hj = cs * h[i] + sn * h[i + 1];
+    const double hjp1 = -sn * h[i] + cs * h[i + 1];
     if (i < j) {
-      const double sn = s->cs[2 * i], cs = s->cs[2 * i + 1];
-      const double hj = cs * h[i] + sn * h[i + 1];
-      const double hjp1 = -sn * h[i] + cs * h[i + 1];
       h[i] = hj;
       h[i + 1] = hjp1;
     }
+  }
   double hjj = 0.0, hj1 = 0.0;
 #pragma unroll
   for (int i = 0; i < kMaxBasis; ++i)
@@ -161,8 +179,9 @@ __device__ void givens_step(GmresState *s, int j) {
     if (i < j) s->H[i * M + j] = h[i];
   s->H[j * M + j] = cs * hjj + sn * hj1;
   s->H[(j + 1) * M + j] = 0.0;
-  const double gj = s->g[j];
-  s->g[j + 1] = -sn * gj;
+  const double gj = g.gj;
+  const double gj1 = -sn * gj;
+  s->g[j + 1] = gj1;
   s->g[j] = cs * gj;
   s->iter += 1;
   s->jlast = j;
@@ -171,7 +190,7 @@ __device__ void givens_step(GmresState *s, int j) {
     return;
   }
   s->invhn = 1.0 / hn;
-  s->l2res = fabs(s->g[j + 1]);
+  s->l2res = fabs(gj1);
   if (s->l2res <= s->target || s->iter >= s->maxits) s->done = 1;
 }
 
@@ -238,41 +257,58 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__rest
     if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    out[k] = sh[0];
+  if (threadIdx.x == 0) out[k] = sh[0];
+  if (gs) {
     // after the norm of step j: the Givens update (single block launch)
-    if (gs) givens_step(gs, j);
+    __shared__ GivensStage stage;
+    givens_step(gs, j, gs->dots + kMaxBasis, sh[0], stage);
   }
 }
 
 // After the mode-3 sweep of Arnoldi step j: out[0..j] = h2 = V^T w1 and
-// ||w1||^2, summed over the partials (8 lanes per value, fixed butterfly);
-// then ||w1 - V h2||^2 = ||w1||^2 - ||h2||^2 (V orthonormal) goes to the
-// norm slot and the Givens update runs.  One block.
-__global__ __launch_bounds__(kBlock) void k_reduce_cgs2(const double *__restrict__ partial, int nparts,
-                                                        double *__restrict__ out, GmresState *gs, int j) {
+// ||w1||^2, summed over the partials (32 lanes per value, each lane's slice
+// loaded in independent batches, then a fixed butterfly); then
+// ||w1 - V h2||^2 = ||w1||^2 - ||h2||^2 (V orthonormal) goes to the norm slot
+// and the Givens update runs.  One block of kReduceThreads.
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_cgs2(const double *__restrict__ partial, int nparts,
+                                                                double *__restrict__ out, GmresState *gs, int j) {
   if (gs->done) return;
   const int ny = j + 1;
-  __shared__ double w1sq;
-  const int k = threadIdx.x / 8, sub = threadIdx.x % 8;
+  __shared__ double sh[kMaxBasis + 1];
+  const int k = threadIdx.x / 32, sub = threadIdx.x % 32;
   double t = 0.0;
-  if (k <= ny)
-    for (int b = sub; b < nparts; b += 8) t += partial[(size_t)k * kDotBlocks + b];
-  t += __shfl_xor(t, 1, 8);
-  t += __shfl_xor(t, 2, 8);
-  t += __shfl_xor(t, 4, 8);
-  if (sub == 0) {
+  if (k <= ny) {
+    const double *row = partial + (size_t)k * kDotBlocks;
+    for (int b0 = sub; b0 < nparts; b0 += 32 * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = b0 + 32 * u < nparts ? row[b0 + 32 * u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t += v[u];
+    }
+  }
+  t += __shfl_xor(t, 16, 32);
+  t += __shfl_xor(t, 8, 32);
+  t += __shfl_xor(t, 4, 32);
+  t += __shfl_xor(t, 2, 32);
+  t += __shfl_xor(t, 1, 32);
+  if (sub == 0 && k <= ny) {
+    sh[k] = t;
     if (k < ny) out[k] = t;
-    if (k == ny) w1sq = t;
   }
   __syncthreads();
+  __shared__ double wsq;
   if (threadIdx.x == 0) {
     double h2sq = 0.0;
-    for (int i = 0; i < ny; ++i) h2sq += out[i] * out[i];
-    const double r = w1sq - h2sq;
-    gs->dots[2 * kMaxBasis] = r > 0.0 ? r : 0.0;
-    givens_step(gs, j);
+    for (int i = 0; i < ny; ++i) h2sq += sh[i] * sh[i];
+    const double r = sh[ny] - h2sq;
+    wsq = r > 0.0 ? r : 0.0;
+    gs->dots[2 * kMaxBasis] = wsq;
   }
+  __syncthreads();
+  __shared__ GivensStage stage;
+  givens_step(gs, j, sh, wsq, stage);
 }
 
 // y = a x + b y, a possibly scaled by a device scalar (*as)
@@ -503,7 +539,7 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
       sweep<0>(c, k->w, k->V, n, j + 1, nullptr, k->st->dots, done);
       hipLaunchKernelGGL((k_sweep<3>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, k->w, k->V, n, j + 1,
                          k->st->dots, k->partial, done);
-      hipLaunchKernelGGL(k_reduce_cgs2, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->nparts,
+      hipLaunchKernelGGL(k_reduce_cgs2, dim3(1), dim3(kReduceThreads), 0, c->stream, k->partial, k->nparts,
                          k->st->dots + kMaxBasis, k->st, j);
     }
   }
