@@ -82,17 +82,17 @@ void destroy(aerofx_ctx *c) {
   delete c;
 }
 
-// Whole-row chunks with at most kBlock items and kMaxRows rows.
-std::vector<int> chunk_rows(const std::vector<int> &rowptr, int N, const char **why) {
+// Whole-row chunks with at most `limit` items and kMaxRows rows.
+std::vector<int> chunk_rows(const std::vector<int> &rowptr, int N, int limit, const char **why) {
   std::vector<int> ch(1, 0);
   int acc = 0, rows = 0;
   for (int n = 0; n < N; ++n) {
     int cnt = rowptr[n + 1] - rowptr[n];
-    if (cnt > kBlock) {
-      *why = "node degree exceeds 256";
+    if (cnt > limit) {
+      *why = "node degree exceeds the rows a block can hold";
       return {};
     }
-    if (acc + cnt > kBlock || rows == kMaxRows) {
+    if (acc + cnt > limit || rows == kMaxRows) {
       ch.push_back(n);
       acc = 0;
       rows = 0;
@@ -221,8 +221,8 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
       }
     }
     const char *why = nullptr;
-    std::vector<int> gchunk = chunk_rows(rowptr, N, &why);
-    std::vector<int> fchunk = why ? std::vector<int>() : chunk_rows(eup, N, &why);
+    std::vector<int> gchunk = chunk_rows(rowptr, N, kBlock, &why);
+    std::vector<int> fchunk = why ? std::vector<int>() : chunk_rows(eup, N, kFluxBlock, &why);
     if (why) {
       destroy(c);
       return set_error(AEROFX_ERR_MESH, std::string("ctx_create: ") + why);
