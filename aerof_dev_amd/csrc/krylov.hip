@@ -38,6 +38,7 @@
 namespace aerofx {
 
 constexpr int kDotBlocks = 1024;  // fixed: the reduction order never depends on the device
+constexpr int kSweepBlocks = 256;  // blocks of the basis sweeps: one per CU
 constexpr int kMaxBasis = 32;     // basis vectors handled by one sweep (m <= 31)
 constexpr int kWave = 64;
 
@@ -199,14 +200,18 @@ __device__ void givens_step(GmresState *s, int j, const double *h2, double wsq, 
 // mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
 // mode 3: w -= sum_k h_k V_k, then partial_k = sum w V_k and partial_ny = sum w w
 // The grid has nparts blocks (fixed by n); block b sums a contiguous slice
-// into partial[.][b].
+// into partial[.][b].  A lane takes two consecutive rows per step (16-B
+// loads: half the load instructions, twice the bytes in flight) and adds
+// them in row order.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__ w, const double *__restrict__ V,
                                                   size_t ldv, int ny, const double *__restrict__ h,
                                                   double *__restrict__ partial, const int *__restrict__ done) {
   if (done && *done) return;
-  const size_t slice = (n + gridDim.x - 1) / gridDim.x;
-  const size_t b0 = blockIdx.x * slice, b1 = min(n, b0 + slice);
+  // slices hold an even number of rows, so that row pairs never straddle two blocks
+  const size_t slice = ((n + gridDim.x - 1) / gridDim.x + 1) & ~(size_t)1;
+  // rounding the slice up can leave the last blocks without rows (b0 >= n)
+  const size_t b0 = min(n, blockIdx.x * slice), b1 = min(n, b0 + slice);
   double acc[kMaxBasis];
 #pragma unroll
   for (int k = 0; k < kMaxBasis; ++k) acc[k] = 0.0;
@@ -215,26 +220,46 @@ __global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k) hk[k] = k < ny ? h[k] : 0.0;
   }
-  for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
-    double v[kMaxBasis];
-#pragma unroll
-    for (int k = 0; k < kMaxBasis; ++k) v[k] = k < ny ? V[(size_t)k * ldv + i] : 0.0;
-    double wi = w[i];
+  // one row of the sweep: w_i update and accumulation, in row order per lane
+  auto row = [&](size_t i, double wi, const double *vk) {
     if (MODE != 0) {
 #pragma unroll
       for (int k = 0; k < kMaxBasis; ++k)
-        if (k < ny) wi -= hk[k] * v[k];
+        if (k < ny) wi -= hk[k] * vk[k];
       if (ny > 0) w[i] = wi;
     }
     if (MODE == 2) {
       acc[0] += wi * wi;
     } else if (MODE == 3) {
 #pragma unroll
-      for (int k = 0; k < kMaxBasis; ++k) acc[k] += k == ny ? wi * wi : wi * v[k];
+      for (int k = 0; k < kMaxBasis; ++k) acc[k] += k == ny ? wi * wi : wi * vk[k];
     } else {
 #pragma unroll
-      for (int k = 0; k < kMaxBasis; ++k) acc[k] += wi * v[k];
+      for (int k = 0; k < kMaxBasis; ++k) acc[k] += wi * vk[k];
     }
+  };
+  const size_t even_end = b0 + ((b1 - b0) & ~(size_t)1);
+  for (size_t i = b0 + 2 * threadIdx.x; i < even_end; i += 2 * kBlock) {
+    double2 v[kMaxBasis];
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k)
+      v[k] = k < ny ? *reinterpret_cast<const double2 *>(V + (size_t)k * ldv + i) : make_double2(0.0, 0.0);
+    const double2 w2 = *reinterpret_cast<const double2 *>(w + i);
+    double vk[kMaxBasis];
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) vk[k] = v[k].x;
+    row(i, w2.x, vk);
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) vk[k] = v[k].y;
+    row(i + 1, w2.y, vk);
+  }
+  // an odd row count (last block only): the lane whose next pair would start
+  // at the last row takes it, after its other rows
+  if (even_end < b1 && threadIdx.x == ((b1 - 1 - b0) / 2) % kBlock) {
+    double vk[kMaxBasis];
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) vk[k] = k < ny ? V[(size_t)k * ldv + b1 - 1] : 0.0;
+    row(b1 - 1, w[b1 - 1], vk);
   }
   if (MODE == 2)
     block_reduce_store_1(acc[0], partial);
@@ -434,6 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_fd_diff(size_t n, const double *__re
 struct Krylov {
   int m = 0;
   size_t n = 0;
+  size_t ldv = 0;             // basis stride: n rounded up to 256 B, so every V_k is 16-B aligned
   int nparts = 0;             // blocks of the dot-product sweeps (fixed by n)
   double *V = nullptr;        // [m+1][n] basis
   double *w = nullptr, *z = nullptr, *r = nullptr, *x = nullptr, *b = nullptr, *R = nullptr;
@@ -469,11 +495,13 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
   Krylov *k = new Krylov();
   k->m = m;
   k->n = 5 * (size_t)c->N;
-  // about 8 entries per lane per sweep, between 64 and kDotBlocks blocks
-  k->nparts = (int)std::min<size_t>(kDotBlocks, std::max<size_t>(64, (k->n + 8 * kBlock - 1) / (8 * kBlock)));
+  // about 8 entries per lane per sweep, between 64 blocks and one per CU (the
+  // sweeps hold up to 32 basis rows per lane: one wave per SIMD)
+  k->nparts = (int)std::min<size_t>(kSweepBlocks, std::max<size_t>(64, (k->n + 8 * kBlock - 1) / (8 * kBlock)));
   void *p = nullptr;
   int rc;
-  if ((rc = ctx_alloc(c, sizeof(double) * (size_t)(m + 1) * k->n, &p))) { delete k; return rc; }
+  k->ldv = (k->n + 31) & ~(size_t)31;
+  if ((rc = ctx_alloc(c, sizeof(double) * (size_t)(m + 1) * k->ldv, &p))) { delete k; return rc; }
   k->V = (double *)p;
   double **vecs[6] = {&k->w, &k->z, &k->r, &k->x, &k->b, &k->R};
   for (double **v : vecs) {
@@ -520,11 +548,11 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
   int *done = &k->st->done;
   int rc;
   for (int j = 0; j < m; ++j) {
-    double *Vj = k->V + (size_t)j * n;
+    double *Vj = k->V + (size_t)j * k->ldv;
     {
       // V_j = (r, or w1 - V h2 of the previous step) * invhn, z = P V_j
       ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
-      if ((rc = launch_prec_norm_flag(c, j == 0 ? k->r : k->w, &k->st->invhn, k->V, n, j,
+      if ((rc = launch_prec_norm_flag(c, j == 0 ? k->r : k->w, &k->st->invhn, k->V, k->ldv, j,
                                       k->st->dots + kMaxBasis, Vj, k->z, done, trans)))
         return rc;
     }
@@ -536,8 +564,8 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
       ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
       // CGS2 in two passes: h1 = V^T w; w1 = w - V h1, h2 = V^T w1, ||w1||;
       // the second update w1 - V h2 is applied by the next step's first kernel
-      sweep<0>(c, k->w, k->V, n, j + 1, nullptr, k->st->dots, done);
-      hipLaunchKernelGGL((k_sweep<3>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, k->w, k->V, n, j + 1,
+      sweep<0>(c, k->w, k->V, k->ldv, j + 1, nullptr, k->st->dots, done);
+      hipLaunchKernelGGL((k_sweep<3>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, k->w, k->V, k->ldv, j + 1,
                          k->st->dots, k->partial, done);
       hipLaunchKernelGGL(k_reduce_cgs2, dim3(1), dim3(kReduceThreads), 0, c->stream, k->partial, k->nparts,
                          k->st->dots + kMaxBasis, k->st, j);
@@ -622,7 +650,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
     const GmresState &hs = *k->h_st;
     if (hs.jlast >= 0) {
       hipLaunchKernelGGL(k_gmres_backsolve, dim3(1), dim3(64), 0, c->stream, k->st);
-      hipLaunchKernelGGL(k_combine, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, n, k->st);
+      hipLaunchKernelGGL(k_combine, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, k->ldv, k->st);
       if ((rc = launch_prec_apply(c, k->w, k->r, trans))) return rc;
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->r, 1.0, x, nullptr);
     }
