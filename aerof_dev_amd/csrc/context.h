@@ -156,6 +156,11 @@ int launch_prec_apply_flag(aerofx_ctx *c, const double *y, double *x, const int 
 // last Gram-Schmidt update and normalisation fused with the preconditioner)
 int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale, const double *Vb, size_t ldv,
                           int nsub, const double *h, double *V, double *x, const int *done, bool trans = false);
+// Gram-Schmidt update pass of Arnoldi step j fused with the preconditioner
+// (delayed reorthogonalisation; coefficients in coef, layout in krylov.hip)
+constexpr int kMaxBasisCoef = 32;
+int launch_dcgs_update(aerofx_ctx *c, double *V, size_t ldv, int j, const double *w, const double *coef,
+                       const int *done, const int *jlast, double *x, bool trans);
 void free_krylov(aerofx_ctx *c);
 void free_adjoint(aerofx_ctx *c);
 // (dR/dU)^T lam -> Ub and/or (dR/dX)^T lam -> Xout (adjoint.hip), enqueued on c->stream

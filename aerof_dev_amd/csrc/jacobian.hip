@@ -341,6 +341,82 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_norm(int N, const T *__restri
   for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
 }
 
+// Gram-Schmidt update pass of Arnoldi step j with delayed reorthogonalisation,
+// fused with the preconditioner (coef: a at [0, 32), d at [32, 65), 1/tau at
+// [65], 1/(tau rho) at [66]):
+//   v_j     = (u_j - sum_{k<j} a_k V_k) / tau                      (slot j, in place)
+//   u_{j+1} = (w - sum_{k<j} d_k V_k - d_j v_j) / (tau rho)        (slot j+1)
+//   x       = P u_{j+1}  (or P^T)
+// When the cycle has stopped, v_j is still finished if column j belongs to
+// the solution (jlast >= j), and nothing else is written.
+template <class T, bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_dcgs_update(int N, const T *__restrict__ Di, double *__restrict__ V,
+                                                        size_t ldv, int j, const double *__restrict__ w,
+                                                        const double *__restrict__ coef,
+                                                        const int *__restrict__ done, const int *__restrict__ jlast,
+                                                        double *__restrict__ x) {
+  const bool stop = *done != 0;
+  if (stop && *jlast < j) return;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double *u = V + (size_t)j * ldv + 5 * (size_t)n;
+  double vj[5], un[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    vj[k] = u[k];
+    un[k] = stop ? 0.0 : w[5 * (size_t)n + k];
+  }
+  // basis vectors four at a time, loads of a group in flight together;
+  // both combinations keep ascending k
+  int b = 0;
+  for (; b + 4 <= j; b += 4) {
+    double vb[4][5];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) vb[q][k] = V[(size_t)(b + q) * ldv + 5 * (size_t)n + k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double ab = coef[b + q], db = coef[kMaxBasisCoef + b + q];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        vj[k] -= ab * vb[q][k];
+        un[k] -= db * vb[q][k];
+      }
+    }
+  }
+  for (; b < j; ++b) {
+    const double ab = coef[b], db = coef[kMaxBasisCoef + b];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double vbk = V[(size_t)b * ldv + 5 * (size_t)n + k];
+      vj[k] -= ab * vbk;
+      un[k] -= db * vbk;
+    }
+  }
+  const double it = coef[2 * kMaxBasisCoef + 1];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    vj[k] *= it;
+    u[k] = vj[k];
+  }
+  if (stop) return;
+  const double dj = coef[kMaxBasisCoef + j], itr = coef[2 * kMaxBasisCoef + 2];
+  double *un_out = V + (size_t)(j + 1) * ldv + 5 * (size_t)n;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    un[k] = (un[k] - dj * vj[k]) * itr;
+    un_out[k] = un[k];
+  }
+  if (TRANS)
+    amtv5_add(Di + blk64_base(n), un, acc);
+  else
+    amv5_add(Di + blk64_base(n), un, acc);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -479,6 +555,28 @@ int launch_prec_norm_flag(aerofx_ctx *c, const double *src, const double *scale,
   else
     jacobi_norm_launch<double>(c, src, scale, Vb, ldv, nsub, h, V, x, done, trans);
   return check_launch("prec launch");
+}
+
+template <class T>
+static void dcgs_update_launch(aerofx_ctx *c, double *V, size_t ldv, int j, const double *w, const double *coef,
+                               const int *done, const int *jlast, double *x, bool trans) {
+  const int nb = (c->N + kBlock - 1) / kBlock;
+  if (trans)
+    hipLaunchKernelGGL((k_dcgs_update<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const T *)c->d_jacDi, V, ldv, j, w, coef, done, jlast, x);
+  else
+    hipLaunchKernelGGL((k_dcgs_update<T, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
+                       (const T *)c->d_jacDi, V, ldv, j, w, coef, done, jlast, x);
+}
+
+int launch_dcgs_update(aerofx_ctx *c, double *V, size_t ldv, int j, const double *w, const double *coef,
+                       const int *done, const int *jlast, double *x, bool trans) {
+  if (!c->d_jacDi) return set_error(AEROFX_ERR_STATE, "prec_apply: preconditioner not set up");
+  if (c->jac_precision)
+    dcgs_update_launch<float>(c, V, ldv, j, w, coef, done, jlast, x, trans);
+  else
+    dcgs_update_launch<double>(c, V, ldv, j, w, coef, done, jlast, x, trans);
+  return check_launch("dcgs update launch");
 }
 
 int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans) {

@@ -9,14 +9,14 @@
 // when the flag is set.  The host synchronises once per cycle.
 //
 // Orthogonalisation is classical Gram-Schmidt with one reorthogonalisation
-// pass (CGS2) in FP64, fused into three sweeps over the basis per step:
-//   sweep 1  h1 = V^T w
-//   sweep 2  w -= V h1, then h2 = V^T w   (one read of V serves both)
-//   sweep 3  w -= V h2, then ||w||^2
-// instead of the reference's j+1 dependent dot/axpy pairs (modified
-// Gram-Schmidt, KspSolver.C:404-411).  Reductions are deterministic: a fixed
-// grid of kDotBlocks blocks sums contiguous slices (wave butterflies in a
-// fixed pattern), and one block sums the partials in order.
+// pass (CGS2) in FP64, the reorthogonalisation of each new vector delayed to
+// the next Arnoldi step (DCGS2) so that a step reads the basis twice and
+// reduces once: one sweep takes V^T [u_j, w] and the norms, one update pass
+// finishes v_j and forms the next once-projected vector u_{j+1} (see
+// k_dcgs_reduce for the algebra), instead of the reference's j+1 dependent
+// dot/axpy pairs (modified Gram-Schmidt, KspSolver.C:404-411).  Reductions
+// are deterministic: a fixed grid of blocks sums contiguous slices (wave
+// butterflies in a fixed pattern), and one block sums the partials in order.
 //
 // The Newton driver follows TsSolver/NewtonSolver for a steady problem with
 // one Newton iteration per pseudo-time step (NewtonSolver.h:70-231,
@@ -40,16 +40,22 @@ namespace aerofx {
 constexpr int kDotBlocks = 1024;  // fixed: the reduction order never depends on the device
 constexpr int kSweepBlocks = 256;  // blocks of the basis sweeps: one per CU
 constexpr int kMaxBasis = 32;     // basis vectors handled by one sweep (m <= 31)
+static_assert(kMaxBasis == kMaxBasisCoef, "coefficient layout shared with the update kernel");
 constexpr int kWave = 64;
 
 // Device-resident GMRES state (one per right-hand side).
+// Coefficients of the Gram-Schmidt update kernel, in GmresState::coef:
+// a at [0, 32), d at [32, 65), 1/tau at [65], 1/(tau rho) at [66].
+constexpr int kCoefD = kMaxBasis, kCoefInvTau = 2 * kMaxBasis + 1, kCoefInvTauRho = 2 * kMaxBasis + 2;
 struct GmresState {
-  double H[(kMaxBasis + 1) * kMaxBasis];  // row-major (i, j) -> H[i * kMaxBasis + j]
-  double cs[2 * kMaxBasis];               // (sin, cos) per rotation
+  double H[(kMaxBasis + 1) * kMaxBasis];   // rotated columns (R), row-major (i, j) -> H[i * kMaxBasis + j]
+  double Hh[(kMaxBasis + 1) * kMaxBasis];  // the Hessenberg columns themselves, same layout
+  double cs[2 * kMaxBasis];                // (sin, cos) per rotation
   double g[kMaxBasis + 1];
   double y[kMaxBasis];
-  double dots[3 * kMaxBasis];             // h1 | h2 | ||w||^2
-  double beta, target, l2res, invhn;
+  double cpart[kMaxBasis + 1];             // once-projected coefficients of the newest column
+  double coef[2 * kMaxBasis + 4];
+  double beta, target, l2res, invhn, rho;
   int iter, jlast, done, maxits, m;
 };
 
@@ -113,32 +119,23 @@ __device__ __forceinline__ void block_reduce_store_1(double v, double *__restric
   }
 }
 
-// Arnoldi step j: Hessenberg column, rotations, residual estimate, stop flag
-// (KspSolver.C:404-447, 919-962).
-// Givens update of Arnoldi step j.  The block's threads stage what it reads
-// (the column h1 + h2 of H, ||w||, the stored rotations, g[j]) in LDS in one
-// parallel round; thread 0 then runs the rotation recurrence from LDS with
-// unconditional loads, so the update costs one global latency.  Every thread
-// of the block must call it (it synchronises).
+// Givens update of Hessenberg column jc (KspSolver.C:404-447, 919-962):
+// rotations, residual estimate, stop flag.  The column (jc + 2 entries, the
+// last one the subdiagonal) is in LDS; the block's threads stage the stored
+// rotations and g[jc] in LDS in one parallel round, then thread 0 runs the
+// rotation recurrence from LDS with unconditional loads.  Every thread of the
+// block must call it (it synchronises).
 struct GivensStage {
-  double h[kMaxBasis + 1];
   double rot[2 * kMaxBasis];
   double gj;
   int done;
 };
 
-__device__ void givens_step(GmresState *s, int j, const double *h2, double wsq, GivensStage &g) {
+__device__ void givens_col(GmresState *s, int jc, const double *col, GivensStage &g) {
   const int t = threadIdx.x;
-  const double hn = sqrt(wsq);
-  if (t <= kMaxBasis) {
-    double v = 0.0;
-    if (t <= j) v = s->dots[t] + h2[t];
-    if (t == j + 1) v = hn;
-    g.h[t] = v;
-  }
   if (t < 2 * kMaxBasis) g.rot[t] = s->cs[t];
   if (t == 0) {
-    g.gj = s->g[j];
+    g.gj = s->g[jc];
     g.done = s->done;
   }
   __syncthreads();
@@ -146,13 +143,14 @@ __device__ void givens_step(GmresState *s, int j, const double *h2, double wsq, 
   const int M = kMaxBasis;
   double h[kMaxBasis + 1];
 #pragma unroll
-  for (int i = 0; i <= kMaxBasis; ++i) h[i] = g.h[i];
+  for (int i = 0; i <= kMaxBasis; ++i) h[i] = i <= jc + 1 ? col[i] : 0.0;
+  const double hn = col[jc + 1];
 #pragma unroll
   for (int i = 0; i < kMaxBasis - 1; ++i) {
     const double sn = g.rot[2 * i], cs = g.rot[2 * i + 1];
     const double hj = cs * h[i] + sn * h[i + 1];
     const double hjp1 = -sn * h[i] + cs * h[i + 1];
-    if (i < j) {
+    if (i < jc) {
       h[i] = hj;
       h[i + 1] = hjp1;
     }
@@ -160,7 +158,7 @@ __device__ void givens_step(GmresState *s, int j, const double *h2, double wsq, 
   double hjj = 0.0, hj1 = 0.0;
 #pragma unroll
   for (int i = 0; i < kMaxBasis; ++i)
-    if (i == j) {
+    if (i == jc) {
       hjj = h[i];
       hj1 = h[i + 1];
     }
@@ -173,105 +171,124 @@ __device__ void givens_step(GmresState *s, int j, const double *h2, double wsq, 
     cs = sqrt(hjj * hjj) * coef;
     sn = cs * hj1 / hjj;
   }
-  s->cs[2 * j + 1] = cs;
-  s->cs[2 * j] = sn;
+  s->cs[2 * jc + 1] = cs;
+  s->cs[2 * jc] = sn;
 #pragma unroll
   for (int i = 0; i < kMaxBasis; ++i)
-    if (i < j) s->H[i * M + j] = h[i];
-  s->H[j * M + j] = cs * hjj + sn * hj1;
-  s->H[(j + 1) * M + j] = 0.0;
+    if (i < jc) s->H[i * M + jc] = h[i];
+  s->H[jc * M + jc] = cs * hjj + sn * hj1;
+  s->H[(jc + 1) * M + jc] = 0.0;
   const double gj = g.gj;
   const double gj1 = -sn * gj;
-  s->g[j + 1] = gj1;
-  s->g[j] = cs * gj;
+  s->g[jc + 1] = gj1;
+  s->g[jc] = cs * gj;
   s->iter += 1;
-  s->jlast = j;
+  s->jlast = jc;
   if (hn == 0.0) {
     s->done = 1;
     return;
   }
-  s->invhn = 1.0 / hn;
   s->l2res = fabs(gj1);
   if (s->l2res <= s->target || s->iter >= s->maxits) s->done = 1;
 }
 
-// mode 0: partial_k = sum w V_k
-// mode 1: w -= sum_k h_k V_k, then partial_k = sum w V_k
-// mode 2: w -= sum_k h_k V_k, then partial_0 = sum w w
-// mode 3: w -= sum_k h_k V_k, then partial_k = sum w V_k and partial_ny = sum w w
-// The grid has nparts blocks (fixed by n); block b sums a contiguous slice
-// into partial[.][b].  A lane takes two consecutive rows per step (16-B
-// loads: half the load instructions, twice the bytes in flight) and adds
-// them in row order.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_sweep(size_t n, double *__restrict__ w, const double *__restrict__ V,
-                                                  size_t ldv, int ny, const double *__restrict__ h,
-                                                  double *__restrict__ partial, const int *__restrict__ done) {
-  if (done && *done) return;
-  // slices hold an even number of rows, so that row pairs never straddle two blocks
-  const size_t slice = ((n + gridDim.x - 1) / gridDim.x + 1) & ~(size_t)1;
-  // rounding the slice up can leave the last blocks without rows (b0 >= n)
-  const size_t b0 = min(n, blockIdx.x * slice), b1 = min(n, b0 + slice);
-  double acc[kMaxBasis];
-#pragma unroll
-  for (int k = 0; k < kMaxBasis; ++k) acc[k] = 0.0;
-  double hk[kMaxBasis];
-  if (MODE != 0) {
-#pragma unroll
-    for (int k = 0; k < kMaxBasis; ++k) hk[k] = k < ny ? h[k] : 0.0;
+// Row pairs of a block's slice: the grid has nparts blocks (fixed by n, at
+// most one per CU); block b covers a contiguous slice of even length, a lane
+// takes two consecutive rows per step (16-B loads) and handles them in row
+// order; an odd last row goes to the lane whose next pair would start there.
+struct RowPairs {
+  size_t b0, b1, even_end;
+  __device__ RowPairs(size_t n) {
+    const size_t slice = ((n + gridDim.x - 1) / gridDim.x + 1) & ~(size_t)1;
+    b0 = min(n, blockIdx.x * slice);  // rounding the slice up can leave the last blocks empty
+    b1 = min(n, b0 + slice);
+    even_end = b0 + ((b1 - b0) & ~(size_t)1);
   }
-  // one row of the sweep: w_i update and accumulation, in row order per lane
-  auto row = [&](size_t i, double wi, const double *vk) {
-    if (MODE != 0) {
+  __device__ bool has_tail() const { return even_end < b1 && threadIdx.x == ((b1 - 1 - b0) / 2) % kBlock; }
+};
+
+// partial[b] = sum over block b's slice of w_i^2
+__global__ __launch_bounds__(kBlock) void k_sumsq(size_t n, const double *__restrict__ w,
+                                                  double *__restrict__ partial) {
+  const RowPairs rp(n);
+  double acc = 0.0;
+  for (size_t i = rp.b0 + 2 * threadIdx.x; i < rp.even_end; i += 2 * kBlock) {
+    const double2 w2 = *reinterpret_cast<const double2 *>(w + i);
+    acc += w2.x * w2.x;
+    acc += w2.y * w2.y;
+  }
+  if (rp.has_tail()) acc += w[rp.b1 - 1] * w[rp.b1 - 1];
+  block_reduce_store_1(acc, partial);
+}
+
+// Gram-Schmidt with delayed reorthogonalisation (DCGS2: classical
+// Gram-Schmidt twice, the second pass for a vector folded into the next
+// Arnoldi step).  At step j the basis slots hold v_0 .. v_{j-1} (final,
+// orthonormal) and u_j = (A' v_{j-1} - V c) / rho, projected once (c, rho
+// from step j-1; u_0 = r / ||r||), and w = A' u_j, A' = H1 P.  One sweep
+// over the slots gives, per block,
+//   a = V_{j-1}^T u_j,  b = V_{j-1}^T w,  alpha = u.u,  beta = u.w,  gamma = w.w
+// (partial rows 0.., 32.., 64, 65, 66).  The final sweep of a cycle (FINAL)
+// needs a and alpha only.
+template <bool FINAL>
+__global__ __launch_bounds__(kBlock) void k_dots2(size_t n, const double *__restrict__ V, size_t ldv, int j,
+                                                  const double *__restrict__ w, double *__restrict__ partial,
+                                                  const int *__restrict__ done) {
+  if (*done) return;
+  const RowPairs rp(n);
+  const double *u = V + (size_t)j * ldv;
+  double aa[kMaxBasis], bb[kMaxBasis], al = 0.0, be = 0.0, ga = 0.0;
 #pragma unroll
-      for (int k = 0; k < kMaxBasis; ++k)
-        if (k < ny) wi -= hk[k] * vk[k];
-      if (ny > 0) w[i] = wi;
+  for (int k = 0; k < kMaxBasis; ++k) aa[k] = bb[k] = 0.0;
+  auto row = [&](double ui, double wi, const double *vk) {
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) {
+      aa[k] += vk[k] * ui;
+      if (!FINAL) bb[k] += vk[k] * wi;
     }
-    if (MODE == 2) {
-      acc[0] += wi * wi;
-    } else if (MODE == 3) {
-#pragma unroll
-      for (int k = 0; k < kMaxBasis; ++k) acc[k] += k == ny ? wi * wi : wi * vk[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < kMaxBasis; ++k) acc[k] += wi * vk[k];
+    al += ui * ui;
+    if (!FINAL) {
+      be += ui * wi;
+      ga += wi * wi;
     }
   };
-  const size_t even_end = b0 + ((b1 - b0) & ~(size_t)1);
-  for (size_t i = b0 + 2 * threadIdx.x; i < even_end; i += 2 * kBlock) {
+  for (size_t i = rp.b0 + 2 * threadIdx.x; i < rp.even_end; i += 2 * kBlock) {
     double2 v[kMaxBasis];
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k)
-      v[k] = k < ny ? *reinterpret_cast<const double2 *>(V + (size_t)k * ldv + i) : make_double2(0.0, 0.0);
-    const double2 w2 = *reinterpret_cast<const double2 *>(w + i);
+      v[k] = k < j ? *reinterpret_cast<const double2 *>(V + (size_t)k * ldv + i) : make_double2(0.0, 0.0);
+    const double2 u2 = *reinterpret_cast<const double2 *>(u + i);
+    const double2 w2 = FINAL ? make_double2(0.0, 0.0) : *reinterpret_cast<const double2 *>(w + i);
     double vk[kMaxBasis];
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k) vk[k] = v[k].x;
-    row(i, w2.x, vk);
+    row(u2.x, w2.x, vk);
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k) vk[k] = v[k].y;
-    row(i + 1, w2.y, vk);
+    row(u2.y, w2.y, vk);
   }
-  // an odd row count (last block only): the lane whose next pair would start
-  // at the last row takes it, after its other rows
-  if (even_end < b1 && threadIdx.x == ((b1 - 1 - b0) / 2) % kBlock) {
+  if (rp.has_tail()) {
+    const size_t i = rp.b1 - 1;
     double vk[kMaxBasis];
 #pragma unroll
-    for (int k = 0; k < kMaxBasis; ++k) vk[k] = k < ny ? V[(size_t)k * ldv + b1 - 1] : 0.0;
-    row(b1 - 1, w[b1 - 1], vk);
+    for (int k = 0; k < kMaxBasis; ++k) vk[k] = k < j ? V[(size_t)k * ldv + i] : 0.0;
+    row(u[i], FINAL ? 0.0 : w[i], vk);
   }
-  if (MODE == 2)
-    block_reduce_store_1(acc[0], partial);
-  else
-    block_reduce_store_k(acc, MODE == 3 ? ny + 1 : ny, gridDim.x, partial);
+  block_reduce_store_k(aa, j, gridDim.x, partial);
+  __syncthreads();  // the reduction's LDS is reused
+  if (!FINAL) {
+    block_reduce_store_k(bb, j, gridDim.x, partial + (size_t)kMaxBasis * kDotBlocks);
+    __syncthreads();
+  }
+  double sc[kMaxBasis];
+#pragma unroll
+  for (int k = 0; k < kMaxBasis; ++k) sc[k] = k == 0 ? al : k == 1 ? be : k == 2 ? ga : 0.0;
+  block_reduce_store_k(sc, 3, gridDim.x, partial + (size_t)2 * kMaxBasis * kDotBlocks);
 }
 
 // out[k] = sum_b partial[k][b] in a fixed tree, one block per k
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial, int nparts,
-                                                            double *__restrict__ out, const int *__restrict__ done,
-                                                            GmresState *gs, int j) {
-  if (done && *done) return;
+                                                            double *__restrict__ out) {
   __shared__ double sh[kBlock];
   const int k = blockIdx.x;
   double s = 0.0;
@@ -283,57 +300,126 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__rest
     __syncthreads();
   }
   if (threadIdx.x == 0) out[k] = sh[0];
-  if (gs) {
-    // after the norm of step j: the Givens update (single block launch)
-    __shared__ GivensStage stage;
-    givens_step(gs, j, gs->dots + kMaxBasis, sh[0], stage);
-  }
 }
 
-// After the mode-3 sweep of Arnoldi step j: out[0..j] = h2 = V^T w1 and
-// ||w1||^2, summed over the partials (32 lanes per value, each lane's slice
-// loaded in independent batches, then a fixed butterfly); then
-// ||w1 - V h2||^2 = ||w1||^2 - ||h2||^2 (V orthonormal) goes to the norm slot
-// and the Givens update runs.  One block of kReduceThreads.
+// After the sweep of step j (one block of kReduceThreads): sums the partials
+// (16 lanes per value, fixed butterfly), then
+//  1. tau = ||u_j - V a|| = sqrt(alpha - a.a): v_j = (u_j - V a) / tau.
+//  2. Column j-1 is complete: A' v_{j-1} = V (c + rho a) + rho tau v_j; it goes
+//     to Hh and through the Givens update (stop test).  At j = 0 instead the
+//     right-hand side g_0 = ||r|| becomes ||r|| tau (v_0 = u_0 / tau).
+//  3. Unless stopped (or FINAL): with g = Hbar_{j-1} a (A' V_{j-1} = V_j Hbar),
+//     A' v_j = (w - V_j g) / tau, so h = V_j^T A' v_j = (p - g) / tau with
+//     p = (b, (beta - a.b) / tau), and ||A' v_j||^2 = (gamma - 2 g.p + g.g) / tau^2.
+//     The next vector u_{j+1} = (A' v_j - V_j h) / rho' = (w - V_j d) / (tau rho'),
+//     d = g + tau h, rho'^2 = ||A' v_j||^2 - h.h (its exact norm is taken at
+//     the next step, so rho' only scales).  If rho'^2 is not above 1e-28 of
+//     ||A' v_j||^2 the Krylov space is exhausted: column j is final as it
+//     is, and the cycle stops.
+// coef (a, d, 1/tau, 1/(tau rho')) feeds k_dcgs_update.
 constexpr int kReduceThreads = 1024;
-__global__ __launch_bounds__(kReduceThreads) void k_reduce_cgs2(const double *__restrict__ partial, int nparts,
-                                                                double *__restrict__ out, GmresState *gs, int j) {
-  if (gs->done) return;
-  const int ny = j + 1;
-  __shared__ double sh[kMaxBasis + 1];
-  const int k = threadIdx.x / 32, sub = threadIdx.x % 32;
-  double t = 0.0;
-  if (k <= ny) {
-    const double *row = partial + (size_t)k * kDotBlocks;
-    for (int b0 = sub; b0 < nparts; b0 += 32 * 8) {
+template <bool FINAL>
+__global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__restrict__ partial, int nparts,
+                                                                GmresState *st, int j) {
+  if (st->done) return;
+  __shared__ double sa[kMaxBasis], sb[kMaxBasis], ssc[3], col[kMaxBasis + 2], sg[kMaxBasis + 1];
+  __shared__ double stau;
+  __shared__ int sbreak;
+  __shared__ GivensStage gstage;
+  const int t = threadIdx.x, grp = t / 16, sub = t % 16;
+  const int nval = FINAL ? j + 1 : 2 * j + 3;
+  if (grp < nval) {
+    int row;
+    if (grp < j) row = grp;                                           // a
+    else if (!FINAL && grp < 2 * j) row = kMaxBasis + (grp - j);      // b
+    else row = 2 * kMaxBasis + (grp - (FINAL ? j : 2 * j));           // alpha, beta, gamma
+    const double *pr = partial + (size_t)row * kDotBlocks;
+    double s = 0.0;
+    for (int b0 = sub; b0 < nparts; b0 += 16 * 8) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = b0 + 32 * u < nparts ? row[b0 + 32 * u] : 0.0;
+      for (int u = 0; u < 8; ++u) v[u] = b0 + 16 * u < nparts ? pr[b0 + 16 * u] : 0.0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) t += v[u];
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    s += __shfl_xor(s, 8, 16);
+    s += __shfl_xor(s, 4, 16);
+    s += __shfl_xor(s, 2, 16);
+    s += __shfl_xor(s, 1, 16);
+    if (sub == 0) {
+      if (grp < j) sa[grp] = s;
+      else if (!FINAL && grp < 2 * j) sb[grp - j] = s;
+      else ssc[grp - (FINAL ? j : 2 * j)] = s;
     }
   }
-  t += __shfl_xor(t, 16, 32);
-  t += __shfl_xor(t, 8, 32);
-  t += __shfl_xor(t, 4, 32);
-  t += __shfl_xor(t, 2, 32);
-  t += __shfl_xor(t, 1, 32);
-  if (sub == 0 && k <= ny) {
-    sh[k] = t;
-    if (k < ny) out[k] = t;
+  __syncthreads();
+  if (t == 0) {
+    double asq = 0.0;
+    for (int k = 0; k < j; ++k) asq += sa[k] * sa[k];
+    const double r = ssc[0] - asq;
+    stau = r > 0.0 ? sqrt(r) : 0.0;
   }
   __syncthreads();
-  __shared__ double wsq;
-  if (threadIdx.x == 0) {
-    double h2sq = 0.0;
-    for (int i = 0; i < ny; ++i) h2sq += sh[i] * sh[i];
-    const double r = sh[ny] - h2sq;
-    wsq = r > 0.0 ? r : 0.0;
-    gs->dots[2 * kMaxBasis] = wsq;
+  const double tau = stau;
+  if (j >= 1) {
+    // column j-1, complete now
+    const double rho = st->rho;
+    if (t <= kMaxBasis + 1) {
+      double v = 0.0;
+      if (t < j) v = st->cpart[t] + rho * sa[t];
+      if (t == j) v = rho * tau;
+      col[t] = v;
+      if (t <= j) st->Hh[t * kMaxBasis + (j - 1)] = v;
+    }
+    __syncthreads();
+    givens_col(st, j - 1, col, gstage);
+    __syncthreads();
+    if (FINAL || st->done) return;
+  } else if (t == 0) {
+    st->g[0] *= tau;
+  }
+  if (FINAL) return;
+  // g = Hbar_{j-1} a (rows 0..j), in ascending column order
+  if (t <= j) {
+    double gt = 0.0;
+    for (int cc = 0; cc < j; ++cc) gt += st->Hh[t * kMaxBasis + cc] * sa[cc];
+    sg[t] = gt;
   }
   __syncthreads();
-  __shared__ GivensStage stage;
-  givens_step(gs, j, sh, wsq, stage);
+  if (t == 0) {
+    double ab = 0.0;
+    for (int k = 0; k < j; ++k) ab += sa[k] * sb[k];
+    const double inv_tau = 1.0 / tau;
+    double gp = 0.0, gg = 0.0, hh = 0.0;
+    for (int k = 0; k <= j; ++k) {
+      const double p = k < j ? sb[k] : (ssc[1] - ab) * inv_tau;
+      const double h = (p - sg[k]) * inv_tau;
+      gp += sg[k] * p;
+      gg += sg[k] * sg[k];
+      hh += h * h;
+      st->cpart[k] = h;
+      st->coef[kCoefD + k] = sg[k] + tau * h;
+      col[k] = h;
+    }
+    for (int k = 0; k < j; ++k) st->coef[k] = sa[k];
+    const double av2 = (ssc[2] - 2.0 * gp + gg) * inv_tau * inv_tau;
+    const double r2 = av2 - hh;
+    st->coef[kCoefInvTau] = inv_tau;
+    if (r2 > 1e-28 * av2) {
+      const double rho = sqrt(r2);
+      st->rho = rho;
+      st->coef[kCoefInvTauRho] = inv_tau / rho;
+      sbreak = 0;
+    } else {
+      // exhausted Krylov space: column j is h with a zero subdiagonal
+      col[j + 1] = 0.0;
+      for (int k = 0; k <= j; ++k) st->Hh[k * kMaxBasis + j] = col[k];
+      st->Hh[(j + 1) * kMaxBasis + j] = 0.0;
+      sbreak = 1;
+    }
+  }
+  __syncthreads();
+  if (sbreak) givens_col(st, j, col, gstage);  // sets the stop flag (zero subdiagonal)
 }
 
 // y = a x + b y, a possibly scaled by a device scalar (*as)
@@ -463,7 +549,7 @@ struct Krylov {
   int nparts = 0;             // blocks of the dot-product sweeps (fixed by n)
   double *V = nullptr;        // [m+1][n] basis
   double *w = nullptr, *z = nullptr, *r = nullptr, *x = nullptr, *b = nullptr, *R = nullptr;
-  double *partial = nullptr;  // [kMaxBasis][kDotBlocks]
+  double *partial = nullptr;  // [2 kMaxBasis + 3][kDotBlocks]
   double *scal = nullptr;     // device scalars
   GmresState *st = nullptr;   // device
   GmresState *h_st = nullptr; // pinned host mirror
@@ -508,7 +594,7 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
     if ((rc = ctx_alloc(c, sizeof(double) * k->n, &p))) { delete k; return rc; }
     *v = (double *)p;
   }
-  if ((rc = ctx_alloc(c, sizeof(double) * kMaxBasis * kDotBlocks, &p))) { delete k; return rc; }
+  if ((rc = ctx_alloc(c, sizeof(double) * (2 * kMaxBasis + 3) * kDotBlocks, &p))) { delete k; return rc; }
   k->partial = (double *)p;
   if ((rc = ctx_alloc(c, sizeof(double) * 16, &p))) { delete k; return rc; }
   k->scal = (double *)p;
@@ -524,14 +610,11 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
 
 static inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-template <int MODE>
-static void sweep(aerofx_ctx *c, double *w, const double *V, size_t ldv, int ny, const double *h, double *out,
-                  const int *done, GmresState *gs = nullptr, int j = -1) {
+// out[0] = w.w (fixed reduction order)
+static void norm2(aerofx_ctx *c, const double *w, double *out) {
   Krylov *k = c->ksp;
-  hipLaunchKernelGGL((k_sweep<MODE>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, V, ldv, ny, h,
-                     k->partial, done);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(MODE == 2 ? 1 : ny), dim3(kBlock), 0, c->stream, k->partial,
-                     k->nparts, out, done, MODE == 2 ? gs : nullptr, j);
+  hipLaunchKernelGGL(k_sumsq, dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, k->partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->nparts, out);
 }
 
 static int fetch_state(aerofx_ctx *c) {
@@ -541,35 +624,48 @@ static int fetch_state(aerofx_ctx *c) {
 }
 
 // The m Arnoldi steps of one cycle (no host decisions: a converged cycle
-// leaves the remaining kernels to return at once on the stop flag).
+// leaves the remaining kernels to return at once on the stop flag).  Per
+// step: H1 apply, one sweep over the basis (k_dots2), its reduction with the
+// small Gram-Schmidt algebra and the Givens update of the previous column
+// (k_dcgs_reduce), and one update pass over the basis fused with the
+// preconditioner (k_dcgs_update); a last sweep and reduction complete the
+// cycle's final column.
 static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
   Krylov *k = c->ksp;
   const size_t n = k->n;
   int *done = &k->st->done;
   int rc;
+  {
+    // u_0 = r / ||r||, z = P u_0
+    ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+    if ((rc = launch_prec_norm_flag(c, k->r, &k->st->invhn, k->V, k->ldv, 0, nullptr, k->V, k->z, done, trans)))
+      return rc;
+  }
   for (int j = 0; j < m; ++j) {
-    double *Vj = k->V + (size_t)j * k->ldv;
-    {
-      // V_j = (r, or w1 - V h2 of the previous step) * invhn, z = P V_j
-      ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
-      if ((rc = launch_prec_norm_flag(c, j == 0 ? k->r : k->w, &k->st->invhn, k->V, k->ldv, j,
-                                      k->st->dots + kMaxBasis, Vj, k->z, done, trans)))
-        return rc;
-    }
     {
       ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
       if ((rc = launch_h1_apply_flag(c, k->z, k->w, done, trans))) return rc;
     }
     {
       ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
-      // CGS2 in two passes: h1 = V^T w; w1 = w - V h1, h2 = V^T w1, ||w1||;
-      // the second update w1 - V h2 is applied by the next step's first kernel
-      sweep<0>(c, k->w, k->V, k->ldv, j + 1, nullptr, k->st->dots, done);
-      hipLaunchKernelGGL((k_sweep<3>), dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, k->w, k->V, k->ldv, j + 1,
-                         k->st->dots, k->partial, done);
-      hipLaunchKernelGGL(k_reduce_cgs2, dim3(1), dim3(kReduceThreads), 0, c->stream, k->partial, k->nparts,
-                         k->st->dots + kMaxBasis, k->st, j);
+      hipLaunchKernelGGL((k_dots2<false>), dim3(k->nparts), dim3(kBlock), 0, c->stream, n, k->V, k->ldv, j, k->w,
+                         k->partial, done);
+      hipLaunchKernelGGL((k_dcgs_reduce<false>), dim3(1), dim3(kReduceThreads), 0, c->stream, k->partial,
+                         k->nparts, k->st, j);
     }
+    {
+      ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+      if ((rc = launch_dcgs_update(c, k->V, k->ldv, j, k->w, k->st->coef, &k->st->done, &k->st->jlast, k->z,
+                                   trans)))
+        return rc;
+    }
+  }
+  {
+    ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
+    hipLaunchKernelGGL((k_dots2<true>), dim3(k->nparts), dim3(kBlock), 0, c->stream, n, k->V, k->ldv, m, nullptr,
+                       k->partial, done);
+    hipLaunchKernelGGL((k_dcgs_reduce<true>), dim3(1), dim3(kReduceThreads), 0, c->stream, k->partial, k->nparts,
+                       k->st, m);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "arnoldi launch");
@@ -643,7 +739,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->w, 1.0, k->r,
                          nullptr);
     }
-    sweep<2>(c, k->r, nullptr, 0, 0, nullptr, k->scal, nullptr);
+    norm2(c, k->r, k->scal);
     hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(64), 0, c->stream, k->st, k->scal, cycle == 0 ? 1 : 0, eps);
     if ((rc = arnoldi_cycle(c, m, trans))) return rc;
     if ((rc = fetch_state(c))) return rc;
@@ -672,10 +768,9 @@ static int jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *prod,
   Krylov *k = c->ksp;
   const size_t n = k->n;
   // computeEpsilon
-  sweep<2>(c, const_cast<double *>(p), nullptr, 0, 0, nullptr, k->scal, nullptr);
+  norm2(c, p, k->scal);
   hipLaunchKernelGGL(k_eps_sum, dim3(kDotBlocks), dim3(kBlock), 0, c->stream, n, U, k->partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, kDotBlocks, k->scal + 1,
-                     nullptr, nullptr, -1);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, kDotBlocks, k->scal + 1);
   hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, c->stream, k->scal, (double)n);
   double *Fa = k->w, *Fb = k->z, *Q = k->r;
   // F(U + eps p)
@@ -732,7 +827,7 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
   }
   if ((rc = launch_jacobian(c, o->cfl > 0.0 ? c->d_dt : nullptr, o->precision))) return rc;
   // ||rhs||
-  sweep<2>(c, const_cast<double *>(rhs), nullptr, 0, 0, nullptr, k->scal, nullptr);
+  norm2(c, rhs, k->scal);
   hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
@@ -752,7 +847,7 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 1.0, k->b,
                          nullptr);
     }
-    sweep<2>(c, k->b, nullptr, 0, 0, nullptr, k->scal, nullptr);
+    norm2(c, k->b, k->scal);
     e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
@@ -887,7 +982,7 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     // spatial residual and its norm (TsDesc::computeResidualNorm); one
     // synchronisation per step reads the norm and the error counters
     if ((rc = launch_residual(c, U, k->R, true))) return rc;
-    sweep<2>(c, k->R, nullptr, 0, 0, nullptr, k->scal, nullptr);
+    norm2(c, k->R, k->scal);
     hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
