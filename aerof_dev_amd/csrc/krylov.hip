@@ -323,10 +323,15 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
                                                                 GmresState *st, int j) {
   if (st->done) return;
   __shared__ double sa[kMaxBasis], sb[kMaxBasis], ssc[3], col[kMaxBasis + 2], sg[kMaxBasis + 1];
+  __shared__ double sH[(kMaxBasis + 1) * kMaxBasis];  // Hbar_{j-1}, rows 0..j
   __shared__ double stau;
   __shared__ int sbreak;
   __shared__ GivensStage gstage;
   const int t = threadIdx.x, grp = t / 16, sub = t % 16;
+  // columns 0..j-2 of the Hessenberg matrix (final), loaded with the partials
+  if (!FINAL)
+    for (int e = t; e < (j + 1) * kMaxBasis; e += kReduceThreads)
+      if (e % kMaxBasis + 1 < j) sH[e] = st->Hh[e];
   const int nval = FINAL ? j + 1 : 2 * j + 3;
   if (grp < nval) {
     int row;
@@ -369,7 +374,10 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
       if (t < j) v = st->cpart[t] + rho * sa[t];
       if (t == j) v = rho * tau;
       col[t] = v;
-      if (t <= j) st->Hh[t * kMaxBasis + (j - 1)] = v;
+      if (t <= j) {
+        st->Hh[t * kMaxBasis + (j - 1)] = v;
+        sH[t * kMaxBasis + (j - 1)] = v;
+      }
     }
     __syncthreads();
     givens_col(st, j - 1, col, gstage);
@@ -382,7 +390,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
   // g = Hbar_{j-1} a (rows 0..j), in ascending column order
   if (t <= j) {
     double gt = 0.0;
-    for (int cc = 0; cc < j; ++cc) gt += st->Hh[t * kMaxBasis + cc] * sa[cc];
+    for (int cc = 0; cc < j; ++cc) gt += sH[t * kMaxBasis + cc] * sa[cc];
     sg[t] = gt;
   }
   __syncthreads();
@@ -466,14 +474,24 @@ __global__ void k_gmres_start(GmresState *s, const double *__restrict__ nrm2, in
 }
 
 // back-substitution (KspSolver.C:968-982)
-__global__ void k_gmres_backsolve(GmresState *s) {
-  if (threadIdx.x != 0) return;
-  const int M = kMaxBasis, j = s->jlast;
-  for (int i = j; i >= 0; --i) {
-    double yi = s->g[i];
-    for (int k = j; k >= i + 1; --k) yi -= s->H[i * M + k] * s->y[k];
-    s->y[i] = yi / s->H[i * M + i];
+// (the triangle and g are staged in LDS by the block first, then one lane
+// runs the recurrence in the reference's order)
+__global__ __launch_bounds__(256) void k_gmres_backsolve(GmresState *s) {
+  constexpr int M = kMaxBasis;
+  __shared__ double sH[M * M], sg[M], sy[M];
+  const int j = s->jlast;
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x) sH[e] = s->H[e];
+  if (threadIdx.x < M) sg[threadIdx.x] = s->g[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = j; i >= 0; --i) {
+      double yi = sg[i];
+      for (int k = j; k >= i + 1; --k) yi -= sH[i * M + k] * sy[k];
+      sy[i] = yi / sH[i * M + i];
+    }
   }
+  __syncthreads();
+  if ((int)threadIdx.x <= j) s->y[threadIdx.x] = sy[threadIdx.x];
 }
 
 // count nodes whose density or pressure is not positive
@@ -745,7 +763,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
     if ((rc = fetch_state(c))) return rc;
     const GmresState &hs = *k->h_st;
     if (hs.jlast >= 0) {
-      hipLaunchKernelGGL(k_gmres_backsolve, dim3(1), dim3(64), 0, c->stream, k->st);
+      hipLaunchKernelGGL(k_gmres_backsolve, dim3(1), dim3(256), 0, c->stream, k->st);
       hipLaunchKernelGGL(k_combine, grid_for(n), dim3(kBlock), 0, c->stream, n, k->w, k->V, k->ldv, k->st);
       if ((rc = launch_prec_apply(c, k->w, k->r, trans))) return rc;
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->r, 1.0, x, nullptr);
