@@ -104,6 +104,14 @@ __device__ __forceinline__ void block_reduce_store_k(double (&v)[kMaxBasis], int
   (void)nparts;
 }
 
+// sum over lanes 0..31 (the values of lanes 32..63 are summed separately),
+// fixed butterfly; every lane of the half-wave gets the total
+__device__ __forceinline__ double wsum32(double v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+  return v;
+}
+
 __device__ __forceinline__ void block_reduce_store_1(double v, double *__restrict__ partial) {
   __shared__ double sh1[kBlock / kWave];
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
@@ -358,11 +366,13 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
     }
   }
   __syncthreads();
-  if (t == 0) {
-    double asq = 0.0;
-    for (int k = 0; k < j; ++k) asq += sa[k] * sa[k];
-    const double r = ssc[0] - asq;
-    stau = r > 0.0 ? sqrt(r) : 0.0;
+  if (t < 32) {
+    // a.a over one half-wave, fixed butterfly
+    const double asq = wsum32(t < j ? sa[t] * sa[t] : 0.0);
+    if (t == 0) {
+      const double r = ssc[0] - asq;
+      stau = r > 0.0 ? sqrt(r) : 0.0;
+    }
   }
   __syncthreads();
   const double tau = stau;
@@ -394,37 +404,41 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
     sg[t] = gt;
   }
   __syncthreads();
-  if (t == 0) {
-    double ab = 0.0;
-    for (int k = 0; k < j; ++k) ab += sa[k] * sb[k];
+  if (t < 32) {
+    // one lane per basis index k <= j; sums over the half-wave, fixed butterfly
+    const int k = t;
+    const double ab = wsum32(k < j ? sa[k] * sb[k] : 0.0);
     const double inv_tau = 1.0 / tau;
-    double gp = 0.0, gg = 0.0, hh = 0.0;
-    for (int k = 0; k <= j; ++k) {
-      const double p = k < j ? sb[k] : (ssc[1] - ab) * inv_tau;
-      const double h = (p - sg[k]) * inv_tau;
-      gp += sg[k] * p;
-      gg += sg[k] * sg[k];
-      hh += h * h;
+    double p = 0.0, h = 0.0, gk = 0.0;
+    if (k <= j) {
+      gk = sg[k];
+      p = k < j ? sb[k] : (ssc[1] - ab) * inv_tau;
+      h = (p - gk) * inv_tau;
       st->cpart[k] = h;
-      st->coef[kCoefD + k] = sg[k] + tau * h;
+      st->coef[kCoefD + k] = gk + tau * h;
       col[k] = h;
+      if (k < j) st->coef[k] = sa[k];
     }
-    for (int k = 0; k < j; ++k) st->coef[k] = sa[k];
-    const double av2 = (ssc[2] - 2.0 * gp + gg) * inv_tau * inv_tau;
-    const double r2 = av2 - hh;
-    st->coef[kCoefInvTau] = inv_tau;
-    if (r2 > 1e-28 * av2) {
-      const double rho = sqrt(r2);
-      st->rho = rho;
-      st->coef[kCoefInvTauRho] = inv_tau / rho;
-      sbreak = 0;
-    } else {
-      // exhausted Krylov space: column j is h with a zero subdiagonal
-      col[j + 1] = 0.0;
-      for (int k = 0; k <= j; ++k) st->Hh[k * kMaxBasis + j] = col[k];
-      st->Hh[(j + 1) * kMaxBasis + j] = 0.0;
-      sbreak = 1;
+    const double gp = wsum32(gk * p), gg = wsum32(gk * gk), hh = wsum32(h * h);
+    if (k == 0) {
+      const double av2 = (ssc[2] - 2.0 * gp + gg) * inv_tau * inv_tau;
+      const double r2 = av2 - hh;
+      st->coef[kCoefInvTau] = inv_tau;
+      if (r2 > 1e-28 * av2) {
+        const double rho = sqrt(r2);
+        st->rho = rho;
+        st->coef[kCoefInvTauRho] = inv_tau / rho;
+        sbreak = 0;
+      } else {
+        sbreak = 1;
+      }
     }
+  }
+  __syncthreads();
+  if (sbreak && t <= j + 1) {
+    // exhausted Krylov space: column j is h with a zero subdiagonal
+    if (t == j + 1) col[t] = 0.0;
+    st->Hh[t * kMaxBasis + j] = t <= j ? col[t] : 0.0;
   }
   __syncthreads();
   if (sbreak) givens_col(st, j, col, gstage);  // sets the stop flag (zero subdiagonal)
