@@ -260,13 +260,22 @@ __global__ __launch_bounds__(kBlock) void k_dots2(size_t n, const double *__rest
       ga += wi * wi;
     }
   };
-  for (size_t i = rp.b0 + 2 * threadIdx.x; i < rp.even_end; i += 2 * kBlock) {
-    double2 v[kMaxBasis];
+  // software-pipelined: the next row pair's loads are in flight while the
+  // current pair is added (one wave per SIMD: nothing else hides latency)
+  auto load = [&](size_t i, double2 (&v)[kMaxBasis], double2 &u2, double2 &w2) {
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k)
       v[k] = k < j ? *reinterpret_cast<const double2 *>(V + (size_t)k * ldv + i) : make_double2(0.0, 0.0);
-    const double2 u2 = *reinterpret_cast<const double2 *>(u + i);
-    const double2 w2 = FINAL ? make_double2(0.0, 0.0) : *reinterpret_cast<const double2 *>(w + i);
+    u2 = *reinterpret_cast<const double2 *>(u + i);
+    w2 = FINAL ? make_double2(0.0, 0.0) : *reinterpret_cast<const double2 *>(w + i);
+  };
+  size_t i = rp.b0 + 2 * threadIdx.x;
+  double2 v[kMaxBasis], u2, w2;
+  if (i < rp.even_end) load(i, v, u2, w2);
+  while (i < rp.even_end) {
+    const size_t inext = i + 2 * kBlock;
+    double2 vn[kMaxBasis], un2, wn2;
+    if (inext < rp.even_end) load(inext, vn, un2, wn2);
     double vk[kMaxBasis];
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k) vk[k] = v[k].x;
@@ -274,6 +283,11 @@ __global__ __launch_bounds__(kBlock) void k_dots2(size_t n, const double *__rest
 #pragma unroll
     for (int k = 0; k < kMaxBasis; ++k) vk[k] = v[k].y;
     row(u2.y, w2.y, vk);
+#pragma unroll
+    for (int k = 0; k < kMaxBasis; ++k) v[k] = vn[k];
+    u2 = un2;
+    w2 = wn2;
+    i = inext;
   }
   if (rp.has_tail()) {
     const size_t i = rp.b1 - 1;
