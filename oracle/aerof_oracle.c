@@ -1383,9 +1383,15 @@ static int fd_function(const or_mesh *m, const or_params *p, const double *Q, co
 int or_jvp_fd(const or_mesh *m, const or_params *p, const double *U, const double *pv, int order,
               const double *dt, double *prod) {
   size_t n = 5 * (size_t)m->N;
-  double *Q = (double *)malloc(sizeof(double) * n);
+  double *Q = (double *)calloc(n ? n : 1, sizeof(double));
   double *Fa = (double *)malloc(sizeof(double) * n);
   double *Fb = (double *)malloc(sizeof(double) * n);
+  if (!Q || !Fa || !Fb) {
+    free(Q);
+    free(Fa);
+    free(Fb);
+    return -1;
+  }
   double eps = fd_epsilon(n, U, pv);
   int ierr = 0;
   for (size_t k = 0; k < n; ++k) Q[k] = U[k] + eps * pv[k];
