@@ -104,20 +104,74 @@ __device__ __forceinline__ void bmv5_add(const T *__restrict__ a, const double *
     acc[i] += r[5 * i] * b[0] + r[5 * i + 1] * b[1] + r[5 * i + 2] * b[2] + r[5 * i + 3] * b[3] + r[5 * i + 4] * b[4];
 }
 
+// Block H1 apply, entry-parallel: a block covers whole rows (<= kBEnt
+// entries, <= kBRows rows); four lanes take one entry, four columns each, so
+// a wave reads the interleaved blocks of 16 consecutive entries at once and
+// the 16-column rows of P as 32-B pieces.  Products go through LDS and each
+// (row, component, column) sums its diagonal product and then its entries in
+// order: per column exactly the single-vector operator's sums.
+constexpr int kBEnt = 64, kBRows = 16, kBCols = 4;
+
 template <class T>
-__global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
-                                                   const double *__restrict__ P, double *__restrict__ Y) {
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int n = gid / kS, c = gid % kS;
-  if (n >= t.N) return;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  bmv5_add(D + blk64_base(n), P + (size_t)n * 5 * kS, c, acc);
-  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) {
-    const int m = t.adj[e];
-    bmv5_add(B + blk64_base(e), P + (size_t)m * 5 * kS, c, acc);
+__device__ __forceinline__ void bmv5x4(const T *__restrict__ a, const double *__restrict__ p, int cg,
+                                       double acc[5][kBCols]) {
+  double b[5][kBCols], r[25];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const double4 v = *reinterpret_cast<const double4 *>(p + k * kS + kBCols * cg);
+    b[k][0] = v.x;
+    b[k][1] = v.y;
+    b[k][2] = v.z;
+    b[k][3] = v.w;
   }
 #pragma unroll
-  for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = acc[k];
+  for (int k = 0; k < 25; ++k) r[k] = (double)a[k * kBlkStride];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int u = 0; u < kBCols; ++u)
+      acc[i][u] = r[5 * i] * b[0][u] + r[5 * i + 1] * b[1][u] + r[5 * i + 2] * b[2][u] + r[5 * i + 3] * b[3][u] +
+                  r[5 * i + 4] * b[4][u];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, int nchunks, const int *__restrict__ chunk,
+                                                   const T *__restrict__ D, const T *__restrict__ B,
+                                                   const double *__restrict__ P, double *__restrict__ Y) {
+  static_assert(kBEnt * kS / kBCols == 256, "four lanes per entry, 256 lanes per block");
+  __shared__ double sc[kBEnt * 5 * kS];
+  __shared__ double sd[kBRows * 5 * kS];
+  const int c = xcd_chunk(blockIdx.x, nchunks);
+  const int r0 = chunk[c], r1 = chunk[c + 1];
+  const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
+  const int tid = threadIdx.x, ent = tid / (kS / kBCols), cg = tid % (kS / kBCols);
+  double acc[5][kBCols];
+  if (ent < e1 - e0) {
+    const int e = e0 + ent;
+    bmv5x4(B + blk64_base(e), P + (size_t)t.adj[e] * 5 * kS, cg, acc);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int u = 0; u < kBCols; ++u) sc[(ent * 5 + k) * kS + kBCols * cg + u] = acc[k][u];
+  }
+  if (ent < r1 - r0) {
+    const int n = r0 + ent;
+    bmv5x4(D + blk64_base(n), P + (size_t)n * 5 * kS, cg, acc);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int u = 0; u < kBCols; ++u) sd[(ent * 5 + k) * kS + kBCols * cg + u] = 0.0 + acc[k][u];
+  }
+  __syncthreads();
+  const int nr = r1 - r0;
+  for (int idx = tid; idx < nr * 5 * kS; idx += 256) {
+    const int rr = idx / (5 * kS), q = idx - rr * 5 * kS;
+    const int n = r0 + rr;
+    double sum = sd[rr * 5 * kS + q];
+    const int b = t.rowptr[n] - e0, eend = t.rowptr[n + 1] - e0;
+    for (int e = b; e < eend; ++e) sum += sc[e * 5 * kS + q];
+    Y[(size_t)n * 5 * kS + q] = sum;
+  }
 }
 
 template <class T>
@@ -386,6 +440,8 @@ struct BlockSolver {
   double *nrm = nullptr;      // [16]
   int *flag = nullptr;
   float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
+  int *bchunk = nullptr;      // row chunks of the block H1 apply (<= kBEnt entries, <= kBRows rows)
+  int nbchunks = 0;
   long long f32_version = -1;     // assembly the FP32 copies were made from
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -447,7 +503,8 @@ void apply_ops(aerofx_ctx *c, BlockSolver *s, const T *D, const T *B, const T *D
                double *W) {
   const int nb = (int)((c->N * (size_t)kS + 255) / 256);
   hipLaunchKernelGGL((k_bjacobi_apply<T>), dim3(nb), dim3(256), 0, s->stream, c->N, Di, Vj, Z);
-  hipLaunchKernelGGL((k_bh1_apply<T>), dim3(nb), dim3(256), 0, s->stream, c->tab, D, B, Z, W);
+  hipLaunchKernelGGL((k_bh1_apply<T>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks, s->bchunk, D,
+                     B, Z, W);
 }
 
 // one inner cycle: from s->R (the residual block), builds V, Hbar, S0
@@ -609,6 +666,46 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
     return nullptr;
   }
   s->flag = (int *)v;
+  {
+    // row chunks for the block H1 apply, from the adjacency row pointers
+    std::vector<int> rp(c->N + 1), ch(1, 0);
+    hipError_t e0 = hipMemcpy(rp.data(), c->tab.rowptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost);
+    if (e0 != hipSuccess) {
+      delete s;
+      *rc = set_hip_error(e0, "block solver row pointers");
+      return nullptr;
+    }
+    int acc = 0, rows = 0;
+    for (int n = 0; n < c->N; ++n) {
+      const int cnt = rp[n + 1] - rp[n];
+      if (cnt > kBEnt) {
+        delete s;
+        *rc = set_error(AEROFX_ERR_MESH, "block_gmres: a node has more than 64 neighbours");
+        return nullptr;
+      }
+      if (acc + cnt > kBEnt || rows == kBRows) {
+        ch.push_back(n);
+        acc = 0;
+        rows = 0;
+      }
+      acc += cnt;
+      rows += 1;
+    }
+    ch.push_back(c->N);
+    s->nbchunks = (int)ch.size() - 1;
+    if ((r = ctx_alloc(c, sizeof(int) * ch.size(), &v))) {
+      delete s;
+      *rc = r;
+      return nullptr;
+    }
+    s->bchunk = (int *)v;
+    e0 = hipMemcpy(s->bchunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice);
+    if (e0 != hipSuccess) {
+      delete s;
+      *rc = set_hip_error(e0, "block solver chunks");
+      return nullptr;
+    }
+  }
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_nrm, sizeof(double) * kS, hipHostMallocDefault);
   if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_flag, sizeof(int) * 4, hipHostMallocDefault);
@@ -708,8 +805,8 @@ extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, dou
   int cycle = 0;
   for (;; ++cycle) {
     // R = B - A64 X (FP64 refinement residual)
-    hipLaunchKernelGGL((k_bh1_apply<double>), dim3(nbk), dim3(256), 0, s->stream, c->tab,
-                       (const double *)c->d_jacD, (const double *)c->d_jacB, s->X, s->R);
+    hipLaunchKernelGGL((k_bh1_apply<double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
+                       s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->X, s->R);
     hipLaunchKernelGGL(k_bsub_from, g1(blk), dim3(256), 0, s->stream, blk, s->B, s->R);
     hipLaunchKernelGGL(k_colnorm_partial, dim3(kChunks), dim3(256), 0, s->stream, s->npad, s->R, s->nrm);
     hipLaunchKernelGGL(k_colnorm_reduce, dim3(1), dim3(64), 0, s->stream, s->nrm, s->nrm + (size_t)kChunks * kS);
