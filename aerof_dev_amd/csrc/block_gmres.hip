@@ -112,18 +112,27 @@ __device__ __forceinline__ void bmv5_add(const T *__restrict__ a, const double *
 // order: per column exactly the single-vector operator's sums.
 constexpr int kBEnt = 64, kBRows = 16, kBCols = 4;
 
-template <class T>
-__device__ __forceinline__ void bmv5x4(const T *__restrict__ a, const double *__restrict__ p, int cg,
+__device__ __forceinline__ void load4(const double *p, double b[kBCols]) {
+  const double4 v = *reinterpret_cast<const double4 *>(p);
+  b[0] = v.x;
+  b[1] = v.y;
+  b[2] = v.z;
+  b[3] = v.w;
+}
+__device__ __forceinline__ void load4(const float *p, double b[kBCols]) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  b[0] = v.x;
+  b[1] = v.y;
+  b[2] = v.z;
+  b[3] = v.w;
+}
+
+template <class T, class TP>
+__device__ __forceinline__ void bmv5x4(const T *__restrict__ a, const TP *__restrict__ p, int cg,
                                        double acc[5][kBCols]) {
   double b[5][kBCols], r[25];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const double4 v = *reinterpret_cast<const double4 *>(p + k * kS + kBCols * cg);
-    b[k][0] = v.x;
-    b[k][1] = v.y;
-    b[k][2] = v.z;
-    b[k][3] = v.w;
-  }
+  for (int k = 0; k < 5; ++k) load4(p + k * kS + kBCols * cg, b[k]);
 #pragma unroll
   for (int k = 0; k < 25; ++k) r[k] = (double)a[k * kBlkStride];
 #pragma unroll
@@ -134,10 +143,10 @@ __device__ __forceinline__ void bmv5x4(const T *__restrict__ a, const double *__
                   r[5 * i + 4] * b[4][u];
 }
 
-template <class T>
+template <class T, class TP>
 __global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, int nchunks, const int *__restrict__ chunk,
                                                    const T *__restrict__ D, const T *__restrict__ B,
-                                                   const double *__restrict__ P, double *__restrict__ Y) {
+                                                   const TP *__restrict__ P, double *__restrict__ Y) {
   static_assert(kBEnt * kS / kBCols == 256, "four lanes per entry, 256 lanes per block");
   __shared__ double sc[kBEnt * 5 * kS];
   __shared__ double sd[kBRows * 5 * kS];
@@ -174,16 +183,16 @@ __global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, int nchunks, con
   }
 }
 
-template <class T>
+template <class T, class TO>
 __global__ __launch_bounds__(256) void k_bjacobi_apply(int N, const T *__restrict__ Di, const double *__restrict__ P,
-                                                       double *__restrict__ Y) {
+                                                       TO *__restrict__ Y) {
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int n = gid / kS, c = gid % kS;
   if (n >= N) return;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   bmv5_add(Di + blk64_base(n), P + (size_t)n * 5 * kS, c, acc);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = acc[k];
+  for (int k = 0; k < 5; ++k) Y[((size_t)n * 5 + k) * kS + c] = (TO)acc[k];
 }
 
 // Y = B - Y (refinement residual), over the padded block
@@ -440,6 +449,7 @@ struct BlockSolver {
   double *nrm = nullptr;      // [16]
   int *flag = nullptr;
   float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
+  float *Zf = nullptr;        // [npad][16] FP32 preconditioned block (input of the FP32 H1 apply)
   int *bchunk = nullptr;      // row chunks of the block H1 apply (<= kBEnt entries, <= kBRows rows)
   int nbchunks = 0;
   long long f32_version = -1;     // assembly the FP32 copies were made from
@@ -498,13 +508,21 @@ void cholqr2(BlockSolver *s, const double *W, double *Q, double *Rout) {
   hipLaunchKernelGGL(k_mm16, dim3(1), dim3(256), 0, s->stream, s->Rt2, s->Rt, Rout);
 }
 
-template <class T>
-void apply_ops(aerofx_ctx *c, BlockSolver *s, const T *D, const T *B, const T *Di, const double *Vj, double *Z,
-               double *W) {
+// Z = P V_j, W = H1 Z: FP64 operators on FP64 vectors, or FP32 operators on
+// an FP32 Z (the products are formed in FP64)
+void apply_ops64(aerofx_ctx *c, BlockSolver *s, const double *Vj) {
   const int nb = (int)((c->N * (size_t)kS + 255) / 256);
-  hipLaunchKernelGGL((k_bjacobi_apply<T>), dim3(nb), dim3(256), 0, s->stream, c->N, Di, Vj, Z);
-  hipLaunchKernelGGL((k_bh1_apply<T>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks, s->bchunk, D,
-                     B, Z, W);
+  hipLaunchKernelGGL((k_bjacobi_apply<double, double>), dim3(nb), dim3(256), 0, s->stream, c->N,
+                     (const double *)c->d_jacDi, Vj, s->Z);
+  hipLaunchKernelGGL((k_bh1_apply<double, double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab,
+                     s->nbchunks, s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->Z, s->W);
+}
+
+void apply_ops32(aerofx_ctx *c, BlockSolver *s, const double *Vj) {
+  const int nb = (int)((c->N * (size_t)kS + 255) / 256);
+  hipLaunchKernelGGL((k_bjacobi_apply<float, float>), dim3(nb), dim3(256), 0, s->stream, c->N, s->Di32, Vj, s->Zf);
+  hipLaunchKernelGGL((k_bh1_apply<float, float>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
+                     s->bchunk, s->D32, s->B32, s->Zf, s->W);
 }
 
 // one inner cycle: from s->R (the residual block), builds V, Hbar, S0
@@ -522,10 +540,9 @@ void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
   for (int j = 0; j < s->m; ++j) {
     const double *Vj = s->V + (size_t)j * blk;
     if (prec)
-      apply_ops<float>(c, s, s->D32, s->B32, s->Di32, Vj, s->Z, s->W);
+      apply_ops32(c, s, Vj);
     else
-      apply_ops<double>(c, s, (const double *)c->d_jacD, (const double *)c->d_jacB, (const double *)c->d_jacDi, Vj,
-                        s->Z, s->W);
+      apply_ops64(c, s, Vj);
     // block CGS2
     tsg_dots(s, j + 1, s->V, s->W, s->G);
     tsg_update(s, j + 1, s->V, s->G, -1.0, 1.0, s->W);
@@ -740,6 +757,8 @@ int ensure_f32(aerofx_ctx *c, BlockSolver *s) {
     s->B32 = (float *)v;
     if ((rc = ctx_alloc(c, nd * sizeof(float), &v))) return rc;
     s->Di32 = (float *)v;
+    if ((rc = ctx_alloc(c, s->npad * kS * sizeof(float), &v))) return rc;
+    s->Zf = (float *)v;
   }
   hipLaunchKernelGGL(k_f64_to_f32, g1(nd), dim3(256), 0, s->stream, nd, (const double *)c->d_jacD, s->D32);
   hipLaunchKernelGGL(k_f64_to_f32, g1(nbk), dim3(256), 0, s->stream, nbk, (const double *)c->d_jacB, s->B32);
@@ -805,7 +824,7 @@ extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, dou
   int cycle = 0;
   for (;; ++cycle) {
     // R = B - A64 X (FP64 refinement residual)
-    hipLaunchKernelGGL((k_bh1_apply<double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
+    hipLaunchKernelGGL((k_bh1_apply<double, double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
                        s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->X, s->R);
     hipLaunchKernelGGL(k_bsub_from, g1(blk), dim3(256), 0, s->stream, blk, s->B, s->R);
     hipLaunchKernelGGL(k_colnorm_partial, dim3(kChunks), dim3(256), 0, s->stream, s->npad, s->R, s->nrm);
@@ -861,9 +880,10 @@ extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, dou
     // X += P (V Y)
     tsg_update(s, m, s->V, s->Y, 1.0, 0.0, s->W);
     if (precision)
-      hipLaunchKernelGGL((k_bjacobi_apply<float>), dim3(nbk), dim3(256), 0, s->stream, c->N, s->Di32, s->W, s->Z);
+      hipLaunchKernelGGL((k_bjacobi_apply<float, double>), dim3(nbk), dim3(256), 0, s->stream, c->N, s->Di32, s->W,
+                         s->Z);
     else
-      hipLaunchKernelGGL((k_bjacobi_apply<double>), dim3(nbk), dim3(256), 0, s->stream, c->N,
+      hipLaunchKernelGGL((k_bjacobi_apply<double, double>), dim3(nbk), dim3(256), 0, s->stream, c->N,
                          (const double *)c->d_jacDi, s->W, s->Z);
     hipLaunchKernelGGL(k_badd, g1(blk), dim3(256), 0, s->stream, blk, s->Z, s->X);
     st->inner_its += m;
