@@ -51,7 +51,7 @@ struct DevTables {
   const int *mirror;     // [2E] entry (m, n) of the entry (n, m)
   // flux pass A: edges grouped by their lower node i (edge order)
   int nfchunks;
-  const int *fchunk;     // [nfchunks+1] row starts
+  const int4 *fchunk4;   // [nfchunks] (first row, end row, first edge, end edge) of each chunk
   const int *eup;        // [N+1] first edge whose lower node is i
   const int2 *eij;       // [E] edge endpoints (i, j), i < j
 

@@ -227,6 +227,10 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
       destroy(c);
       return set_error(AEROFX_ERR_MESH, std::string("ctx_create: ") + why);
     }
+    // one 16-B record per flux chunk: a block reads its rows and edges at once
+    std::vector<int4> fchunk4(fchunk.empty() ? 0 : fchunk.size() - 1);
+    for (size_t k = 0; k + 1 < fchunk.size(); ++k)
+      fchunk4[k] = make_int4(fchunk[k], fchunk[k + 1], eup[fchunk[k]], eup[fchunk[k + 1]]);
     // node-derived tables
     std::vector<double> X4(4 * (size_t)N), lsq6(6 * (size_t)N), invcv(N), fnorm3(3 * (size_t)F);
     for (int n = 0; n < N; ++n) {
@@ -252,7 +256,7 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     t.F = F;
     if ((rc = upload(c, X4, &t.X4)) || (rc = upload(c, lsq6, &t.lsq6)) || (rc = upload(c, invcv, &t.invcv)) || (rc = upload(c, c->cv, &t.cv)) ||
         (rc = upload(c, gchunk, &t.gchunk)) || (rc = upload(c, rowptr, &t.rowptr)) ||
-        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, eid, &t.eidx)) || (rc = upload(c, mirror, &t.mirror)) || (rc = upload(c, fchunk, &t.fchunk)) || (rc = upload(c, eup, &t.eup)) ||
+        (rc = upload(c, adj, &t.adj)) || (rc = upload(c, eid, &t.eidx)) || (rc = upload(c, mirror, &t.mirror)) || (rc = upload(c, fchunk4, &t.fchunk4)) || (rc = upload(c, eup, &t.eup)) ||
         (rc = upload(c, eij, &t.eij)) || (rc = upload(c, ledge, &t.ledge)) ||
         (rc = upload(c, c->normals, &t.normal)) || (rc = upload(c, lrow, &t.lrow)) ||
         (rc = upload(c, frow, &t.frow)) || (rc = upload(c, fidx, &t.fidx)) || (rc = upload(c, fcode, &t.fcode)) ||

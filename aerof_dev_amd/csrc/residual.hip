@@ -138,9 +138,8 @@ __global__ __launch_bounds__(kFluxBlock) void k_flux_edges(DevTables t, const do
                                                        double *__restrict__ Fm, double *__restrict__ P,
                                                        int *__restrict__ err) {
   __shared__ double sf[5 * (kFluxBlock + 1)];
-  const int c = xcd_chunk(blockIdx.x, t.nfchunks);
-  const int r0 = t.fchunk[c], r1 = t.fchunk[c + 1];
-  const int l0 = t.eup[r0], l1 = t.eup[r1];
+  const int4 ch = t.fchunk4[xcd_chunk(blockIdx.x, t.nfchunks)];
+  const int r0 = ch.x, r1 = ch.y, l0 = ch.z, l1 = ch.w;
   const int tid = threadIdx.x;
   const Gas &g = fc.gas;
   if (tid < l1 - l0) {
