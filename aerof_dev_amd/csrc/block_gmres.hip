@@ -279,8 +279,10 @@ __global__ __launch_bounds__(256) void k_tsg_reduce2(const double *__restrict__ 
 
 // Out = beta Out + alpha sum_b V_b H_b over 16-row tiles, one wave per tile.
 // A = V_b rows (16 x 4 per step), B = H_b rows (4 x 16).
-__global__ __launch_bounds__(512) void k_tsg_update(size_t npad, int nb, const double *__restrict__ V, size_t ldv,
-                             const double *__restrict__ H, double alpha, double beta, double *__restrict__ Out) {
+// Out may be one of the V_b (in place): a wave reads all of its 16 rows of
+// every V_b before it writes them.
+__global__ __launch_bounds__(512) void k_tsg_update(size_t npad, int nb, const double *V, size_t ldv,
+                             const double *__restrict__ H, double alpha, double beta, double *Out) {
   const int lane = threadIdx.x & 63;
   const size_t tile = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (tile * 16 >= npad) return;
@@ -309,9 +311,7 @@ __global__ __launch_bounds__(512) void k_tsg_update(size_t npad, int nb, const d
 // numerically rank-deficient block) restarts the factorisation of G + s I
 // with s = 1e-13 max diag(G) (shifted Cholesky-QR; the second Cholesky-QR
 // pass restores orthogonality); flag = 1 only if that fails too.
-__global__ void k_chol16(const double *__restrict__ G, double *__restrict__ R, double *__restrict__ Rinv,
-                         int *__restrict__ flag) {
-  if (threadIdx.x != 0) return;
+__device__ void chol16(const double *G, double *R, double *Rinv, int *flag) {
   double a[kTile], r[kTile], ri[kTile];
   double dmax = 0.0;
   for (int e = 0; e < kTile; ++e) {
@@ -360,6 +360,11 @@ __global__ void k_chol16(const double *__restrict__ G, double *__restrict__ R, d
   }
 }
 
+__global__ void k_chol16(const double *__restrict__ G, double *__restrict__ R, double *__restrict__ Rinv,
+                         int *__restrict__ flag) {
+  if (threadIdx.x == 0) chol16(G, R, Rinv, flag);
+}
+
 // C = A B (16x16), and optionally D += E
 __global__ void k_mm16(const double *__restrict__ A, const double *__restrict__ B, double *__restrict__ C) {
   const int e = threadIdx.x;
@@ -370,10 +375,125 @@ __global__ void k_mm16(const double *__restrict__ A, const double *__restrict__ 
   C[e] = s;
 }
 
-__global__ void k_add_tiles(int ntiles, const double *__restrict__ A, const double *__restrict__ B,
-                            double *__restrict__ C) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < ntiles * kTile) C[e] = A[e] + B[e];
+// ---------------------------------------------------------------------------
+// block Gram-Schmidt with delayed reorthogonalisation (block DCGS2): the
+// 16x16 tile algebra of one Arnoldi step, one block of 256 threads (one per
+// tile element; thread 0 runs the Cholesky factorisations)
+// ---------------------------------------------------------------------------
+// c[e] = sum_l op(a)[i][l] b[l][k] for the thread's element e = (i, k)
+__device__ __forceinline__ double tdot(const double *a, const double *b, bool ta) {
+  const int i = threadIdx.x / kS, k = threadIdx.x % kS;
+  double s = 0.0;
+#pragma unroll
+  for (int l = 0; l < kS; ++l) s += (ta ? a[l * kS + i] : a[i * kS + l]) * b[l * kS + k];
+  return s;
+}
+
+// At step j the slots hold V_0..V_{j-1} (final), U_j = W1_{j-1} P^-1 (the
+// previous step's once-projected block, P from its Gram estimate), and
+// W = A' U_j in slot j+1.  G = V^T U_j and G2 = V^T W over slots 0..j+1 give
+// A_b, alpha = U^T U, B_b, beta = U^T W, gamma = W^T W.  Then (tiles):
+//   T^T T = alpha - sum A_b^T A_b;  V_j = (U_j - sum V_b A_b) T^-1
+//   column j-1 of H: C_b + A_b P (b < j), T P (row j)     [S0 <- T S0 at j = 0]
+//   G_k = sum_c H_kc A_c;  p_k = B_k (k < j), p_j = T^-T (beta - sum A_b^T B_b)
+//   h_k = (p_k - G_k) T^-1
+//   Gram = T^-T (gamma - sum (G_k^T p_k + p_k^T G_k - G_k^T G_k)) T^-1 - sum h_k^T h_k = P'^T P'
+//   U_{j+1} = W M - sum_{k<=j} V_k D_k,  M = T^-1 P'^-1,  D_k = (G_k T^-1 + h_k) P'^-1
+// and (C, P) <- (h, P').  The update coefficients go to cV (V_j over slots
+// 0..j) and cU (U_{j+1} over slots 0..j+1).  FINAL (j = m): only T and column
+// m-1.  Work tiles: wk[0..7] scalars-like tiles, then G_k, p_j and h_k.
+constexpr int kWkTiles = 8 + 3 * (kMaxM + 2);
+template <bool FINAL>
+__global__ __launch_bounds__(256) void k_bdcgs_tiles(const double *__restrict__ G, const double *__restrict__ G2,
+                                                     double *__restrict__ Hbar, int m, int j, double *__restrict__ Cp,
+                                                     double *__restrict__ Pp, double *__restrict__ S0,
+                                                     double *__restrict__ cV, double *__restrict__ cU,
+                                                     double *__restrict__ wk, int *__restrict__ flag) {
+  const int e = threadIdx.x;
+  double *S = wk, *T = wk + kTile, *Ti = wk + 2 * kTile, *Pn = wk + 3 * kTile, *Pi = wk + 4 * kTile;
+  double *tmp = wk + 6 * kTile, *tmp2 = wk + 7 * kTile;
+  double *Gk = wk + 8 * kTile, *pj = Gk + (size_t)(kMaxM + 2) * kTile, *hk = pj + kTile;
+  auto At = [&](int b) { return G + (size_t)b * kTile; };
+  auto Bt = [&](int b) { return G2 + (size_t)b * kTile; };
+  auto Ht = [&](int r, int c) { return Hbar + ((size_t)r * m + c) * kTile; };
+  // T^T T = alpha - sum A_b^T A_b
+  {
+    double sv = At(j)[e];
+    for (int b = 0; b < j; ++b) sv -= tdot(At(b), At(b), true);
+    S[e] = sv;
+  }
+  __syncthreads();
+  if (e == 0) chol16(S, T, Ti, flag);
+  __syncthreads();
+  if (j >= 1) {
+    for (int b = 0; b < j; ++b) Ht(b, j - 1)[e] = Cp[(size_t)b * kTile + e] + tdot(At(b), Pp, false);
+    Ht(j, j - 1)[e] = tdot(T, Pp, false);
+  } else {
+    tmp[e] = tdot(T, S0, false);
+    __syncthreads();
+    S0[e] = tmp[e];
+  }
+  if (FINAL) return;
+  __syncthreads();
+  // G_k = sum_{c<j} H_kc A_c (H_kc = 0 for k > c + 1)
+  for (int k = 0; k <= j; ++k) {
+    double sv = 0.0;
+    for (int c = k > 0 ? k - 1 : 0; c < j; ++c) sv += tdot(Ht(k, c), At(c), false);
+    Gk[(size_t)k * kTile + e] = sv;
+  }
+  // p_j = T^-T (beta - sum A_b^T B_b)
+  {
+    double sv = Bt(j)[e];
+    for (int b = 0; b < j; ++b) sv -= tdot(At(b), Bt(b), true);
+    tmp[e] = sv;
+  }
+  __syncthreads();
+  pj[e] = tdot(Ti, tmp, true);
+  __syncthreads();
+  auto pk = [&](int k) { return k < j ? Bt(k) : pj; };
+  // h_k = (p_k - G_k) T^-1, and Q = gamma - sum (G_k^T p_k + p_k^T G_k - G_k^T G_k)
+  {
+    double q = Bt(j + 1)[e];
+    for (int k = 0; k <= j; ++k) {
+      const double *g = Gk + (size_t)k * kTile, *pp = pk(k);
+      q -= tdot(g, pp, true) + tdot(pp, g, true) - tdot(g, g, true);
+    }
+    tmp[e] = q;
+  }
+  for (int k = 0; k <= j; ++k) {
+    const double *g = Gk + (size_t)k * kTile, *pp = pk(k);
+    const int i = e / kS, c = e % kS;
+    double sv = 0.0;
+#pragma unroll
+    for (int l = 0; l < kS; ++l) sv += (pp[i * kS + l] - g[i * kS + l]) * Ti[l * kS + c];
+    hk[(size_t)k * kTile + e] = sv;
+  }
+  __syncthreads();
+  // Gram = T^-T Q T^-1 - sum h_k^T h_k
+  tmp2[e] = tdot(tmp, Ti, false);
+  __syncthreads();
+  {
+    double sv = tdot(Ti, tmp2, true);
+    for (int k = 0; k <= j; ++k) sv -= tdot(hk + (size_t)k * kTile, hk + (size_t)k * kTile, true);
+    S[e] = sv;
+  }
+  __syncthreads();
+  if (e == 0) chol16(S, Pn, Pi, flag);
+  __syncthreads();
+  // M = T^-1 P'^-1 -> cU[j+1]; D_k = (G_k T^-1 + h_k) P'^-1 -> cU[k] = -D_k
+  cU[(size_t)(j + 1) * kTile + e] = tdot(Ti, Pi, false);
+  for (int k = 0; k <= j; ++k) {
+    tmp[e] = tdot(Gk + (size_t)k * kTile, Ti, false) + hk[(size_t)k * kTile + e];
+    __syncthreads();
+    cU[(size_t)k * kTile + e] = -tdot(tmp, Pi, false);
+    __syncthreads();
+  }
+  // V_j = U_j T^-1 - sum V_b (A_b T^-1)
+  for (int b = 0; b < j; ++b) cV[(size_t)b * kTile + e] = -tdot(At(b), Ti, false);
+  cV[(size_t)j * kTile + e] = Ti[e];
+  // (C, P) <- (h, P')
+  for (int k = 0; k <= j; ++k) Cp[(size_t)k * kTile + e] = hk[(size_t)k * kTile + e];
+  Pp[e] = Pn[e];
 }
 
 // Y[:, c] = X[:, c] / d_c with d_c = sqrt(nrm2[c]) (1 for a zero column)
@@ -446,6 +566,9 @@ struct BlockSolver {
   double *G2 = nullptr;       // scratch tiles [kMaxM+1]
   double *Rt = nullptr, *Ri = nullptr, *Rt2 = nullptr, *Ri2 = nullptr;  // tiles
   double *Y = nullptr;        // [m] tiles
+  double *Cp = nullptr, *Pp = nullptr;   // once-projected coefficients (kMaxM+1 tiles) and scale tile of the newest block
+  double *cV = nullptr, *cU = nullptr;   // update coefficients (kMaxM+2 tiles each)
+  double *wk = nullptr;                  // kWkTiles work tiles of the step algebra
   double *nrm = nullptr;      // [16]
   int *flag = nullptr;
   float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
@@ -510,19 +633,19 @@ void cholqr2(BlockSolver *s, const double *W, double *Q, double *Rout) {
 
 // Z = P V_j, W = H1 Z: FP64 operators on FP64 vectors, or FP32 operators on
 // an FP32 Z (the products are formed in FP64)
-void apply_ops64(aerofx_ctx *c, BlockSolver *s, const double *Vj) {
+void apply_ops64(aerofx_ctx *c, BlockSolver *s, const double *Vj, double *W) {
   const int nb = (int)((c->N * (size_t)kS + 255) / 256);
   hipLaunchKernelGGL((k_bjacobi_apply<double, double>), dim3(nb), dim3(256), 0, s->stream, c->N,
                      (const double *)c->d_jacDi, Vj, s->Z);
   hipLaunchKernelGGL((k_bh1_apply<double, double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab,
-                     s->nbchunks, s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->Z, s->W);
+                     s->nbchunks, s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->Z, W);
 }
 
-void apply_ops32(aerofx_ctx *c, BlockSolver *s, const double *Vj) {
+void apply_ops32(aerofx_ctx *c, BlockSolver *s, const double *Vj, double *W) {
   const int nb = (int)((c->N * (size_t)kS + 255) / 256);
   hipLaunchKernelGGL((k_bjacobi_apply<float, float>), dim3(nb), dim3(256), 0, s->stream, c->N, s->Di32, Vj, s->Zf);
   hipLaunchKernelGGL((k_bh1_apply<float, float>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
-                     s->bchunk, s->D32, s->B32, s->Zf, s->W);
+                     s->bchunk, s->D32, s->B32, s->Zf, W);
 }
 
 // one inner cycle: from s->R (the residual block), builds V, Hbar, S0
@@ -537,23 +660,26 @@ void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
   hipLaunchKernelGGL(k_scale_cols, g1(blk), dim3(256), 0, s->stream, blk, s->R, cn, s->Z);
   cholqr2(s, s->Z, s->V, s->S0);
   hipLaunchKernelGGL(k_scale_tile_cols, dim3(1), dim3(256), 0, s->stream, s->S0, cn);
+  // block Arnoldi with delayed reorthogonalisation: per step one operator
+  // apply into slot j+1, the Gram tiles of slots 0..j+1 against U_j and W,
+  // the tile algebra, and two in-place updates (V_j, then U_{j+1})
   for (int j = 0; j < s->m; ++j) {
-    const double *Vj = s->V + (size_t)j * blk;
+    double *Uj = s->V + (size_t)j * blk, *Wn = s->V + (size_t)(j + 1) * blk;
     if (prec)
-      apply_ops32(c, s, Vj);
+      apply_ops32(c, s, Uj, Wn);
     else
-      apply_ops64(c, s, Vj);
-    // block CGS2
-    tsg_dots(s, j + 1, s->V, s->W, s->G);
-    tsg_update(s, j + 1, s->V, s->G, -1.0, 1.0, s->W);
-    tsg_dots(s, j + 1, s->V, s->W, s->G2);
-    tsg_update(s, j + 1, s->V, s->G2, -1.0, 1.0, s->W);
-    // H(0..j, j) = G + G2, stored as column j of Hbar
-    for (int i = 0; i <= j; ++i)
-      hipLaunchKernelGGL(k_add_tiles, dim3(1), dim3(256), 0, s->stream, 1, s->G + (size_t)i * kTile,
-                         s->G2 + (size_t)i * kTile, s->Hbar + ((size_t)i * s->m + j) * kTile);
-    cholqr2(s, s->W, s->V + (size_t)(j + 1) * blk, s->Hbar + ((size_t)(j + 1) * s->m + j) * kTile);
+      apply_ops64(c, s, Uj, Wn);
+    tsg_dots(s, j + 2, s->V, Uj, s->G);
+    tsg_dots(s, j + 2, s->V, Wn, s->G2);
+    hipLaunchKernelGGL((k_bdcgs_tiles<false>), dim3(1), dim3(256), 0, s->stream, s->G, s->G2, s->Hbar, s->m, j, s->Cp,
+                       s->Pp, s->S0, s->cV, s->cU, s->wk, s->flag);
+    tsg_update(s, j + 1, s->V, s->cV, 1.0, 0.0, Uj);
+    tsg_update(s, j + 2, s->V, s->cU, 1.0, 0.0, Wn);
   }
+  // the last block's reorthogonalisation completes column m-1
+  tsg_dots(s, s->m + 1, s->V, s->V + (size_t)s->m * blk, s->G);
+  hipLaunchKernelGGL((k_bdcgs_tiles<true>), dim3(1), dim3(256), 0, s->stream, s->G, s->G2, s->Hbar, s->m, s->m, s->Cp,
+                     s->Pp, s->S0, s->cV, s->cU, s->wk, s->flag);
 }
 
 // host least squares: min || E1 S0 - Hbar Y ||_F, Hbar ((m+1)16 x m16) block
@@ -671,6 +797,9 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
       (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G)) || (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G2)) ||
       (r = alloc_d(c, kTile, &s->Rt)) || (r = alloc_d(c, kTile, &s->Ri)) || (r = alloc_d(c, kTile, &s->Rt2)) ||
       (r = alloc_d(c, kTile, &s->Ri2)) || (r = alloc_d(c, (size_t)m * kTile, &s->Y)) ||
+      (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->Cp)) || (r = alloc_d(c, kTile, &s->Pp)) ||
+      (r = alloc_d(c, (size_t)(kMaxM + 2) * kTile, &s->cV)) || (r = alloc_d(c, (size_t)(kMaxM + 2) * kTile, &s->cU)) ||
+      (r = alloc_d(c, (size_t)kWkTiles * kTile, &s->wk)) ||
       (r = alloc_d(c, (size_t)kChunks * kS + 2 * kS, &s->nrm))) {
     delete s;
     *rc = r;
