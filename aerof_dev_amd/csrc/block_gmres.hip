@@ -218,24 +218,35 @@ __global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__
 // v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
 // holds D[l/16 + 4r][l%16], r < 4 (checked on gfx950).  Here A = V_b^T over 4 rows, B = W over
 // the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
+// With TWO, the same sweep also forms V_b^T W2 into partial2 (one read of
+// the V_b for two right-hand blocks).
+template <bool TWO>
 __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
                            const double *__restrict__ V, size_t ldv, const double *__restrict__ W,
-                           double *__restrict__ partial) {
+                           const double *__restrict__ W2, double *__restrict__ partial,
+                           double *__restrict__ partial2) {
   const int lane = threadIdx.x & 63;
   const int chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (chunk >= nchunks) return;
   const size_t r0 = (size_t)chunk * rows_per_chunk;
   const size_t r1 = min(npad, r0 + rows_per_chunk);
-  f64x4 acc[kMaxM + 1];
+  f64x4 acc[kMaxM + 1], acc2[TWO ? kMaxM + 1 : 1];
 #pragma unroll
   for (int b = 0; b <= kMaxM; ++b) acc[b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  if (TWO) {
+#pragma unroll
+    for (int b = 0; b <= kMaxM; ++b) acc2[TWO ? b : 0] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  }
   const int rr = lane >> 4, cc = lane & 15;
   // rows_per_chunk is a multiple of 16: four MFMA steps per iteration, their
   // loads issued together
   for (size_t k0 = r0; k0 < r1; k0 += 16) {
-    double w[4];
+    double w[4], w2[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) w[u] = W[(k0 + 4 * u + rr) * kS + cc];
+    for (int u = 0; u < 4; ++u) {
+      w[u] = W[(k0 + 4 * u + rr) * kS + cc];
+      if (TWO) w2[u] = W2[(k0 + 4 * u + rr) * kS + cc];
+    }
 #pragma unroll
     for (int b = 0; b <= kMaxM; ++b)
       if (b < nb) {
@@ -243,7 +254,10 @@ __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_c
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = V[(size_t)b * ldv + (k0 + 4 * u + rr) * kS + cc];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], w[u], acc[b], 0, 0, 0);
+        for (int u = 0; u < 4; ++u) {
+          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], w[u], acc[b], 0, 0, 0);
+          if (TWO) acc2[TWO ? b : 0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[u], w2[u], acc2[TWO ? b : 0], 0, 0, 0);
+        }
       }
   }
 #pragma unroll
@@ -252,6 +266,11 @@ __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_c
       double *dst = partial + ((size_t)chunk * mp1 + b) * kTile;
 #pragma unroll
       for (int r = 0; r < 4; ++r) dst[(rr + 4 * r) * kS + cc] = acc[b][r];
+      if (TWO) {
+        double *dst2 = partial2 + ((size_t)chunk * mp1 + b) * kTile;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst2[(rr + 4 * r) * kS + cc] = acc2[TWO ? b : 0][r];
+      }
     }
 }
 
@@ -559,6 +578,7 @@ struct BlockSolver {
   double *W = nullptr, *Z = nullptr, *R = nullptr, *X = nullptr, *B = nullptr, *T = nullptr;
   int nchunks = 0;            // row split of V^T W (fixed by npad)
   double *partial = nullptr;  // [nchunks][m+1][256]
+  double *partial2 = nullptr; // [nchunks][m+1][256], second right-hand block of a fused sweep
   double *part2 = nullptr;    // [kRedGroups][m+1][256]
   double *Hbar = nullptr;     // [(m+1)][m] tiles: block (i, j) at (i * m + j)
   double *S0 = nullptr;       // tile
@@ -601,15 +621,27 @@ int alloc_d(aerofx_ctx *c, size_t count, double **p) {
 
 BlockSolver *get_block(aerofx_ctx *c, int m, int *rc);
 
-void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *out) {
+// out[b] = V_b^T W (and out2[b] = V_b^T W2 when W2 is given), b < nb
+void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *out, const double *W2 = nullptr,
+              double *out2 = nullptr) {
   // rows per chunk: a multiple of 16 (four MFMA steps), fixed by npad alone
   const size_t rows = std::max<size_t>(16, ((s->npad + s->nchunks - 1) / s->nchunks + 15) / 16 * 16);
   const int wpb = s->shape.waves_dots;
-  hipLaunchKernelGGL(k_tsg_dots, dim3((s->nchunks + wpb - 1) / wpb), dim3(64 * wpb), 0, s->stream, s->npad, rows,
-                     s->nchunks, s->m + 1, nb, V, s->npad * kS, W, s->partial);
+  const dim3 grid((s->nchunks + wpb - 1) / wpb), block(64 * wpb);
+  if (W2)
+    hipLaunchKernelGGL((k_tsg_dots<true>), grid, block, 0, s->stream, s->npad, rows, s->nchunks, s->m + 1, nb, V,
+                       s->npad * kS, W, W2, s->partial, s->partial2);
+  else
+    hipLaunchKernelGGL((k_tsg_dots<false>), grid, block, 0, s->stream, s->npad, rows, s->nchunks, s->m + 1, nb, V,
+                       s->npad * kS, W, nullptr, s->partial, nullptr);
   hipLaunchKernelGGL(k_tsg_reduce1, dim3(nb, kRedGroups), dim3(256), 0, s->stream, s->partial, s->nchunks, s->m + 1,
                      s->part2);
   hipLaunchKernelGGL(k_tsg_reduce2, dim3(nb), dim3(256), 0, s->stream, s->part2, s->m + 1, out);
+  if (W2) {
+    hipLaunchKernelGGL(k_tsg_reduce1, dim3(nb, kRedGroups), dim3(256), 0, s->stream, s->partial2, s->nchunks,
+                       s->m + 1, s->part2);
+    hipLaunchKernelGGL(k_tsg_reduce2, dim3(nb), dim3(256), 0, s->stream, s->part2, s->m + 1, out2);
+  }
 }
 
 void tsg_update(BlockSolver *s, int nb, const double *V, const double *H, double alpha, double beta, double *Out) {
@@ -669,8 +701,7 @@ void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
       apply_ops32(c, s, Uj, Wn);
     else
       apply_ops64(c, s, Uj, Wn);
-    tsg_dots(s, j + 2, s->V, Uj, s->G);
-    tsg_dots(s, j + 2, s->V, Wn, s->G2);
+    tsg_dots(s, j + 2, s->V, Uj, s->G, Wn, s->G2);
     hipLaunchKernelGGL((k_bdcgs_tiles<false>), dim3(1), dim3(256), 0, s->stream, s->G, s->G2, s->Hbar, s->m, j, s->Cp,
                        s->Pp, s->S0, s->cV, s->cU, s->wk, s->flag);
     tsg_update(s, j + 1, s->V, s->cV, 1.0, 0.0, Uj);
@@ -792,6 +823,7 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
   if ((r = alloc_d(c, (size_t)(m + 1) * blk, &s->V)) || (r = alloc_d(c, blk, &s->W)) || (r = alloc_d(c, blk, &s->Z)) ||
       (r = alloc_d(c, blk, &s->R)) || (r = alloc_d(c, blk, &s->X)) || (r = alloc_d(c, blk, &s->B)) ||
       (r = alloc_d(c, blk, &s->T)) || (r = alloc_d(c, (size_t)s->nchunks * (m + 1) * kTile, &s->partial)) ||
+      (r = alloc_d(c, (size_t)s->nchunks * (m + 1) * kTile, &s->partial2)) ||
       (r = alloc_d(c, (size_t)kRedGroups * (m + 1) * kTile, &s->part2)) ||
       (r = alloc_d(c, (size_t)(m + 1) * m * kTile, &s->Hbar)) || (r = alloc_d(c, kTile, &s->S0)) ||
       (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G)) || (r = alloc_d(c, (size_t)(kMaxM + 1) * kTile, &s->G2)) ||
