@@ -323,6 +323,60 @@ __global__ __launch_bounds__(512) void k_tsg_update(size_t npad, int nb, const d
   }
 }
 
+// Both updates of a block Arnoldi step in one sweep, one wave per 16-row tile:
+//   V_j     = sum_{b<j} V_b cV_b + U_j cV_j                  (slot j, in place)
+//   U_{j+1} = sum_{b<j} V_b cU_b + V_j cU_j + W cU_{j+1}      (slot j+1 holds W, in place)
+// The slots 0..j-1 are read once for both sums; the new V_j tile goes
+// through LDS from the MFMA output layout to the input layout.
+__global__ __launch_bounds__(512) void k_tsg_update2(size_t npad, int j, double *V, size_t ldv,
+                                                     const double *__restrict__ cV, const double *__restrict__ cU) {
+  __shared__ double sv[8][kTile];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t tile = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (tile * 16 >= npad) return;
+  const size_t r0 = tile * 16;
+  const int ri = lane & 15, kk = lane >> 4;
+  f64x4 a1 = (f64x4){0.0, 0.0, 0.0, 0.0}, a2 = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < j; ++b) {
+    const double *Vb = V + (size_t)b * ldv + (r0 + ri) * kS;
+    const double *Hv = cV + (size_t)b * kTile, *Hu = cU + (size_t)b * kTile;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double x = Vb[4 * q + kk];
+      a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Hv[(4 * q + kk) * kS + ri], a1, 0, 0, 0);
+      a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Hu[(4 * q + kk) * kS + ri], a2, 0, 0, 0);
+    }
+  }
+  double *Uj = V + (size_t)j * ldv, *Wn = V + (size_t)(j + 1) * ldv;
+  double w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    w[q] = Wn[(r0 + ri) * kS + 4 * q + kk];
+    a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Uj[(r0 + ri) * kS + 4 * q + kk],
+                                              cV[(size_t)j * kTile + (4 * q + kk) * kS + ri], a1, 0, 0, 0);
+  }
+  // V_j: D[kk + 4 r][ri] -> row r0 + kk + 4 r, column ri
+  double *t = sv[wave];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    t[(kk + 4 * r) * kS + ri] = a1[r];
+    Uj[(r0 + kk + 4 * r) * kS + ri] = a1[r];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ri * kS + 4 * q + kk], cU[(size_t)j * kTile + (4 * q + kk) * kS + ri],
+                                              a2, 0, 0, 0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(w[q], cU[(size_t)(j + 1) * kTile + (4 * q + kk) * kS + ri], a2, 0, 0,
+                                              0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Wn[(r0 + kk + 4 * r) * kS + ri] = a2[r];
+}
+
 // ---------------------------------------------------------------------------
 // 16x16 helpers (one lane): Cholesky G = R^T R, inverse of R, products
 // ---------------------------------------------------------------------------
@@ -644,6 +698,14 @@ void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *
   }
 }
 
+// the step's two in-place updates (V_j, U_{j+1}) in one sweep
+void tsg_update2(BlockSolver *s, int j) {
+  const int wpb = s->shape.waves_update;
+  const size_t tiles = s->npad / 16;
+  hipLaunchKernelGGL(k_tsg_update2, dim3((unsigned)((tiles + wpb - 1) / wpb)), dim3(64 * wpb), 0, s->stream, s->npad,
+                     j, s->V, s->npad * kS, s->cV, s->cU);
+}
+
 void tsg_update(BlockSolver *s, int nb, const double *V, const double *H, double alpha, double beta, double *Out) {
   const int wpb = s->shape.waves_update;
   const size_t tiles = s->npad / 16;
@@ -693,8 +755,9 @@ void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
   cholqr2(s, s->Z, s->V, s->S0);
   hipLaunchKernelGGL(k_scale_tile_cols, dim3(1), dim3(256), 0, s->stream, s->S0, cn);
   // block Arnoldi with delayed reorthogonalisation: per step one operator
-  // apply into slot j+1, the Gram tiles of slots 0..j+1 against U_j and W,
-  // the tile algebra, and two in-place updates (V_j, then U_{j+1})
+  // apply into slot j+1, one sweep for the Gram tiles of slots 0..j+1
+  // against U_j and W, the tile algebra, and one sweep for the two in-place
+  // updates (V_j, then U_{j+1})
   for (int j = 0; j < s->m; ++j) {
     double *Uj = s->V + (size_t)j * blk, *Wn = s->V + (size_t)(j + 1) * blk;
     if (prec)
@@ -704,8 +767,7 @@ void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
     tsg_dots(s, j + 2, s->V, Uj, s->G, Wn, s->G2);
     hipLaunchKernelGGL((k_bdcgs_tiles<false>), dim3(1), dim3(256), 0, s->stream, s->G, s->G2, s->Hbar, s->m, j, s->Cp,
                        s->Pp, s->S0, s->cV, s->cU, s->wk, s->flag);
-    tsg_update(s, j + 1, s->V, s->cV, 1.0, 0.0, Uj);
-    tsg_update(s, j + 2, s->V, s->cU, 1.0, 0.0, Wn);
+    tsg_update2(s, j);
   }
   // the last block's reorthogonalisation completes column m-1
   tsg_dots(s, s->m + 1, s->V, s->V + (size_t)s->m * blk, s->G);
