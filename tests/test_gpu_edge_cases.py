@@ -136,3 +136,54 @@ def test_rejected_calls_leave_the_context_working(gpu):
     h1.evaluate(Ud, dt, precision=0)
     its, res = GmresSolver(op, m=20, eps=1e-8, maxits=300).solve(v, w)
     assert 0 < its <= 300 and res <= 1e-8 * float(torch.linalg.norm(v))
+
+
+def _single_tet():
+    """One tet (4 nodes, 20 unknowns), its faces oriented outward: two
+    farfield faces and two slip walls."""
+    X = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 1.0, 0.05], [0.1, 0.2, 1.0]])
+    tets = np.array([[0, 1, 2, 3]], np.int32)
+    faces = []
+    cen = X.mean(axis=0)
+    for f in ([1, 2, 3], [0, 3, 2], [0, 1, 3], [0, 2, 1]):
+        a, b, c = X[f]
+        if np.dot(np.cross(b - a, c - a), (a + b + c) / 3 - cen) < 0:
+            f = [f[0], f[2], f[1]]
+        faces.append(f)
+    codes = np.array([AEROFX_BC_OUTLET, AEROFX_BC_INLET, 2, 2], np.int32)
+    return M.Mesh(X, tets, np.array(faces, np.int32), codes)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gmres_exhausts_small_krylov_space(gpu, precision):
+    """20 unknowns, GMRES(30), tolerance 1e-14: the Krylov space is exhausted
+    (or the tolerance met) inside the first cycle; the solution must solve the
+    system that the H1 operator applies, built here column by column."""
+    import torch
+    from aerof_dev_amd.operators import GmresSolver, MatVecProdH1, SpaceOperator
+    m = _single_tet()
+    p = FlowParams(mach=0.5, alpha=0.1)
+    U = init_state(p, m.X, "noise", 0.01, seed=9)
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U).to(gpu)
+    h1 = MatVecProdH1(op)
+    dt = torch.empty(m.num_nodes, dtype=torch.float64, device=gpu)
+    h1.timestep(Ud, 10.0, dt)
+    h1.evaluate(Ud, dt, precision=precision)
+    n = 5 * m.num_nodes
+    A = np.empty((n, n))
+    for k in range(n):
+        e = torch.zeros(n, dtype=torch.float64, device=gpu)
+        e[k] = 1.0
+        y = torch.empty_like(e)
+        h1.apply(e.view(-1, 5), y.view(-1, 5))
+        A[:, k] = y.cpu().numpy()
+    b = np.random.default_rng(4).standard_normal(n)
+    x = torch.empty(n, dtype=torch.float64, device=gpu)
+    its, res = GmresSolver(op, m=30, eps=1e-14, maxits=60).solve(torch.from_numpy(b).to(gpu).view(-1, 5),
+                                                                x.view(-1, 5))
+    xs = np.linalg.solve(A, b)
+    xg = x.cpu().numpy()
+    assert its <= 30, its
+    assert np.linalg.norm(A @ xg - b) <= 1e-10 * np.linalg.norm(b)
+    assert np.linalg.norm(xg - xs) <= 1e-8 * np.linalg.norm(xs)
