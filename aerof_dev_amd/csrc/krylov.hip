@@ -139,13 +139,18 @@ struct GivensStage {
   int done;
 };
 
-__device__ void givens_col(GmresState *s, int jc, const double *col, GivensStage &g) {
+// Loads what the update reads into the stage (the caller may do this early,
+// overlapping it with its own loads, and must synchronise before the update).
+__device__ __forceinline__ void givens_stage(const GmresState *s, int jc, GivensStage &g) {
   const int t = threadIdx.x;
   if (t < 2 * kMaxBasis) g.rot[t] = s->cs[t];
-  if (t == 0) {
-    g.gj = s->g[jc];
-    g.done = s->done;
-  }
+  if (t == 0) g.gj = s->g[jc];
+}
+
+__device__ void givens_col(GmresState *s, int jc, const double *col, GivensStage &g, bool staged = false) {
+  const int t = threadIdx.x;
+  if (!staged) givens_stage(s, jc, g);
+  if (t == 0) g.done = s->done;
   __syncthreads();
   if (t != 0 || g.done) return;
   const int M = kMaxBasis;
@@ -350,6 +355,8 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
   __shared__ int sbreak;
   __shared__ GivensStage gstage;
   const int t = threadIdx.x, grp = t / 16, sub = t % 16;
+  // the Givens inputs of column j-1, loaded with the partials
+  if (j >= 1) givens_stage(st, j - 1, gstage);
   // columns 0..j-2 of the Hessenberg matrix (final), loaded with the partials
   if (!FINAL)
     for (int e = t; e < (j + 1) * kMaxBasis; e += kReduceThreads)
@@ -404,7 +411,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
       }
     }
     __syncthreads();
-    givens_col(st, j - 1, col, gstage);
+    givens_col(st, j - 1, col, gstage, true);
     __syncthreads();
     if (FINAL || st->done) return;
   } else if (t == 0) {
