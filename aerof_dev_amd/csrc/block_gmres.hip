@@ -336,25 +336,26 @@ __global__ __launch_bounds__(512) void k_tsg_update2(size_t npad, int j, double 
   if (tile * 16 >= npad) return;
   const size_t r0 = tile * 16;
   const int ri = lane & 15, kk = lane >> 4;
+  // MFMA step q takes k = 4 kk + q from lane (ri, kk): a lane's four A
+  // values are one contiguous 32-B piece of its row
   f64x4 a1 = (f64x4){0.0, 0.0, 0.0, 0.0}, a2 = (f64x4){0.0, 0.0, 0.0, 0.0};
   for (int b = 0; b < j; ++b) {
-    const double *Vb = V + (size_t)b * ldv + (r0 + ri) * kS;
+    const double4 x4 = *reinterpret_cast<const double4 *>(V + (size_t)b * ldv + (r0 + ri) * kS + 4 * kk);
+    const double x[4] = {x4.x, x4.y, x4.z, x4.w};
     const double *Hv = cV + (size_t)b * kTile, *Hu = cU + (size_t)b * kTile;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double x = Vb[4 * q + kk];
-      a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Hv[(4 * q + kk) * kS + ri], a1, 0, 0, 0);
-      a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, Hu[(4 * q + kk) * kS + ri], a2, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[q], Hv[(4 * kk + q) * kS + ri], a1, 0, 0, 0);
+      a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[q], Hu[(4 * kk + q) * kS + ri], a2, 0, 0, 0);
     }
   }
   double *Uj = V + (size_t)j * ldv, *Wn = V + (size_t)(j + 1) * ldv;
-  double w[4];
+  const double4 w4 = *reinterpret_cast<const double4 *>(Wn + (r0 + ri) * kS + 4 * kk);
+  const double4 u4 = *reinterpret_cast<const double4 *>(Uj + (r0 + ri) * kS + 4 * kk);
+  const double w[4] = {w4.x, w4.y, w4.z, w4.w}, uu[4] = {u4.x, u4.y, u4.z, u4.w};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    w[q] = Wn[(r0 + ri) * kS + 4 * q + kk];
-    a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Uj[(r0 + ri) * kS + 4 * q + kk],
-                                              cV[(size_t)j * kTile + (4 * q + kk) * kS + ri], a1, 0, 0, 0);
-  }
+  for (int q = 0; q < 4; ++q)
+    a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(uu[q], cV[(size_t)j * kTile + (4 * kk + q) * kS + ri], a1, 0, 0, 0);
   // V_j: D[kk + 4 r][ri] -> row r0 + kk + 4 r, column ri
   double *t = sv[wave];
 #pragma unroll
@@ -367,11 +368,11 @@ __global__ __launch_bounds__(512) void k_tsg_update2(size_t npad, int j, double 
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int q = 0; q < 4; ++q)
-    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ri * kS + 4 * q + kk], cU[(size_t)j * kTile + (4 * q + kk) * kS + ri],
+    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(t[ri * kS + 4 * kk + q], cU[(size_t)j * kTile + (4 * kk + q) * kS + ri],
                                               a2, 0, 0, 0);
 #pragma unroll
   for (int q = 0; q < 4; ++q)
-    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(w[q], cU[(size_t)(j + 1) * kTile + (4 * q + kk) * kS + ri], a2, 0, 0,
+    a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(w[q], cU[(size_t)(j + 1) * kTile + (4 * kk + q) * kS + ri], a2, 0, 0,
                                               0);
 #pragma unroll
   for (int r = 0; r < 4; ++r) Wn[(r0 + kk + 4 * r) * kS + ri] = a2[r];
