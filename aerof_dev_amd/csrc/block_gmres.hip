@@ -219,8 +219,9 @@ __global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__
 // holds D[l/16 + 4r][l%16], r < 4 (checked on gfx950).  Here A = V_b^T over 4 rows, B = W over
 // the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
 // With TWO, the same sweep also forms V_b^T W2 into partial2 (one read of
-// the V_b for two right-hand blocks).
-template <bool TWO>
+// the V_b for two right-hand blocks).  NBMAX (>= nb) sizes the accumulators:
+// fewer registers, more waves per SIMD for the early steps of a cycle.
+template <bool TWO, int NBMAX>
 __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
                            const double *__restrict__ V, size_t ldv, const double *__restrict__ W,
                            const double *__restrict__ W2, double *__restrict__ partial,
@@ -230,12 +231,12 @@ __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_c
   if (chunk >= nchunks) return;
   const size_t r0 = (size_t)chunk * rows_per_chunk;
   const size_t r1 = min(npad, r0 + rows_per_chunk);
-  f64x4 acc[kMaxM + 1], acc2[TWO ? kMaxM + 1 : 1];
+  f64x4 acc[NBMAX], acc2[TWO ? NBMAX : 1];
 #pragma unroll
-  for (int b = 0; b <= kMaxM; ++b) acc[b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < NBMAX; ++b) acc[b] = (f64x4){0.0, 0.0, 0.0, 0.0};
   if (TWO) {
 #pragma unroll
-    for (int b = 0; b <= kMaxM; ++b) acc2[TWO ? b : 0] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < NBMAX; ++b) acc2[TWO ? b : 0] = (f64x4){0.0, 0.0, 0.0, 0.0};
   }
   const int rr = lane >> 4, cc = lane & 15;
   // rows_per_chunk is a multiple of 16: four MFMA steps per iteration, their
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_c
       if (TWO) w2[u] = W2[(k0 + 4 * u + rr) * kS + cc];
     }
 #pragma unroll
-    for (int b = 0; b <= kMaxM; ++b)
+    for (int b = 0; b < NBMAX; ++b)
       if (b < nb) {
         double v[4];
 #pragma unroll
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_c
       }
   }
 #pragma unroll
-  for (int b = 0; b <= kMaxM; ++b)
+  for (int b = 0; b < NBMAX; ++b)
     if (b < nb) {
       double *dst = partial + ((size_t)chunk * mp1 + b) * kTile;
 #pragma unroll
@@ -683,12 +684,21 @@ void tsg_dots(BlockSolver *s, int nb, const double *V, const double *W, double *
   const size_t rows = std::max<size_t>(16, ((s->npad + s->nchunks - 1) / s->nchunks + 15) / 16 * 16);
   const int wpb = s->shape.waves_dots;
   const dim3 grid((s->nchunks + wpb - 1) / wpb), block(64 * wpb);
-  if (W2)
-    hipLaunchKernelGGL((k_tsg_dots<true>), grid, block, 0, s->stream, s->npad, rows, s->nchunks, s->m + 1, nb, V,
-                       s->npad * kS, W, W2, s->partial, s->partial2);
-  else
-    hipLaunchKernelGGL((k_tsg_dots<false>), grid, block, 0, s->stream, s->npad, rows, s->nchunks, s->m + 1, nb, V,
-                       s->npad * kS, W, nullptr, s->partial, nullptr);
+#define TSG_DOTS(TWO, NBM)                                                                                     \
+  hipLaunchKernelGGL((k_tsg_dots<TWO, NBM>), grid, block, 0, s->stream, s->npad, rows, s->nchunks, s->m + 1, nb, V, \
+                     s->npad * kS, W, W2, s->partial, s->partial2)
+  if (W2) {
+    if (nb <= 4) TSG_DOTS(true, 4);
+    else if (nb <= 8) TSG_DOTS(true, 8);
+    else if (nb <= 12) TSG_DOTS(true, 12);
+    else TSG_DOTS(true, kMaxM + 1);
+  } else {
+    if (nb <= 4) TSG_DOTS(false, 4);
+    else if (nb <= 8) TSG_DOTS(false, 8);
+    else if (nb <= 12) TSG_DOTS(false, 12);
+    else TSG_DOTS(false, kMaxM + 1);
+  }
+#undef TSG_DOTS
   hipLaunchKernelGGL(k_tsg_reduce1, dim3(nb, kRedGroups), dim3(256), 0, s->stream, s->partial, s->nchunks, s->m + 1,
                      s->part2);
   hipLaunchKernelGGL(k_tsg_reduce2, dim3(nb), dim3(256), 0, s->stream, s->part2, s->m + 1, out);
