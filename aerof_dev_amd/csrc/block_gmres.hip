@@ -836,46 +836,54 @@ void host_lsq(int m, const std::vector<double> &Ht, const std::vector<double> &S
 }
 
 // times each candidate launch shape on the current buffers; keeps the fastest
+// Times the candidate waves per workgroup of the two sweeps a step runs (the
+// two-right-hand-block Gram sweep and the fused update) at a mid-cycle
+// width; each candidate is timed twice over three launches and its best is
+// kept, so one noisy measurement does not decide.
 void autotune(aerofx_ctx *c, BlockSolver *s) {
   if (s->shape.tuned) return;
   hipEvent_t a, b;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
-  const int cand[] = {1, 2, 4};  // waves per workgroup (<= 256 threads: the dots keep 17 tiles in registers)
-  float best = 1e30f;
-  int nb = std::min(s->m, 8) + 1;
-  for (int w : cand) {
-    s->shape.waves_dots = w;
-    tsg_dots(s, nb, s->V, s->W, s->G);  // warm
+  const int j = std::min(s->m, 8) - 1;  // step whose sweeps are timed
+  const size_t blk = s->npad * kS;
+  double *U = s->V + (size_t)j * blk, *W = s->V + (size_t)(j + 1) * blk;
+  // defined coefficients for the update's timing runs (the slots are
+  // rewritten by every cycle before they are read)
+  (void)hipMemsetAsync(s->cV, 0, (size_t)(kMaxM + 2) * kTile * sizeof(double), s->stream);
+  (void)hipMemsetAsync(s->cU, 0, (size_t)(kMaxM + 2) * kTile * sizeof(double), s->stream);
+  auto time3 = [&](auto &&launch) {
+    launch();  // warm
     (void)hipEventRecord(a, s->stream);
-    for (int r = 0; r < 3; ++r) tsg_dots(s, nb, s->V, s->W, s->G);
+    for (int r = 0; r < 3; ++r) launch();
     (void)hipEventRecord(b, s->stream);
     (void)hipEventSynchronize(b);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, a, b);
-    if (ms < best) {
-      best = ms;
-      c->blk_tune[0] = w;
+    return ms;
+  };
+  const int cand[] = {1, 2, 4};  // waves per workgroup (<= 256 threads)
+  float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  for (int pass = 0; pass < 2; ++pass)
+    for (int k = 0; k < 3; ++k) {
+      s->shape.waves_dots = cand[k];
+      best[k] = std::min(best[k], time3([&] { tsg_dots(s, j + 2, s->V, U, s->G, W, s->G2); }));
     }
-  }
-  s->shape.waves_dots = c->blk_tune[0];
-  best = 1e30f;
+  int kb = 0;
+  for (int k = 1; k < 3; ++k)
+    if (best[k] < best[kb]) kb = k;
+  c->blk_tune[0] = s->shape.waves_dots = cand[kb];
   const int cand_u[] = {1, 2, 4, 8};
-  for (int w : cand_u) {
-    s->shape.waves_update = w;
-    tsg_update(s, nb, s->V, s->G, 0.0, 1.0, s->W);  // W unchanged: beta 1, alpha 0
-    (void)hipEventRecord(a, s->stream);
-    for (int r = 0; r < 3; ++r) tsg_update(s, nb, s->V, s->G, 0.0, 1.0, s->W);
-    (void)hipEventRecord(b, s->stream);
-    (void)hipEventSynchronize(b);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    if (ms < best) {
-      best = ms;
-      c->blk_tune[1] = w;
+  for (float &x : best) x = 1e30f;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int k = 0; k < 4; ++k) {
+      s->shape.waves_update = cand_u[k];
+      best[k] = std::min(best[k], time3([&] { tsg_update2(s, j); }));
     }
-  }
-  s->shape.waves_update = c->blk_tune[1];
+  kb = 0;
+  for (int k = 1; k < 4; ++k)
+    if (best[k] < best[kb]) kb = k;
+  c->blk_tune[1] = s->shape.waves_update = cand_u[kb];
   s->shape.tuned = true;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
