@@ -224,6 +224,82 @@ def adjoint_bench(dev, reps=5):
     return res
 
 
+def shape_bench(dev, order=1, ncheck=1):
+    """Adjoint shape gradient on the 192x64x64 bump channel (815k nodes):
+    the wall lift J (pressure force on the slip walls along z) and its
+    derivatives with respect to 8 Gaussian bump amplitudes with seeded centres.
+    Steady solve (mixed precision) to 1e-11, mixed-precision adjoint solve,
+    (dR/dX)^T psi, dJ/da_k = (dJ/dX + (dR/dX)^T psi) . dX/da_k; the first
+    ncheck amplitudes against central differences of re-converged J (step
+    1e-6; each side starts from the converged unperturbed state and drops
+    its own initial residual by 1e-9)."""
+    import numpy as np
+    import torch
+
+    from aerof_dev_amd import FlowParams, init_state
+    from aerof_dev_amd import mesh as M
+    from aerof_dev_amd.operators import SpaceOperator, adjoint_solve, steady_solve
+
+    zdir, h = (0.0, 0.0, 1.0), 1e-6
+    centres = np.random.default_rng(2024).uniform(0.9, 2.1, size=8)
+
+    def mesh(amps):
+        bumps = [(0.05, 1.5, 0.2)] + [(a, c, 0.2) for a, c in zip(amps, centres)]
+        return M.kuhn_box(192, 64, 64, lo=(0.0, 0.0, 0.0), hi=(3.0, 1.0, 1.0), face_codes=M.CHANNEL, bumps=bumps)
+
+    def converge(op, U, eps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = steady_solve(op, U, precision=1, max_steps=3000, eps=eps, cfl0=10.0, cfl_max=1e4)
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    params = FlowParams(mach=0.675, second_order=order == 2)
+    m = mesh([0.0] * 8)
+    op = SpaceOperator(m, params)
+    U = torch.from_numpy(init_state(params, m.X, "freestream")).to(dev)
+    r, t_steady = converge(op, U, 1e-11)
+    dJdU = torch.empty_like(U)
+    dJdX = torch.empty((m.num_nodes, 3), dtype=torch.float64, device=dev)
+    op.wall_force(U, zdir, dJdU, dJdX)
+    psi = torch.empty_like(U)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = adjoint_solve(op, U, (-dJdU).contiguous(), psi, precision=1, cfl=1e4, eps=1e-11, max_its=3000)
+    gx = torch.empty_like(dJdX)
+    op.compute_transpose_derivative_of_residual(U, psi, None, gx)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    total = (dJdX + gx).cpu().numpy()
+    grads = []
+    for k in range(8):
+        amps = [0.0] * 8
+        amps[k] = 1.0
+        grads.append(float(np.sum(total * (mesh(amps).X - m.X))))
+    op.close()
+    checks = []
+    for k in range(ncheck):
+        Js = []
+        for sgn in (1.0, -1.0):
+            amps = [0.0] * 8
+            amps[k] = sgn * h
+            opk = SpaceOperator(mesh(amps), params)
+            Uk = U.clone()
+            converge(opk, Uk, 1e-9)
+            Js.append(opk.wall_force(Uk, zdir))
+            opk.close()
+        fd = (Js[0] - Js[1]) / (2 * h)
+        checks.append({"k": k, "adjoint": grads[k], "fd": fd, "rel_diff": abs(grads[k] - fd) / abs(fd)})
+    res = {"workload": f"192x64x64 bump channel ({m.num_nodes} nodes), {'second' if order == 2 else 'first'} order, "
+                       "wall lift, 8 bump amplitudes",
+           "steady_mixed_s": round(t_steady, 3), "steady_steps": r["steps"], "steady_krylov_its": r["krylov_its"],
+           "steady_rel_res": float(r["res"] / r["res0"]), "adjoint_mixed_s": round(t_adj, 3),
+           "adjoint_outer_its": st["its"], "adjoint_krylov_its": st["krylov_its"], "adjoint_converged": st["converged"],
+           "dJ_da": grads, "fd_checks": checks}
+    log(f"[bench] shape: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +314,8 @@ def main():
                     help="also time 16-RHS block GMRES against 16 single-RHS FP64 solves (128^3 box)")
     ap.add_argument("--adjoint", action="store_true",
                     help="also time the adjoint kernels on the 192x64x64 channel")
+    ap.add_argument("--shape", action="store_true",
+                    help="also run the adjoint shape gradient (8 bump amplitudes) on the 192x64x64 channel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -350,7 +428,7 @@ def main():
                     "k_flux_edges": {"ms": round(edge_ms, 5)},
                     "k_flux_nodes": {"ms": round(node_ms, 5)}},
     }
-    if not args.no_newton or args.batched or args.adjoint:
+    if not args.no_newton or args.batched or args.adjoint or args.shape:
         op.close()
     if not args.no_newton:
         out["newton"] = newton_bench(dev, args.newton_steps)
@@ -358,6 +436,8 @@ def main():
         out["batched"] = batched_bench(dev, args.n)
     if args.adjoint:
         out["adjoint"] = adjoint_bench(dev)
+    if args.shape:
+        out["shape"] = shape_bench(dev)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mesh, params, U, args.cpu_budget)
