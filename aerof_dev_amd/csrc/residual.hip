@@ -218,7 +218,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double
                                                        const double *__restrict__ Fm,
                                                        const double *__restrict__ P, FluxConsts fc, int scale,
                                                        double *__restrict__ R) {
-  const int n = blockIdx.x * kBlock + threadIdx.x;
+  const int nb = (t.N + kBlock - 1) / kBlock;
+  const int n = xcd_chunk(blockIdx.x, nb) * kBlock + (int)threadIdx.x;
   if (n >= t.N) return;
   double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const int q0 = t.lrow[n], q1 = t.lrow[n + 1];

@@ -387,8 +387,8 @@ def main():
     log(f"[bench] {args.steps} steps, {ms_step:.4f} ms/step; kernels {timers}")
 
     def per_launch(name):
-        tot, cnt = timers[name]
-        return tot / cnt
+        tot, cnt = timers.get(name, (0.0, 0))
+        return tot / cnt if cnt else 0.0
     grad_ms = per_launch("nodalGrad")
     edge_ms = per_launch("fvTerm")
     node_ms = per_launch("fvTermNodes")
@@ -421,10 +421,10 @@ def main():
                    "gpu_rcm_s": round(t_rcm, 3), "gpu_edge_colouring_s": round(t_col, 3), "edge_colours": ncolors},
         "roofline": roofline,
         "kernels": {"k_gradients": {"ms": round(grad_ms, 5),
-                                    "algorithmic_GBs": round(grad_bytes / (grad_ms * 1e-3) / 1e9, 1)},
+                                    "algorithmic_GBs": round(grad_bytes / (grad_ms * 1e-3) / 1e9, 1) if grad_ms else None},
                     "k_flux_edges+k_flux_nodes": {
                         "ms": round(edge_ms + node_ms, 5),
-                        "algorithmic_GBs": round(flux_bytes / ((edge_ms + node_ms) * 1e-3) / 1e9, 1)},
+                        "algorithmic_GBs": round(flux_bytes / ((edge_ms + node_ms) * 1e-3) / 1e9, 1) if edge_ms + node_ms else None},
                     "k_flux_edges": {"ms": round(edge_ms, 5)},
                     "k_flux_nodes": {"ms": round(node_ms, 5)}},
     }
