@@ -13,7 +13,7 @@
 // their own slots and each node sums its slots in a fixed order):
 //   1. forward residual: V, LSQ gradients, R (scaled)
 //   2. k_adj_edges  one thread per edge: mu = lambda/V, g = mu_i - mu_j;
-//      g^T dF/d(V_ij, V_ji, n) by 13 forward-mode passes of the Roe flux,
+//      g^T dF/d(V_ij, V_ji, n) by the reverse mode of the Roe flux,
 //      then reverse through the Van Albada reconstruction by hand
 //      -> per edge: Vbar_i, Vbar_j, Gbar_i, Gbar_j, dxbar, nbar
 //   3. k_adj_nodes  one thread per node: sums the edge slots, adds the
@@ -33,9 +33,8 @@
 namespace aerofx {
 
 struct Adjoint {
-  double *EV = nullptr;   // [E][10] Vbar_i, Vbar_j
-  double *EG = nullptr;   // [E][30] Gbar_i[d*5+k], Gbar_j
-  double *EX = nullptr;   // [E][6]  dxbar, nbar
+  double *EV = nullptr;   // [E][20] Vbar_i, c_i, Vbar_j, c_j (Gbar_i[d*5+k] = c_i[k] dx[d])
+  double *EX = nullptr;   // [E][6]  dxbar, nbar (coordinate derivatives only)
   double *Vb = nullptr;   // [N][5]
   double *Gb = nullptr;   // [N][16]
   double *Xb = nullptr;   // [N][4]
@@ -131,6 +130,244 @@ __device__ __forceinline__ void face_flux(const FluxConsts &fc, int code, const 
 
 __device__ __forceinline__ double seed(int dir, int k) { return dir == k ? 1.0 : 0.0; }
 
+// Reverse mode of roeflux7 (device_physics.h) with evitno = 0, the edge
+// call: given the weights g on the five flux components, returns
+// bL = g^T dphi/dUg, bR = g^T dphi/dUd and be = g^T dphi/den.  The forward
+// statements are repeated as written (one rounding each, the same upwind
+// branch), then each is reversed in the opposite order; one forward and one
+// reverse sweep replace the 13 directional passes of the Dual form.
+__device__ __forceinline__ void roeflux7_vjp(const Gas &gs, const double e[3], const double L[5], const double R[5],
+                                             const double g[5], double bL[5], double bR[5], double be[3]) {
+  const double gam = gs.gam, gam1 = gs.gam1, invgam1 = gs.invgam1, ps = gs.pstiff;
+  // ---- forward
+  const double rn2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+  const double rnorm = sqrt(rn2);
+  const double invnorm = 1.0 / rnorm;
+  const double n0 = e[0] * invnorm, n1 = e[1] * invnorm, n2 = e[2] * invnorm;
+  const double vitg2 = L[1] * L[1] + L[2] * L[2] + L[3] * L[3];
+  const double vitd2 = R[1] * R[1] + R[2] * R[2] + R[3] * R[3];
+  const double s1 = sqrt(L[0]), s2 = sqrt(R[0]);
+  const double ener1 = (L[4] + gam * ps) * invgam1 + 0.5 * L[0] * vitg2;
+  const double ener2 = (R[4] + gam * ps) * invgam1 + 0.5 * R[0] * vitd2;
+  const double usro = 1.0 / (s1 + s2);
+  const double m2 = s1 * L[1] + s2 * R[1], m3 = s1 * L[2] + s2 * R[2], m4n = s1 * L[3] + s2 * R[3];
+  const double u2 = m2 * usro, u3 = m3 * usro, u4 = m4n * usro;
+  const double A1 = (ener1 + L[4]) / s1, A2 = (ener2 + R[4]) / s2;
+  const double u5 = (A1 + A2) * usro;
+  const double VdotNt = n0 * u2 + n1 * u3 + n2 * u4;
+  const bool left = VdotNt > 0.0;
+  const double *S = left ? L : R;
+  const double vit2 = left ? vitg2 : vitd2;
+  const double VdotN = S[1] * n0 + S[2] * n1 + S[3] * n2;
+  const double H0 = gam * (S[4] + ps) + 0.5 * gam1 * S[0] * vit2;
+  const double H = H0 / (gam1 * S[0]);
+  const double p0 = S[0] * VdotN;
+  const double p1 = p0 * S[1] + S[4] * n0, p2 = p0 * S[2] + S[4] * n1, p3 = p0 * S[3] + S[4] * n2;
+  const double p4 = p0 * H;
+  const double qir = 0.5 * (u2 * u2 + u3 * u3 + u4 * u4);
+  const double cr2 = gam1 * (u5 - qir);
+  const double cr = sqrt(cr2);
+  const double icr2 = 1.0 / cr2;
+  const double vp4 = VdotNt + cr, vp5 = VdotNt - cr;
+  const double r = cr, sm = -cr, cb = -cr, itb = -1.0 / cr;
+  const double d1 = -L[0] + R[0];
+  const double d2 = -L[0] * L[1] + R[0] * R[1];
+  const double d3 = -L[0] * L[2] + R[0] * R[2];
+  const double d4 = -L[0] * L[3] + R[0] * R[3];
+  const double a1 = VdotNt * d1 - n0 * d2 - n1 * d3 - n2 * d4;
+  const double a2 = qir * d1 - u2 * d2 - u3 * d3 - u4 * d4 + (ener2 - ener1);
+  double flur4 = 0.0, flur5 = 0.0, mm = 0.0, tt = 0.0;
+  if (!left) {
+    mm = vp4 > 0.0 ? vp4 : 0.0;
+    tt = cb * a1 + gam1 * sm * itb * a2;
+    flur4 = mm * tt;
+  } else {
+    mm = vp5 < 0.0 ? vp5 : 0.0;
+    tt = -cb * a1 + (-gam1 * r * itb * a2);
+    flur5 = -mm * tt;
+  }
+  const double fs = flur4 + flur5;
+  const double rs = r * flur4 + sm * flur5;
+  const double X0 = fs, X1 = n0 * rs + u2 * fs, X2 = n1 * rs + u3 * fs, X3 = n2 * rs + u4 * fs,
+               X4 = VdotNt * rs + u5 * fs;
+  const double q0 = p0 - 0.5 * X0 * icr2, q1 = p1 - X1 * 0.5 * icr2, q2 = p2 - X2 * 0.5 * icr2,
+               q3 = p3 - X3 * 0.5 * icr2, q4 = p4 - X4 * 0.5 * icr2;
+  // ---- reverse
+  double bLl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, bRr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // phi_k = q_k rnorm
+  double b_rnorm = g[0] * q0 + g[1] * q1 + g[2] * q2 + g[3] * q3 + g[4] * q4;
+  const double bq0 = g[0] * rnorm, bq1 = g[1] * rnorm, bq2 = g[2] * rnorm, bq3 = g[3] * rnorm, bq4 = g[4] * rnorm;
+  // q_k = p_k - X_k icr2 / 2
+  const double c = 0.5 * icr2;
+  const double b_icr2 = -0.5 * (bq0 * X0 + bq1 * X1 + bq2 * X2 + bq3 * X3 + bq4 * X4);
+  const double bX0 = -c * bq0, bX1 = -c * bq1, bX2 = -c * bq2, bX3 = -c * bq3, bX4 = -c * bq4;
+  double b_p0 = bq0;
+  const double b_p1 = bq1, b_p2 = bq2, b_p3 = bq3, b_p4 = bq4;
+  // X_k
+  double b_fs = bX0 + u2 * bX1 + u3 * bX2 + u4 * bX3 + u5 * bX4;
+  const double b_rs = n0 * bX1 + n1 * bX2 + n2 * bX3 + VdotNt * bX4;
+  double b_n0 = rs * bX1, b_n1 = rs * bX2, b_n2 = rs * bX3;
+  double b_u2 = fs * bX1, b_u3 = fs * bX2, b_u4 = fs * bX3, b_u5 = fs * bX4;
+  double b_VdotNt = rs * bX4;
+  // rs = r flur4 + sm flur5, fs = flur4 + flur5
+  const double b_flur4 = b_fs + r * b_rs, b_flur5 = b_fs + sm * b_rs;
+  double b_r = flur4 * b_rs, b_sm = flur5 * b_rs, b_cb = 0.0, b_itb = 0.0, b_a1 = 0.0, b_a2 = 0.0;
+  double b_vp4 = 0.0, b_vp5 = 0.0;
+  if (!left) {
+    // flur4 = mm tt, tt = cb a1 + ((gam1 sm) itb) a2
+    const double b_mm = tt * b_flur4, b_tt = mm * b_flur4;
+    b_vp4 = vp4 > 0.0 ? b_mm : 0.0;
+    b_cb += a1 * b_tt;
+    b_a1 += cb * b_tt;
+    const double k = gam1 * sm * itb;
+    b_a2 += k * b_tt;
+    const double b_k = a2 * b_tt;
+    b_sm += gam1 * itb * b_k;
+    b_itb += gam1 * sm * b_k;
+  } else {
+    // flur5 = -mm tt, tt = -cb a1 + (((-gam1) r) itb) a2
+    const double b_mm = -tt * b_flur5, b_tt = -mm * b_flur5;
+    b_vp5 = vp5 < 0.0 ? b_mm : 0.0;
+    b_cb += -a1 * b_tt;
+    b_a1 += -cb * b_tt;
+    const double k = -gam1 * r * itb;
+    b_a2 += k * b_tt;
+    const double b_k = a2 * b_tt;
+    b_r += -gam1 * itb * b_k;
+    b_itb += -gam1 * r * b_k;
+  }
+  // a2 = qir d1 - u2 d2 - u3 d3 - u4 d4 + (ener2 - ener1)
+  double b_qir = d1 * b_a2;
+  double b_d1 = qir * b_a2, b_d2 = -u2 * b_a2, b_d3 = -u3 * b_a2, b_d4 = -u4 * b_a2;
+  b_u2 += -d2 * b_a2;
+  b_u3 += -d3 * b_a2;
+  b_u4 += -d4 * b_a2;
+  double b_ener1 = -b_a2, b_ener2 = b_a2;
+  // a1 = VdotNt d1 - n0 d2 - n1 d3 - n2 d4
+  b_VdotNt += d1 * b_a1;
+  b_d1 += VdotNt * b_a1;
+  b_n0 += -d2 * b_a1;
+  b_n1 += -d3 * b_a1;
+  b_n2 += -d4 * b_a1;
+  b_d2 += -n0 * b_a1;
+  b_d3 += -n1 * b_a1;
+  b_d4 += -n2 * b_a1;
+  // d1..d4
+  bLl[0] += -b_d1 - L[1] * b_d2 - L[2] * b_d3 - L[3] * b_d4;
+  bRr[0] += b_d1 + R[1] * b_d2 + R[2] * b_d3 + R[3] * b_d4;
+  bLl[1] += -L[0] * b_d2;
+  bLl[2] += -L[0] * b_d3;
+  bLl[3] += -L[0] * b_d4;
+  bRr[1] += R[0] * b_d2;
+  bRr[2] += R[0] * b_d3;
+  bRr[3] += R[0] * b_d4;
+  // r = cr, sm = -cr, cb = -cr, itb = -1/cr; vp4 = VdotNt + cr, vp5 = VdotNt - cr
+  double b_cr = b_r - b_sm - b_cb + b_itb / (cr * cr) + b_vp4 - b_vp5;
+  b_VdotNt += b_vp4 + b_vp5;
+  // icr2 = 1/cr2, cr = sqrt(cr2), cr2 = gam1 (u5 - qir)
+  const double b_cr2 = -icr2 * icr2 * b_icr2 + 0.5 * b_cr / cr;
+  b_u5 += gam1 * b_cr2;
+  b_qir += -gam1 * b_cr2;
+  // qir = (u2^2 + u3^2 + u4^2) / 2
+  b_u2 += u2 * b_qir;
+  b_u3 += u3 * b_qir;
+  b_u4 += u4 * b_qir;
+  // p1..p4 = p0 S_k + S4 n_k, p0 H
+  b_p0 += S[1] * b_p1 + S[2] * b_p2 + S[3] * b_p3 + H * b_p4;
+  double bS[5];
+  bS[1] = p0 * b_p1;
+  bS[2] = p0 * b_p2;
+  bS[3] = p0 * b_p3;
+  bS[4] = n0 * b_p1 + n1 * b_p2 + n2 * b_p3;
+  b_n0 += S[4] * b_p1;
+  b_n1 += S[4] * b_p2;
+  b_n2 += S[4] * b_p3;
+  const double b_H = p0 * b_p4;
+  // p0 = S0 VdotN
+  bS[0] = VdotN * b_p0;
+  const double b_VdotN = S[0] * b_p0;
+  // H = H0 / (gam1 S0), H0 = gam (S4 + ps) + gam1 S0 vit2 / 2
+  const double b_H0 = b_H / (gam1 * S[0]);
+  bS[0] += -H * b_H / S[0] + 0.5 * gam1 * vit2 * b_H0;
+  bS[4] += gam * b_H0;
+  const double b_vit2 = 0.5 * gam1 * S[0] * b_H0;
+  // VdotN = S1 n0 + S2 n1 + S3 n2
+  bS[1] += n0 * b_VdotN;
+  bS[2] += n1 * b_VdotN;
+  bS[3] += n2 * b_VdotN;
+  b_n0 += S[1] * b_VdotN;
+  b_n1 += S[2] * b_VdotN;
+  b_n2 += S[3] * b_VdotN;
+  double b_vitg2 = 0.0, b_vitd2 = 0.0;
+  if (left) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) bLl[k] += bS[k];
+    b_vitg2 += b_vit2;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) bRr[k] += bS[k];
+    b_vitd2 += b_vit2;
+  }
+  // VdotNt = n0 u2 + n1 u3 + n2 u4
+  b_n0 += u2 * b_VdotNt;
+  b_n1 += u3 * b_VdotNt;
+  b_n2 += u4 * b_VdotNt;
+  b_u2 += n0 * b_VdotNt;
+  b_u3 += n1 * b_VdotNt;
+  b_u4 += n2 * b_VdotNt;
+  // u5 = (A1 + A2) usro, A1 = (ener1 + L4) / s1, A2 = (ener2 + R4) / s2
+  double b_usro = (A1 + A2) * b_u5;
+  const double b_A1 = usro * b_u5, b_A2 = usro * b_u5;
+  b_ener1 += b_A1 / s1;
+  bLl[4] += b_A1 / s1;
+  double b_s1 = -A1 / s1 * b_A1;
+  b_ener2 += b_A2 / s2;
+  bRr[4] += b_A2 / s2;
+  double b_s2 = -A2 / s2 * b_A2;
+  // u_c = m_c usro, m_c = s1 L_c + s2 R_c
+  b_usro += m2 * b_u2 + m3 * b_u3 + m4n * b_u4;
+  const double b_m2 = usro * b_u2, b_m3 = usro * b_u3, b_m4 = usro * b_u4;
+  b_s1 += L[1] * b_m2 + L[2] * b_m3 + L[3] * b_m4;
+  b_s2 += R[1] * b_m2 + R[2] * b_m3 + R[3] * b_m4;
+  bLl[1] += s1 * b_m2;
+  bLl[2] += s1 * b_m3;
+  bLl[3] += s1 * b_m4;
+  bRr[1] += s2 * b_m2;
+  bRr[2] += s2 * b_m3;
+  bRr[3] += s2 * b_m4;
+  // usro = 1 / (s1 + s2)
+  b_s1 += -usro * usro * b_usro;
+  b_s2 += -usro * usro * b_usro;
+  // ener = (p + gam ps) / gam1 + rho vit / 2
+  bLl[4] += invgam1 * b_ener1;
+  bLl[0] += 0.5 * vitg2 * b_ener1;
+  b_vitg2 += 0.5 * L[0] * b_ener1;
+  bRr[4] += invgam1 * b_ener2;
+  bRr[0] += 0.5 * vitd2 * b_ener2;
+  b_vitd2 += 0.5 * R[0] * b_ener2;
+  // s = sqrt(rho)
+  bLl[0] += 0.5 * b_s1 / s1;
+  bRr[0] += 0.5 * b_s2 / s2;
+  // vit2 = |u|^2
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
+    bLl[k] += 2.0 * L[k] * b_vitg2;
+    bRr[k] += 2.0 * R[k] * b_vitd2;
+  }
+  // n = e invnorm, invnorm = 1/rnorm, rnorm = sqrt(rn2)
+  const double b_invnorm = e[0] * b_n0 + e[1] * b_n1 + e[2] * b_n2;
+  b_rnorm += -invnorm * invnorm * b_invnorm;
+  const double b_rn2 = 0.5 * b_rnorm / rnorm;
+  be[0] = invnorm * b_n0 + 2.0 * e[0] * b_rn2;
+  be[1] = invnorm * b_n1 + 2.0 * e[1] * b_rn2;
+  be[2] = invnorm * b_n2 + 2.0 * e[2] * b_rn2;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    bL[k] = bLl[k];
+    bR[k] = bRr[k];
+  }
+}
+
 // d VA(a, b) / da and / db of the Van Albada function (RecFcn.h:269-281)
 __device__ __forceinline__ void va_partials(double a, double b, double &da, double &db) {
   const double eps0 = 1.e-16;
@@ -143,12 +380,11 @@ __device__ __forceinline__ void va_partials(double a, double b, double &da, doub
 // ---------------------------------------------------------------------------
 // 2. per edge
 // ---------------------------------------------------------------------------
-template <bool SECOND>
+template <bool SECOND, bool WANTX>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const double *__restrict__ V,
                                                          const double *__restrict__ grad,
                                                          const double *__restrict__ lam, FluxConsts fc,
-                                                         double *__restrict__ EV, double *__restrict__ EG,
-                                                         double *__restrict__ EX) {
+                                                         double *__restrict__ EV, double *__restrict__ EX) {
   const long long l = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
   if (l >= t.E) return;
   const int2 ij = t.eij[l];
@@ -191,25 +427,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const doub
   double fb[13];
 #pragma unroll
   for (int d = 0; d < 13; ++d) fb[d] = 0.0;
-  if (!(sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) < 1e-18)) {
-#pragma unroll 1
-    for (int dir = 0; dir < 13; ++dir) {
-      Dual L[5], Rr[5], en[3], phi[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        L[k] = Dual(VL[k], seed(dir, k));
-        Rr[k] = Dual(VR[k], seed(dir, 5 + k));
-      }
-#pragma unroll
-      for (int a = 0; a < 3; ++a) en[a] = Dual(nv[a], seed(dir, 10 + a));
-      roeflux7(fc.gas, en, Dual(0.0), L, Rr, phi);
-      double s = 0.0;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) s += g[k] * phi[k].d;
-      fb[dir] = s;
-    }
-  }
-  double vbi[5], vbj[5], gbi[15], gbj[15], dxb[3] = {0.0, 0.0, 0.0};
+  if (!(sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) < 1e-18))
+    roeflux7_vjp(fc.gas, nv, VL, VR, g, fb, fb + 5, fb + 10);
+  double vbi[5], vbj[5], ci[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, cj[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double dxb[3] = {0.0, 0.0, 0.0};
   if (SECOND) {
     const double beta = fc.beta, beta2 = 1.0 - 2.0 * beta;
 #pragma unroll
@@ -230,11 +451,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const doub
       const double ddVijb = 2.0 * beta * A, ddVjib = 2.0 * beta * A2;
       vbi[k] = Lb - dVjib;
       vbj[k] = Rb + dVjib;
+      // Gbar_i[5a+k] = ddVijb dx[a]: the node pass forms the products
+      ci[k] = ddVijb;
+      cj[k] = ddVjib;
+      if (WANTX) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        gbi[5 * a + k] = ddVijb * dx[a];
-        gbj[5 * a + k] = ddVjib * dx[a];
-        dxb[a] += ddVijb * gi[5 * a + k] + ddVjib * gj[5 * a + k];
+        for (int a = 0; a < 3; ++a) dxb[a] += ddVijb * gi[5 * a + k] + ddVjib * gj[5 * a + k];
       }
     }
   } else {
@@ -244,25 +466,24 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const doub
       vbj[k] = fb[5 + k];
     }
   }
-  double *ev = EV + 10 * (size_t)l;
+  double2 *ev = reinterpret_cast<double2 *>(EV + 20 * (size_t)l);
+  ev[0] = make_double2(vbi[0], vbi[1]);
+  ev[1] = make_double2(vbi[2], vbi[3]);
+  ev[2] = make_double2(vbi[4], ci[0]);
+  ev[3] = make_double2(ci[1], ci[2]);
+  ev[4] = make_double2(ci[3], ci[4]);
+  ev[5] = make_double2(vbj[0], vbj[1]);
+  ev[6] = make_double2(vbj[2], vbj[3]);
+  ev[7] = make_double2(vbj[4], cj[0]);
+  ev[8] = make_double2(cj[1], cj[2]);
+  ev[9] = make_double2(cj[3], cj[4]);
+  if (WANTX) {
+    double *ex = EX + 6 * (size_t)l;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    ev[k] = vbi[k];
-    ev[5 + k] = vbj[k];
-  }
-  if (SECOND) {
-    double *eg = EG + 30 * (size_t)l;
-#pragma unroll
-    for (int q = 0; q < 15; ++q) {
-      eg[q] = gbi[q];
-      eg[15 + q] = gbj[q];
+    for (int a = 0; a < 3; ++a) {
+      ex[a] = dxb[a];
+      ex[3 + a] = fb[10 + a];
     }
-  }
-  double *ex = EX + 6 * (size_t)l;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    ex[a] = dxb[a];
-    ex[3 + a] = fb[10 + a];
   }
 }
 
@@ -272,8 +493,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const doub
 __global__ __launch_bounds__(kAdjBlock) void k_adj_nodes(DevTables t, const double *__restrict__ V,
                                                          const double *__restrict__ lam, FluxConsts fc,
                                                          const double *__restrict__ EV,
-                                                         const double *__restrict__ EG,
-                                                         const double *__restrict__ EX, int second,
+                                                         const double *__restrict__ EX, int second, int want_x,
                                                          double *__restrict__ Vb, double *__restrict__ Gb,
                                                          double *__restrict__ Xb) {
   const int n = blockIdx.x * kAdjBlock + threadIdx.x;
@@ -281,28 +501,42 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_nodes(DevTables t, const doub
   double vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gb[15], xb[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < 15; ++q) gb[q] = 0.0;
+  const double4 xn = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)n);
+  // one edge slot: Vbar, and Gbar = c dx^T with dx = X_j - X_i as the edge
+  // pass formed it (the same products, added in the same order)
+  auto add_slot = [&](const double *ev, int m, bool upper) {
+    double c[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      vb[k] += ev[k];
+      c[k] = ev[5 + k];
+    }
+    if (second) {
+      const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
+      const double dx[3] = {upper ? xm.x - xn.x : xn.x - xm.x, upper ? xm.y - xn.y : xn.y - xm.y,
+                            upper ? xm.z - xn.z : xn.z - xm.z};
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) gb[5 * a + k] += c[k] * dx[a];
+    }
+  };
   // edges (m, n), m < n: the j-side slots
   for (int q = t.lrow[n]; q < t.lrow[n + 1]; ++q) {
     const size_t l = t.ledge[q];
+    add_slot(EV + 20 * l + 10, t.eij[l].x, false);
+    if (want_x) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) vb[k] += EV[10 * l + 5 + k];
-    if (second) {
-#pragma unroll
-      for (int r = 0; r < 15; ++r) gb[r] += EG[30 * l + 15 + r];
+      for (int a = 0; a < 3; ++a) xb[a] += EX[6 * l + a];
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) xb[a] += EX[6 * l + a];
   }
   // edges (n, m), n < m: the i-side slots
   for (int l = t.eup[n]; l < t.eup[n + 1]; ++l) {
+    add_slot(EV + 20 * (size_t)l, t.eij[l].y, true);
+    if (want_x) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) vb[k] += EV[10 * (size_t)l + k];
-    if (second) {
-#pragma unroll
-      for (int r = 0; r < 15; ++r) gb[r] += EG[30 * (size_t)l + r];
+      for (int a = 0; a < 3; ++a) xb[a] -= EX[6 * (size_t)l + a];
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) xb[a] -= EX[6 * (size_t)l + a];
   }
   const int f0 = t.frow[n], f1 = t.frow[n + 1];
   if (f0 < f1) {
@@ -707,7 +941,7 @@ int ensure_adjoint(aerofx_ctx *c) {
     std::vector<int> fill(frow.begin(), frow.end() - 1);
     for (size_t k = 0; k < 3 * (size_t)F; ++k) nfslot[fill[c->faces[k]]++] = (int)k;
   }
-  if ((rc = alloc_n(c, 10 * (size_t)E, &a->EV)) || (rc = alloc_n(c, 30 * (size_t)E, &a->EG)) ||
+  if ((rc = alloc_n(c, 20 * (size_t)E, &a->EV)) ||
       (rc = alloc_n(c, 6 * (size_t)E, &a->EX)) || (rc = alloc_n(c, 5 * (size_t)N, &a->Vb)) ||
       (rc = alloc_n(c, 16 * (size_t)N, &a->Gb)) || (rc = alloc_n(c, 4 * (size_t)N, &a->Xb)) ||
       (rc = alloc_n(c, 6 * (size_t)N, &a->S)) || (rc = alloc_n(c, 5 * (size_t)N, &a->R)) ||
@@ -743,16 +977,20 @@ int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub
   if ((rc = launch_residual(c, U, a->R, true))) return rc;
   // 2.
   if (t.E > 0) {
-    if (second)
-      hipLaunchKernelGGL((k_adj_edges<true>), grid_for(t.E), dim3(kAdjBlock), 0, c->stream, t, c->d_V, c->d_grad,
-                         lam, c->fc, a->EV, a->EG, a->EX);
+    const dim3 g = grid_for(t.E), b(kAdjBlock);
+    if (second && want_x)
+      hipLaunchKernelGGL((k_adj_edges<true, true>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
+    else if (second)
+      hipLaunchKernelGGL((k_adj_edges<true, false>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
+    else if (want_x)
+      hipLaunchKernelGGL((k_adj_edges<false, true>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
     else
-      hipLaunchKernelGGL((k_adj_edges<false>), grid_for(t.E), dim3(kAdjBlock), 0, c->stream, t, c->d_V, c->d_grad,
-                         lam, c->fc, a->EV, a->EG, a->EX);
+      hipLaunchKernelGGL((k_adj_edges<false, false>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV,
+                         a->EX);
   }
   // 3.
-  hipLaunchKernelGGL(k_adj_nodes, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, lam, c->fc, a->EV,
-                     a->EG, a->EX, second, a->Vb, a->Gb, a->Xb);
+  hipLaunchKernelGGL(k_adj_nodes, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, lam, c->fc, a->EV, a->EX,
+                     second, want_x, a->Vb, a->Gb, a->Xb);
   // 4.
   hipLaunchKernelGGL(k_adj_lsq, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, U, c->gas, a->Vb, a->Gb,
                      second, want_x, Ub, a->Xb, a->S);
