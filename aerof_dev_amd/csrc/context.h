@@ -103,6 +103,12 @@ struct aerofx_ctx {
   void *d_jacD = nullptr, *d_jacB = nullptr, *d_jacDi = nullptr;  // current precision's
   void *jac_store[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // [precision][D, B, Di]
   long long jac_version = 0;  // bumped at every assembly
+  // off-diagonal blocks in mirror order (slot e holds the block of the
+  // mirror entry), so the transposed apply reads them as the forward one
+  // does; rebuilt before transposed use when the assembly is newer
+  void *jac_mirror[2] = {nullptr, nullptr};
+  long long mirror_version = -1;
+  int mirror_precision = -1;
   double *d_fjac = nullptr;    // [3F][25] boundary-face Jacobians, per node face slot
   double *d_dt = nullptr;
   aerofx::Krylov *ksp = nullptr;
@@ -148,6 +154,9 @@ int launch_timestep(aerofx_ctx *c, double cfl, double *dt);
 int launch_jacobian(aerofx_ctx *c, const double *dt, int precision);
 // trans: apply the transposed operator (adjoint solves)
 int launch_h1_apply(aerofx_ctx *c, const double *p, double *y, bool trans = false);
+// builds the mirrored off-diagonal blocks of the current assembly if they are
+// older than it (enqueued on c->stream; call outside graph capture)
+int ensure_mirror(aerofx_ctx *c);
 int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans = false);
 // variants that return at once when *done != 0 (device-resident Krylov loops)
 int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans = false);

@@ -239,7 +239,7 @@ __device__ __forceinline__ void amtv5_add(const T *__restrict__ a, const double 
 // contiguous segments), the products go through LDS, and each row sums its
 // diagonal product and then its entries in order -- the sums of the
 // reference's node loop followed by its edge loop.
-template <class T, bool TRANS>
+template <class T, bool TRANS, bool MIRRORED = false>
 __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__restrict__ D, const T *__restrict__ B,
                                                      const double *__restrict__ p, double *__restrict__ y,
                                                      const int *__restrict__ done) {
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__res
 #pragma unroll
     for (int k = 0; k < 5; ++k) pm[k] = p[5 * (size_t)m + k];
     if (TRANS)
-      amtv5_add(B + blk64_base(t.mirror[e]), pm, cc);
+      amtv5_add(B + blk64_base(MIRRORED ? e : t.mirror[e]), pm, cc);
     else
       amv5_add(B + blk64_base(e), pm, cc);
 #pragma unroll
@@ -417,6 +417,20 @@ __global__ __launch_bounds__(kBlock) void k_dcgs_update(int N, const T *__restri
   for (int k = 0; k < 5; ++k) x[5 * (size_t)n + k] = acc[k];
 }
 
+// BT block e = B block mirror[e] (entries (n, m) and (m, n) swap places), so
+// that lane e of the transposed apply reads slot e: 64 consecutive slots per
+// 512-B (FP64) segment instead of 64 scattered blocks
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mirror_blocks(int nent, const int *__restrict__ mirror,
+                                                          const T *__restrict__ B, T *__restrict__ BT) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nent) return;
+  const T *src = B + blk64_base(mirror[e]);
+  T *dst = BT + blk64_base(e);
+#pragma unroll
+  for (int k = 0; k < 25; ++k) dst[k * kBlkStride] = src[k * kBlkStride];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -489,10 +503,43 @@ int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *
   return check_launch("invert launch");
 }
 
+int ensure_mirror(aerofx_ctx *c) {
+  if (!c->d_jacB) return 0;
+  const int prec = c->jac_precision;
+  if (c->mirror_version == c->jac_version && c->mirror_precision == prec) return 0;
+  const size_t nent = 2 * (size_t)c->E;
+  if (!c->jac_mirror[prec]) {
+    const size_t esz = prec ? sizeof(float) : sizeof(double);
+    int rc = ctx_alloc(c, esz * blk64_count(nent), &c->jac_mirror[prec]);
+    if (rc) return rc;
+  }
+  if (nent) {
+    const dim3 g((unsigned)((nent + kBlock - 1) / kBlock));
+    if (prec)
+      hipLaunchKernelGGL((k_mirror_blocks<float>), g, dim3(kBlock), 0, c->stream, (int)nent, c->tab.mirror,
+                         (const float *)c->d_jacB, (float *)c->jac_mirror[prec]);
+    else
+      hipLaunchKernelGGL((k_mirror_blocks<double>), g, dim3(kBlock), 0, c->stream, (int)nent, c->tab.mirror,
+                         (const double *)c->d_jacB, (double *)c->jac_mirror[prec]);
+    int rc = check_launch("mirror launch");
+    if (rc) return rc;
+  }
+  c->mirror_version = c->jac_version;
+  c->mirror_precision = prec;
+  return 0;
+}
+
+// The transposed apply reads the mirrored blocks when they belong to the
+// current assembly (slot e, coalesced), else the blocks of the mirror
+// entries in place.
 template <class T>
 static void h1_launch(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
   const int nb = c->tab.ngchunks;
-  if (trans)
+  const bool mirrored = c->mirror_version == c->jac_version && c->mirror_precision == c->jac_precision;
+  if (trans && mirrored)
+    hipLaunchKernelGGL((k_h1_apply<T, true, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
+                       (const T *)c->d_jacD, (const T *)c->jac_mirror[c->jac_precision], p, y, done);
+  else if (trans)
     hipLaunchKernelGGL((k_h1_apply<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
                        (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
   else

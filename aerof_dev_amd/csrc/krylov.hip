@@ -773,6 +773,9 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
                  bool trans = false) {
   int rc = ensure_krylov(c, m);
   if (rc) return rc;
+  // the captured transposed cycle reads the mirrored blocks: bring them up
+  // to date before any capture
+  if (trans && (rc = ensure_mirror(c))) return rc;
   Krylov *k = c->ksp;
   const size_t n = k->n;
   *k->h_st = GmresState{};
@@ -927,7 +930,8 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
 
 int aerofx_jvp_transpose(aerofx_ctx *c, const double *p, double *JTp) {
   if (!c || !p || !JTp) return set_error(AEROFX_ERR_ARG, "jvp_transpose: null argument");
-  return launch_h1_apply(c, p, JTp, true);
+  int rc = ensure_mirror(c);
+  return rc ? rc : launch_h1_apply(c, p, JTp, true);
 }
 
 int aerofx_prec_apply_transpose(aerofx_ctx *c, const double *y, double *x) {
