@@ -19,43 +19,10 @@ struct Gas {
   double gam, gam1, invgam1, pstiff;
 };
 
-// Division and square root for the flux kernels.  With AEROFX_FAST_DIV the
-// quotient is a * r with r = 1/b from v_rcp_f64 refined by two Newton steps
-// and one residual correction (error <= 1 ulp, no range scaling: operands
-// are O(1) flow quantities); the square root is x * rsqrt(x) refined the same
-// way.  Without it, IEEE-rounded division and sqrt are used.
-__device__ __forceinline__ double ddiv(double a, double b) {
-#ifdef AEROFX_FAST_DIV
-  double r = __builtin_amdgcn_rcp(b);
-  double e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  double q = a * r;
-  double res = fma(-b, q, a);
-  return fma(res, r, q);
-#else
-  return a / b;
-#endif
-}
-
-__device__ __forceinline__ double dsqrt(double x) {
-#ifdef AEROFX_FAST_DIV
-  double y = __builtin_amdgcn_rsq(x);
-  double h = 0.5 * y;
-  double g = x * y;
-  double r = fma(-g, h, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  r = fma(-g, h, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  double d = fma(-g, g, x);
-  return fma(d, h, g);
-#else
-  return sqrt(x);
-#endif
-}
+// Division and square root of the flux routines: IEEE-rounded, as the
+// reference's; the Dual overloads (device_dual.h) carry derivatives.
+__device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
 
 template <class T>
 __device__ __forceinline__ void prim_from_cons(const T U[5], const Gas &g, T V[5]) {

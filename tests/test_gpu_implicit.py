@@ -162,3 +162,35 @@ def test_h1_transpose_dot(gpu, precision):
     h1.apply_transpose(x, HTx)
     torch.cuda.synchronize()
     assert float((HTx - qv).norm()) <= 1e-7 * float(qv.norm())
+
+
+def test_transpose_follows_reassembly(gpu):
+    """The transposed apply and the transposed GMRES cycle read the
+    off-diagonal blocks in mirror order, copied once per assembly: after a new
+    assembly (another state, then another precision) they must act with the
+    new blocks, exactly as a context that only ever saw those blocks."""
+    import torch
+    from aerof_dev_amd.operators import GmresSolver, MatVecProdH1, SpaceOperator
+    m, p, U, op, om = _setup(gpu, channel=True)
+    rng = np.random.default_rng(21)
+    qv = torch.from_numpy(rng.normal(size=U.shape)).to(gpu)
+    U2 = U * (1.0 + 0.02 * rng.standard_normal(U.shape))
+    h1 = MatVecProdH1(op)
+    for Ucur, prec in ((U, 0), (U2, 0), (U2, 1), (U, 1)):
+        Ud = torch.from_numpy(np.ascontiguousarray(Ucur)).to(gpu)
+        h1.evaluate(Ud, precision=prec)
+        got = torch.empty_like(Ud)
+        h1.apply_transpose(qv, got)
+        xg = torch.empty_like(Ud)
+        GmresSolver(op, 20, 1e-6, 40).solveT(qv, xg)
+        fresh = SpaceOperator(m, p)
+        h1f = MatVecProdH1(fresh)
+        h1f.evaluate(Ud, precision=prec)
+        want = torch.empty_like(Ud)
+        h1f.apply_transpose(qv, want)
+        xw = torch.empty_like(Ud)
+        GmresSolver(fresh, 20, 1e-6, 40).solveT(qv, xw)
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), prec
+        assert torch.equal(xg, xw), prec
+        fresh.close()
