@@ -104,75 +104,91 @@ __device__ __forceinline__ void bmv5_add(const T *__restrict__ a, const double *
     acc[i] += r[5 * i] * b[0] + r[5 * i + 1] * b[1] + r[5 * i + 2] * b[2] + r[5 * i + 3] * b[3] + r[5 * i + 4] * b[4];
 }
 
-// Block H1 apply, entry-parallel: a block covers whole rows (<= kBEnt
-// entries, <= kBRows rows); four lanes take one entry, four columns each, so
-// a wave reads the interleaved blocks of 16 consecutive entries at once and
-// the 16-column rows of P as 32-B pieces.  Products go through LDS and each
+// Block H1 apply, entry-parallel: a block covers whole rows (<= NE entries,
+// <= NR rows); kS/NC lanes take one entry, NC columns each, so a wave reads
+// the interleaved blocks of 64/(kS/NC) consecutive entries at once and the
+// 16-column rows of P in NC-wide pieces.  Products go through LDS and each
 // (row, component, column) sums its diagonal product and then its entries in
-// order: per column exactly the single-vector operator's sums.
-constexpr int kBEnt = 64, kBRows = 16, kBCols = 4;
+// order: per column exactly the single-vector operator's sums.  Two shapes:
+// 32 entries / 8 rows / 2 columns per lane (25 KB of LDS, six blocks per CU)
+// when every node has at most 32 neighbours, else 64 / 16 / 4 (50 KB).
+constexpr int kBEntSmall = 32, kBRowsSmall = 8, kBEnt = 64, kBRows = 16;
 
-__device__ __forceinline__ void load4(const double *p, double b[kBCols]) {
-  const double4 v = *reinterpret_cast<const double4 *>(p);
-  b[0] = v.x;
-  b[1] = v.y;
-  b[2] = v.z;
-  b[3] = v.w;
+template <int NC>
+__device__ __forceinline__ void loadc(const double *p, double b[NC]) {
+  if (NC == 4) {
+    const double4 v = *reinterpret_cast<const double4 *>(p);
+    b[0] = v.x;
+    b[1] = v.y;
+    b[2] = v.z;
+    b[3] = v.w;
+  } else {
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    b[0] = v.x;
+    b[1] = v.y;
+  }
 }
-__device__ __forceinline__ void load4(const float *p, double b[kBCols]) {
-  const float4 v = *reinterpret_cast<const float4 *>(p);
-  b[0] = v.x;
-  b[1] = v.y;
-  b[2] = v.z;
-  b[3] = v.w;
+template <int NC>
+__device__ __forceinline__ void loadc(const float *p, double b[NC]) {
+  if (NC == 4) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    b[0] = v.x;
+    b[1] = v.y;
+    b[2] = v.z;
+    b[3] = v.w;
+  } else {
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    b[0] = v.x;
+    b[1] = v.y;
+  }
 }
 
-template <class T, class TP>
-__device__ __forceinline__ void bmv5x4(const T *__restrict__ a, const TP *__restrict__ p, int cg,
-                                       double acc[5][kBCols]) {
-  double b[5][kBCols], r[25];
+template <int NC, class T, class TP>
+__device__ __forceinline__ void bmv5xc(const T *__restrict__ a, const TP *__restrict__ p, int cg,
+                                       double acc[5][NC]) {
+  double b[5][NC], r[25];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) load4(p + k * kS + kBCols * cg, b[k]);
+  for (int k = 0; k < 5; ++k) loadc<NC>(p + k * kS + NC * cg, b[k]);
 #pragma unroll
   for (int k = 0; k < 25; ++k) r[k] = (double)a[k * kBlkStride];
 #pragma unroll
   for (int i = 0; i < 5; ++i)
 #pragma unroll
-    for (int u = 0; u < kBCols; ++u)
+    for (int u = 0; u < NC; ++u)
       acc[i][u] = r[5 * i] * b[0][u] + r[5 * i + 1] * b[1][u] + r[5 * i + 2] * b[2][u] + r[5 * i + 3] * b[3][u] +
                   r[5 * i + 4] * b[4][u];
 }
 
-template <class T, class TP>
+template <class T, class TP, int NE, int NR, int NC>
 __global__ __launch_bounds__(256) void k_bh1_apply(DevTables t, int nchunks, const int *__restrict__ chunk,
                                                    const T *__restrict__ D, const T *__restrict__ B,
                                                    const TP *__restrict__ P, double *__restrict__ Y) {
-  static_assert(kBEnt * kS / kBCols == 256, "four lanes per entry, 256 lanes per block");
-  __shared__ double sc[kBEnt * 5 * kS];
-  __shared__ double sd[kBRows * 5 * kS];
+  static_assert(NE * kS / NC == 256, "kS/NC lanes per entry, 256 lanes per block");
+  __shared__ double sc[NE * 5 * kS];
+  __shared__ double sd[NR * 5 * kS];
   const int c = xcd_chunk(blockIdx.x, nchunks);
   const int r0 = chunk[c], r1 = chunk[c + 1];
   const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
-  const int tid = threadIdx.x, ent = tid / (kS / kBCols), cg = tid % (kS / kBCols);
-  double acc[5][kBCols];
-  if (ent < e1 - e0) {
+  const int tid = threadIdx.x, ent = tid / (kS / NC), cg = tid % (kS / NC);
+  const int ne = e1 - e0, nr = r1 - r0;
+  double acc[5][NC];
+  if (ent < ne) {
     const int e = e0 + ent;
-    bmv5x4(B + blk64_base(e), P + (size_t)t.adj[e] * 5 * kS, cg, acc);
+    bmv5xc<NC>(B + blk64_base(e), P + (size_t)t.adj[e] * 5 * kS, cg, acc);
 #pragma unroll
     for (int k = 0; k < 5; ++k)
 #pragma unroll
-      for (int u = 0; u < kBCols; ++u) sc[(ent * 5 + k) * kS + kBCols * cg + u] = acc[k][u];
+      for (int u = 0; u < NC; ++u) sc[(ent * 5 + k) * kS + NC * cg + u] = acc[k][u];
   }
-  if (ent < r1 - r0) {
+  if (ent < nr) {
     const int n = r0 + ent;
-    bmv5x4(D + blk64_base(n), P + (size_t)n * 5 * kS, cg, acc);
+    bmv5xc<NC>(D + blk64_base(n), P + (size_t)n * 5 * kS, cg, acc);
 #pragma unroll
     for (int k = 0; k < 5; ++k)
 #pragma unroll
-      for (int u = 0; u < kBCols; ++u) sd[(ent * 5 + k) * kS + kBCols * cg + u] = 0.0 + acc[k][u];
+      for (int u = 0; u < NC; ++u) sd[(ent * 5 + k) * kS + NC * cg + u] = 0.0 + acc[k][u];
   }
   __syncthreads();
-  const int nr = r1 - r0;
   for (int idx = tid; idx < nr * 5 * kS; idx += 256) {
     const int rr = idx / (5 * kS), q = idx - rr * 5 * kS;
     const int n = r0 + rr;
@@ -649,8 +665,9 @@ struct BlockSolver {
   int *flag = nullptr;
   float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
   float *Zf = nullptr;        // [npad][16] FP32 preconditioned block (input of the FP32 H1 apply)
-  int *bchunk = nullptr;      // row chunks of the block H1 apply (<= kBEnt entries, <= kBRows rows)
+  int *bchunk = nullptr;      // row chunks of the block H1 apply (see k_bh1_apply for the limits)
   int nbchunks = 0;
+  bool bsmall = false;        // chunks of <= kBEntSmall entries and <= kBRowsSmall rows
   long long f32_version = -1;     // assembly the FP32 copies were made from
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -736,21 +753,30 @@ void cholqr2(BlockSolver *s, const double *W, double *Q, double *Rout) {
   hipLaunchKernelGGL(k_mm16, dim3(1), dim3(256), 0, s->stream, s->Rt2, s->Rt, Rout);
 }
 
+// W = H1 P over the solver's chunks, in the shape the chunks were built for
+template <class T, class TP>
+static void bh1_launch(aerofx_ctx *c, BlockSolver *s, const T *D, const T *B, const TP *P, double *W) {
+  if (s->bsmall)
+    hipLaunchKernelGGL((k_bh1_apply<T, TP, kBEntSmall, kBRowsSmall, 2>), dim3(s->nbchunks), dim3(256), 0, s->stream,
+                       c->tab, s->nbchunks, s->bchunk, D, B, P, W);
+  else
+    hipLaunchKernelGGL((k_bh1_apply<T, TP, kBEnt, kBRows, 4>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab,
+                       s->nbchunks, s->bchunk, D, B, P, W);
+}
+
 // Z = P V_j, W = H1 Z: FP64 operators on FP64 vectors, or FP32 operators on
 // an FP32 Z (the products are formed in FP64)
 void apply_ops64(aerofx_ctx *c, BlockSolver *s, const double *Vj, double *W) {
   const int nb = (int)((c->N * (size_t)kS + 255) / 256);
   hipLaunchKernelGGL((k_bjacobi_apply<double, double>), dim3(nb), dim3(256), 0, s->stream, c->N,
                      (const double *)c->d_jacDi, Vj, s->Z);
-  hipLaunchKernelGGL((k_bh1_apply<double, double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab,
-                     s->nbchunks, s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->Z, W);
+  bh1_launch(c, s, (const double *)c->d_jacD, (const double *)c->d_jacB, (const double *)s->Z, W);
 }
 
 void apply_ops32(aerofx_ctx *c, BlockSolver *s, const double *Vj, double *W) {
   const int nb = (int)((c->N * (size_t)kS + 255) / 256);
   hipLaunchKernelGGL((k_bjacobi_apply<float, float>), dim3(nb), dim3(256), 0, s->stream, c->N, s->Di32, Vj, s->Zf);
-  hipLaunchKernelGGL((k_bh1_apply<float, float>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
-                     s->bchunk, s->D32, s->B32, s->Zf, W);
+  bh1_launch(c, s, (const float *)s->D32, (const float *)s->B32, (const float *)s->Zf, W);
 }
 
 // one inner cycle: from s->R (the residual block), builds V, Hbar, S0
@@ -934,15 +960,19 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
       *rc = set_hip_error(e0, "block solver row pointers");
       return nullptr;
     }
+    int maxdeg = 0;
+    for (int n = 0; n < c->N; ++n) maxdeg = std::max(maxdeg, rp[n + 1] - rp[n]);
+    if (maxdeg > kBEnt) {
+      delete s;
+      *rc = set_error(AEROFX_ERR_MESH, "block_gmres: a node has more than 64 neighbours");
+      return nullptr;
+    }
+    s->bsmall = maxdeg <= kBEntSmall;
+    const int lim_ent = s->bsmall ? kBEntSmall : kBEnt, lim_rows = s->bsmall ? kBRowsSmall : kBRows;
     int acc = 0, rows = 0;
     for (int n = 0; n < c->N; ++n) {
       const int cnt = rp[n + 1] - rp[n];
-      if (cnt > kBEnt) {
-        delete s;
-        *rc = set_error(AEROFX_ERR_MESH, "block_gmres: a node has more than 64 neighbours");
-        return nullptr;
-      }
-      if (acc + cnt > kBEnt || rows == kBRows) {
+      if (acc + cnt > lim_ent || rows == lim_rows) {
         ch.push_back(n);
         acc = 0;
         rows = 0;
@@ -1066,8 +1096,7 @@ extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, dou
   int cycle = 0;
   for (;; ++cycle) {
     // R = B - A64 X (FP64 refinement residual)
-    hipLaunchKernelGGL((k_bh1_apply<double, double>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab, s->nbchunks,
-                       s->bchunk, (const double *)c->d_jacD, (const double *)c->d_jacB, s->X, s->R);
+    bh1_launch(c, s, (const double *)c->d_jacD, (const double *)c->d_jacB, (const double *)s->X, s->R);
     hipLaunchKernelGGL(k_bsub_from, g1(blk), dim3(256), 0, s->stream, blk, s->B, s->R);
     hipLaunchKernelGGL(k_colnorm_partial, dim3(kChunks), dim3(256), 0, s->stream, s->npad, s->R, s->nrm);
     hipLaunchKernelGGL(k_colnorm_reduce, dim3(1), dim3(64), 0, s->stream, s->nrm, s->nrm + (size_t)kChunks * kS);
