@@ -665,9 +665,11 @@ struct BlockSolver {
   int *flag = nullptr;
   float *D32 = nullptr, *B32 = nullptr, *Di32 = nullptr;
   float *Zf = nullptr;        // [npad][16] FP32 preconditioned block (input of the FP32 H1 apply)
-  int *bchunk = nullptr;      // row chunks of the block H1 apply (see k_bh1_apply for the limits)
-  int nbchunks = 0;
-  bool bsmall = false;        // chunks of <= kBEntSmall entries and <= kBRowsSmall rows
+  // row chunks of the block H1 apply: [0] <= kBEnt entries, <= kBRows rows;
+  // [1] <= kBEntSmall, <= kBRowsSmall (built when every node fits)
+  int *bchunk[2] = {nullptr, nullptr};
+  int nbchunks[2] = {0, 0};
+  bool bsmall = false;
   long long f32_version = -1;     // assembly the FP32 copies were made from
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -756,12 +758,17 @@ void cholqr2(BlockSolver *s, const double *W, double *Q, double *Rout) {
 // W = H1 P over the solver's chunks, in the shape the chunks were built for
 template <class T, class TP>
 static void bh1_launch(aerofx_ctx *c, BlockSolver *s, const T *D, const T *B, const TP *P, double *W) {
-  if (s->bsmall)
-    hipLaunchKernelGGL((k_bh1_apply<T, TP, kBEntSmall, kBRowsSmall, 2>), dim3(s->nbchunks), dim3(256), 0, s->stream,
-                       c->tab, s->nbchunks, s->bchunk, D, B, P, W);
-  else
-    hipLaunchKernelGGL((k_bh1_apply<T, TP, kBEnt, kBRows, 4>), dim3(s->nbchunks), dim3(256), 0, s->stream, c->tab,
-                       s->nbchunks, s->bchunk, D, B, P, W);
+  // the narrow shape pays at FP32 (6 workgroups per CU); at FP64 its
+  // register count leaves 4 waves per SIMD and the wide shape is faster
+  if constexpr (sizeof(T) == sizeof(float)) {
+    if (s->bsmall) {
+      hipLaunchKernelGGL((k_bh1_apply<T, TP, kBEntSmall, kBRowsSmall, 2>), dim3(s->nbchunks[1]), dim3(256), 0,
+                         s->stream, c->tab, s->nbchunks[1], s->bchunk[1], D, B, P, W);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_bh1_apply<T, TP, kBEnt, kBRows, 4>), dim3(s->nbchunks[0]), dim3(256), 0, s->stream, c->tab,
+                     s->nbchunks[0], s->bchunk[0], D, B, P, W);
 }
 
 // Z = P V_j, W = H1 Z: FP64 operators on FP64 vectors, or FP32 operators on
@@ -953,7 +960,7 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
   s->flag = (int *)v;
   {
     // row chunks for the block H1 apply, from the adjacency row pointers
-    std::vector<int> rp(c->N + 1), ch(1, 0);
+    std::vector<int> rp(c->N + 1);
     hipError_t e0 = hipMemcpy(rp.data(), c->tab.rowptr, sizeof(int) * rp.size(), hipMemcpyDeviceToHost);
     if (e0 != hipSuccess) {
       delete s;
@@ -968,31 +975,34 @@ BlockSolver *get_block(aerofx_ctx *c, int m, int *rc) {
       return nullptr;
     }
     s->bsmall = maxdeg <= kBEntSmall;
-    const int lim_ent = s->bsmall ? kBEntSmall : kBEnt, lim_rows = s->bsmall ? kBRowsSmall : kBRows;
-    int acc = 0, rows = 0;
-    for (int n = 0; n < c->N; ++n) {
-      const int cnt = rp[n + 1] - rp[n];
-      if (acc + cnt > lim_ent || rows == lim_rows) {
-        ch.push_back(n);
-        acc = 0;
-        rows = 0;
+    for (int shape = 0; shape < (s->bsmall ? 2 : 1); ++shape) {
+      const int lim_ent = shape ? kBEntSmall : kBEnt, lim_rows = shape ? kBRowsSmall : kBRows;
+      std::vector<int> ch(1, 0);
+      int acc = 0, rows = 0;
+      for (int n = 0; n < c->N; ++n) {
+        const int cnt = rp[n + 1] - rp[n];
+        if (acc + cnt > lim_ent || rows == lim_rows) {
+          ch.push_back(n);
+          acc = 0;
+          rows = 0;
+        }
+        acc += cnt;
+        rows += 1;
       }
-      acc += cnt;
-      rows += 1;
-    }
-    ch.push_back(c->N);
-    s->nbchunks = (int)ch.size() - 1;
-    if ((r = ctx_alloc(c, sizeof(int) * ch.size(), &v))) {
-      delete s;
-      *rc = r;
-      return nullptr;
-    }
-    s->bchunk = (int *)v;
-    e0 = hipMemcpy(s->bchunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice);
-    if (e0 != hipSuccess) {
-      delete s;
-      *rc = set_hip_error(e0, "block solver chunks");
-      return nullptr;
+      ch.push_back(c->N);
+      s->nbchunks[shape] = (int)ch.size() - 1;
+      if ((r = ctx_alloc(c, sizeof(int) * ch.size(), &v))) {
+        delete s;
+        *rc = r;
+        return nullptr;
+      }
+      s->bchunk[shape] = (int *)v;
+      e0 = hipMemcpy(s->bchunk[shape], ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice);
+      if (e0 != hipSuccess) {
+        delete s;
+        *rc = set_hip_error(e0, "block solver chunks");
+        return nullptr;
+      }
     }
   }
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
