@@ -92,8 +92,12 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *
 // Phase 2: one lane per (row, element) adds the entries in entry order, then
 // the faces in face order, scales by 1/V and adds 1/dt -- the reference's
 // per-node order.
+// Two roejac6 evaluations inline take 256 VGPRs + 18 AGPRs, one wave per
+// SIMD; asking for two waves spills ~18 registers to scratch and is faster
+// (assembly 50 -> 38 ms over a 96x32x32 FP64 steady solve; three waves spill
+// 116 and are slower).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
                                                      const double *__restrict__ dt, double *__restrict__ fjac,
                                                      T *__restrict__ D, T *__restrict__ B) {
   __shared__ double sd[25 * kPad];
