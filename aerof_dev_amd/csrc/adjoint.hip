@@ -12,7 +12,7 @@
 // Stages (all deterministic: per-edge, per-tet and per-face results go to
 // their own slots and each node sums its slots in a fixed order):
 //   1. forward residual: V, LSQ gradients, R (scaled)
-//   2. k_adj_edges  one thread per edge: mu = lambda/V, g = mu_i - mu_j;
+//   2. k_adj_edges_u / _x  one thread per edge: mu = lambda/V, g = mu_i - mu_j;
 //      g^T dF/d(V_ij, V_ji, n) by the reverse mode of the Roe flux,
 //      then reverse through the Van Albada reconstruction by hand
 //      -> per edge: Vbar_i, Vbar_j, Gbar_i, Gbar_j, dxbar, nbar
@@ -381,10 +381,9 @@ __device__ __forceinline__ void va_partials(double a, double b, double &da, doub
 // 2. per edge
 // ---------------------------------------------------------------------------
 template <bool SECOND, bool WANTX>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const double *__restrict__ V,
-                                                         const double *__restrict__ grad,
-                                                         const double *__restrict__ lam, FluxConsts fc,
-                                                         double *__restrict__ EV, double *__restrict__ EX) {
+__device__ __forceinline__ void adj_edge(const DevTables &t, const double *__restrict__ V,
+                                         const double *__restrict__ grad, const double *__restrict__ lam,
+                                         const FluxConsts &fc, double *__restrict__ EV, double *__restrict__ EX) {
   const long long l = (long long)blockIdx.x * kAdjBlock + threadIdx.x;
   if (l >= t.E) return;
   const int2 ij = t.eij[l];
@@ -485,6 +484,23 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_edges(DevTables t, const doub
       ex[3 + a] = fb[10 + a];
     }
   }
+}
+
+// The (dR/dU)^T-only edge pass fits two waves per SIMD without spilling
+// (256 VGPRs; 2.10 -> 1.83 ms at 815k nodes); with the coordinate slots it
+// would spill 68 registers and is left at the compiler's choice.
+template <bool SECOND>
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_edges_u(
+    DevTables t, const double *__restrict__ V, const double *__restrict__ grad, const double *__restrict__ lam,
+    FluxConsts fc, double *__restrict__ EV, double *__restrict__ EX) {
+  adj_edge<SECOND, false>(t, V, grad, lam, fc, EV, EX);
+}
+template <bool SECOND>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_edges_x(DevTables t, const double *__restrict__ V,
+                                                           const double *__restrict__ grad,
+                                                           const double *__restrict__ lam, FluxConsts fc,
+                                                           double *__restrict__ EV, double *__restrict__ EX) {
+  adj_edge<SECOND, true>(t, V, grad, lam, fc, EV, EX);
 }
 
 // ---------------------------------------------------------------------------
@@ -979,14 +995,13 @@ int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub
   if (t.E > 0) {
     const dim3 g = grid_for(t.E), b(kAdjBlock);
     if (second && want_x)
-      hipLaunchKernelGGL((k_adj_edges<true, true>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
+      hipLaunchKernelGGL((k_adj_edges_x<true>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
     else if (second)
-      hipLaunchKernelGGL((k_adj_edges<true, false>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
+      hipLaunchKernelGGL((k_adj_edges_u<true>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
     else if (want_x)
-      hipLaunchKernelGGL((k_adj_edges<false, true>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
+      hipLaunchKernelGGL((k_adj_edges_x<false>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
     else
-      hipLaunchKernelGGL((k_adj_edges<false, false>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV,
-                         a->EX);
+      hipLaunchKernelGGL((k_adj_edges_u<false>), g, b, 0, c->stream, t, c->d_V, c->d_grad, lam, c->fc, a->EV, a->EX);
   }
   // 3.
   hipLaunchKernelGGL(k_adj_nodes, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, lam, c->fc, a->EV, a->EX,
