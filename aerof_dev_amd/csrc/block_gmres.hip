@@ -236,9 +236,12 @@ __global__ void k_f64_to_f32(size_t len, const double *__restrict__ a, float *__
 // the same rows: both are the 4x16 row slab [k0 + l/16][l%16].
 // With TWO, the same sweep also forms V_b^T W2 into partial2 (one read of
 // the V_b for two right-hand blocks).  NBMAX (>= nb) sizes the accumulators:
-// fewer registers, more waves per SIMD for the early steps of a cycle.
+// fewer registers, more waves per SIMD for the early steps of a cycle.  The
+// 12-block instance fits 254 VGPRs with no accumulation registers when asked
+// for two waves per SIMD (it used 254 + 16, one wave): 16-RHS mixed solve at
+// 128^3 3.92 -> 3.76 s.  The 17-block one would spill.
 template <bool TWO, int NBMAX>
-__global__ __launch_bounds__(256) void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBMAX == 12 ? 2 : 1))) void k_tsg_dots(size_t npad, size_t rows_per_chunk, int nchunks, int mp1, int nb,
                            const double *__restrict__ V, size_t ldv, const double *__restrict__ W,
                            const double *__restrict__ W2, double *__restrict__ partial,
                            double *__restrict__ partial2) {
