@@ -186,7 +186,11 @@ int aerofx_jvp(aerofx_ctx *ctx, const double *p, double *Jp);
 /* JacobiPrec::apply (KspPrec.C:87-111): x = D^-1 y. */
 int aerofx_prec_apply(aerofx_ctx *ctx, const double *y, double *x);
 /* MatVecProdH1::applyTranspose (MatVecProd.C:1189-1206,
- * SubDomain.C:3958-3990): JTp = H1^T p, per node in the reference's order. */
+ * SubDomain.C:3958-3990): JTp = H1^T p, per node in the reference's order.
+ * The first transposed use after an assembly (here, in
+ * aerofx_gmres_transpose or aerofx_adjoint_solve) copies the off-diagonal
+ * blocks into mirror order, into one more buffer of the blocks' size in the
+ * current precision, allocated on first use and kept with the context. */
 int aerofx_jvp_transpose(aerofx_ctx *ctx, const double *p, double *JTp);
 /* Transposed block-Jacobi: x = (D^-1)^T y (the preconditioner of the
  * transposed solve, KspSolver::solveT, KspSolver.C:754). */
@@ -202,7 +206,8 @@ int aerofx_jvp_fd(aerofx_ctx *ctx, const double *U, const double *p, double *Jp,
                   int order, const double *dt);
 /* GmresSolver::solve (KspSolver.C:353-492): right-preconditioned GMRES(m)
  * on the operators of the last aerofx_jacobian_setup, x0 = 0, stop at
- * ||r|| <= eps ||b|| or maxits.  Orthogonalisation is CGS2 in FP64. */
+ * ||r|| <= eps ||b|| or maxits.  Orthogonalisation is CGS2 in FP64, the
+ * reorthogonalisation of each vector delayed to the next step. */
 int aerofx_gmres(aerofx_ctx *ctx, const double *b, double *x, int m,
                  double eps, int maxits, int *its, double *final_res);
 
@@ -228,7 +233,10 @@ typedef struct {
  * the Krylov basis and its block Gram-Schmidt (FP64 MFMA tall-skinny GEMMs)
  * are FP64 either way.  use_graph != 0 replays each cycle as one captured
  * hipGraph.  Stops when every ||b_k - A x_k|| <= eps ||b_k|| or after
- * max_cycles.  No counterpart in the reference: its multi-RHS
+ * max_cycles.  Nodes may have at most 64 neighbours (AEROFX_ERR_MESH
+ * otherwise); a block space of 16 m vectors that exhausts a small system
+ * makes the block rank-deficient (AEROFX_ERR_STATE): use a smaller m.
+ * No counterpart in the reference: its multi-RHS
  * GmresSolver::solve(VecSet&, VecSet&) (KspSolver.C:988-1135) is dead code. */
 int aerofx_block_gmres(aerofx_ctx *ctx, int nrhs, const double *B, double *X,
                        int m, double eps, int max_cycles, int precision,
