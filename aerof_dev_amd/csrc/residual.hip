@@ -24,7 +24,10 @@
 
 namespace aerofx {
 
-constexpr int kGradChunk = 4;  // neighbours per load batch of the gradient pass
+// neighbours per load batch of the gradient pass (measured at 128^3: 2 ->
+// 0.317 ms, 4 -> 0.310, 6 -> 0.307, 8 -> 0.301; from 10 on the batch no
+// longer stays in registers)
+constexpr int kGradChunk = 8;
 
 // ---------------------------------------------------------------------------
 // 1. gradients (+ primitive state): one lane per node, neighbours in
