@@ -16,8 +16,10 @@ constexpr int kNumXcd = 8;        // MI355X: 8 XCDs, blocks dealt round-robin
 constexpr int kPad = kBlock + 1;  // LDS row stride (odd: conflict-free)
 // threads (= edges) per block of the flux pass; also the largest number of
 // upper edges one node may have (measured at 128^3: 256 -> 0.645 ms,
-// 128 -> 0.629 ms, 64 -> 0.618 ms for the edge-flux kernel)
+// 128 -> 0.629 ms, 64 -> 0.618 ms for the edge-flux kernel).  A mesh whose
+// nodes all have at most kFluxBlockSmall upper edges gets the small blocks.
 constexpr int kFluxBlock = 128;
+constexpr int kFluxBlockSmall = 64;
 
 // Block b of a launch runs on XCD b % 8; remap so that each XCD sweeps one
 // contiguous range of chunks (its L2 then holds that range's neighbours).
@@ -51,6 +53,7 @@ struct DevTables {
   const int *mirror;     // [2E] entry (m, n) of the entry (n, m)
   // flux pass A: edges grouped by their lower node i (edge order)
   int nfchunks;
+  int fblock;            // edges per flux block: kFluxBlockSmall or kFluxBlock
   const int4 *fchunk4;   // [nfchunks] (first row, end row, first edge, end edge) of each chunk
   const int *eup;        // [N+1] first edge whose lower node is i
   const int2 *eij;       // [E] edge endpoints (i, j), i < j

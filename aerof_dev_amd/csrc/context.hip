@@ -222,7 +222,10 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     }
     const char *why = nullptr;
     std::vector<int> gchunk = chunk_rows(rowptr, N, kBlock, &why);
-    std::vector<int> fchunk = why ? std::vector<int>() : chunk_rows(eup, N, kFluxBlock, &why);
+    int maxup = 0;
+    for (int n = 0; n < N; ++n) maxup = std::max(maxup, eup[n + 1] - eup[n]);
+    const int fblock = maxup <= kFluxBlockSmall ? kFluxBlockSmall : kFluxBlock;
+    std::vector<int> fchunk = why ? std::vector<int>() : chunk_rows(eup, N, fblock, &why);
     if (why) {
       destroy(c);
       return set_error(AEROFX_ERR_MESH, std::string("ctx_create: ") + why);
@@ -266,6 +269,7 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     }
     t.ngchunks = (int)gchunk.size() - 1;
     t.nfchunks = (int)fchunk.size() - 1;
+    t.fblock = fblock;
     void *pv = nullptr;
     if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
     c->d_V = static_cast<double *>(pv);

@@ -130,17 +130,17 @@ __global__ __launch_bounds__(kBlock) void k_gradients(DevTables t, const double 
 
 // ---------------------------------------------------------------------------
 // 2. edge fluxes, computed once per edge.  A block covers whole rows (all
-//    edges whose lower node lies in [r0, r1), <= kFluxBlock edges); lane k of the
+//    edges whose lower node lies in [r0, r1), <= FB edges); lane k of the
 //    block takes edge l0 + k.  Node i data are read directly (the ~7 lanes of
 //    a row hit the same lines); only the fluxes go through LDS, for the
 //    per-row sums P_i in edge order.
 // ---------------------------------------------------------------------------
-template <bool SECOND>
-__global__ __launch_bounds__(kFluxBlock) void k_flux_edges(DevTables t, const double *__restrict__ V,
+template <bool SECOND, int FB>
+__global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ grad, FluxConsts fc,
                                                        double *__restrict__ Fm, double *__restrict__ P,
                                                        int *__restrict__ err) {
-  __shared__ double sf[5 * (kFluxBlock + 1)];
+  __shared__ double sf[5 * (FB + 1)];
   const int4 ch = t.fchunk4[xcd_chunk(blockIdx.x, t.nfchunks)];
   const int r0 = ch.x, r1 = ch.y, l0 = ch.z, l1 = ch.w;
   const int tid = threadIdx.x;
@@ -198,18 +198,18 @@ __global__ __launch_bounds__(kFluxBlock) void k_flux_edges(DevTables t, const do
     double *dst = Fm + 5 * (size_t)l;  // contiguous in edge order
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      sf[k * (kFluxBlock + 1) + tid] = f[k];
+      sf[k * (FB + 1) + tid] = f[k];
       dst[k] = f[k];
     }
   }
   __syncthreads();
   const int nr = r1 - r0;
-  for (int idx = tid; idx < nr * 5; idx += kFluxBlock) {
+  for (int idx = tid; idx < nr * 5; idx += FB) {
     const int rr = idx / 5, q = idx - 5 * rr;
     const int r = r0 + rr;
     const int b = t.eup[r] - l0, eend = t.eup[r + 1] - l0;
     double s = 0.0;
-    for (int e = b; e < eend; ++e) s += sf[q * (kFluxBlock + 1) + e];
+    for (int e = b; e < eend; ++e) s += sf[q * (FB + 1) + e];
     P[5 * (size_t)r + q] = s;
   }
 }
@@ -291,11 +291,18 @@ int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale) {
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_TERM);
-    if (c->p.second_order)
-      hipLaunchKernelGGL((k_flux_edges<true>), dim3(t.nfchunks), dim3(kFluxBlock), 0, c->stream, t, c->d_V,
+    const dim3 g(t.nfchunks);
+    if (c->p.second_order && t.fblock == kFluxBlockSmall)
+      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t, c->d_V,
                          c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err);
+    else if (c->p.second_order)
+      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, c->d_V, c->d_grad,
+                         c->fc, c->d_Fm, c->d_P, c->d_err);
+    else if (t.fblock == kFluxBlockSmall)
+      hipLaunchKernelGGL((k_flux_edges<false, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t,
+                         c->d_V, c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err);
     else
-      hipLaunchKernelGGL((k_flux_edges<false>), dim3(t.nfchunks), dim3(kFluxBlock), 0, c->stream, t, c->d_V,
+      hipLaunchKernelGGL((k_flux_edges<false, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, c->d_V,
                          c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err);
   }
   {
