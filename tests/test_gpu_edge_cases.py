@@ -187,3 +187,41 @@ def test_gmres_exhausts_small_krylov_space(gpu, precision):
     assert its <= 30, its
     assert np.linalg.norm(A @ xg - b) <= 1e-10 * np.linalg.norm(b)
     assert np.linalg.norm(xg - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def _cone(k):
+    """k tets around the axis from the base centre to the apex (node 0):
+    the apex has k + 1 neighbours, all numbered after it."""
+    ang = 2.0 * np.pi * np.arange(k) / k
+    rim = np.stack([np.cos(ang), np.sin(ang), np.zeros(k)], axis=1)
+    X = np.vstack([[0.0, 0.0, 1.0], [0.0, 0.0, 0.0], rim])
+    tets, faces = [], []
+    for i in range(k):
+        a, b = 2 + i, 2 + (i + 1) % k
+        t = [0, 1, a, b]
+        q = X[t]
+        if np.dot(np.cross(q[1] - q[0], q[2] - q[0]), q[3] - q[0]) < 0:
+            t = [0, 1, b, a]
+        tets.append(t)
+        for f in ([0, a, b], [1, b, a]):
+            r = X[f]
+            if np.dot(np.cross(r[1] - r[0], r[2] - r[0]), (r[0] + r[1] + r[2]) / 3.0 - [0.0, 0.0, 0.4]) < 0:
+                f = [f[0], f[2], f[1]]
+            faces.append(f)
+    return M.Mesh(X, np.array(tets, np.int32), np.array(faces, np.int32),
+                  np.full(len(faces), AEROFX_BC_INLET, np.int32))
+
+
+@pytest.mark.parametrize("second_order", [True, False])
+def test_node_with_many_upper_edges(gpu, second_order):
+    """A node with 81 upper edges: more than the 64-edge flux blocks hold, so
+    the context takes 128-edge blocks; the residual still matches."""
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _cone(80)
+    p = FlowParams(mach=0.5, alpha=0.1, second_order=second_order)
+    U = init_state(p, m.X, "noise", 0.01, seed=12)
+    ref = O.OracleMesh(m, p).residual(U, want_abs=True)
+    assert ref["ierr"] == 0
+    ierr, Rg = _residual(SpaceOperator(m, p), U, gpu)
+    assert ierr == 0
+    _check(Rg, ref)
