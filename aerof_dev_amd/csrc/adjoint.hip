@@ -873,7 +873,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_xgather(DevTables t, const do
                                                            const int *__restrict__ nfslot,
                                                            const double *__restrict__ FX,
                                                            double *__restrict__ out) {
-  const int n = blockIdx.x * kAdjBlock + threadIdx.x;
+  // XCD-aware block order (each XCD a contiguous node range): 0.1 ms less at
+  // 815k nodes; the same order made the node-sum and LSQ reverse passes slower
+  const int nb = (t.N + kAdjBlock - 1) / kAdjBlock;
+  const int n = xcd_chunk(blockIdx.x, nb) * kAdjBlock + (int)threadIdx.x;
   if (n >= t.N) return;
   double xb[3] = {Xb[4 * (size_t)n], Xb[4 * (size_t)n + 1], Xb[4 * (size_t)n + 2]};
   if (second) {
