@@ -16,6 +16,8 @@ LIB_PATH = os.environ.get("AEROFX_LIB") or os.path.join(_HERE, "_lib", "libaerof
 AEROFX_BC_SLIP_WALL = 2
 AEROFX_BC_INLET = 4
 AEROFX_BC_OUTLET = 5
+AEROFX_MVP_H1 = 0
+AEROFX_MVP_FD = 1
 TIMER_NAMES = ("nodalGrad", "fvTerm", "fvTermNodes", "fvJac", "fluidPrecSetup", "jacApply",
                "precApply", "gramSchmidt")
 
@@ -55,7 +57,8 @@ class NewtonOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_steps", C.c_int), ("eps", C.c_double),
                 ("cfl0", C.c_double), ("cfl_max", C.c_double), ("ser", C.c_double),
                 ("cfl_coef1", C.c_double), ("cfl_coef2", C.c_double),
-                ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double)]
+                ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double),
+                ("mvp", C.c_int), ("fd_order", C.c_int)]
 
 
 class BlockStats(C.Structure):
@@ -85,6 +88,7 @@ SIGNATURES = {
     "aerofx_version": (C.c_int, []),
     "aerofx_mesh_box_sizes": (C.c_int, [C.POINTER(BoxSpec), c_int_p, c_int_p, c_int_p]),
     "aerofx_mesh_box": (C.c_int, [C.POINTER(BoxSpec), c_double_p, c_int_p, c_int_p, c_int_p]),
+    "aerofx_mesh_box_bump_derivative": (C.c_int, [C.POINTER(BoxSpec), C.c_int, C.c_int, c_double_p]),
     "aerofx_mesh_edges": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p]),
     "aerofx_mesh_rcm": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p]),
     "aerofx_mesh_renumber": (C.c_int, [C.c_int, C.c_int, C.c_int, c_int_p, c_double_p, c_int_p, c_int_p]),
@@ -96,6 +100,7 @@ SIGNATURES = {
     "aerofx_ctx_sizes": (C.c_int, [C.c_void_p, c_int_p, c_int_p, c_int_p]),
     "aerofx_ctx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "aerofx_ctx_sync": (C.c_int, [C.c_void_p]),
+    "aerofx_ctx_set_residual_kernels": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_timers": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_longlong)]),
     "aerofx_ctx_tables": (C.c_int, [C.c_void_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p]),
@@ -112,6 +117,8 @@ SIGNATURES = {
     "aerofx_gmres": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                c_int_p, c_double_p]),
     "aerofx_jvp_fd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "aerofx_mvp_select": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "aerofx_mvp_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_jvp_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_prec_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_gmres_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
@@ -124,6 +131,9 @@ SIGNATURES = {
     "aerofx_adjoint_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(AdjointOpts),
                                        C.POINTER(AdjointStats)]),
     "aerofx_adjoint_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_direct_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.POINTER(AdjointOpts), C.POINTER(AdjointStats)]),
+    "aerofx_residual_derivative": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_steady_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats),
                                       c_double_p, C.c_int]),
 }

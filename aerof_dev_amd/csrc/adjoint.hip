@@ -28,6 +28,7 @@
 
 #include "context.h"
 #include "device_dual.h"
+#include "device_sens.h"
 #include "host_internal.h"
 
 namespace aerofx {
@@ -58,27 +59,6 @@ constexpr int kAdjBlock = 256;
 // ElemTet edge -> (end, end) and edge -> (face, face) tables (ElemTetCore.C:19-21)
 __constant__ int cEdgeEnd[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
 __constant__ int cEdgeFace[6][2] = {{0, 1}, {2, 0}, {1, 2}, {0, 3}, {3, 1}, {2, 3}};
-
-// (1/r11, 1/r22, 1/r33, r12/r11, r23/r22, psi) from the LSQ moments
-// M = sum dx dx^T (xx, xy, xz, yy, yz, zz), in the order of the set-up
-// (SubDomainCore.C:1755-1777, SubDomain.C:137-159).
-template <class T>
-__device__ __forceinline__ void lsq_params(const T M[6], T L[6]) {
-  T r11 = gsqrt(M[0]);
-  T or11 = 1.0 / r11;
-  T r12 = M[1] * or11;
-  T r13 = M[2] * or11;
-  T r22 = gsqrt(M[3] - r12 * r12);
-  T r23 = (M[4] - r12 * r13) / r22;
-  T r33 = gsqrt(M[5] - (r13 * r13 + r23 * r23));
-  T o11 = 1.0 / r11, o22 = 1.0 / r22, o33 = 1.0 / r33;
-  L[0] = o11;
-  L[1] = o22;
-  L[2] = o33;
-  L[3] = r12 * o11;
-  L[4] = r23 * o22;
-  L[5] = (r12 * r23 - r13 * r22) * o11 * o22;
-}
 
 // LSQ weights W(dx; L) as the gradient pass evaluates them.
 __device__ __forceinline__ void lsq_weights(const double L[6], double dx0, double dx1, double dx2, double w[3]) {
@@ -114,17 +94,6 @@ __device__ __forceinline__ void lsq_weights_rev(const double L[6], double dx0, d
     Lb[3] += -alpha2 * wb[0] - c2 * dx0;
     Lb[4] += -alpha3 * wb[1] - c3 * dx1;
     Lb[5] += alpha3 * wb[0] + c3 * dx0;
-  }
-}
-
-// boundary flux of one face node; nv = (1/3) face normal
-template <class T>
-__device__ __forceinline__ void face_flux(const FluxConsts &fc, int code, const T nv[3], const T V[5], T fl[5]) {
-  if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
-    wallflux(nv, 0.0, V, fl);
-  } else {
-    const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
-    boundflux5(fc.gas, nv, 0.0, V, ub, fl);
   }
 }
 
@@ -941,34 +910,22 @@ int upload_vec(aerofx_ctx *c, const std::vector<T> &v, T **out) {
 int ensure_adjoint(aerofx_ctx *c) {
   if (c->adj) return 0;
   const int N = c->N, E = c->E, T = c->T, F = c->F;
-  if ((int)c->tets.size() != 4 * T || (int)c->tet_edges.size() != 6 * T || (int)c->faces.size() != 3 * F)
-    return set_error(AEROFX_ERR_STATE, "adjoint: element tables missing from the context");
+  int rc = ensure_elem_tables(c);
+  if (rc) return rc;
   Adjoint *a = new Adjoint();
-  int rc = 0;
-  // node -> (tet, vertex) slots in tet order; node -> (face, vertex) in the order of tab.fidx
-  std::vector<int> ntrow(N + 1, 0), ntslot(4 * (size_t)T), nfslot(3 * (size_t)F);
-  for (size_t k = 0; k < 4 * (size_t)T; ++k) ntrow[c->tets[k] + 1]++;
-  for (int n = 0; n < N; ++n) ntrow[n + 1] += ntrow[n];
-  {
-    std::vector<int> fill(ntrow.begin(), ntrow.end() - 1);
-    for (size_t k = 0; k < 4 * (size_t)T; ++k) ntslot[fill[c->tets[k]]++] = (int)k;
-  }
-  {
-    std::vector<int> frow(N + 1, 0);
-    for (size_t k = 0; k < 3 * (size_t)F; ++k) frow[c->faces[k] + 1]++;
-    for (int n = 0; n < N; ++n) frow[n + 1] += frow[n];
-    std::vector<int> fill(frow.begin(), frow.end() - 1);
-    for (size_t k = 0; k < 3 * (size_t)F; ++k) nfslot[fill[c->faces[k]]++] = (int)k;
-  }
+  const ElemTables &el = *c->elem;
+  a->tets = el.tets;
+  a->tedge = el.tedge;
+  a->faces = el.faces;
+  a->ntrow = el.ntrow;
+  a->ntslot = el.ntslot;
+  a->nfslot = el.nfslot;
   if ((rc = alloc_n(c, 20 * (size_t)E, &a->EV)) ||
       (rc = alloc_n(c, 6 * (size_t)E, &a->EX)) || (rc = alloc_n(c, 5 * (size_t)N, &a->Vb)) ||
       (rc = alloc_n(c, 16 * (size_t)N, &a->Gb)) || (rc = alloc_n(c, 4 * (size_t)N, &a->Xb)) ||
       (rc = alloc_n(c, 6 * (size_t)N, &a->S)) || (rc = alloc_n(c, 5 * (size_t)N, &a->R)) ||
       (rc = alloc_n(c, 12 * (size_t)std::max(T, 1), &a->TX)) ||
-      (rc = alloc_n(c, 9 * (size_t)std::max(F, 1), &a->FX)) || (rc = alloc_n(c, (size_t)F + 1, &a->Jf)) || (rc = upload_vec(c, c->tets, &a->tets)) ||
-      (rc = upload_vec(c, c->tet_edges, &a->tedge)) || (rc = upload_vec(c, c->faces, &a->faces)) ||
-      (rc = upload_vec(c, ntrow, &a->ntrow)) || (rc = upload_vec(c, ntslot, &a->ntslot)) ||
-      (rc = upload_vec(c, nfslot, &a->nfslot))) {
+      (rc = alloc_n(c, 9 * (size_t)std::max(F, 1), &a->FX)) || (rc = alloc_n(c, (size_t)F + 1, &a->Jf))) {
     delete a;
     return rc;
   }
@@ -984,6 +941,47 @@ int ensure_adjoint(aerofx_ctx *c) {
 inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kAdjBlock - 1) / kAdjBlock)); }
 
 }  // namespace
+
+int ensure_elem_tables(aerofx_ctx *c) {
+  if (c->elem) return 0;
+  const int N = c->N, E = c->E, T = c->T, F = c->F;
+  if ((int)c->tets.size() != 4 * T || (int)c->tet_edges.size() != 6 * T || (int)c->faces.size() != 3 * F)
+    return set_error(AEROFX_ERR_STATE, "element tables missing from the context");
+  // node -> (tet, vertex) slots in tet order; node -> (face, vertex) in the
+  // order of tab.fidx; edge -> (tet, local edge) slots in tet order
+  std::vector<int> ntrow(N + 1, 0), ntslot(4 * (size_t)T), nfslot(3 * (size_t)F);
+  std::vector<int> etrow(E + 1, 0), etslot(6 * (size_t)T);
+  for (size_t k = 0; k < 4 * (size_t)T; ++k) ntrow[c->tets[k] + 1]++;
+  for (int n = 0; n < N; ++n) ntrow[n + 1] += ntrow[n];
+  {
+    std::vector<int> fill(ntrow.begin(), ntrow.end() - 1);
+    for (size_t k = 0; k < 4 * (size_t)T; ++k) ntslot[fill[c->tets[k]]++] = (int)k;
+  }
+  {
+    std::vector<int> frow(N + 1, 0);
+    for (size_t k = 0; k < 3 * (size_t)F; ++k) frow[c->faces[k] + 1]++;
+    for (int n = 0; n < N; ++n) frow[n + 1] += frow[n];
+    std::vector<int> fill(frow.begin(), frow.end() - 1);
+    for (size_t k = 0; k < 3 * (size_t)F; ++k) nfslot[fill[c->faces[k]]++] = (int)k;
+  }
+  for (size_t k = 0; k < 6 * (size_t)T; ++k) etrow[c->tet_edges[k] + 1]++;
+  for (int l = 0; l < E; ++l) etrow[l + 1] += etrow[l];
+  {
+    std::vector<int> fill(etrow.begin(), etrow.end() - 1);
+    for (size_t k = 0; k < 6 * (size_t)T; ++k) etslot[fill[c->tet_edges[k]]++] = (int)k;
+  }
+  ElemTables *el = new ElemTables();
+  int rc;
+  if ((rc = upload_vec(c, c->tets, &el->tets)) || (rc = upload_vec(c, c->tet_edges, &el->tedge)) ||
+      (rc = upload_vec(c, c->faces, &el->faces)) || (rc = upload_vec(c, ntrow, &el->ntrow)) ||
+      (rc = upload_vec(c, ntslot, &el->ntslot)) || (rc = upload_vec(c, nfslot, &el->nfslot)) ||
+      (rc = upload_vec(c, etrow, &el->etrow)) || (rc = upload_vec(c, etslot, &el->etslot))) {
+    delete el;
+    return rc;
+  }
+  c->elem = el;
+  return 0;
+}
 
 int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout) {
   int rc = ensure_adjoint(c);

@@ -792,7 +792,7 @@ void apply_ops32(aerofx_ctx *c, BlockSolver *s, const double *Vj, double *W) {
 // one inner cycle: from s->R (the residual block), builds V, Hbar, S0
 void enqueue_cycle(aerofx_ctx *c, BlockSolver *s, int prec) {
   const size_t blk = s->npad * kS;
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->flag, 1);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s->stream, s->flag, 1, nullptr);
   // equilibrate the residual columns first: they converge at different
   // rates, and Cholesky-QR squares the condition number of the block
   double *cn = s->nrm + (size_t)kChunks * kS + kS;
@@ -1050,6 +1050,10 @@ int ensure_f32(aerofx_ctx *c, BlockSolver *s) {
   // the FP32 block-Jacobi inverse is LINPACK in float on the FP32 blocks,
   // as the single-RHS mixed path builds it
   if ((rc = launch_invert_f32(c, s->stream, s->D32, s->Di32))) return rc;
+  hipError_t e = hipMemcpyAsync(c->h_err + 1, c->d_err + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return set_hip_error(e, "block_gmres FP32 inverse");
+  if (c->h_err[1]) return set_error(AEROFX_ERR_STATE, "block_gmres: zero pivot in an FP32 diagonal block");
   s->f32_version = c->jac_version;
   return 0;
 }
@@ -1085,9 +1089,16 @@ extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, dou
   BlockSolver *s = get_block(c, m, &rc);
   if (!s) return rc;
   *st = aerofx_block_stats{};
-  // order after the caller's work on the context stream
-  hipEvent_t ev;
-  hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  // order after the caller's work on the context stream; the event is
+  // destroyed on every return
+  struct EventGuard {
+    hipEvent_t ev = nullptr;
+    ~EventGuard() {
+      if (ev) (void)hipEventDestroy(ev);
+    }
+  } guard;
+  hipError_t e = hipEventCreateWithFlags(&guard.ev, hipEventDisableTiming);
+  hipEvent_t ev = guard.ev;
   if (e == hipSuccess) e = hipEventRecord(ev, c->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(s->stream, ev, 0);
   if (e != hipSuccess) return set_hip_error(e, "block_gmres stream order");
@@ -1177,7 +1188,6 @@ extern "C" int aerofx_block_gmres(aerofx_ctx *c, int nrhs, const double *Bv, dou
   if (e == hipSuccess) e = hipEventRecord(ev, s->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, ev, 0);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  (void)hipEventDestroy(ev);
   if (e != hipSuccess) return set_hip_error(e, "block_gmres finish");
   st->cycles = cycle;
   st->res_max = worst;

@@ -20,6 +20,16 @@ constexpr int kPad = kBlock + 1;  // LDS row stride (odd: conflict-free)
 // nodes all have at most kFluxBlockSmall upper edges gets the small blocks.
 constexpr int kFluxBlock = 128;
 constexpr int kFluxBlockSmall = 64;
+// LDS-staged residual passes: strips of consecutive nodes (gradients) or
+// rows (fluxes) whose touched nodes are loaded into LDS once per block.
+//   gradient strip: <= kGradStrip nodes (one lane each), <= kGradHalo staged
+//                   nodes of 8 doubles (V, X): 61,440 B of LDS
+//   flux strip:     <= kFluxStrip edges (one lane each), <= kFluxHalo staged
+//                   nodes of 23 doubles (V, X, gradients): 35,328 B of LDS
+constexpr int kGradStrip = 128;
+constexpr int kGradHalo = 960;
+constexpr int kFluxStrip = 256;
+constexpr int kFluxHalo = 192;
 
 // Block b of a launch runs on XCD b % 8; remap so that each XCD sweeps one
 // contiguous range of chunks (its L2 then holds that range's neighbours).
@@ -67,6 +77,17 @@ struct DevTables {
   const int *fcode;      // [F] boundary code
   const double *fnorm3;  // [F][3] (1/3) face normal
   const double *fnorm;   // [F][3] face normal
+  // LDS-staged passes (ngb == 0: the mesh does not fit, per-lane kernels run)
+  int ngb;               // gradient strips
+  const int4 *gb4;       // [ngb] (first node, end node, first halo entry, end halo entry)
+  const int *gown;       // [ngb] halo position of the strip's first node
+  const int *ghalo;      // staged node ids, ascending within each strip
+  const uint16_t *gloc;  // [2E] halo position of each adjacency entry's neighbour
+  int nfb;               // flux strips
+  const int4 *fb4;       // [nfb] (first row, end row, first edge, end edge)
+  const int2 *fbh;       // [nfb] (first halo entry, end halo entry)
+  const int *fhalo;      // staged node ids, ascending within each strip
+  const uint32_t *floc;  // [E] halo position of i | halo position of j << 16
 };
 
 struct FluxConsts {
@@ -79,6 +100,21 @@ int set_hip_error(hipError_t e, const char *what);
 struct Krylov;
 struct Adjoint;
 struct BlockSolver;
+struct Tangent;
+
+// Element tables of the geometry derivatives (device copies, built on first
+// use): the adjoint and the tangent passes sum per-element contributions per
+// node or per edge in element order.
+struct ElemTables {
+  int *tets = nullptr;    // [T][4]
+  int *tedge = nullptr;   // [T][6] edge of each local tet edge
+  int *faces = nullptr;   // [F][3]
+  int *ntrow = nullptr;   // [N+1]
+  int *ntslot = nullptr;  // [4T] 4 t + local vertex, ascending t per node
+  int *nfslot = nullptr;  // [3F] 3 f + local vertex, in the order of DevTables::fidx
+  int *etrow = nullptr;   // [E+1]
+  int *etslot = nullptr;  // [6T] 6 t + local edge, ascending t per edge
+};
 
 }  // namespace aerofx
 
@@ -100,6 +136,7 @@ struct aerofx_ctx {
   double *d_Fm = nullptr;    // [E][5] edge fluxes in edge order
   double *d_P = nullptr;     // [N][5] per-node sums over upper edges
   int *d_err = nullptr;      // [0] reconstruction errors, [1] singular blocks, [2] bad states
+  int residual_kernels = 0;  // AEROFX_RESIDUAL_* (auto: LDS-staged when the mesh fits)
   int *h_err = nullptr;
   // implicit operators (jacobian.hip) and Krylov workspace (krylov.hip)
   int jac_precision = -1;    // 0: FP64 blocks, 1: FP32 blocks
@@ -113,9 +150,20 @@ struct aerofx_ctx {
   long long mirror_version = -1;
   int mirror_precision = -1;
   double *d_fjac = nullptr;    // [3F][25] boundary-face Jacobians, per node face slot
+  // MatVecProd of aerofx_gmres and the Newton driver (AEROFX_MVP_*) and the
+  // scratch of the finite-difference products, allocated on first use:
+  // Q, Fa, Fb, F0 [5N] each, reduction partials [kDotBlocks], scalars [16]
+  int mvp_type = AEROFX_MVP_H1, mvp_order = 1;
+  const double *mvp_Q = nullptr, *mvp_F0 = nullptr, *mvp_dt = nullptr;
+  double *d_fd = nullptr;
+  // direct sensitivity solves: residuals and corrections of up to 16
+  // right-hand sides plus one product vector (allocated on first use)
+  double *d_direct = nullptr;
   double *d_dt = nullptr;
   aerofx::Krylov *ksp = nullptr;
   aerofx::Adjoint *adj = nullptr;
+  aerofx::ElemTables *elem = nullptr;
+  aerofx::Tangent *tan = nullptr;
   aerofx::BlockSolver *blk = nullptr;
   int blk_tune[2] = {0, 0};  // autotuned waves per workgroup of the block GEMMs
   // per-kernel timers (the reference's Timer buckets, Timer.h:12-15)
@@ -145,10 +193,12 @@ struct ProfScope {
 // device allocation owned by the context
 int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out);
 
-__global__ void k_zero_int(int *p, int n);
+__global__ void k_zero_int(int *p, int n, const int *done);
 
-// residual passes (residual.hip); all enqueue on c->stream
-int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale);
+// residual passes (residual.hip); all enqueue on c->stream; every kernel
+// returns at once when done != nullptr and *done != 0 (device-resident
+// Krylov loops with a matrix-free operator)
+int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done = nullptr);
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz);
 int launch_primitive(aerofx_ctx *c, const double *U, double *V);
 
@@ -175,10 +225,18 @@ int launch_dcgs_update(aerofx_ctx *c, double *V, size_t ldv, int j, const double
                        const int *done, const int *jlast, double *x, bool trans);
 void free_krylov(aerofx_ctx *c);
 void free_adjoint(aerofx_ctx *c);
+void free_tangent(aerofx_ctx *c);
+// c->elem (adjoint.hip)
+int ensure_elem_tables(aerofx_ctx *c);
+// dR = (dR/dU) dU + (dR/dX) dX at U (tangent.hip; dU, dX may be NULL),
+// enqueued on c->stream; the forward residual it runs leaves R (raw flux
+// sums) in the tangent workspace
+int launch_tangent(aerofx_ctx *c, const double *U, const double *dU, const double *dX, double *dR);
 // (dR/dU)^T lam -> Ub and/or (dR/dX)^T lam -> Xout (adjoint.hip), enqueued on c->stream
 int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout);
 void free_block(aerofx_ctx *c);
-// LINPACK inverse of FP32 5x5 blocks on a given stream (error count in d_err[1])
+// LINPACK inverse of FP32 5x5 blocks on a given stream (zero-pivot count in
+// d_err[1], cleared first)
 int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *Di);
 
 }  // namespace aerofx

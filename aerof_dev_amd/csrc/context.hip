@@ -74,6 +74,8 @@ void destroy(aerofx_ctx *c) {
   collect_timers(c);
   free_krylov(c);
   free_adjoint(c);
+  free_tangent(c);
+  delete c->elem;  // device memory is in c->allocs
   free_block(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   for (void *p : c->allocs) (void)hipFree(p);
@@ -270,6 +272,38 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     t.ngchunks = (int)gchunk.size() - 1;
     t.nfchunks = (int)fchunk.size() - 1;
     t.fblock = fblock;
+    // LDS-staged strips (both passes or neither)
+    {
+      HaloStrips gs, fs;
+      std::vector<int> jcol(E);
+      for (int l = 0; l < E; ++l) jcol[l] = ptr[2 * (size_t)l + 1];
+      if (halo_strips(N, rowptr.data(), adj.data(), kGradStrip, 1 << 30, kGradHalo, gs) &&
+          halo_strips(N, eup.data(), jcol.data(), kFluxStrip, kFluxStrip, kFluxHalo, fs)) {
+        const int ngb = (int)gs.own.size(), nfb = (int)fs.own.size();
+        std::vector<int4> gb4(ngb), fb4(nfb);
+        std::vector<int2> fbh(nfb);
+        for (int b = 0; b < ngb; ++b) gb4[b] = make_int4(gs.start[b], gs.start[b + 1], gs.hptr[b], gs.hptr[b + 1]);
+        for (int b = 0; b < nfb; ++b) {
+          fb4[b] = make_int4(fs.start[b], fs.start[b + 1], eup[fs.start[b]], eup[fs.start[b + 1]]);
+          fbh[b] = make_int2(fs.hptr[b], fs.hptr[b + 1]);
+        }
+        // position of each edge's lower node: rows are consecutive from own[b]
+        std::vector<uint32_t> floc(E);
+        for (int b = 0; b < nfb; ++b)
+          for (int r = fs.start[b]; r < fs.start[b + 1]; ++r)
+            for (int l = eup[r]; l < eup[r + 1]; ++l)
+              floc[l] = (uint32_t)(fs.own[b] + (r - fs.start[b])) | ((uint32_t)fs.loc[l] << 16);
+        if ((rc = upload(c, gb4, &t.gb4)) || (rc = upload(c, gs.own, &t.gown)) ||
+            (rc = upload(c, gs.halo, &t.ghalo)) || (rc = upload(c, gs.loc, &t.gloc)) ||
+            (rc = upload(c, fb4, &t.fb4)) || (rc = upload(c, fbh, &t.fbh)) || (rc = upload(c, fs.halo, &t.fhalo)) ||
+            (rc = upload(c, floc, &t.floc))) {
+          destroy(c);
+          return rc;
+        }
+        t.ngb = ngb;
+        t.nfb = nfb;
+      }
+    }
     void *pv = nullptr;
     if ((rc = ctx_alloc(c, 5 * sizeof(double) * (size_t)N, &pv))) { destroy(c); return rc; }
     c->d_V = static_cast<double *>(pv);
@@ -317,6 +351,14 @@ int aerofx_ctx_set_stream(aerofx_ctx *c, void *stream, int use_context_stream) {
   if (!c) return set_error(AEROFX_ERR_ARG, "set_stream: null ctx");
   c->stream = use_context_stream ? c->own_stream : static_cast<hipStream_t>(stream);
   return AEROFX_OK;
+}
+
+int aerofx_ctx_set_residual_kernels(aerofx_ctx *c, int family) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "set_residual_kernels: null ctx");
+  if (family != AEROFX_RESIDUAL_AUTO && family != AEROFX_RESIDUAL_PER_LANE)
+    return set_error(AEROFX_ERR_ARG, "set_residual_kernels: AEROFX_RESIDUAL_AUTO or AEROFX_RESIDUAL_PER_LANE");
+  c->residual_kernels = family;
+  return family == AEROFX_RESIDUAL_AUTO && c->tab.ngb > 0 ? AEROFX_RESIDUAL_STAGED : AEROFX_RESIDUAL_PER_LANE;
 }
 
 int aerofx_ctx_sync(aerofx_ctx *c) {

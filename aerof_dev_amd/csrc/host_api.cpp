@@ -87,6 +87,21 @@ int aerofx_mesh_box(const aerofx_box_spec *spec, double *X, int *tets, int *face
   return AEROFX_OK;
 }
 
+int aerofx_mesh_box_bump_derivative(const aerofx_box_spec *spec, int bump, int param, double *dX) {
+  if (!spec || !dX) return set_error(AEROFX_ERR_ARG, "mesh_box_bump_derivative: null argument");
+  if (spec->nbumps < 0 || (spec->nbumps > 0 && (!spec->bump_amp || !spec->bump_center || !spec->bump_width)))
+    return set_error(AEROFX_ERR_ARG, "mesh_box_bump_derivative: bad bump arrays");
+  try {
+    BoxSpec b = to_spec(spec);
+    int N, T, F;
+    kuhn_box_sizes(b, N, T, F);
+    kuhn_box_bump_derivative(b, bump, param, dX);
+  } catch (std::exception &e) {
+    return set_error(AEROFX_ERR_ARG, e.what());
+  }
+  return AEROFX_OK;
+}
+
 int aerofx_mesh_edges(int N, int T, const int *tets, int *num_edges, int *ptr) {
   if (N <= 0 || T <= 0 || !tets || !num_edges) return set_error(AEROFX_ERR_ARG, "mesh_edges: bad argument");
   try {

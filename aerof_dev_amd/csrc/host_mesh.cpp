@@ -61,29 +61,59 @@ void kuhn_box_sizes(const BoxSpec &s, int &N, int &T, int &F) {
   N = (int)n; T = (int)t; F = (int)f;
 }
 
+// Lattice position of node (i, j, k) with the interior jitter, before the
+// wall bumps.
+static void lattice_node(const BoxSpec &s, int i, int j, int k, double x[3]) {
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const int n = i + (nx + 1) * (j + (ny + 1) * k);
+  const double h[3] = {(s.hi[0] - s.lo[0]) / nx, (s.hi[1] - s.lo[1]) / ny, (s.hi[2] - s.lo[2]) / nz};
+  const uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
+  x[0] = s.lo[0] + h[0] * i;
+  x[1] = s.lo[1] + h[1] * j;
+  x[2] = s.lo[2] + h[2] * k;
+  if (i == nx) x[0] = s.hi[0];
+  if (j == ny) x[1] = s.hi[1];
+  if (k == nz) x[2] = s.hi[2];
+  bool interior = i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz;
+  if (interior && s.jitter != 0.0) {
+    for (int a = 0; a < 3; ++a) {
+      uint32_t ctr[4] = {(uint32_t)n, (uint32_t)a, 0x4a495454u /* "JITT" */, 0u}, o[4];
+      philox4x32_10(ctr, key, o);
+      double u = u01_53(o[0], o[1]);
+      x[a] += (2.0 * u - 1.0) * s.jitter * h[a];
+    }
+  }
+}
+
+void kuhn_box_bump_derivative(const BoxSpec &s, int b, int param, double *dX) {
+  if (b < 0 || b >= s.nbumps || (param != 0 && param != 1))
+    throw std::invalid_argument("kuhn_box_bump_derivative: bad bump or parameter");
+  for (int k = 0; k <= s.nz; ++k)
+    for (int j = 0; j <= s.ny; ++j)
+      for (int i = 0; i <= s.nx; ++i) {
+        const int n = i + (s.nx + 1) * (j + (s.ny + 1) * k);
+        double x[3];
+        lattice_node(s, i, j, k, x);
+        // z += a exp(-t^2) (1 - zeta), t = (x - c) / w
+        const double zeta = (x[2] - s.lo[2]) / (s.hi[2] - s.lo[2]);
+        const double t = (x[0] - s.bump_center[b]) / s.bump_width[b];
+        const double g = std::exp(-t * t) * (1.0 - zeta);
+        dX[3 * (size_t)n] = 0.0;
+        dX[3 * (size_t)n + 1] = 0.0;
+        dX[3 * (size_t)n + 2] = param == 0 ? g : s.bump_amp[b] * g * 2.0 * t / s.bump_width[b];
+      }
+}
+
 void kuhn_box(const BoxSpec &s, double *X, int *tets, int *faces, int *codes) {
   const int nx = s.nx, ny = s.ny, nz = s.nz;
   auto id = [&](int i, int j, int k) { return i + (nx + 1) * (j + (ny + 1) * k); };
-  const double h[3] = {(s.hi[0] - s.lo[0]) / nx, (s.hi[1] - s.lo[1]) / ny, (s.hi[2] - s.lo[2]) / nz};
-  const uint32_t key[2] = {(uint32_t)s.seed, (uint32_t)(s.seed >> 32)};
   // node coordinates: lattice, interior jitter, then the wall bumps
   for (int k = 0; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j)
       for (int i = 0; i <= nx; ++i) {
         int n = id(i, j, k);
-        double x[3] = {s.lo[0] + h[0] * i, s.lo[1] + h[1] * j, s.lo[2] + h[2] * k};
-        if (i == nx) x[0] = s.hi[0];
-        if (j == ny) x[1] = s.hi[1];
-        if (k == nz) x[2] = s.hi[2];
-        bool interior = i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz;
-        if (interior && s.jitter != 0.0) {
-          for (int a = 0; a < 3; ++a) {
-            uint32_t ctr[4] = {(uint32_t)n, (uint32_t)a, 0x4a495454u /* "JITT" */, 0u}, o[4];
-            philox4x32_10(ctr, key, o);
-            double u = u01_53(o[0], o[1]);
-            x[a] += (2.0 * u - 1.0) * s.jitter * h[a];
-          }
-        }
+        double x[3];
+        lattice_node(s, i, j, k, x);
         if (s.nbumps > 0) {
           double zeta = (x[2] - s.lo[2]) / (s.hi[2] - s.lo[2]);
           double dz = 0.0;
@@ -300,6 +330,48 @@ void node_csr(int N, int E, const int *ptr, std::vector<int> &rowptr, std::vecto
     col[fill[i]] = j; eid[fill[i]++] = l;
     col[fill[j]] = i; eid[fill[j]++] = l;
   }
+}
+
+bool halo_strips(int N, const int *rowptr, const int *col, int max_rows, int max_items, int cap,
+                 HaloStrips &hs) {
+  hs = HaloStrips();
+  if (cap > 65536) return false;  // positions are 16-bit
+  hs.start.push_back(0);
+  hs.hptr.push_back(0);
+  hs.loc.assign((size_t)rowptr[N], 0);
+  std::vector<int> stamp(N, -1), pos(N, 0), cur;
+  int blk = 0, r0 = 0;
+  auto close = [&](int r1) {
+    std::sort(cur.begin(), cur.end());
+    for (size_t k = 0; k < cur.size(); ++k) pos[cur[k]] = (int)k;
+    for (int e = rowptr[r0]; e < rowptr[r1]; ++e) hs.loc[e] = (uint16_t)pos[col[e]];
+    hs.own.push_back(pos[r0]);
+    hs.halo.insert(hs.halo.end(), cur.begin(), cur.end());
+    hs.max_halo = std::max(hs.max_halo, (int)cur.size());
+    hs.start.push_back(r1);
+    hs.hptr.push_back((int)hs.halo.size());
+    cur.clear();
+    ++blk;
+    r0 = r1;
+  };
+  for (int r = 0; r < N; ++r) {
+    int fresh = stamp[r] != blk;
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) fresh += stamp[col[e]] != blk;
+    const bool fits = r - r0 < max_rows && rowptr[r + 1] - rowptr[r0] <= max_items &&
+                      (int)cur.size() + fresh <= cap;
+    if (!fits) {
+      if (r == r0) return false;  // this row alone exceeds a limit
+      close(r);
+      // the row starts a new block: count again against the empty halo
+      fresh = 1 + (rowptr[r + 1] - rowptr[r]);
+      if (1 > max_rows || rowptr[r + 1] - rowptr[r] > max_items || fresh > cap) return false;
+    }
+    if (stamp[r] != blk) { stamp[r] = blk; cur.push_back(r); }
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e)
+      if (stamp[col[e]] != blk) { stamp[col[e]] = blk; cur.push_back(col[e]); }
+  }
+  if (r0 < N) close(N);
+  return true;
 }
 
 // ---------------------------------------------------------------------------

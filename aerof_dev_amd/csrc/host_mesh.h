@@ -30,6 +30,8 @@ struct BoxSpec {
 // triangles oriented so that 0.5*(x2-x0)^(x1-x0) points outward.
 void kuhn_box_sizes(const BoxSpec &s, int &N, int &T, int &F);
 void kuhn_box(const BoxSpec &s, double *X, int *tets, int *faces, int *codes);
+// Closed-form dX/dp [N][3] of bump b's amplitude (param 0) or centre (1).
+void kuhn_box_bump_derivative(const BoxSpec &s, int b, int param, double *dX);
 
 // Unique edges sorted lexicographically by (min id, max id); optional
 // tet -> edge map [T][6] in ElemTet local-edge order.
@@ -50,6 +52,23 @@ void lsq_factors(int N, int E, const double *X, const int *ptr, double *R);
 // of the incident edges' ids).  eid[k] = edge id of entry k.
 void node_csr(int N, int E, const int *ptr, std::vector<int> &rowptr,
               std::vector<int> &col, std::vector<int> &eid);
+
+// Strips of consecutive rows for kernels that stage the data of every node a
+// block touches in LDS once.  Row r owns the entries [rowptr[r], rowptr[r+1])
+// with target nodes col[e].  Block b covers rows [start[b], start[b+1]); its
+// halo is the sorted set of the rows themselves and all targets of their
+// entries, halo[hptr[b] .. hptr[b+1]).  A block closes before it would exceed
+// max_rows rows, max_items entries or cap halo nodes.  loc[e] is the position
+// of col[e] in its block's halo and own[b] the position of row start[b] (the
+// rows are consecutive ids, so row r sits at own[b] + r - start[b]).
+// Returns false if a single row alone exceeds a limit.
+struct HaloStrips {
+  std::vector<int> start, hptr, halo, own;
+  std::vector<uint16_t> loc;
+  int max_halo = 0;
+};
+bool halo_strips(int N, const int *rowptr, const int *col, int max_rows, int max_items, int cap,
+                 HaloStrips &hs);
 
 // Reference RCM rule over the edge graph with self loops: renum[old] = new.
 void rcm(int N, int E, const int *ptr, int *renum);

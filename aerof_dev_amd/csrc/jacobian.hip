@@ -490,7 +490,7 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   if ((rc = check_launch("jacobian launch"))) return rc;
   {
     ProfScope ps(c, AEROFX_TIMER_PREC_SETUP);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 1, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 1, 1, nullptr);
     if (precision)
       hipLaunchKernelGGL((k_invert<float>), dim3(nb), dim3(kBlock), 0, c->stream, c->N, (const float *)c->d_jacD,
                          (float *)c->d_jacDi, c->d_err + 1);
@@ -503,6 +503,7 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
 
 int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *Di) {
   const int nb = (c->N + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, c->d_err + 1, 1, nullptr);
   hipLaunchKernelGGL((k_invert<float>), dim3(nb), dim3(kBlock), 0, stream, c->N, D, Di, c->d_err + 1);
   return check_launch("invert launch");
 }

@@ -222,7 +222,8 @@ struct RowPairs {
 
 // partial[b] = sum over block b's slice of w_i^2
 __global__ __launch_bounds__(kBlock) void k_sumsq(size_t n, const double *__restrict__ w,
-                                                  double *__restrict__ partial) {
+                                                  double *__restrict__ partial, const int *__restrict__ done) {
+  if (done && *done) return;
   const RowPairs rp(n);
   double acc = 0.0;
   for (size_t i = rp.b0 + 2 * threadIdx.x; i < rp.even_end; i += 2 * kBlock) {
@@ -315,7 +316,8 @@ __global__ __launch_bounds__(kBlock) void k_dots2(size_t n, const double *__rest
 
 // out[k] = sum_b partial[k][b] in a fixed tree, one block per k
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const double *__restrict__ partial, int nparts,
-                                                            double *__restrict__ out) {
+                                                            double *__restrict__ out, const int *__restrict__ done) {
+  if (done && *done) return;
   __shared__ double sh[kBlock];
   const int k = blockIdx.x;
   double s = 0.0;
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_check_states(int N, const double *__
 }
 
 // ---------------------------------------------------------------------------
-// MatVecProdFD (MatVecProd.C:262-372, 879-929)
+// MatVecProdFD (MatVecProd.C:88-118, 262-372, 879-929)
 // ---------------------------------------------------------------------------
 // partial sums of eps0 |U| + eps0 over all entries (computeEpsilon)
 __global__ __launch_bounds__(kBlock) void k_eps_sum(size_t n, const double *__restrict__ U,
@@ -556,7 +558,8 @@ __global__ __launch_bounds__(kBlock) void k_eps_sum(size_t n, const double *__re
 }
 
 // scal[0] = p.p, scal[1] = sum(eps0 |U| + eps0) -> scal[2] = eps, [3] = 1/eps, [4] = 0.5/eps
-__global__ void k_fd_eps(double *scal, double size) {
+__global__ void k_fd_eps(double *scal, double size, const int *__restrict__ done) {
+  if (done && *done) return;
   if (threadIdx.x != 0) return;
   const double norm = sqrt(scal[0]);
   const double eps = norm < 1.0e-14 ? 1.e-6 : scal[1] / (size * norm);
@@ -568,7 +571,8 @@ __global__ void k_fd_eps(double *scal, double size) {
 // Q = U + s eps p (s = +1 or -1)
 __global__ __launch_bounds__(kBlock) void k_fd_shift(size_t n, const double *__restrict__ U,
                                                      const double *__restrict__ p, const double *__restrict__ scal,
-                                                     double s, double *__restrict__ Q) {
+                                                     double s, double *__restrict__ Q, const int *__restrict__ done) {
+  if (done && *done) return;
   const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
   if (i >= n) return;
   const double ep = scal[2] * p[i];
@@ -579,7 +583,8 @@ __global__ __launch_bounds__(kBlock) void k_fd_shift(size_t n, const double *__r
 // F += (1/dt) (1 Q + (-1) Un)
 __global__ __launch_bounds__(kBlock) void k_fd_time(size_t n, const double *__restrict__ Q,
                                                     const double *__restrict__ U, const double *__restrict__ dt,
-                                                    double *__restrict__ F) {
+                                                    double *__restrict__ F, const int *__restrict__ done) {
+  if (done && *done) return;
   const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
   if (i >= n) return;
   const double invDt = 1.0 / dt[i / 5];
@@ -589,10 +594,88 @@ __global__ __launch_bounds__(kBlock) void k_fd_time(size_t n, const double *__re
 // prod = scal[idx] (Fa - Fb)
 __global__ __launch_bounds__(kBlock) void k_fd_diff(size_t n, const double *__restrict__ Fa,
                                                     const double *__restrict__ Fb, const double *__restrict__ scal,
-                                                    int idx, double *__restrict__ prod) {
+                                                    int idx, double *__restrict__ prod, const int *__restrict__ done) {
+  if (done && *done) return;
   const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
   if (i >= n) return;
   prod[i] = scal[idx] * (Fa[i] - Fb[i]);
+}
+
+// err[3] += err[0]: every residual pass restarts err[0]; the products of
+// MatVecProdFD sum the counts of all their residual evaluations in err[3]
+__global__ void k_err_accumulate(int *err, const int *__restrict__ done) {
+  if (done && *done) return;
+  if (threadIdx.x == 0) err[3] += err[0];
+}
+
+// Blocks of the row-pair sweeps over n values: about 8 entries per lane per
+// sweep, between 64 blocks and one per CU (fixed by n, so the reduction
+// order never depends on the call).
+static int sweep_parts(size_t n) {
+  return (int)std::min<size_t>(kSweepBlocks, std::max<size_t>(64, (n + 8 * kBlock - 1) / (8 * kBlock)));
+}
+
+static inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+// Scratch of the finite-difference products.
+struct FdScratch {
+  double *Q = nullptr, *Fa = nullptr, *Fb = nullptr;  // [5N]
+  double *partial = nullptr;                           // [kDotBlocks]
+  double *scal = nullptr;                              // [8], layout of k_fd_eps
+};
+
+// computeEpsilon, the state part: scal[1] = sum(eps0 |U| + eps0).
+static void enqueue_fd_state_sum(aerofx_ctx *c, const FdScratch &s, const double *U) {
+  const size_t n = 5 * (size_t)c->N;
+  hipLaunchKernelGGL(k_eps_sum, dim3(kDotBlocks), dim3(kBlock), 0, c->stream, n, U, s.partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, s.partial, kDotBlocks, s.scal + 1,
+                     nullptr);
+}
+
+// One MatVecProdFD::apply (MatVecProd.C:262-372) with the state sum already
+// in scal[1]: eps from ||p|| (computeEpsilon), then
+//   order 1: prod = (F(U + eps p) - F0) / eps
+//   order 2: prod = (F(U + eps p) - F(U - eps p)) / (2 eps)
+// with F(Q) = R(Q) + (Q - U) / dt (dt NULL: R only).  The residual passes'
+// error counts are added to err[3].  Every kernel returns at once when
+// *done != 0 (done may be NULL).
+static int enqueue_fd_product(aerofx_ctx *c, const FdScratch &s, int nparts, const double *U, const double *dt,
+                              const double *F0, const double *p, double *prod, int order, const int *done) {
+  const size_t n = 5 * (size_t)c->N;
+  int rc;
+  hipLaunchKernelGGL(k_sumsq, dim3(nparts), dim3(kBlock), 0, c->stream, n, p, s.partial, done);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, s.partial, nparts, s.scal, done);
+  hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, c->stream, s.scal, (double)n, done);
+  hipLaunchKernelGGL(k_fd_shift, grid_for(n), dim3(kBlock), 0, c->stream, n, U, p, s.scal, 1.0, s.Q, done);
+  if ((rc = launch_residual(c, s.Q, s.Fa, true, done))) return rc;
+  hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, done);
+  if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Q, U, dt, s.Fa, done);
+  if (order == 1) {
+    hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Fa, F0, s.scal, 3, prod, done);
+  } else {
+    hipLaunchKernelGGL(k_fd_shift, grid_for(n), dim3(kBlock), 0, c->stream, n, U, p, s.scal, -1.0, s.Q, done);
+    if ((rc = launch_residual(c, s.Q, s.Fb, true, done))) return rc;
+    hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, done);
+    if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Q, U, dt, s.Fb, done);
+    hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Fa, s.Fb, s.scal, 4, prod, done);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "finite-difference product launch");
+}
+
+// The whole of aerofx_jvp_fd: MatVecProdFD::evaluate at U (F0 = F(U) in
+// s.Fb for order 1) and one apply; the summed error count is left in err[3].
+static int enqueue_jvp_fd(aerofx_ctx *c, const FdScratch &s, int nparts, const double *U, const double *p,
+                          double *prod, int order, const double *dt) {
+  int rc;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 3, 1, nullptr);
+  enqueue_fd_state_sum(c, s, U);
+  if (order == 1) {
+    // F(U): the time term vanishes at Q = Un
+    if ((rc = launch_residual(c, U, s.Fb, true))) return rc;
+    hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, nullptr);
+  }
+  return enqueue_fd_product(c, s, nparts, U, dt, s.Fb, p, prod, order, nullptr);
 }
 
 struct Krylov {
@@ -611,7 +694,8 @@ struct Krylov {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t order_ev = nullptr;
-  int graph_key = -1;
+  long long graph_key = -1;
+  const void *graph_fd[3] = {nullptr, nullptr, nullptr};  // Q, F0, dt captured with a matrix-free operator
   double *h_scal = nullptr;   // pinned
 };
 
@@ -634,9 +718,8 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
   Krylov *k = new Krylov();
   k->m = m;
   k->n = 5 * (size_t)c->N;
-  // about 8 entries per lane per sweep, between 64 blocks and one per CU (the
-  // sweeps hold up to 32 basis rows per lane: one wave per SIMD)
-  k->nparts = (int)std::min<size_t>(kSweepBlocks, std::max<size_t>(64, (k->n + 8 * kBlock - 1) / (8 * kBlock)));
+  // the sweeps hold up to 32 basis rows per lane: one wave per SIMD
+  k->nparts = sweep_parts(k->n);
   void *p = nullptr;
   int rc;
   k->ldv = (k->n + 31) & ~(size_t)31;
@@ -661,13 +744,62 @@ static int ensure_krylov(aerofx_ctx *c, int m) {
   return 0;
 }
 
-static inline dim3 grid_for(size_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
-
 // out[0] = w.w (fixed reduction order)
 static void norm2(aerofx_ctx *c, const double *w, double *out) {
   Krylov *k = c->ksp;
-  hipLaunchKernelGGL(k_sumsq, dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, k->partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->nparts, out);
+  hipLaunchKernelGGL(k_sumsq, dim3(k->nparts), dim3(kBlock), 0, c->stream, k->n, w, k->partial, nullptr);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, k->nparts, out, nullptr);
+}
+
+// The finite-difference scratch of the context (allocated on first use).
+static int ensure_fd(aerofx_ctx *c, FdScratch *s) {
+  const size_t n = 5 * (size_t)c->N;
+  if (!c->d_fd) {
+    void *p = nullptr;
+    int rc = ctx_alloc(c, sizeof(double) * (4 * n + kDotBlocks + 16), &p);
+    if (rc) return rc;
+    c->d_fd = static_cast<double *>(p);
+  }
+  s->Q = c->d_fd;
+  s->Fa = s->Q + n;
+  s->Fb = s->Fa + n;
+  s->partial = c->d_fd + 4 * n;
+  s->scal = s->partial + kDotBlocks;
+  return 0;
+}
+static double *fd_own_F0(aerofx_ctx *c) { return c->d_fd + 3 * 5 * (size_t)c->N; }
+
+// y = A p with the selected MatVecProd (transposed solves: always H1^T).
+static int launch_operator(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
+  if (trans || c->mvp_type == AEROFX_MVP_H1) return launch_h1_apply_flag(c, p, y, done, trans);
+  FdScratch s;
+  int rc = ensure_fd(c, &s);
+  return rc ? rc
+            : enqueue_fd_product(c, s, sweep_parts(5 * (size_t)c->N), c->mvp_Q, c->mvp_dt, c->mvp_F0, p, y,
+                                 c->mvp_order, done);
+}
+
+// MatVecProdFD::evaluate: selects the finite-difference operator about Q.
+// F0 NULL: F(Q) = R(Q) is evaluated into the context's own buffer.  Enqueues
+// the state part of computeEpsilon and clears the error sum err[3].
+static int select_fd(aerofx_ctx *c, const double *Q, const double *F0, const double *dt, int order) {
+  FdScratch s;
+  int rc = ensure_fd(c, &s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 3, 1, nullptr);
+  if (!F0) {
+    if ((rc = launch_residual(c, Q, fd_own_F0(c), true))) return rc;
+    hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, nullptr);
+    F0 = fd_own_F0(c);
+  }
+  enqueue_fd_state_sum(c, s, Q);
+  c->mvp_type = AEROFX_MVP_FD;
+  c->mvp_Q = Q;
+  c->mvp_F0 = F0;
+  c->mvp_dt = dt;
+  c->mvp_order = order;
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "mvp_select launch");
 }
 
 static int fetch_state(aerofx_ctx *c) {
@@ -697,7 +829,7 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
   for (int j = 0; j < m; ++j) {
     {
       ProfScope ps(c, AEROFX_TIMER_JAC_APPLY);
-      if ((rc = launch_h1_apply_flag(c, k->z, k->w, done, trans))) return rc;
+      if ((rc = launch_operator(c, k->z, k->w, done, trans))) return rc;
     }
     {
       ProfScope ps(c, AEROFX_TIMER_GRAM_SCHMIDT);
@@ -730,12 +862,16 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
 static int arnoldi_cycle(aerofx_ctx *c, int m, bool trans) {
   Krylov *k = c->ksp;
   if (c->prof) return enqueue_arnoldi(c, m, trans);
-  const int key = (m * 2 + (trans ? 1 : 0)) * 2 + c->jac_precision;
+  const bool fd = !trans && c->mvp_type == AEROFX_MVP_FD;
+  const long long key = (((((long long)m * 2 + (trans ? 1 : 0)) * 2 + c->jac_precision) * 2 + (fd ? 1 : 0)) * 3 +
+                         (fd ? c->mvp_order : 0)) * 2 + c->residual_kernels;
+  const void *fdp[3] = {fd ? c->mvp_Q : nullptr, fd ? c->mvp_F0 : nullptr, fd ? c->mvp_dt : nullptr};
+  const bool same_fd = k->graph_fd[0] == fdp[0] && k->graph_fd[1] == fdp[1] && k->graph_fd[2] == fdp[2];
   hipError_t e = hipSuccess;
   if (!k->cap_stream) e = hipStreamCreateWithFlags(&k->cap_stream, hipStreamNonBlocking);
   if (e == hipSuccess && !k->order_ev) e = hipEventCreateWithFlags(&k->order_ev, hipEventDisableTiming);
   if (e != hipSuccess) return set_hip_error(e, "arnoldi stream");
-  if (!k->exec || k->graph_key != key) {
+  if (!k->exec || k->graph_key != key || !same_fd) {
     if (k->exec) (void)hipGraphExecDestroy(k->exec);
     if (k->graph) (void)hipGraphDestroy(k->graph);
     k->exec = nullptr;
@@ -751,6 +887,7 @@ static int arnoldi_cycle(aerofx_ctx *c, int m, bool trans) {
     if (e == hipSuccess) e = hipGraphInstantiate(&k->exec, k->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) return set_hip_error(e, "arnoldi graph");
     k->graph_key = key;
+    for (int q = 0; q < 3; ++q) k->graph_fd[q] = fdp[q];
   }
   hipEvent_t ev = k->order_ev;
   e = hipEventRecord(ev, c->stream);
@@ -790,7 +927,7 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
     if (cycle == 0) {
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, 0.0, k->r, nullptr);
     } else {
-      if ((rc = launch_h1_apply(c, x, k->w, trans))) return rc;
+      if ((rc = launch_operator(c, x, k->w, nullptr, trans))) return rc;
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, 0.0, k->r, nullptr);
       hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->w, 1.0, k->r,
                          nullptr);
@@ -816,38 +953,6 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
   return 0;
 }
 
-// MatVecProdFD::evaluate + apply: prod = (F(U + eps p) - F(U)) / eps (order 1)
-// or (F(U + eps p) - F(U - eps p)) / (2 eps) (order 2), F = R + time term.
-static int jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *prod, int order, const double *dt) {
-  int rc = ensure_krylov(c, c->ksp ? c->ksp->m : 1);
-  if (rc) return rc;
-  Krylov *k = c->ksp;
-  const size_t n = k->n;
-  // computeEpsilon
-  norm2(c, p, k->scal);
-  hipLaunchKernelGGL(k_eps_sum, dim3(kDotBlocks), dim3(kBlock), 0, c->stream, n, U, k->partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, c->stream, k->partial, kDotBlocks, k->scal + 1);
-  hipLaunchKernelGGL(k_fd_eps, dim3(1), dim3(64), 0, c->stream, k->scal, (double)n);
-  double *Fa = k->w, *Fb = k->z, *Q = k->r;
-  // F(U + eps p)
-  hipLaunchKernelGGL(k_fd_shift, grid_for(n), dim3(kBlock), 0, c->stream, n, U, p, k->scal, 1.0, Q);
-  if ((rc = launch_residual(c, Q, Fa, true))) return rc;
-  if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, Q, U, dt, Fa);
-  if (order == 1) {
-    // F(U): the time term vanishes at Q = Un
-    if ((rc = launch_residual(c, U, Fb, true))) return rc;
-    if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, U, U, dt, Fb);
-    hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, Fa, Fb, k->scal, 3, prod);
-  } else {
-    hipLaunchKernelGGL(k_fd_shift, grid_for(n), dim3(kBlock), 0, c->stream, n, U, p, k->scal, -1.0, Q);
-    if ((rc = launch_residual(c, Q, Fb, true))) return rc;
-    if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, Q, U, dt, Fb);
-    hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, Fa, Fb, k->scal, 4, prod);
-  }
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : set_hip_error(e, "jvp_fd launch");
-}
-
 }  // namespace aerofx
 
 using namespace aerofx;
@@ -857,12 +962,50 @@ extern "C" {
 int aerofx_jvp_fd(aerofx_ctx *c, const double *U, const double *p, double *Jp, int order, const double *dt) {
   if (!c || !U || !p || !Jp) return set_error(AEROFX_ERR_ARG, "jvp_fd: null argument");
   if (order != 1 && order != 2) return set_error(AEROFX_ERR_ARG, "jvp_fd: order 1 or 2");
-  int rc = jvp_fd(c, U, p, Jp, order, dt);
+  FdScratch s;
+  int rc = ensure_fd(c, &s);
   if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if ((rc = enqueue_jvp_fd(c, s, sweep_parts(5 * (size_t)c->N), U, p, Jp, order, dt))) return rc;
+  hipError_t e = hipMemcpyAsync(c->h_err + 3, c->d_err + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "jvp_fd sync");
-  return c->h_err[0];
+  return c->h_err[3];
+}
+
+int aerofx_mvp_select(aerofx_ctx *c, int type, const double *Q, const double *F0, const double *dt, int order) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "mvp_select: null ctx");
+  if (type == AEROFX_MVP_H1) {
+    c->mvp_type = AEROFX_MVP_H1;
+    c->mvp_Q = c->mvp_F0 = c->mvp_dt = nullptr;
+    return AEROFX_OK;
+  }
+  if (type != AEROFX_MVP_FD) return set_error(AEROFX_ERR_ARG, "mvp_select: AEROFX_MVP_H1 or AEROFX_MVP_FD");
+  if (!Q) return set_error(AEROFX_ERR_ARG, "mvp_select: null state");
+  if (order != 1 && order != 2) return set_error(AEROFX_ERR_ARG, "mvp_select: order 1 or 2");
+  int rc = select_fd(c, Q, F0, dt, order);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(c->h_err + 3, c->d_err + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "mvp_select sync");
+  return c->h_err[3];
+}
+
+int aerofx_mvp_apply(aerofx_ctx *c, const double *p, double *prod) {
+  if (!c || !p || !prod) return set_error(AEROFX_ERR_ARG, "mvp_apply: null argument");
+  if (c->mvp_type == AEROFX_MVP_H1) {
+    if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "mvp_apply: operators not set up");
+    int rc = launch_h1_apply(c, p, prod);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "mvp_apply sync");
+  }
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 3, 1, nullptr);
+  int rc = launch_operator(c, p, prod, nullptr, false);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(c->h_err + 3, c->d_err + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "mvp_apply sync");
+  return c->h_err[3];
 }
 
 int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, double *psi,
@@ -882,11 +1025,13 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
     if ((rc = launch_timestep(c, o->cfl, c->d_dt))) return rc;
   }
   if ((rc = launch_jacobian(c, o->cfl > 0.0 ? c->d_dt : nullptr, o->precision))) return rc;
-  // ||rhs||
+  // ||rhs|| and the block inversion's zero-pivot count
   norm2(c, rhs, k->scal);
   hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->h_err + 1, c->d_err + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
+  if (c->h_err[1]) return set_error(AEROFX_ERR_STATE, "adjoint_solve: zero pivot in a diagonal block");
   const double rnorm0 = std::sqrt(k->h_scal[0]);
   hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, c->stream, n, 0.0, psi);
   double res = rnorm0;
@@ -924,6 +1069,105 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
   st->its = it;
   st->res0 = rnorm0;
   st->res = res;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return AEROFX_OK;
+}
+
+int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *rhs, double *x,
+                        const aerofx_adjoint_opts *o, aerofx_adjoint_stats *st) {
+  if (!c || !U || !rhs || !x || !o || !st) return set_error(AEROFX_ERR_ARG, "direct_solve: null argument");
+  if (nrhs < 1 || nrhs > 16) return set_error(AEROFX_ERR_ARG, "direct_solve: 1 <= nrhs <= 16");
+  if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "direct_solve: precision 0 or 1");
+  int rc = ensure_krylov(c, o->gmres_m);
+  if (rc) return rc;
+  Krylov *k = c->ksp;
+  const size_t n = k->n;
+  if (!c->d_direct) {
+    void *p = nullptr;
+    if ((rc = ctx_alloc(c, sizeof(double) * (2 * 16 + 1) * n, &p))) return rc;
+    c->d_direct = static_cast<double *>(p);
+  }
+  double *Rb = c->d_direct, *Db = Rb + 16 * n, *Ax = Db + 16 * n;
+  // the inner solves use the assembled H1, whatever operator is selected
+  struct Restore {
+    aerofx_ctx *c;
+    int type;
+    ~Restore() { c->mvp_type = type; }
+  } restore{c, c->mvp_type};
+  c->mvp_type = AEROFX_MVP_H1;
+  auto t0 = std::chrono::steady_clock::now();
+  *st = aerofx_adjoint_stats{};
+  // A = H1 + diag(1/dt) at U and its block-Jacobi inverse (block GMRES takes
+  // FP64 blocks and makes its own FP32 copies for precision 1)
+  const int jac_prec = nrhs > 1 ? 0 : o->precision;
+  if ((rc = launch_primitive(c, U, c->d_V))) return rc;
+  if (o->cfl > 0.0 && (rc = launch_timestep(c, o->cfl, c->d_dt))) return rc;
+  if ((rc = launch_jacobian(c, o->cfl > 0.0 ? c->d_dt : nullptr, jac_prec))) return rc;
+  std::vector<double> bn(nrhs), rn(nrhs);
+  auto norms = [&](const double *base, std::vector<double> &out) -> int {
+    for (int q = 0; q < nrhs; ++q) {
+      norm2(c, base + (size_t)q * n, k->scal + q);
+    }
+    hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double) * nrhs, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "direct_solve sync");
+    for (int q = 0; q < nrhs; ++q) out[q] = std::sqrt(k->h_scal[q]);
+    return 0;
+  };
+  if ((rc = norms(rhs, bn))) return rc;
+  if (c->h_err[1]) return set_error(AEROFX_ERR_STATE, "direct_solve: zero pivot in a diagonal block");
+  hipLaunchKernelGGL(k_fill, grid_for(n * nrhs), dim3(kBlock), 0, c->stream, n * nrhs, 0.0, x);
+  double worst = 0.0, worst_abs = 0.0, worst_b = 0.0;
+  int it;
+  for (it = 0;; ++it) {
+    // r_k = rhs_k - (dR/dU) x_k: exact second-order products, FP64
+    for (int q = 0; q < nrhs; ++q) {
+      double *r = Rb + (size_t)q * n;
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, rhs + (size_t)q * n, 0.0,
+                         r, nullptr);
+      if (it > 0) {
+        if ((rc = launch_tangent(c, U, x + (size_t)q * n, nullptr, Ax))) return rc;
+        hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, Ax, 1.0, r, nullptr);
+      }
+    }
+    if ((rc = norms(Rb, rn))) return rc;
+    if (it > 0 && c->h_err[0]) return set_error(AEROFX_ERR_STATE, "direct_solve: non-physical reconstructed state");
+    worst = 0.0;
+    for (int q = 0; q < nrhs; ++q) {
+      const double rel = bn[q] > 0.0 ? rn[q] / bn[q] : 0.0;
+      if (rel >= worst) {
+        worst = rel;
+        worst_abs = rn[q];
+        worst_b = bn[q];
+      }
+    }
+    if (worst <= o->eps) {
+      st->converged = 1;
+      break;
+    }
+    if (it >= o->max_its) break;
+    // A d_k = r_k with H1 + 1/dt and block-Jacobi
+    if (nrhs == 1) {
+      GmresOut go;
+      if ((rc = gmres(c, Rb, Db, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go))) return rc;
+      st->krylov_its += go.its;
+    } else {
+      aerofx_block_stats bs;
+      const int bm = std::min(o->gmres_m, 16);
+      if ((rc = aerofx_block_gmres(c, nrhs, Rb, Db, bm, o->gmres_eps, std::max(1, o->gmres_maxits / bm), o->precision,
+                                   1, &bs)) < 0)
+        return rc;
+      st->krylov_its += bs.inner_its;
+    }
+    hipLaunchKernelGGL(k_axpby, grid_for(n * nrhs), dim3(kBlock), 0, c->stream, n * nrhs, 1.0, nullptr, Db, 1.0, x,
+                       nullptr);
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "direct_solve sync");
+  st->its = it;
+  st->res0 = worst_b;
+  st->res = worst_abs;
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return AEROFX_OK;
 }
@@ -1014,36 +1258,70 @@ int aerofx_gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
                  double *final_res) {
   if (!c || !b || !x) return set_error(AEROFX_ERR_ARG, "gmres: null argument");
   if (!c->d_jacD) return set_error(AEROFX_ERR_STATE, "gmres: operators not set up");
+  const bool fd = c->mvp_type == AEROFX_MVP_FD;
+  if (fd) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 3, 1, nullptr);
   GmresOut o;
   int rc = gmres(c, b, x, m, eps, maxits, &o);
   if (rc) return rc;
   if (its) *its = o.its;
   if (final_res) *final_res = o.res;
-  hipError_t e = hipStreamSynchronize(c->stream);
-  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "gmres sync");
+  hipError_t e = fd ? hipMemcpyAsync(c->h_err + 3, c->d_err + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream)
+                    : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "gmres sync");
+  return fd ? c->h_err[3] : AEROFX_OK;
 }
 
 int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, aerofx_newton_stats *st,
                         double *history, int hist_cap) {
   if (!c || !U || !o || !st) return set_error(AEROFX_ERR_ARG, "steady_solve: null argument");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "steady_solve: precision 0 or 1");
+  if (o->mvp != AEROFX_MVP_H1 && o->mvp != AEROFX_MVP_FD)
+    return set_error(AEROFX_ERR_ARG, "steady_solve: mvp AEROFX_MVP_H1 or AEROFX_MVP_FD");
+  if (o->mvp == AEROFX_MVP_FD && o->fd_order != 1 && o->fd_order != 2)
+    return set_error(AEROFX_ERR_ARG, "steady_solve: fd_order 1 or 2");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
   Krylov *k = c->ksp;
   const size_t n = k->n;
+  // the caller's operator selection is restored on return
+  struct Restore {
+    aerofx_ctx *c;
+    int type, order;
+    const double *Q, *F0, *dt;
+    ~Restore() {
+      c->mvp_type = type;
+      c->mvp_order = order;
+      c->mvp_Q = Q;
+      c->mvp_F0 = F0;
+      c->mvp_dt = dt;
+    }
+  } restore{c, c->mvp_type, c->mvp_order, c->mvp_Q, c->mvp_F0, c->mvp_dt};
+  c->mvp_type = AEROFX_MVP_H1;
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_newton_stats{};
   double res0 = 0.0, res = 0.0, cfl = o->cfl0;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err, 4, nullptr);
   int step;
   for (step = 0; step <= o->max_steps; ++step) {
     // spatial residual and its norm (TsDesc::computeResidualNorm); one
-    // synchronisation per step reads the norm and the error counters
+    // synchronisation per step reads the norm and the error counters of the
+    // previous step's block inversion [1], update [2] and finite-difference
+    // products [3]
     if ((rc = launch_residual(c, U, k->R, true))) return rc;
     norm2(c, k->R, k->scal);
     hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return set_hip_error(e, "steady_solve sync");
+    if (step > 0 && c->h_err[1]) {
+      st->steps = step;
+      return set_error(AEROFX_ERR_STATE, "steady_solve: zero pivot in a diagonal block");
+    }
+    if (step > 0 && c->h_err[3]) {
+      st->steps = step;
+      return set_error(AEROFX_ERR_STATE, "steady_solve: non-physical reconstruction in a finite-difference product");
+    }
     if (step > 0 && c->h_err[2]) {
       st->steps = step;
       return set_error(AEROFX_ERR_STATE, "steady_solve: update produced non-physical states");
@@ -1067,6 +1345,11 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
                    o->cfl_max);
     if ((rc = launch_timestep(c, cfl, c->d_dt))) return rc;
     if ((rc = launch_jacobian(c, c->d_dt, o->precision))) return rc;
+    if (o->mvp == AEROFX_MVP_FD) {
+      // MatVecProdFD::evaluate at the Newton state: F(U) = R(U) (the time
+      // term vanishes at Q = Un); one residual per Krylov product
+      if ((rc = select_fd(c, U, k->R, c->d_dt, o->fd_order))) return rc;
+    }
     // b = -F
     hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 0.0, k->b,
                        nullptr);
@@ -1075,7 +1358,7 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     st->krylov_its += go.its;
     // Q += dQ; the state check is read at the next step's synchronisation
     hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->x, 1.0, U, nullptr);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 2, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 2, 1, nullptr);
     hipLaunchKernelGGL(k_check_states, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N, U,
                        c->gas, c->d_err + 2);
     st->final_cfl = cfl;

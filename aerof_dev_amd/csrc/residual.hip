@@ -51,7 +51,9 @@ template <bool FROM_U, bool OUT3>
 __global__ __launch_bounds__(kBlock) void k_gradients(DevTables t, const double *__restrict__ W, Gas gas,
                                                       double *__restrict__ Vout, double *__restrict__ grad,
                                                       double *__restrict__ ddx, double *__restrict__ ddy,
-                                                      double *__restrict__ ddz, int *__restrict__ err) {
+                                                      double *__restrict__ ddz, int *__restrict__ err,
+                                                      const int *__restrict__ done) {
+  if (done && *done) return;
   const int nb = (t.N + kBlock - 1) / kBlock;
   const int n = xcd_chunk(blockIdx.x, nb) * kBlock + (int)threadIdx.x;
   if (err && blockIdx.x == 0 && threadIdx.x == 0) *err = 0;  // the flux pass follows in stream order
@@ -139,7 +141,8 @@ template <bool SECOND, int FB>
 __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ grad, FluxConsts fc,
                                                        double *__restrict__ Fm, double *__restrict__ P,
-                                                       int *__restrict__ err) {
+                                                       int *__restrict__ err, const int *__restrict__ done) {
+  if (done && *done) return;
   __shared__ double sf[5 * (FB + 1)];
   const int4 ch = t.fchunk4[xcd_chunk(blockIdx.x, t.nfchunks)];
   const int r0 = ch.x, r1 = ch.y, l0 = ch.z, l1 = ch.w;
@@ -215,12 +218,232 @@ __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__
 }
 
 // ---------------------------------------------------------------------------
+// 1'. gradients with the strip's node data staged in LDS.  A block covers a
+//     strip of consecutive nodes [n0, n1) (one lane each); every node the
+//     strip touches (the strip and all neighbours, ascending ids) is loaded
+//     once: consecutive lanes take consecutive staged nodes, so the loads
+//     coalesce, and U -> V is converted once per staged node instead of once
+//     per adjacency entry.  The sums are the per-lane kernel's, term for term.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kGradStrip) void k_gradients_staged(DevTables t, const double *__restrict__ U, Gas gas,
+                                                                 double *__restrict__ Vout,
+                                                                 double *__restrict__ grad, int *__restrict__ err,
+                                                                 const int *__restrict__ done) {
+  if (done && *done) return;
+  __shared__ double sv[8 * kGradHalo];  // field f of staged node h at sv[f * kGradHalo + h]: V (5), X (3)
+  const int b = xcd_chunk(blockIdx.x, t.ngb);
+  const int4 s = t.gb4[b];
+  const int tid = threadIdx.x;
+  if (err && blockIdx.x == 0 && tid == 0) *err = 0;  // the flux pass follows in stream order
+  const int H = s.w - s.z;
+  // every staged node's loads are issued before any is used: one memory
+  // latency per block instead of one per staged node
+  constexpr int PER = (kGradHalo + kGradStrip - 1) / kGradStrip;
+  double u[PER][5];
+  double4 x[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int h = tid + r * kGradStrip;
+    const int g = h < H ? t.ghalo[s.z + h] : t.ghalo[s.z];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[r][k] = U[5 * (size_t)g + k];
+    x[r] = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)g);
+  }
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int h = tid + r * kGradStrip;
+    if (h < H) {
+      double v[5];
+      prim_from_cons(u[r], gas, v);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sv[k * kGradHalo + h] = v[k];
+      sv[5 * kGradHalo + h] = x[r].x;
+      sv[6 * kGradHalo + h] = x[r].y;
+      sv[7 * kGradHalo + h] = x[r].z;
+    }
+  }
+  __syncthreads();
+  const int n = s.x + tid;
+  if (n >= s.y) return;
+  const int hn = t.gown[b] + tid;
+  double Vn[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    Vn[k] = sv[k * kGradHalo + hn];
+    Vout[5 * (size_t)n + k] = Vn[k];
+  }
+  const double xn0 = sv[5 * kGradHalo + hn], xn1 = sv[6 * kGradHalo + hn], xn2 = sv[7 * kGradHalo + hn];
+  const double *L = t.lsq6 + 6 * (size_t)n;
+  const double or11 = L[0], or22 = L[1], or33 = L[2], r12or11 = L[3], r23or22 = L[4], psi = L[5];
+  double acc[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+  const int e0 = t.rowptr[n], e1 = t.rowptr[n + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int hm = t.gloc[e];
+    const double dx0 = sv[5 * kGradHalo + hm] - xn0, dx1 = sv[6 * kGradHalo + hm] - xn1,
+                 dx2 = sv[7 * kGradHalo + hm] - xn2;
+    const double alpha1 = dx0 * or11 * or11;
+    const double alpha2 = (dx1 - r12or11 * dx0) * or22 * or22;
+    const double alpha3 = (dx2 - r23or22 * dx1 + psi * dx0) * or33 * or33;
+    const double w0 = alpha1 - r12or11 * alpha2 + psi * alpha3;
+    const double w1 = alpha2 - r23or22 * alpha3;
+    const double w2 = alpha3;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const double d = sv[q * kGradHalo + hm] - Vn[q];
+      acc[q] += w0 * d;
+      acc[5 + q] += w1 * d;
+      acc[10 + q] += w2 * d;
+    }
+  }
+  double2 *g2 = reinterpret_cast<double2 *>(grad + 16 * (size_t)n);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) g2[q] = make_double2(acc[2 * q], acc[2 * q + 1]);
+  g2[7] = make_double2(acc[14], 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// 2'. edge fluxes with the strip's node data staged in LDS.  A block covers
+//     the edges of consecutive rows (lower nodes) [r0, r1); the rows and the
+//     upper ends of their edges (ascending ids) are loaded once, field by
+//     field with consecutive lanes on consecutive doubles of a record
+//     (V: 5, X: 3 of 4, gradients: 15 of 16), so the loads coalesce.  Lane k
+//     computes edge l0 + k from LDS exactly as the per-lane kernel does; the
+//     fluxes go back through the same LDS for coalesced Fm stores and the
+//     per-row sums P.
+// ---------------------------------------------------------------------------
+template <bool SECOND>
+__global__ __launch_bounds__(kFluxStrip) void k_flux_staged(DevTables t, const double *__restrict__ V,
+                                                            const double *__restrict__ grad, FluxConsts fc,
+                                                            double *__restrict__ Fm, double *__restrict__ P,
+                                                            int *__restrict__ err, const int *__restrict__ done) {
+  if (done && *done) return;
+  // staged node h: record sh[h * NF ...] = V (5), X (3), gradients (15); the
+  // odd record length spreads the lanes' reads of random records over the
+  // LDS banks, and consecutive lanes staging one record write consecutive
+  // addresses
+  constexpr int NF = SECOND ? 23 : 5;
+  constexpr int HS = kFluxHalo;
+  constexpr int FS = kFluxStrip + 1;
+  // the flux staging [5][FS] reuses the node staging [HS][NF]
+  __shared__ double sh[NF * HS > 5 * FS ? NF * HS : 5 * FS];
+  __shared__ int sidx[HS];
+  const int b = xcd_chunk(blockIdx.x, t.nfb);
+  const int4 ch = t.fb4[b];
+  const int2 hr = t.fbh[b];
+  const int r0 = ch.x, r1 = ch.y, l0 = ch.z, l1 = ch.w;
+  const int H = hr.y - hr.x;
+  const int tid = threadIdx.x;
+  const Gas &g = fc.gas;
+  for (int h = tid; h < H; h += kFluxStrip) sidx[h] = t.fhalo[hr.x + h];
+  __syncthreads();
+  // all of a lane's staging loads are in flight together: consecutive lanes
+  // take consecutive doubles of the records (V: 5, X: 3 of 4, gradients: 15
+  // of 16), so the loads coalesce, and a block waits one memory latency
+  constexpr int PV = (5 * HS + kFluxStrip - 1) / kFluxStrip;
+  constexpr int PX = (4 * HS + kFluxStrip - 1) / kFluxStrip;
+  constexpr int PG = (16 * HS + kFluxStrip - 1) / kFluxStrip;
+  double rv[PV], rx[SECOND ? PX : 1], rg[SECOND ? PG : 1];
+#pragma unroll
+  for (int r = 0; r < PV; ++r) {
+    const int q = tid + r * kFluxStrip;
+    const int h = q / 5, f = q - 5 * h;
+    rv[r] = q < 5 * H ? V[5 * (size_t)sidx[min(h, H - 1)] + f] : 0.0;
+  }
+  if (SECOND) {
+#pragma unroll
+    for (int r = 0; r < PX; ++r) {
+      const int q = tid + r * kFluxStrip;
+      rx[r] = q < 4 * H ? t.X4[4 * (size_t)sidx[min(q >> 2, H - 1)] + (q & 3)] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < PG; ++r) {
+      const int q = tid + r * kFluxStrip;
+      rg[r] = q < 16 * H ? grad[16 * (size_t)sidx[min(q >> 4, H - 1)] + (q & 15)] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < PV; ++r) {
+    const int q = tid + r * kFluxStrip;
+    const int h = q / 5, f = q - 5 * h;
+    if (q < 5 * H) sh[h * NF + f] = rv[r];
+  }
+  if (SECOND) {
+#pragma unroll
+    for (int r = 0; r < PX; ++r) {
+      const int q = tid + r * kFluxStrip;
+      if (q < 4 * H && (q & 3) < 3) sh[(q >> 2) * NF + 5 + (q & 3)] = rx[r];
+    }
+#pragma unroll
+    for (int r = 0; r < PG; ++r) {
+      const int q = tid + r * kFluxStrip;
+      if (q < 16 * H && (q & 15) < 15) sh[(q >> 4) * NF + 8 + (q & 15)] = rg[r];
+    }
+  }
+  __syncthreads();
+  const int ne = l1 - l0;
+  double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (tid < ne) {
+    const int l = l0 + tid;
+    const uint32_t loc = t.floc[l];
+    const double *si = sh + (int)(loc & 0xffffu) * NF, *sj = sh + (int)(loc >> 16) * NF;
+    double Vi[5], Vj[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      Vi[k] = si[k];
+      Vj[k] = sj[k];
+    }
+    double Vij[5], Vji[5];
+    if (SECOND) {
+      const double dx0 = sj[5] - si[5], dx1 = sj[6] - si[6], dx2 = sj[7] - si[7];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const double ddVij = dx0 * si[8 + k] + dx1 * si[13 + k] + dx2 * si[18 + k];
+        const double ddVji = dx0 * sj[8 + k] + dx1 * sj[13 + k] + dx2 * sj[18 + k];
+        vanalbada(fc.beta, Vi[k], ddVij, Vj[k], ddVji, Vij[k], Vji[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        Vij[k] = Vi[k];
+        Vji[k] = Vj[k];
+      }
+    }
+    // checkReconstructedValues (EdgeCore.C:137-166, VarFcnSGEuler.h:90-110)
+    const int bad = (Vij[0] <= 0.0 || Vij[4] + g.pstiff <= 0.0) + (Vji[0] <= 0.0 || Vji[4] + g.pstiff <= 0.0);
+    if (bad) atomicAdd(err, bad);
+    const double nv[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
+    if (sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) >= 1e-18) roeflux7(g, nv, 0.0, Vij, Vji, f);
+  }
+  __syncthreads();  // the node staging is dead: reuse it for the fluxes
+  double *sf = sh;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) sf[k * FS + tid] = f[k];
+  __syncthreads();
+  for (int q = tid; q < 5 * ne; q += kFluxStrip) {
+    const int e = q / 5, k = q - 5 * e;
+    Fm[5 * (size_t)l0 + q] = sf[k * FS + e];
+  }
+  const int nr = r1 - r0;
+  for (int idx = tid; idx < nr * 5; idx += kFluxStrip) {
+    const int rr = idx / 5, q = idx - 5 * rr;
+    const int r = r0 + rr;
+    const int eb = t.eup[r] - l0, eend = t.eup[r + 1] - l0;
+    double s = 0.0;
+    for (int e = eb; e < eend; ++e) s += sf[q * FS + e];
+    P[5 * (size_t)r + q] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 3. node assembly: lower edges, upper-edge partial sum, faces, 1/V
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ Fm,
                                                        const double *__restrict__ P, FluxConsts fc, int scale,
-                                                       double *__restrict__ R) {
+                                                       double *__restrict__ R, const int *__restrict__ done) {
+  if (done && *done) return;
   const int nb = (t.N + kBlock - 1) / kBlock;
   const int n = xcd_chunk(blockIdx.x, nb) * kBlock + (int)threadIdx.x;
   if (n >= t.N) return;
@@ -260,7 +483,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double
   for (int k = 0; k < 5; ++k) R[5 * (size_t)n + k] = scale ? s[k] * ic : s[k];
 }
 
-__global__ void k_primitive(int N, const double *__restrict__ U, Gas g, double *__restrict__ V) {
+__global__ void k_primitive(int N, const double *__restrict__ U, Gas g, double *__restrict__ V,
+                            const int *__restrict__ done) {
+  if (done && *done) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
   double u[5], v[5];
@@ -271,7 +496,8 @@ __global__ void k_primitive(int N, const double *__restrict__ U, Gas g, double *
   for (int k = 0; k < 5; ++k) V[5 * (size_t)n + k] = v[k];
 }
 
-__global__ void k_zero_int(int *p, int n) {
+__global__ void k_zero_int(int *p, int n, const int *done) {
+  if (done && *done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
@@ -279,36 +505,47 @@ __global__ void k_zero_int(int *p, int n) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale) {
+int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done) {
   const DevTables &t = c->tab;
+  const bool staged = t.ngb > 0 && c->residual_kernels != AEROFX_RESIDUAL_PER_LANE;
   if (c->p.second_order) {
     ProfScope ps(c, AEROFX_TIMER_NODAL_GRAD);
-    hipLaunchKernelGGL((k_gradients<true, false>), dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, t, U, c->gas,
-                       c->d_V, c->d_grad, nullptr, nullptr, nullptr, c->d_err);
+    if (staged)
+      hipLaunchKernelGGL(k_gradients_staged, dim3(t.ngb), dim3(kGradStrip), 0, c->stream, t, U, c->gas, c->d_V,
+                         c->d_grad, c->d_err, done);
+    else
+      hipLaunchKernelGGL((k_gradients<true, false>), dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
+                         t, U, c->gas, c->d_V, c->d_grad, nullptr, nullptr, nullptr, c->d_err, done);
   } else {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err, 1);
-    hipLaunchKernelGGL(k_primitive, dim3((t.N + 255) / 256), dim3(256), 0, c->stream, t.N, U, c->gas, c->d_V);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err, 1, done);
+    hipLaunchKernelGGL(k_primitive, dim3((t.N + 255) / 256), dim3(256), 0, c->stream, t.N, U, c->gas, c->d_V, done);
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_TERM);
     const dim3 g(t.nfchunks);
-    if (c->p.second_order && t.fblock == kFluxBlockSmall)
+    if (staged && c->p.second_order)
+      hipLaunchKernelGGL(k_flux_staged<true>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, c->d_V, c->d_grad,
+                         c->fc, c->d_Fm, c->d_P, c->d_err, done);
+    else if (staged)
+      hipLaunchKernelGGL(k_flux_staged<false>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, c->d_V, c->d_grad,
+                         c->fc, c->d_Fm, c->d_P, c->d_err, done);
+    else if (c->p.second_order && t.fblock == kFluxBlockSmall)
       hipLaunchKernelGGL((k_flux_edges<true, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t, c->d_V,
-                         c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err);
+                         c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err, done);
     else if (c->p.second_order)
       hipLaunchKernelGGL((k_flux_edges<true, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, c->d_V, c->d_grad,
-                         c->fc, c->d_Fm, c->d_P, c->d_err);
+                         c->fc, c->d_Fm, c->d_P, c->d_err, done);
     else if (t.fblock == kFluxBlockSmall)
       hipLaunchKernelGGL((k_flux_edges<false, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t,
-                         c->d_V, c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err);
+                         c->d_V, c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err, done);
     else
       hipLaunchKernelGGL((k_flux_edges<false, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, c->d_V,
-                         c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err);
+                         c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err, done);
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_NODES);
     hipLaunchKernelGGL(k_flux_nodes, dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, t, c->d_V,
-                       c->d_Fm, c->d_P, c->fc, scale ? 1 : 0, R);
+                       c->d_Fm, c->d_P, c->fc, scale ? 1 : 0, R, done);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "residual launch");
@@ -316,13 +553,13 @@ int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale) {
 
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz) {
   hipLaunchKernelGGL((k_gradients<false, true>), dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->tab, V,
-                     c->gas, nullptr, nullptr, ddx, ddy, ddz, nullptr);
+                     c->gas, nullptr, nullptr, ddx, ddy, ddz, nullptr, nullptr);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "gradients launch");
 }
 
 int launch_primitive(aerofx_ctx *c, const double *U, double *V) {
-  hipLaunchKernelGGL(k_primitive, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, U, c->gas, V);
+  hipLaunchKernelGGL(k_primitive, dim3((c->N + 255) / 256), dim3(256), 0, c->stream, c->N, U, c->gas, V, nullptr);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "primitive launch");
 }

@@ -69,10 +69,7 @@ CHANNEL = (AEROFX_BC_INLET, AEROFX_BC_OUTLET, AEROFX_BC_SLIP_WALL, AEROFX_BC_SLI
            AEROFX_BC_SLIP_WALL, AEROFX_BC_SLIP_WALL)
 
 
-def kuhn_box(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
-             jitter: float = 0.0, seed: int = 0, face_codes=FARFIELD_BOX,
-             bumps=()) -> Mesh:
-    """Kuhn-split hex grid; bumps = [(amplitude, x_center, width), ...] on z-."""
+def _box_spec(nx, ny, nz, lo, hi, jitter, seed, face_codes, bumps):
     amp = np.array([b[0] for b in bumps], dtype=np.float64)
     cen = np.array([b[1] for b in bumps], dtype=np.float64)
     wid = np.array([b[2] for b in bumps], dtype=np.float64)
@@ -80,6 +77,14 @@ def kuhn_box(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
                    int(seed) & 0xFFFFFFFFFFFFFFFF, (C.c_int * 6)(*face_codes), len(bumps),
                    _dp(amp) if len(bumps) else None, _dp(cen) if len(bumps) else None,
                    _dp(wid) if len(bumps) else None)
+    return spec, (amp, cen, wid)  # the arrays must outlive the spec
+
+
+def kuhn_box(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+             jitter: float = 0.0, seed: int = 0, face_codes=FARFIELD_BOX,
+             bumps=()) -> Mesh:
+    """Kuhn-split hex grid; bumps = [(amplitude, x_center, width), ...] on z-."""
+    spec, _keep = _box_spec(nx, ny, nz, lo, hi, jitter, seed, face_codes, bumps)
     N, T, F = C.c_int(), C.c_int(), C.c_int()
     L = lib()
     check(L.aerofx_mesh_box_sizes(C.byref(spec), C.byref(N), C.byref(T), C.byref(F)), "mesh_box_sizes")
@@ -89,7 +94,26 @@ def kuhn_box(nx: int, ny: int, nz: int, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
     codes = np.empty((F.value,), np.int32)
     check(L.aerofx_mesh_box(C.byref(spec), _dp(X), _ip(tets), _ip(faces), _ip(codes)), "mesh_box")
     return Mesh(X, tets, faces, codes, dict(nx=nx, ny=ny, nz=nz, lo=tuple(lo), hi=tuple(hi),
-                                            jitter=jitter, seed=seed, bumps=tuple(bumps)))
+                                            jitter=jitter, seed=seed, bumps=tuple(bumps),
+                                            face_codes=tuple(face_codes)))
+
+
+def bump_derivative(mesh: Mesh, bump: int, param: str) -> np.ndarray:
+    """Closed-form dX/dp [N,3] of a kuhn_box mesh for bump `bump`'s
+    'amplitude' or 'center', in the mesh's node order (renumbered meshes
+    included: renumber() records the permutation)."""
+    md = mesh.meta
+    spec, _keep = _box_spec(md["nx"], md["ny"], md["nz"], md["lo"], md["hi"], md["jitter"], md["seed"],
+                            md["face_codes"], md["bumps"])
+    dX = np.empty((mesh.num_nodes, 3), np.float64)
+    check(lib().aerofx_mesh_box_bump_derivative(C.byref(spec), int(bump), {"amplitude": 0, "center": 1}[param],
+                                                _dp(dX)), "mesh_box_bump_derivative")
+    renum = md.get("renum")
+    if renum is not None:
+        out = np.empty_like(dX)
+        out[renum] = dX
+        dX = out
+    return dX
 
 
 def edges(mesh: Mesh) -> np.ndarray:
@@ -128,6 +152,8 @@ def renumber(mesh: Mesh, renum: np.ndarray) -> Mesh:
                                      _ip(tets), _ip(faces)), "mesh_renumber")
     meta = dict(mesh.meta)
     meta["renumbered"] = True
+    prev = meta.get("renum")
+    meta["renum"] = renum.copy() if prev is None else renum[prev]
     return Mesh(X, tets, faces, mesh.codes.copy(), meta)
 
 

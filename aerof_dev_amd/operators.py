@@ -16,7 +16,12 @@ from ._abi import (AdjointOpts, AdjointStats, BlockStats, NewtonOpts, NewtonStat
 from .mesh import FlowParams, Mesh
 
 
-def _ptr(t) -> int:
+def _ptr(t, numel: int) -> int:
+    """Device address of a contiguous float64 tensor of exactly numel elements.
+
+    Every entry point of the C ABI reads and writes implicit extents ([N][5],
+    [N][3], [N] or [nrhs][N][5]); a tensor of another size is rejected here,
+    before any launch could touch memory past its end."""
     if not t.is_cuda:
         raise ValueError("device tensor expected")
     if not t.is_contiguous():
@@ -24,7 +29,14 @@ def _ptr(t) -> int:
     import torch
     if t.dtype != torch.float64:
         raise ValueError("float64 tensor expected")
+    if t.numel() != numel:
+        raise ValueError(f"tensor of {numel} elements expected, got {t.numel()} (shape {tuple(t.shape)})")
     return t.data_ptr()
+
+
+def _optr(t, numel: int):
+    """_ptr for an optional argument (None -> NULL)."""
+    return None if t is None else _ptr(t, numel)
 
 
 class SpaceOperator:
@@ -51,6 +63,7 @@ class SpaceOperator:
         N, E, F = C.c_int(), C.c_int(), C.c_int()
         check(lib().aerofx_ctx_sizes(h, C.byref(N), C.byref(E), C.byref(F)), "ctx_sizes")
         self.num_nodes, self.num_edges, self.num_faces = N.value, E.value, F.value
+        self.n5, self.n3 = 5 * self.num_nodes, 3 * self.num_nodes  # element counts of [N][5], [N][3]
         self.set_stream(stream)
 
     def close(self):
@@ -89,6 +102,13 @@ class SpaceOperator:
     def sync(self):
         check(lib().aerofx_ctx_sync(self._h), "sync")
 
+    def set_residual_kernels(self, family: str = "auto") -> str:
+        """'auto' (LDS-staged strips when the mesh fits) or 'per_lane';
+        returns the family the next residual uses ('staged' or 'per_lane')."""
+        code = {"auto": 0, "per_lane": 1}[family]
+        rc = check(lib().aerofx_ctx_set_residual_kernels(self._h, code), "set_residual_kernels")
+        return {1: "per_lane", 2: "staged"}[rc]
+
     def tables(self):
         """Host copies: edge table, ctrlVol, edge normals, face normals, LSQ R."""
         N, E, F = self.num_nodes, self.num_edges, self.num_faces
@@ -106,35 +126,42 @@ class SpaceOperator:
         """computeResidual (SpaceOperator.C:524-634): R = fluxes / V.
 
         Returns the count of non-physical reconstructions (0 when valid)."""
-        return check(lib().aerofx_residual_ex(self._h, _ptr(U), _ptr(R), 1 if fluxes_only else 0),
+        return check(lib().aerofx_residual_ex(self._h, _ptr(U, self.n5), _ptr(R, self.n5), 1 if fluxes_only else 0),
                      "residual")
 
     def compute_residual_async(self, U, R):
-        check(lib().aerofx_residual_ex(self._h, _ptr(U), _ptr(R), 2), "residual")
+        check(lib().aerofx_residual_ex(self._h, _ptr(U, self.n5), _ptr(R, self.n5), 2), "residual")
 
     def compute_gradients(self, V, ddx, ddy, ddz):
         """DistNodalGrad::compute, least squares, on primitive V."""
-        check(lib().aerofx_gradients(self._h, _ptr(V), _ptr(ddx), _ptr(ddy), _ptr(ddz)), "gradients")
+        check(lib().aerofx_gradients(self._h, _ptr(V, self.n5), _ptr(ddx, self.n5), _ptr(ddy, self.n5),
+                                       _ptr(ddz, self.n5)), "gradients")
 
     def conservative_to_primitive(self, U, V):
-        check(lib().aerofx_primitive(self._h, _ptr(U), _ptr(V)), "primitive")
+        check(lib().aerofx_primitive(self._h, _ptr(U, self.n5), _ptr(V, self.n5)), "primitive")
 
     def wall_force(self, U, direction=(0.0, 0.0, 1.0), dJdU=None, dJdX=None) -> float:
         """Pressure force on the slip walls along direction; optional dJ/dU [N,5], dJ/dX [N,3]."""
         d = (C.c_double * 3)(*direction)
         J = C.c_double()
-        check(lib().aerofx_wall_force(self._h, _ptr(U), d, C.byref(J),
-                                      _ptr(dJdU) if dJdU is not None else None,
-                                      _ptr(dJdX) if dJdX is not None else None), "wall_force")
+        check(lib().aerofx_wall_force(self._h, _ptr(U, self.n5), d, C.byref(J), _optr(dJdU, self.n5),
+                                      _optr(dJdX, self.n3)), "wall_force")
         return J.value
+
+    def compute_derivative_of_residual(self, U, dR, dU=None, dX=None) -> int:
+        """dR = (dR/dU) dU + (dR/dX) dX at U (SpaceOperator::computeDerivativeOfResidual,
+        SpaceOperator.C:917-1070); dU [N,5] and dX [N,3] may be None (zero).
+        Returns the count of non-physical reconstructions (0 when valid)."""
+        return check(lib().aerofx_residual_derivative(self._h, _ptr(U, self.n5), _optr(dU, self.n5),
+                                                      _optr(dX, self.n3), _ptr(dR, self.n5)),
+                     "residual_derivative")
 
     def compute_transpose_derivative_of_residual(self, U, lam, dRdU_T_lam=None, dRdX_T_lam=None):
         """(dR/dU)^T lam -> dRdU_T_lam [N,5] and (dR/dX)^T lam -> dRdX_T_lam [N,3]
         (SpaceOperator::computeTransposeDerivativeOfResidual, SpaceOperator.C:1235-1304).
         Either output may be None.  The MUSCL gradients are recomputed."""
-        check(lib().aerofx_adjoint_residual(self._h, _ptr(U), _ptr(lam),
-                                            _ptr(dRdU_T_lam) if dRdU_T_lam is not None else None,
-                                            _ptr(dRdX_T_lam) if dRdX_T_lam is not None else None),
+        check(lib().aerofx_adjoint_residual(self._h, _ptr(U, self.n5), _ptr(lam, self.n5),
+                                            _optr(dRdU_T_lam, self.n5), _optr(dRdX_T_lam, self.n3)),
               "adjoint_residual")
 
 
@@ -151,21 +178,21 @@ class MatVecProdH1:
         self.op = op
 
     def evaluate(self, U, dt=None, precision: int = 0):
-        check(lib().aerofx_jacobian_setup(self.op._h, _ptr(U), _ptr(dt) if dt is not None else None,
+        check(lib().aerofx_jacobian_setup(self.op._h, _ptr(U, self.op.n5), _optr(dt, self.op.num_nodes),
                                           int(precision)), "jacobian_setup")
 
     def apply(self, p, prod):
-        check(lib().aerofx_jvp(self.op._h, _ptr(p), _ptr(prod)), "jvp")
+        check(lib().aerofx_jvp(self.op._h, _ptr(p, self.op.n5), _ptr(prod, self.op.n5)), "jvp")
 
     def prec_apply(self, y, x):
-        check(lib().aerofx_prec_apply(self.op._h, _ptr(y), _ptr(x)), "prec_apply")
+        check(lib().aerofx_prec_apply(self.op._h, _ptr(y, self.op.n5), _ptr(x, self.op.n5)), "prec_apply")
 
     def apply_transpose(self, p, prod):
         """MatVecProdH1::applyTranspose (MatVecProd.C:1190-1206)."""
-        check(lib().aerofx_jvp_transpose(self.op._h, _ptr(p), _ptr(prod)), "jvp_transpose")
+        check(lib().aerofx_jvp_transpose(self.op._h, _ptr(p, self.op.n5), _ptr(prod, self.op.n5)), "jvp_transpose")
 
     def prec_apply_transpose(self, y, x):
-        check(lib().aerofx_prec_apply_transpose(self.op._h, _ptr(y), _ptr(x)), "prec_apply_transpose")
+        check(lib().aerofx_prec_apply_transpose(self.op._h, _ptr(y, self.op.n5), _ptr(x, self.op.n5)), "prec_apply_transpose")
 
     def blocks(self):
         """Host copies: D [N,5,5] and per edge (A_ij, A_ji) [E,2,5,5]."""
@@ -177,23 +204,42 @@ class MatVecProdH1:
         return D, A
 
     def timestep(self, U, cfl, dt):
-        check(lib().aerofx_timestep(self.op._h, _ptr(U), float(cfl), _ptr(dt)), "timestep")
+        check(lib().aerofx_timestep(self.op._h, _ptr(U, self.op.n5), float(cfl), _ptr(dt, self.op.num_nodes)), "timestep")
 
 
 class MatVecProdFD:
     """MatVecProdFD<5,5> (MatVecProd.h:83-160): finite-difference JVP of the
-    residual plus the backward-Euler time term (dt given) about U."""
+    residual plus the backward-Euler time term (dt given) about U.
+
+    evaluate() selects it as the operator of the context's GMRES (the
+    block-Jacobi preconditioner stays the one of the last
+    MatVecProdH1.evaluate); apply() is one product.  Q, F0 and dt must stay
+    alive and unchanged while it is selected.  standalone_apply() is the
+    self-contained aerofx_jvp_fd (F(U) recomputed per call)."""
 
     def __init__(self, op: SpaceOperator, order: int = 1):
         self.op, self.order = op, order
-        self.U = self.dt = None
+        self.U = self.dt = self.F0 = None
 
-    def evaluate(self, U, dt=None):
-        self.U, self.dt = U, dt
+    def evaluate(self, U, dt=None, F0=None) -> int:
+        from ._abi import AEROFX_MVP_FD
+        self.U, self.dt, self.F0 = U, dt, F0
+        return check(lib().aerofx_mvp_select(self.op._h, AEROFX_MVP_FD, _ptr(U, self.op.n5), _optr(F0, self.op.n5),
+                                             _optr(dt, self.op.num_nodes), int(self.order)), "mvp_select")
+
+    def release(self):
+        """Selects the assembled H1 again as the GMRES operator."""
+        from ._abi import AEROFX_MVP_H1
+        check(lib().aerofx_mvp_select(self.op._h, AEROFX_MVP_H1, None, None, None, 1), "mvp_select")
+        self.U = self.dt = self.F0 = None
 
     def apply(self, p, prod) -> int:
-        return check(lib().aerofx_jvp_fd(self.op._h, _ptr(self.U), _ptr(p), _ptr(prod), int(self.order),
-                                         _ptr(self.dt) if self.dt is not None else None), "jvp_fd")
+        return check(lib().aerofx_mvp_apply(self.op._h, _ptr(p, self.op.n5), _ptr(prod, self.op.n5)), "mvp_apply")
+
+    def standalone_apply(self, U, p, prod, dt=None) -> int:
+        return check(lib().aerofx_jvp_fd(self.op._h, _ptr(U, self.op.n5), _ptr(p, self.op.n5),
+                                         _ptr(prod, self.op.n5), int(self.order),
+                                         _optr(dt, self.op.num_nodes)), "jvp_fd")
 
 
 class GmresSolver:
@@ -203,17 +249,21 @@ class GmresSolver:
         self.op, self.m, self.eps, self.maxits = op, m, eps, maxits
 
     def solve(self, b, x):
+        """Returns (iterations, final residual estimate); raises if a
+        finite-difference product met a non-physical state."""
         its = C.c_int()
         res = C.c_double()
-        check(lib().aerofx_gmres(self.op._h, _ptr(b), _ptr(x), self.m, self.eps, self.maxits, C.byref(its),
+        bad = check(lib().aerofx_gmres(self.op._h, _ptr(b, self.op.n5), _ptr(x, self.op.n5), self.m, self.eps, self.maxits, C.byref(its),
                                  C.byref(res)), "gmres")
+        if bad:
+            raise RuntimeError(f"gmres: {bad} non-physical reconstructions in finite-difference products")
         return its.value, res.value
 
     def solveT(self, b, x):
         """GmresSolver::solveT (KspSolver.C:754): the transposed system."""
         its = C.c_int()
         res = C.c_double()
-        check(lib().aerofx_gmres_transpose(self.op._h, _ptr(b), _ptr(x), self.m, self.eps, self.maxits,
+        check(lib().aerofx_gmres_transpose(self.op._h, _ptr(b, self.op.n5), _ptr(x, self.op.n5), self.m, self.eps, self.maxits,
                                            C.byref(its), C.byref(res)), "gmres_transpose")
         return its.value, res.value
 
@@ -231,8 +281,12 @@ class BlockGmresSolver:
 
     def solve(self, B, X) -> dict:
         """B, X: device float64 [nrhs, N, 5], nrhs <= 16."""
+        nrhs = int(B.shape[0]) if B.dim() > 0 else 0
+        if not 1 <= nrhs <= 16:
+            raise ValueError(f"1 to 16 right-hand sides expected, got {nrhs}")
         st = BlockStats()
-        check(lib().aerofx_block_gmres(self.op._h, int(B.shape[0]), _ptr(B), _ptr(X), self.m, self.eps,
+        check(lib().aerofx_block_gmres(self.op._h, nrhs, _ptr(B, nrhs * self.op.n5), _ptr(X, nrhs * self.op.n5),
+                                       self.m, self.eps,
                                        self.max_cycles, self.precision, 1 if self.use_graph else 0, C.byref(st)),
               "block_gmres")
         return dict(cycles=st.cycles, inner_its=st.inner_its, converged=bool(st.converged), res_max=st.res_max,
@@ -246,21 +300,43 @@ def adjoint_solve(op: SpaceOperator, U, rhs, psi, precision: int = 1, cfl: float
     o = AdjointOpts(int(precision), float(cfl), int(max_its), float(eps), int(gmres_m), int(gmres_maxits),
                     float(gmres_eps))
     st = AdjointStats()
-    check(lib().aerofx_adjoint_solve(op._h, _ptr(U), _ptr(rhs), _ptr(psi), C.byref(o), C.byref(st)),
+    check(lib().aerofx_adjoint_solve(op._h, _ptr(U, op.n5), _ptr(rhs, op.n5), _ptr(psi, op.n5), C.byref(o),
+                                       C.byref(st)),
           "adjoint_solve")
+    return dict(its=st.its, converged=bool(st.converged), krylov_its=st.krylov_its, res0=st.res0, res=st.res,
+                seconds=st.seconds)
+
+
+def direct_solve(op: SpaceOperator, U, rhs, x, precision: int = 1, cfl: float = 1e4, max_its: int = 400,
+                 eps: float = 1e-10, gmres_m: int = 30, gmres_maxits: int = 30, gmres_eps: float = 1e-2):
+    """(dR/dU) x_k = rhs_k for rhs, x [nrhs, N, 5] (or [N, 5]), nrhs <= 16:
+    FP64 outer iterations on the exact products, inner GMRES (one RHS) or
+    block GMRES (several) on H1 + 1/dt with block-Jacobi (FP32 when
+    precision=1)."""
+    nrhs = 1 if rhs.dim() == 2 else int(rhs.shape[0])
+    o = AdjointOpts(int(precision), float(cfl), int(max_its), float(eps), int(gmres_m), int(gmres_maxits),
+                    float(gmres_eps))
+    st = AdjointStats()
+    check(lib().aerofx_direct_solve(op._h, _ptr(U, op.n5), nrhs, _ptr(rhs, nrhs * op.n5), _ptr(x, nrhs * op.n5),
+                                    C.byref(o), C.byref(st)), "direct_solve")
     return dict(its=st.its, converged=bool(st.converged), krylov_its=st.krylov_its, res0=st.res0, res=st.res,
                 seconds=st.seconds)
 
 
 def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200, eps: float = 1e-8,
                  cfl0: float = 10.0, cfl_max: float = 1e4, ser: float = 0.7, gmres_m: int = 30,
-                 gmres_maxits: int = 30, gmres_eps: float = 1e-2):
-    """Pseudo-transient Newton-GMRES to ||R|| <= eps ||R0|| (U updated in place)."""
+                 gmres_maxits: int = 30, gmres_eps: float = 1e-2, mvp: str = "h1", fd_order: int = 1):
+    """Pseudo-transient Newton-GMRES to ||R|| <= eps ||R0|| (U updated in place).
+
+    mvp "h1": Krylov products with the assembled first-order Jacobian (FP32
+    blocks when precision=1); "fd": matrix-free MatVecProdFD products of the
+    FP64 residual (one residual per product), block-Jacobi in `precision`."""
+    code = {"h1": 0, "fd": 1}[mvp]
     o = NewtonOpts(int(precision), int(max_steps), float(eps), float(cfl0), float(cfl_max), float(ser), 0.0, 0.0,
-                   int(gmres_m), int(gmres_maxits), float(gmres_eps))
+                   int(gmres_m), int(gmres_maxits), float(gmres_eps), code, int(fd_order))
     st = NewtonStats()
     hist = np.zeros(max_steps + 1)
-    check(lib().aerofx_steady_solve(op._h, _ptr(U), C.byref(o), C.byref(st), hist.ctypes.data_as(c_double_p),
+    check(lib().aerofx_steady_solve(op._h, _ptr(U, op.n5), C.byref(o), C.byref(st), hist.ctypes.data_as(c_double_p),
                                     max_steps + 1), "steady_solve")
     n = min(st.steps + 1, max_steps + 1)
     return dict(steps=st.steps, converged=bool(st.converged), krylov_its=st.krylov_its, res0=st.res0,

@@ -22,9 +22,10 @@ bracketing each launch on the stream the kernels run on.  traffic (HBM bytes
 per evaluation) comes from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary under
 profiles/.
 
-cpu_baseline: the oracle's single-thread C restatement of the same residual
-on the same mesh and state, timed on the host over a bounded number of
-evaluations.
+cpu_baseline: the oracle's C restatement of the same residual on the same mesh
+and state, on the host's cores (OpenMP over contiguous node partitions with the
+shared-node assembly, as the reference runs its subdomains), timed over a
+bounded number of evaluations; the one-thread figure is reported beside it.
 """
 import argparse
 import json
@@ -55,25 +56,55 @@ def load_traffic(name):
 
 
 def cpu_baseline(mesh, params, U, budget_s):
+    """The oracle's residual on the host: OpenMP over contiguous node
+    partitions with the shared-node assembly, as the reference runs its
+    subdomains (one per thread), and the same on one thread."""
     import numpy as np
     import oracle as O
     t0 = time.time()
     om = O.OracleMesh(mesh, params)
     log(f"[bench] oracle set-up {time.time() - t0:.1f} s")
-    times = []
-    t_start = time.time()
-    while True:
-        t0 = time.perf_counter()
-        out = om.residual(U)
-        times.append(time.perf_counter() - t0)
-        if out["ierr"] != 0:
-            raise RuntimeError("oracle residual reported non-physical states")
-        if time.time() - t_start > budget_s or len(times) >= 10:
-            break
-    med = float(np.median(times))
-    return dict(value=om.E / med / 1e6, unit="Medges/s", cores=1, kind="port",
-                sample=f"{len(times)} full residual evaluations on the same 128^3 mesh and state, "
-                       f"single thread, median {med * 1e3:.0f} ms each")
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+
+    def timed(threads, budget, max_evals):
+        times = []
+        t_start = time.time()
+        while True:
+            t1 = time.perf_counter()
+            out = om.residual_par(U, threads)
+            times.append(time.perf_counter() - t1)
+            if out["ierr"] != 0:
+                raise RuntimeError("oracle residual reported non-physical states")
+            if time.time() - t_start > budget or len(times) >= max_evals:
+                return float(np.median(times)), len(times)
+
+    med, k = timed(cores, budget_s, 10)
+    med1, k1 = timed(1, budget_s / 2, 3)
+    return dict(value=om.E / med / 1e6, unit="Medges/s", cores=cores, kind="port",
+                sample=f"{k} full residual evaluations on the same 128^3 mesh and state, {cores} OpenMP threads "
+                       f"over {cores} node partitions with the shared-node assembly, median {med * 1e3:.0f} ms each",
+                single_thread_value=om.E / med1 / 1e6,
+                single_thread_sample=f"{k1} evaluations on one thread, median {med1 * 1e3:.0f} ms each")
+
+
+def copy_peak(dev, gib=1.0, reps=10):
+    """Measured device-to-device copy rate (bytes read + written per second)."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * n * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    return round(gbs, 1)
 
 
 def newton_bench(dev, max_steps):
@@ -110,6 +141,24 @@ def newton_bench(dev, max_steps):
         log(f"[bench] newton {name}: {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its, "
             f"converged={r['converged']} rel_res={r['res'] / r['res0']:.2e}")
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    # matrix-free Newton-Krylov (MatVecProdFD products of the FP64 residual,
+    # one residual per Krylov product; FP32 block-Jacobi of H1)
+    steady_solve(op, torch.from_numpy(U0).to(dev), precision=1, max_steps=2, mvp="fd")
+    Ud = torch.from_numpy(U0).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = steady_solve(op, Ud, precision=1, max_steps=max_steps, eps=1e-8, cfl0=10.0, cfl_max=1e4, mvp="fd")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    Ufd = Ud.cpu().numpy()
+    out["matrix_free"] = {"seconds": round(wall, 4), "steps": r["steps"], "krylov_its": r["krylov_its"],
+                          "converged": r["converged"], "rel_res": float(r["res"] / r["res0"]),
+                          "preconditioner": "FP32 block-Jacobi of H1",
+                          "speedup_vs_h1_fp64": round(out["fp64"]["seconds"] / wall, 3),
+                          "state_rel_diff_vs_h1_fp64": float(np.linalg.norm(Ufd - states["fp64"]) /
+                                                             np.linalg.norm(states["fp64"]))}
+    log(f"[bench] newton matrix-free: {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its, "
+        f"converged={r['converged']}")
     # per-kernel-bucket breakdown from a separate profiled run of each precision
     for name, prec in (("fp64", 0), ("mixed", 1)):
         Ud = torch.from_numpy(U0).to(dev)
@@ -312,8 +361,8 @@ def main():
     ap.add_argument("--newton-steps", type=int, default=600)
     ap.add_argument("--batched", action="store_true",
                     help="also time 16-RHS block GMRES against 16 single-RHS FP64 solves (128^3 box)")
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also time the adjoint kernels on the 192x64x64 channel")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip timing the adjoint kernels on the 192x64x64 channel")
     ap.add_argument("--shape", action="store_true",
                     help="also run the adjoint shape gradient (8 bump amplitudes) on the 192x64x64 channel")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -428,13 +477,15 @@ def main():
                     "k_flux_edges": {"ms": round(edge_ms, 5)},
                     "k_flux_nodes": {"ms": round(node_ms, 5)}},
     }
-    if not args.no_newton or args.batched or args.adjoint or args.shape:
+    roofline["copy_peak_GBs"] = copy_peak(dev)
+    roofline["frac_of_copy_peak"] = round(achieved / roofline["copy_peak_GBs"], 4)
+    if not args.no_newton or args.batched or not args.no_adjoint or args.shape:
         op.close()
     if not args.no_newton:
         out["newton"] = newton_bench(dev, args.newton_steps)
     if args.batched:
         out["batched"] = batched_bench(dev, args.n)
-    if args.adjoint:
+    if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(dev)
     if args.shape:
         out["shape"] = shape_bench(dev)

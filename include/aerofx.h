@@ -80,6 +80,12 @@ int aerofx_mesh_box_sizes(const aerofx_box_spec *spec, int *num_nodes,
                           int *num_tets, int *num_faces);
 int aerofx_mesh_box(const aerofx_box_spec *spec, double *X, int *tets,
                     int *faces, int *face_codes);
+/* Shape derivative of the synthetic box's coordinates (no reference
+ * counterpart: the design variables of the sensitivity workloads): dX/dp
+ * [N][3] for bump `bump`'s amplitude (param 0) or centre (param 1), in the
+ * generator's node order.  The bump adds a exp(-((x - c)/w)^2)(1 - zeta) to
+ * z, zeta the node's height fraction before the bumps. */
+int aerofx_mesh_box_bump_derivative(const aerofx_box_spec *spec, int bump, int param, double *dX);
 /* Edge count and table (EdgeSet::find/createPointers, EdgeCore.C:55-119):
  * ptr must hold 2*num_edges ints (query with ptr == NULL). */
 int aerofx_mesh_edges(int num_nodes, int num_tets, const int *tets,
@@ -147,6 +153,21 @@ int aerofx_ctx_tables(const aerofx_ctx *ctx, int *ptr, double *ctrl_vol,
 int aerofx_ctx_profile(aerofx_ctx *ctx, int enable);
 int aerofx_ctx_timers(aerofx_ctx *ctx, double *ms, long long *counts);
 
+/* Kernel family of the residual passes (no counterpart in the reference; the
+ * results are bit-identical either way):
+ *   AEROFX_RESIDUAL_AUTO      LDS-staged strips when the mesh fits them
+ *                             (each node with its neighbours within 960
+ *                             staged nodes; each node with at most 256 upper
+ *                             edges, touching at most 191 other nodes through
+ *                             them), otherwise per-lane gathers
+ *   AEROFX_RESIDUAL_PER_LANE  one lane per node or edge gathering from HBM
+ * Returns the family the next residual will use (AEROFX_RESIDUAL_STAGED or
+ * AEROFX_RESIDUAL_PER_LANE), or < 0. */
+#define AEROFX_RESIDUAL_AUTO 0
+#define AEROFX_RESIDUAL_PER_LANE 1
+#define AEROFX_RESIDUAL_STAGED 2
+int aerofx_ctx_set_residual_kernels(aerofx_ctx *ctx, int family);
+
 /* ---- SpaceOperator<5> -------------------------------------------------- */
 /* SpaceOperator<5>::computeResidual (SpaceOperator.C:524-634), NONDESCRIPTOR
  * form: R = (sum of edge and face fluxes)/V.  U, R: device [N][5].
@@ -200,14 +221,44 @@ int aerofx_prec_apply_transpose(aerofx_ctx *ctx, const double *y, double *x);
  * (F(U + eps p) - F(U - eps p)) / (2 eps) (order 2), F = R + (Q - U)/dt
  * (TimeState::add_dAW_dt, backward Euler, Un = U; dt NULL: R only),
  * eps = sum(1e-6 |U| + 1e-6) / (5N ||p||) (1e-6 when ||p|| < 1e-14).
- * Device U, p, Jp [N][5], dt [N].  Returns the reconstruction error count
- * of the last residual (0 when valid) or < 0. */
+ * Device U, p, Jp [N][5], dt [N].  Its scratch (four [N][5] vectors) is
+ * allocated on the first call.  Synchronises; returns the summed
+ * reconstruction error counts of its two residual evaluations (0 when
+ * valid; Jp is then not meaningful) or < 0. */
 int aerofx_jvp_fd(aerofx_ctx *ctx, const double *U, const double *p, double *Jp,
                   int order, const double *dt);
+/* The MatVecProd of aerofx_gmres (ImplicitCoupledTsDesc.C:33-44 picks it
+ * from ImplicitData::mvp):
+ *   AEROFX_MVP_H1  MatVecProdH1: the assembled first-order Jacobian of the
+ *                  last aerofx_jacobian_setup (the default)
+ *   AEROFX_MVP_FD  MatVecProdFD::evaluate (MatVecProd.C:88-118) about Q:
+ *                  products (F(Q + eps p) - F0) / eps (fd_order 1) or
+ *                  (F(Q + eps p) - F(Q - eps p)) / (2 eps) (fd_order 2), eps
+ *                  of computeEpsilon (MatVecProd.C:879-929), with
+ *                  F(Q') = R(Q') + (Q' - Q) / dt (dt NULL: R only); F0 =
+ *                  F(Q) may be passed (the Newton function value the caller
+ *                  already holds) or NULL (evaluated here).  Q, F0 and dt
+ *                  (device [N][5], [N][5], [N]) must stay valid and unchanged
+ *                  while the operator is selected.  One residual per product
+ *                  at fd_order 1, two at fd_order 2.
+ * The preconditioner stays the block-Jacobi of the last aerofx_jacobian_setup;
+ * the transposed solves always use H1^T.  Returns the count of non-physical
+ * reconstructions in the evaluation of F0 (0 when valid), or < 0. */
+#define AEROFX_MVP_H1 0
+#define AEROFX_MVP_FD 1
+int aerofx_mvp_select(aerofx_ctx *ctx, int type, const double *Q, const double *F0,
+                      const double *dt, int fd_order);
+/* MatVecProd::apply (MatVecProd.h:54-80) with the selected operator:
+ * prod = A p.  Synchronises; returns the count of non-physical
+ * reconstructions in its residual evaluations (0 when valid), or < 0. */
+int aerofx_mvp_apply(aerofx_ctx *ctx, const double *p, double *prod);
 /* GmresSolver::solve (KspSolver.C:353-492): right-preconditioned GMRES(m)
- * on the operators of the last aerofx_jacobian_setup, x0 = 0, stop at
+ * with the selected MatVecProd (aerofx_mvp_select; H1 by default) and the
+ * block-Jacobi of the last aerofx_jacobian_setup, x0 = 0, stop at
  * ||r|| <= eps ||b|| or maxits.  Orthogonalisation is CGS2 in FP64, the
- * reorthogonalisation of each vector delayed to the next step. */
+ * reorthogonalisation of each vector delayed to the next step.  Returns 0,
+ * or with AEROFX_MVP_FD the count of non-physical reconstructions met in its
+ * products (> 0; x is then not meaningful), or < 0. */
 int aerofx_gmres(aerofx_ctx *ctx, const double *b, double *x, int m,
                  double eps, int maxits, int *its, double *final_res);
 
@@ -256,6 +307,21 @@ int aerofx_block_gmres(aerofx_ctx *ctx, int nrhs, const double *B, double *X,
 int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const double *lambda,
                             double *dRdU_T_lambda, double *dRdX_T_lambda);
 
+/* Directional derivative of the residual of aerofx_residual at U (device
+ * [N][5]): dR = (dR/dU) dU + (dR/dX) dX for a state perturbation dU (device
+ * [N][5], may be NULL: zero) and a mesh perturbation dX (device [N][3], may
+ * be NULL: zero); dR device [N][5].  Forward mode through every term the
+ * residual computes from U and X (dual normals, control volumes, face
+ * normals, LSQ factors, gradients, reconstruction, fluxes).  Replaces
+ * SpaceOperator::computeDerivativeOfResidual (SpaceOperator.C:917-1070,
+ * EdgeSet::computeDerivativeOfFiniteVolumeTerm, Edge.C:1494-1573) for the
+ * direct shape sensitivity (dX = dX/ds) and, with dX NULL, is the exact
+ * product of the second-order Jacobian (MatVecProdH2::apply,
+ * MatVecProd.C:1293+).  The MUSCL gradients are recomputed.  Synchronises;
+ * returns the non-physical reconstruction count (0 when valid) or < 0. */
+int aerofx_residual_derivative(aerofx_ctx *ctx, const double *U, const double *dU,
+                               const double *dX, double *dR);
+
 /* Pressure force on the slip-wall faces along dir (lift for dir = e_z):
  * J = sum_{wall faces f} sum_{v in f} p_v (n_f / 3) . dir, with
  * dJdU (device [N][5], may be NULL) and dJdX (device [N][3], may be NULL).
@@ -285,6 +351,19 @@ int aerofx_adjoint_solve(aerofx_ctx *ctx, const double *U, const double *rhs,
                          double *psi, const aerofx_adjoint_opts *opts,
                          aerofx_adjoint_stats *stats);
 
+/* Direct sensitivity solve (dR/dU) x_k = rhs_k, k < nrhs <= 16, at U (the
+ * sensitivity handler's ksp->solve(dFdS, dUdS), FluidShapeOptimizationHandler.C:
+ * 1369-1430; rhs_k = -(dR/dX) dX/ds_k from aerofx_residual_derivative):
+ * outer FP64 iterations on the exact second-order products of
+ * aerofx_residual_derivative, each correcting every x_k by GMRES (nrhs 1)
+ * or one block GMRES (nrhs > 1) on H1 + diag(1/dt) with block-Jacobi in the
+ * chosen precision.  rhs, x device [nrhs][N][5].  Stops when every
+ * ||rhs_k - (dR/dU) x_k|| <= eps ||rhs_k||; stats->res0 and res are the
+ * norms of the worst right-hand side and its residual. */
+int aerofx_direct_solve(aerofx_ctx *ctx, const double *U, int nrhs, const double *rhs,
+                        double *x, const aerofx_adjoint_opts *opts,
+                        aerofx_adjoint_stats *stats);
+
 /* ---- steady pseudo-transient Newton-GMRES ------------------------------ */
 typedef struct {
   int precision;        /* 0: all FP64; 1: FP32 Jacobian/JVP/block-Jacobi */
@@ -293,6 +372,8 @@ typedef struct {
   double cfl0, cfl_max, ser, cfl_coef1, cfl_coef2;  /* CflLaw Old */
   int gmres_m, gmres_maxits;
   double gmres_eps;     /* relative Krylov tolerance */
+  int mvp;              /* AEROFX_MVP_H1 (assembled H1) or AEROFX_MVP_FD (matrix-free) */
+  int fd_order;         /* 1 or 2 with AEROFX_MVP_FD */
 } aerofx_newton_opts;
 typedef struct {
   int steps, converged, krylov_its;

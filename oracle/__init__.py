@@ -151,6 +151,16 @@ class OracleMesh:
             out["ddx"], out["ddy"], out["ddz"] = g
         return out
 
+    def residual_par(self, U, nthreads):
+        """The residual on nthreads OpenMP threads over node partitions with
+        the shared-node assembly (CPU timing baseline)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        R = np.empty((self.N, 5))
+        L = lib()
+        L.or_residual_par.restype = C.c_int
+        ierr = L.or_residual_par(C.byref(self.m), C.byref(self.p), _d(U), _d(R), int(nthreads))
+        return dict(R=R, ierr=ierr)
+
     def jvp_fd(self, U, p, order=1, dt=None):
         """MatVecProdFD: (F(U + eps p) - F(U)) / eps, or central (order 2)."""
         U = np.ascontiguousarray(U, dtype=np.float64)
@@ -183,29 +193,36 @@ class OracleMesh:
         lib().or_h1_apply(C.byref(self.m), _d(D), _d(A), _d(p), _d(y))
         return y
 
-    def gmres(self, V, dt, b, precision=0, m=30, eps=1e-2, maxits=30):
+    def gmres(self, V, dt, b, precision=0, m=30, eps=1e-2, maxits=30, U=None, fd_order=1):
+        """GMRES on H1 + 1/dt at V with block-Jacobi; with U given the Krylov
+        products are MatVecProdFD about U (order fd_order)."""
         V = np.ascontiguousarray(V, dtype=np.float64)
         dt = np.ascontiguousarray(dt, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.empty_like(b)
         fr = C.c_double()
+        ierr = C.c_int()
         L = lib()
-        L.or_gmres_h1.restype = C.c_int
-        its = L.or_gmres_h1(C.byref(self.m), C.byref(self.p), _d(V), _d(dt), int(precision), _d(b), _d(x),
-                            int(m), C.c_double(eps), int(maxits), C.byref(fr))
+        L.or_gmres_mvp.restype = C.c_int
+        Uc = np.ascontiguousarray(U, dtype=np.float64) if U is not None else None
+        its = L.or_gmres_mvp(C.byref(self.m), C.byref(self.p), _d(V), _d(Uc) if Uc is not None else None, _d(dt),
+                             int(precision), 1 if Uc is not None else 0, int(fd_order), _d(b), _d(x), int(m),
+                             C.c_double(eps), int(maxits), C.byref(fr), C.byref(ierr))
+        if ierr.value:
+            raise RuntimeError(f"non-physical reconstruction in a finite-difference product ({ierr.value})")
         return x, its, fr.value
 
     def steady_solve(self, U, precision=0, max_steps=50, eps=1e-8, cfl0=10.0, cfl_max=1e4, ser=0.7, m=30,
-                     maxits=30, geps=1e-2):
+                     maxits=30, geps=1e-2, mvp=0, fd_order=1):
         U = np.ascontiguousarray(U, dtype=np.float64).copy()
         hist = np.zeros(max_steps + 1)
         conv, kits = C.c_int(), C.c_int()
         L = lib()
-        L.or_steady_solve.restype = C.c_int
-        steps = L.or_steady_solve(C.byref(self.m), C.byref(self.p), _d(U), int(precision), int(max_steps),
-                                  C.c_double(eps), C.c_double(cfl0), C.c_double(cfl_max), C.c_double(ser),
-                                  int(m), int(maxits), C.c_double(geps), _d(hist), C.byref(conv),
-                                  C.byref(kits))
+        L.or_steady_solve_mvp.restype = C.c_int
+        steps = L.or_steady_solve_mvp(C.byref(self.m), C.byref(self.p), _d(U), int(precision), int(mvp),
+                                      int(fd_order), int(max_steps), C.c_double(eps), C.c_double(cfl0),
+                                      C.c_double(cfl_max), C.c_double(ser), int(m), int(maxits), C.c_double(geps),
+                                      _d(hist), C.byref(conv), C.byref(kits))
         return dict(U=U, steps=steps, converged=bool(conv.value), krylov_its=kits.value,
                     history=hist[:steps + 1].copy())
 

@@ -14,6 +14,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------------ */
 /* Edge table                                                                */
@@ -1183,9 +1186,53 @@ typedef struct {
   const double *Adiag, *Aedge;  /* H1 */
   double *Dinv;                 /* [N][25] inverse diagonal blocks (FP64) */
   float *Dinvf;                 /* or FP32 blocks (mixed precision) */
+  /* MatVecProdFD instead of H1 when U != NULL (MatVecProd.C:88-118,
+   * 262-372): products about U with F0 = F(U), backward-Euler term with dt */
+  const or_params *p;
+  const double *U, *F0, *dt;
+  int fd_order;
+  double loceps;                /* sum(eps0 |U| + eps0), the state part of computeEpsilon */
+  int ierr;                     /* reconstruction errors met in the products */
 } or_ops;
 
-static void ops_matvec(const or_ops *o, const double *p, double *y) { or_h1_apply(o->m, o->Adiag, o->Aedge, p, y); }
+static int fd_function(const or_mesh *m, const or_params *p, const double *Q, const double *U,
+                       const double *dt, double *F);
+
+/* MatVecProdFD::apply with computeEpsilon (MatVecProd.C:262-372, 879-929). */
+static void ops_fd_product(or_ops *o, const double *pv, double *y) {
+  size_t n = 5 * (size_t)o->m->N;
+  double pp = 0.0;
+  for (size_t k = 0; k < n; ++k) pp += pv[k] * pv[k];
+  double norm = sqrt(pp);
+  double eps = norm < 1.0e-14 ? 1.e-6 : o->loceps / ((double)n * norm);
+  double *Q = (double *)calloc(n ? n : 1, sizeof(double)), *Fa = (double *)calloc(n ? n : 1, sizeof(double));
+  if (!Q || !Fa) {
+    free(Q);
+    free(Fa);
+    o->ierr = -1;
+    return;
+  }
+  for (size_t k = 0; k < n; ++k) Q[k] = o->U[k] + eps * pv[k];
+  o->ierr += fd_function(o->m, o->p, Q, o->U, o->dt, Fa);
+  if (o->fd_order == 1) {
+    double inv = 1.0 / eps;
+    for (size_t k = 0; k < n; ++k) y[k] = inv * (Fa[k] - o->F0[k]);
+  } else {
+    double *Fb = (double *)malloc(sizeof(double) * n);
+    for (size_t k = 0; k < n; ++k) Q[k] = o->U[k] - eps * pv[k];
+    o->ierr += fd_function(o->m, o->p, Q, o->U, o->dt, Fb);
+    double half = 0.5 / eps;
+    for (size_t k = 0; k < n; ++k) y[k] = half * (Fa[k] - Fb[k]);
+    free(Fb);
+  }
+  free(Q);
+  free(Fa);
+}
+
+static void ops_matvec(or_ops *o, const double *p, double *y) {
+  if (o->U) ops_fd_product(o, p, y);
+  else or_h1_apply(o->m, o->Adiag, o->Aedge, p, y);
+}
 
 static void ops_prec(const or_ops *o, const double *y, double *x) {
   int N = o->m->N;
@@ -1204,7 +1251,7 @@ static double vdot(size_t n, const double *a, const double *b) {
 /* GmresSolver::solve (KspSolver.C:353-492), typePrec = 2 (right), modified
  * Gram-Schmidt (:404-411), rotations and back-substitution (:919-982).
  * x is set to 0 first (ImplicitCoupledTsDesc.C:355).  Returns iterations. */
-int or_gmres(const or_ops *o, const double *b, double *x, int m, double eps, int maxits, double *final_res) {
+int or_gmres(or_ops *o, const double *b, double *x, int m, double eps, int maxits, double *final_res) {
   size_t n = 5 * (size_t)o->m->N;
   double *V = (double *)malloc(sizeof(double) * n * (size_t)(m + 1));
   double *w = (double *)malloc(sizeof(double) * n), *r = (double *)malloc(sizeof(double) * n);
@@ -1282,14 +1329,37 @@ int or_gmres(const or_ops *o, const double *b, double *x, int m, double eps, int
 
 /* Builds the operators of one Newton step: H1 at V with the dt term and the
  * inverted diagonal blocks (FP64, or FP32 copies for precision 1, as
- * JacobiPrec<float,5>::getData + setup). */
-int or_gmres_h1(const or_mesh *m, const or_params *p, const double *V, const double *dt, int precision,
-                const double *b, double *x, int gm, double eps, int maxits, double *final_res) {
+ * JacobiPrec<float,5>::getData + setup); with mvp 1 the Krylov products are
+ * MatVecProdFD about U (conservative; F0 = F(U) = R(U)) of order fd_order,
+ * the preconditioner unchanged.  *ierr (optional) receives the reconstruction
+ * errors of the products. */
+int or_gmres_mvp(const or_mesh *m, const or_params *p, const double *V, const double *U, const double *dt,
+                 int precision, int mvp, int fd_order, const double *b, double *x, int gm, double eps, int maxits,
+                 double *final_res, int *ierr) {
   int N = m->N, E = m->E;
+  size_t n = 5 * (size_t)N;
   double *Ad = (double *)malloc(sizeof(double) * 25 * (size_t)N);
   double *Ae = (double *)malloc(sizeof(double) * 50 * (size_t)E);
+  double *F0 = NULL;
   or_jacobian_h1(m, p, V, dt, Ad, Ae);
-  or_ops o = {m, Ad, Ae, 0, 0};
+  or_ops o;
+  memset(&o, 0, sizeof(o));
+  o.m = m;
+  o.Adiag = Ad;
+  o.Aedge = Ae;
+  if (mvp == 1) {
+    F0 = (double *)malloc(sizeof(double) * n);
+    o.ierr += or_residual(m, p, U, F0, 0, NULL, NULL, NULL, NULL);
+    o.p = p;
+    o.U = U;
+    o.F0 = F0;
+    o.dt = dt;
+    o.fd_order = fd_order;
+    const double eps0 = 1.e-6;
+    double loceps = 0.0;
+    for (size_t k = 0; k < n; ++k) loceps += eps0 * fabs(U[k]) + eps0;
+    o.loceps = loceps;
+  }
   if (precision) {
     o.Dinvf = (float *)malloc(sizeof(float) * 25 * (size_t)N);
     for (size_t k = 0; k < 25 * (size_t)N; ++k) o.Dinvf[k] = (float)Ad[k];
@@ -1300,8 +1370,14 @@ int or_gmres_h1(const or_mesh *m, const or_params *p, const double *V, const dou
     for (int i = 0; i < N; ++i) or_invert5(o.Dinv + 25 * (size_t)i);
   }
   int its = or_gmres(&o, b, x, gm, eps, maxits, final_res);
-  free(Ad); free(Ae); free(o.Dinv); free(o.Dinvf);
+  if (ierr) *ierr = o.ierr;
+  free(Ad); free(Ae); free(o.Dinv); free(o.Dinvf); free(F0);
   return its;
+}
+
+int or_gmres_h1(const or_mesh *m, const or_params *p, const double *V, const double *dt, int precision,
+                const double *b, double *x, int gm, double eps, int maxits, double *final_res) {
+  return or_gmres_mvp(m, p, V, NULL, dt, precision, 0, 1, b, x, gm, eps, maxits, final_res, NULL);
 }
 
 /* TsParameters::computeCflNumber, CflLaw Old (TsParameters.C:198-200):
@@ -1316,10 +1392,13 @@ double or_cfl_old(int its, double res, double cfl, double cfl0, double cflMax, d
 
 /* Steady pseudo-transient Newton (one Newton iteration per step) with the
  * CflLaw Old (TsParameters.C:198-200); returns steps taken; history[k] =
- * ||R||_2 at step k; *converged set when ||R|| <= eps ||R0||. */
-int or_steady_solve(const or_mesh *m, const or_params *p, double *U, int precision, int max_steps,
-                    double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
-                    double geps, double *history, int *converged, int *krylov_its) {
+ * ||R||_2 at step k; *converged set when ||R|| <= eps ||R0||.  mvp 0: Krylov
+ * products with H1; 1: MatVecProdFD of order fd_order about the step's state
+ * (ImplicitCoupledTsDesc.C:33-44).  Stops early (not converged) when a
+ * residual or a product meets a non-physical reconstruction. */
+int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
+                        int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
+                        double geps, double *history, int *converged, int *krylov_its) {
   int N = m->N;
   size_t n = 5 * (size_t)N;
   double *R = (double *)malloc(sizeof(double) * n), *V = (double *)malloc(sizeof(double) * n);
@@ -1342,11 +1421,20 @@ int or_steady_solve(const or_mesh *m, const or_params *p, double *U, int precisi
     or_timestep(m, p, V, cfl, dt);
     for (size_t k = 0; k < n; ++k) b[k] = -1.0 * R[k];
     double fr;
-    *krylov_its += or_gmres_h1(m, p, V, dt, precision, b, dQ, gm, geps, gmaxits, &fr);
+    int perr = 0;
+    *krylov_its += or_gmres_mvp(m, p, V, U, dt, precision, mvp, fd_order, b, dQ, gm, geps, gmaxits, &fr, &perr);
+    if (perr) break;
     for (size_t k = 0; k < n; ++k) U[k] += dQ[k];
   }
   free(R); free(V); free(dt); free(b); free(dQ);
   return step;
+}
+
+int or_steady_solve(const or_mesh *m, const or_params *p, double *U, int precision, int max_steps,
+                    double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
+                    double geps, double *history, int *converged, int *krylov_its) {
+  return or_steady_solve_mvp(m, p, U, precision, 0, 1, max_steps, eps, cfl0, cfl_max, ser, gm, gmaxits, geps,
+                             history, converged, krylov_its);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1408,4 +1496,151 @@ int or_jvp_fd(const or_mesh *m, const or_params *p, const double *U, const doubl
   }
   free(Q); free(Fa); free(Fb);
   return ierr;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Partitioned residual for the CPU baseline                                 */
+/* ------------------------------------------------------------------------ */
+
+/* The residual as the reference runs it on P subdomains with OpenMP
+ * (Domain.C:868-928): the nodes are split into P contiguous ranges; each
+ * thread runs the edge loops (gradients, then fluxes) over the edges whose
+ * lower node it owns, adds contributions to its own nodes directly and
+ * keeps those to later partitions' nodes in a private buffer; after each
+ * loop the owners add the buffers of the earlier partitions in partition
+ * order (the shared-node assembly).  Faces and the 1/V scaling run per
+ * owned node.  Sums are formed in another order than or_residual's, so the
+ * result agrees with it to rounding; when ierr > 0 the result is not
+ * meaningful.  Returns the non-physical reconstruction count. */
+int or_residual_par(const or_mesh *m, const or_params *p, const double *U, double *R, int nthreads) {
+  const int N = m->N, E = m->E, F = m->F;
+  const double gam = p->gamma, pstiff = p->pstiff;
+  const int P = nthreads < 1 ? 1 : nthreads;
+  double *V = (double *)malloc(sizeof(double) * 5 * (size_t)N);
+  double *G = (double *)malloc(sizeof(double) * 15 * (size_t)N); /* [N][3][5] */
+  int *nlo = (int *)malloc(sizeof(int) * (P + 1));
+  int *elo = (int *)malloc(sizeof(int) * (P + 1));
+  int *hend = (int *)malloc(sizeof(int) * P);     /* one past the largest upper node of each partition */
+  double **hb = (double **)calloc(P, sizeof(double *));
+  int ierr_total = 0, fail = 0;
+  if (!V || !G || !nlo || !elo || !hend || !hb) fail = 1;
+  if (!fail) {
+    for (int q = 0; q <= P; ++q) nlo[q] = (int)((long long)N * q / P);
+    /* edges sorted by lower node: partition q owns [elo[q], elo[q+1]) */
+    for (int q = 0, l = 0; q <= P; ++q) {
+      while (l < E && m->ptr[2 * l] < nlo[q]) ++l;
+      elo[q] = l;
+    }
+    for (int q = 0; q < P; ++q) {
+      int mx = nlo[q + 1];
+      for (int l = elo[q]; l < elo[q + 1]; ++l)
+        if (m->ptr[2 * l + 1] + 1 > mx) mx = m->ptr[2 * l + 1] + 1;
+      hend[q] = mx;
+      size_t len = (size_t)(mx - nlo[q + 1]);
+      hb[q] = (double *)malloc(sizeof(double) * 15 * (len ? len : 1));
+      if (!hb[q]) fail = 1;
+    }
+  }
+  if (fail) {
+    free(V); free(G); free(nlo); free(elo); free(hend);
+    if (hb) for (int q = 0; q < P; ++q) free(hb[q]);
+    free(hb);
+    return -1;
+  }
+#pragma omp parallel num_threads(P) reduction(+ : ierr_total)
+  {
+#ifdef _OPENMP
+    const int q = omp_get_thread_num();
+#else
+    const int q = 0;
+#endif
+    const int a = nlo[q], b = nlo[q + 1];
+    const double *X = m->X;
+    double *H = hb[q];
+    const size_t hlen = (size_t)(hend[q] - b);
+    or_cons_to_prim(b - a, U + 5 * (size_t)a, V + 5 * (size_t)a, gam, pstiff);
+    for (size_t k = 15 * (size_t)a; k < 15 * (size_t)b; ++k) G[k] = 0.0;
+    for (size_t k = 0; k < 15 * hlen; ++k) H[k] = 0.0;
+#pragma omp barrier
+    if (p->second_order) {
+      for (int l = elo[q]; l < elo[q + 1]; ++l) {
+        int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
+        double Wi[3], Wj[3];
+        double dx[3] = {X[3 * j] - X[3 * i], X[3 * j + 1] - X[3 * i + 1], X[3 * j + 2] - X[3 * i + 2]};
+        lsq_local_weights(dx, m->R + 6 * (size_t)i, Wi);
+        dx[0] = -dx[0]; dx[1] = -dx[1]; dx[2] = -dx[2];
+        lsq_local_weights(dx, m->R + 6 * (size_t)j, Wj);
+        double *gi = G + 15 * (size_t)i;
+        double *gj = j < b ? G + 15 * (size_t)j : H + 15 * (size_t)(j - b);
+        for (int k = 0; k < 5; ++k) {
+          double d = V[5 * (size_t)j + k] - V[5 * (size_t)i + k];
+          gi[k] += Wi[0] * d; gi[5 + k] += Wi[1] * d; gi[10 + k] += Wi[2] * d;
+          gj[k] -= Wj[0] * d; gj[5 + k] -= Wj[1] * d; gj[10 + k] -= Wj[2] * d;
+        }
+      }
+#pragma omp barrier
+      for (int r = 0; r < q; ++r)
+        for (int n = (a > nlo[r + 1] ? a : nlo[r + 1]); n < b && n < hend[r]; ++n)
+          for (int k = 0; k < 15; ++k) G[15 * (size_t)n + k] += hb[r][15 * (size_t)(n - nlo[r + 1]) + k];
+#pragma omp barrier
+    }
+    for (size_t k = 5 * (size_t)a; k < 5 * (size_t)b; ++k) R[k] = 0.0;
+    for (size_t k = 0; k < 15 * hlen; ++k) H[k] = 0.0;
+    int ierr = 0;
+    for (int l = elo[q]; l < elo[q + 1]; ++l) {
+      int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
+      double Vi[5], Vj[5], flux[5];
+      const double *vi = V + 5 * (size_t)i, *vj = V + 5 * (size_t)j;
+      if (p->second_order) {
+        double dx[3] = {X[3 * j] - X[3 * i], X[3 * j + 1] - X[3 * i + 1], X[3 * j + 2] - X[3 * i + 2]};
+        double ddVij[5], ddVji[5];
+        const double *gi = G + 15 * (size_t)i, *gj = G + 15 * (size_t)j;
+        for (int k = 0; k < 5; ++k) {
+          ddVij[k] = dx[0] * gi[k] + dx[1] * gi[5 + k] + dx[2] * gi[10 + k];
+          ddVji[k] = dx[0] * gj[k] + dx[1] * gj[5 + k] + dx[2] * gj[10 + k];
+        }
+        or_vanalbada5(p->beta, vi, ddVij, vj, ddVji, Vi, Vj);
+      } else {
+        for (int k = 0; k < 5; ++k) { Vi[k] = vi[k]; Vj[k] = vj[k]; }
+      }
+      if (Vi[0] <= 0.0 || Vi[4] + pstiff <= 0.0) ++ierr;
+      if (Vj[0] <= 0.0 || Vj[4] + pstiff <= 0.0) ++ierr;
+      const double *nl = m->normals + 3 * (size_t)l;
+      if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) continue;
+      or_roeflux7(gam, pstiff, nl, m->normal_vel[l], Vi, Vj, flux);
+      double *rj = j < b ? R + 5 * (size_t)j : H + 5 * (size_t)(j - b);
+      for (int k = 0; k < 5; ++k) {
+        R[5 * (size_t)i + k] += flux[k];
+        rj[k] -= flux[k];
+      }
+    }
+    ierr_total += ierr;
+#pragma omp barrier
+    for (int r = 0; r < q; ++r)
+      for (int n = (a > nlo[r + 1] ? a : nlo[r + 1]); n < b && n < hend[r]; ++n)
+        for (int k = 0; k < 5; ++k) R[5 * (size_t)n + k] += hb[r][5 * (size_t)(n - nlo[r + 1]) + k];
+    const double third = 1.0 / 3.0;
+    for (int f = 0; f < F; ++f) {
+      int code = m->codes[f];
+      double nv[3] = {third * m->fnorm[3 * f], third * m->fnorm[3 * f + 1], third * m->fnorm[3 * f + 2]};
+      for (int c = 0; c < 3; ++c) {
+        int node = m->faces[3 * f + c];
+        if (node < a || node >= b) continue;
+        double flux[5];
+        if (code == 2 || code == -2) or_wallflux(nv, 0.0, V + 5 * (size_t)node, flux);
+        else if (code == 4 || code == -4) or_boundflux5(gam, nv, 0.0, V + 5 * (size_t)node, m->Ub_in, flux);
+        else if (code == 5 || code == -5) or_boundflux5(gam, nv, 0.0, V + 5 * (size_t)node, m->Ub_out, flux);
+        else continue;
+        for (int k = 0; k < 5; ++k) R[5 * (size_t)node + k] += flux[k];
+      }
+    }
+    for (int n = a; n < b; ++n) {
+      double invcv = 1.0 / m->cv[n];
+      for (int k = 0; k < 5; ++k) R[5 * (size_t)n + k] *= invcv;
+    }
+  }
+  free(V); free(G); free(nlo); free(elo); free(hend);
+  for (int q = 0; q < P; ++q) free(hb[q]);
+  free(hb);
+  return ierr_total;
 }

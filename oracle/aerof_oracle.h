@@ -129,6 +129,13 @@ int or_residual_ex(const or_mesh *m, const or_params *p, const double *U,
                    double *R, double *Rabs, int fluxes_only, double *V_out,
                    double *ddx, double *ddy, double *ddz);
 
+/* The residual of or_residual as the reference runs it on P OpenMP threads
+ * over contiguous node partitions with the shared-node assembly of
+ * Domain.C:868-928 (timing baseline; equal to or_residual up to the order
+ * of the sums).  Returns the non-physical reconstruction count, -1 when out
+ * of memory. */
+int or_residual_par(const or_mesh *m, const or_params *p, const double *U, double *R, int nthreads);
+
 /* ---- orderings ----------------------------------------------------------*/
 
 /* Connectivity::renumByComponent(2) over the edge-based node graph with
@@ -147,6 +154,23 @@ int or_color_edges(int N, int E, const int *ptr, int *color);
  * Un = U.  Returns the summed reconstruction error count. */
 int or_jvp_fd(const or_mesh *m, const or_params *p, const double *U, const double *pv, int order,
               const double *dt, double *prod);
+
+/* ---- Newton-Krylov ------------------------------------------------------*/
+
+/* GmresSolver::solve (KspSolver.C:353-492), right block-Jacobi from H1 at V
+ * (+ 1/dt), Krylov products with H1 (mvp 0) or MatVecProdFD about the
+ * conservative state U (mvp 1, order fd_order; MatVecProd.C:88-118,
+ * 262-372).  Returns iterations; *ierr (optional) the reconstruction errors
+ * met in the products. */
+int or_gmres_mvp(const or_mesh *m, const or_params *p, const double *V, const double *U, const double *dt,
+                 int precision, int mvp, int fd_order, const double *b, double *x, int gm, double eps, int maxits,
+                 double *final_res, int *ierr);
+
+/* Pseudo-transient Newton (NewtonSolver.h:70-231, CflLaw Old) with the
+ * Krylov products of mvp (0: H1, 1: MatVecProdFD). */
+int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
+                        int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
+                        double geps, double *history, int *converged, int *krylov_its);
 
 #ifdef __cplusplus
 }

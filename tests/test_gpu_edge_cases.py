@@ -114,16 +114,24 @@ def test_rejected_calls_leave_the_context_working(gpu):
         with pytest.raises(AerofxError, match="Krylov dimension"):
             GmresSolver(op, m=bad_m).solve(v, w)
     B17 = torch.ones((17, m.num_nodes, 5), dtype=torch.float64, device=gpu)
-    with pytest.raises(AerofxError, match="nrhs"):
+    with pytest.raises(ValueError, match="1 to 16 right-hand sides"):
         BlockGmresSolver(op, m=4).solve(B17, torch.zeros_like(B17))
+    from aerof_dev_amd._abi import lib as _lib
+    st = __import__("aerof_dev_amd._abi", fromlist=["BlockStats"]).BlockStats()
+    import ctypes as _C
+    with pytest.raises(AerofxError, match="nrhs"):
+        from aerof_dev_amd._abi import check as _check
+        _check(_lib().aerofx_block_gmres(op._h, 17, B17.data_ptr(), torch.zeros_like(B17).data_ptr(), 4, 1e-8, 10, 0,
+                                         0, _C.byref(st)), "block_gmres")
     with pytest.raises(AerofxError, match="1 <= m <= 16"):
         BlockGmresSolver(op, m=17).solve(B, X)
     with pytest.raises(AerofxError, match="precision 0 or 1"):
         BlockGmresSolver(op, m=4, precision=2).solve(B, X)
     fd = MatVecProdFD(op, order=3)
-    fd.evaluate(Ud)
     with pytest.raises(AerofxError, match="order 1 or 2"):
-        fd.apply(v, w)
+        fd.evaluate(Ud)
+    with pytest.raises(AerofxError, match="order 1 or 2"):
+        fd.standalone_apply(Ud, v, w)
     with pytest.raises(AerofxError, match="precision 0 or 1"):
         steady_solve(op, Ud.clone(), precision=2, max_steps=2)
     with pytest.raises(AerofxError, match="precision 0 or 1"):

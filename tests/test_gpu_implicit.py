@@ -194,3 +194,102 @@ def test_transpose_follows_reassembly(gpu):
         assert torch.equal(got, want), prec
         assert torch.equal(xg, xw), prec
         fresh.close()
+
+
+# ---- matrix-free Newton-Krylov (MatVecProdFD as the GMRES operator) -------
+# The finite-difference step comes from two global sums whose order differs
+# (fixed tree here, sequential in the oracle), so each product differs from
+# the oracle's in the last bits of eps: about 1e-9 relative.  The products
+# are also not linear in p (eps depends on ||p||) and carry the residual's
+# rounding divided by eps (~1e-8 relative), so two GMRES runs that differ
+# only in orthogonalisation order (CGS2 here, MGS in the oracle) agree to
+# that noise times the conditioning of the system, and Newton histories
+# built on them agree step by step only to about 1e-2; both converge to the
+# same discrete steady state.
+
+
+@pytest.mark.parametrize("given_f0", [False, True])
+def test_mvp_fd_products_match_oracle(gpu, given_f0):
+    import torch
+    from aerof_dev_amd.operators import MatVecProdFD, MatVecProdH1
+    m, p, U, op, om = _setup(gpu, channel=True)
+    dt = om.timestep(om.residual(U)["V"], 40.0)
+    Ud = torch.from_numpy(U).to(gpu)
+    dtd = torch.from_numpy(dt).to(gpu)
+    F0 = torch.from_numpy(om.residual(U)["R"]).to(gpu) if given_f0 else None
+    fd = MatVecProdFD(op, 1)
+    assert fd.evaluate(Ud, dtd, F0) == 0
+    rng = np.random.default_rng(21)
+    for _ in range(3):
+        pv = rng.normal(size=U.shape)
+        yd = torch.empty_like(Ud)
+        assert fd.apply(torch.from_numpy(pv).to(gpu), yd) == 0
+        ref = om.jvp_fd(U, pv, 1, dt)
+        assert np.linalg.norm(yd.cpu().numpy() - ref) <= 1e-7 * np.linalg.norm(ref)
+    # switching back to H1 restores the assembled operator's products
+    fd.release()
+    h1 = MatVecProdH1(op)
+    h1.evaluate(Ud, dtd, 0)
+    pv = rng.normal(size=U.shape)
+    y1, y2 = torch.empty_like(Ud), torch.empty_like(Ud)
+    h1.apply(torch.from_numpy(pv).to(gpu), y1)
+    from aerof_dev_amd._abi import lib
+    lib().aerofx_mvp_apply(op._h, torch.from_numpy(pv).to(gpu).data_ptr(), y2.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+
+
+@pytest.mark.parametrize("fd_order", [1, 2])
+def test_gmres_fd_matches_oracle(gpu, fd_order):
+    """GMRES with MatVecProdFD products to a Newton-like tolerance (1e-3): the
+    same iteration count as the oracle's restatement within 2, the same
+    solution within 1e-2, and a true residual of the finite-difference system
+    (the oracle's products) within a few times the tolerance."""
+    import torch
+    from aerof_dev_amd.operators import GmresSolver, MatVecProdFD, MatVecProdH1
+    m, p, U, op, om = _setup(gpu, channel=True)
+    ref = om.residual(U)
+    V = ref["V"]
+    dt = om.timestep(V, 100.0)
+    b = -ref["R"]
+    x_ref, its_ref, _ = om.gmres(V, dt, b, precision=0, m=30, eps=1e-3, maxits=60, U=U, fd_order=fd_order)
+    Ud = torch.from_numpy(U).to(gpu)
+    dtd = torch.from_numpy(dt).to(gpu)
+    MatVecProdH1(op).evaluate(Ud, dtd, 0)
+    fd = MatVecProdFD(op, fd_order)
+    fd.evaluate(Ud, dtd)
+    xd = torch.empty_like(Ud)
+    its, res = GmresSolver(op, 30, 1e-3, 60).solve(torch.from_numpy(b).to(gpu), xd)
+    x = xd.cpu().numpy()
+    fd.release()
+    assert abs(its - its_ref) <= 2 and its < 60
+    assert _rel(x, x_ref) <= 1e-2
+    Ax = om.jvp_fd(U, x, fd_order, dt)
+    assert np.linalg.norm(Ax - b) <= 5e-3 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_steady_solve_fd_tracks_oracle(gpu, precision):
+    """Matrix-free Newton-GMRES (FD products of the FP64 residual, FP64 or FP32
+    block-Jacobi) against the oracle's restatement, and the same steady state
+    as the assembled-H1 Newton."""
+    import torch
+    from aerof_dev_amd.operators import steady_solve
+    m, p, U, op, om = _setup(gpu, n=8, channel=True, noise=0.0)
+    ref = om.steady_solve(U, precision=precision, max_steps=60, eps=1e-8, mvp=1)
+    assert ref["converged"]
+    Ud = torch.from_numpy(U).to(gpu)
+    out = steady_solve(op, Ud, precision=precision, max_steps=60, eps=1e-8, mvp="fd")
+    assert out["converged"]
+    h, hr = out["history"], ref["history"]
+    assert abs(len(h) - len(hr)) <= 4 and len(h) <= 40
+    # the initial residual is the same; the steps follow the oracle within the
+    # finite-difference noise of the inexact (1e-2) Krylov solves
+    assert abs(h[0] - hr[0]) <= 1e-12 * hr[0]
+    assert np.max(np.abs(h[:10] - hr[:10]) / hr[:10]) <= 5e-2
+    Uh = torch.from_numpy(U).to(gpu)
+    out_h1 = steady_solve(op, Uh, precision=0, max_steps=400, eps=1e-8)
+    assert out_h1["converged"] and out_h1["steps"] > out["steps"]
+    a, b, c = Ud.cpu().numpy(), Uh.cpu().numpy(), ref["U"]
+    assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
+    assert np.linalg.norm(a - c) <= 1e-6 * np.linalg.norm(c)

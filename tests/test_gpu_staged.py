@@ -1,0 +1,80 @@
+"""LDS-staged residual kernels against the per-lane kernels and the oracle.
+
+The staged gradient and flux passes form every sum of the per-lane passes
+term for term in the same order (only where the node data come from
+differs), so the two kernel families must give bit-identical residuals; the
+oracle comparison is the usual 1e-12 (normwise and per node, scaled by the
+sum of |contributions|).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from aerof_dev_amd import FlowParams, init_state
+from aerof_dev_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _both(gpu, mesh, params, U):
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    op = SpaceOperator(mesh, params)
+    Ud = torch.from_numpy(U).to(gpu)
+    out = {}
+    for fam in ("per_lane", "auto"):
+        used = op.set_residual_kernels(fam)
+        Rd = torch.full_like(Ud, float("nan"))
+        ierr = op.compute_residual(Ud, Rd)
+        torch.cuda.synchronize()
+        out[used] = (ierr, Rd.cpu().numpy())
+    op.close()
+    return out
+
+
+def _check(Rg, ref):
+    R, Rabs = ref["R"], ref["Rabs"]
+    nw = np.linalg.norm(Rg - R) / np.linalg.norm(R)
+    per = np.max(np.abs(Rg - R) / np.maximum(Rabs, 1e-300))
+    assert nw <= TOL and per <= TOL, (nw, per)
+
+
+CASES = {
+    "box20_noise_o2": (lambda: M.box_mesh(20, jitter=0.15, seed=1234), dict(mach=0.5), ("noise", 0.01, 99)),
+    "box20_noise_o1": (lambda: M.box_mesh(20, jitter=0.15, seed=1234), dict(mach=0.5, second_order=False),
+                       ("noise", 0.01, 99)),
+    "box24_rcm_sine": (lambda: M.box_mesh(24, jitter=0.15, seed=7, rcm_order=True), dict(mach=0.8),
+                       ("sine", 0.02, 5)),
+    "channel_bump": (lambda: M.channel_mesh(24, 8, 8, jitter=0.1, seed=2), dict(mach=0.675, alpha=0.05),
+                     ("noise", 0.01, 4)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_staged_equals_per_lane_and_oracle(gpu, case):
+    make, pkw, (kind, amp, seed) = CASES[case]
+    m = make()
+    p = FlowParams(**pkw)
+    U = init_state(p, m.X, kind, amp, seed=seed)
+    print(f"[staged] {case}: N={m.num_nodes}", flush=True)
+    out = _both(gpu, m, p, U)
+    assert set(out) == {"per_lane", "staged"}, out.keys()
+    (e1, R1), (e2, R2) = out["per_lane"], out["staged"]
+    assert e1 == e2 == 0
+    assert np.array_equal(R1, R2), f"max |diff| {np.max(np.abs(R1 - R2)):.3e}"
+    ref = O.OracleMesh(m, p).residual(U, want_abs=True)
+    _check(R2, ref)
+
+
+def test_staged_counts_nonphysical_reconstructions(gpu):
+    """A node with negative pressure (energy below kinetic): both families
+    count the non-physical reconstructed states as the oracle does."""
+    m = M.box_mesh(6, jitter=0.0, seed=0)
+    p = FlowParams(mach=0.5)
+    U = init_state(p, m.X, "freestream")
+    U[U.shape[0] // 2, 4] = -2.375
+    out = _both(gpu, m, p, U)
+    ref = O.OracleMesh(m, p).residual(U)
+    assert ref["ierr"] > 0
+    assert out["per_lane"][0] == out["staged"][0] == ref["ierr"]

@@ -98,3 +98,25 @@ def test_states():
     Us = init_state(p, m.X, "sine", 0.02, seed=1)
     assert np.abs(Us[:, 0] - 1).max() <= 0.02 + 1e-15
     np.testing.assert_array_equal(init_state(p, m.X, "noise", 0.01, seed=1), Un)  # seeded
+
+
+def test_bump_derivative_closed_form():
+    """dX/da and dX/dc of the bump generator against central differences of
+    the generator itself (the amplitude enters linearly; the centre through
+    exp(-t^2)), also through an RCM renumbering."""
+    base = [(0.05, 1.3, 0.2), (0.03, 1.8, 0.3)]
+    kw = dict(lo=(0.0, 0.0, 0.0), hi=(3.0, 1.0, 1.0), jitter=0.1, seed=4, face_codes=M.CHANNEL)
+    m = M.kuhn_box(12, 4, 4, bumps=base, **kw)
+    h = 1e-6
+    for b in range(2):
+        for k, name in ((0, "amplitude"), (1, "center")):
+            bp = [list(x) for x in base]
+            bm = [list(x) for x in base]
+            bp[b][k] += h
+            bm[b][k] -= h
+            fd = (M.kuhn_box(12, 4, 4, bumps=bp, **kw).X - M.kuhn_box(12, 4, 4, bumps=bm, **kw).X) / (2 * h)
+            dX = M.bump_derivative(m, b, name)
+            assert np.abs(dX - fd).max() <= 1e-8 * max(np.abs(fd).max(), 1e-300)
+            assert np.abs(dX[:, :2]).max() == 0.0 and np.abs(dX[:, 2]).max() > 0.0
+    r = M.renumber(m, M.rcm(m))
+    assert np.array_equal(M.bump_derivative(r, 1, "center")[M.rcm(m)], M.bump_derivative(m, 1, "center"))

@@ -296,3 +296,22 @@ def test_h1_equals_fd_jacobian_at_uniform_state():
         y = om.h1_apply(D, A, d)
         fd = (om.residual(U + h * d)["R"] - om.residual(U - h * d)["R"]) / (2 * h)
         assert np.linalg.norm(y - fd) <= 1e-6 * np.linalg.norm(fd)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_partitioned_residual_matches_serial(threads):
+    """The OpenMP-partitioned residual of the CPU baseline sums the same terms
+    as or_residual in another order (subdomain partials added at the shared
+    nodes afterwards), for both spatial orders."""
+    from aerof_dev_amd import FlowParams, init_state
+    from aerof_dev_amd import mesh as M
+    m = M.box_mesh(9, jitter=0.15, seed=8, rcm_order=True)
+    for second in (True, False):
+        p = FlowParams(mach=0.8, second_order=second)
+        U = init_state(p, m.X, "sine", 0.02, seed=3)
+        om = O.OracleMesh(m, p)
+        ref = om.residual(U, want_abs=True)
+        par = om.residual_par(U, threads)
+        assert par["ierr"] == ref["ierr"] == 0
+        err = np.abs(par["R"] - ref["R"]) / np.maximum(ref["Rabs"], 1e-300)
+        assert err.max() <= 1e-12, err.max()
