@@ -18,6 +18,18 @@ AEROFX_BC_INLET = 4
 AEROFX_BC_OUTLET = 5
 AEROFX_MVP_H1 = 0
 AEROFX_MVP_FD = 1
+AEROFX_ERR_ARG = -1
+AEROFX_ERR_HIP = -2
+AEROFX_ERR_MESH = -3
+AEROFX_ERR_NOMEM = -4
+AEROFX_ERR_STATE = -5
+AEROFX_PHYS_ROEFLUX7 = 0
+AEROFX_PHYS_BOUNDFLUX5 = 1
+AEROFX_PHYS_VANALBADA = 2
+AEROFX_PHYS_ROEJAC6_SW1 = 3
+AEROFX_PHYS_ROEJAC6_SW2 = 4
+AEROFX_PHYS_INVERT5_F64 = 5
+AEROFX_PHYS_INVERT5_F32 = 6
 TIMER_NAMES = ("nodalGrad", "fvTerm", "fvTermNodes", "fvJac", "fluidPrecSetup", "jacApply",
                "precApply", "gramSchmidt")
 
@@ -136,6 +148,8 @@ SIGNATURES = {
     "aerofx_residual_derivative": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_steady_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats),
                                       c_double_p, C.c_int]),
+    "aerofx_physics_eval": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

@@ -349,6 +349,74 @@ def shape_bench(dev, order=1, ncheck=1):
     return res
 
 
+def sensitivity_bench(dev, n=NX, ndesign=16):
+    """Batched direct sensitivities on the n^3 Kuhn box with 8 jittered wall
+    bumps: a mixed-precision steady solve, then 16 right-hand sides
+    -(dR/dX) dX/ds_k (the amplitude and the centre of each bump, closed-form
+    dX/ds from the generator, forward-mode (dR/dX)), solved together (block
+    GMRES with FP32 inner cycles in captured graphs, FP64 refinement with the
+    exact products) and one at a time in FP64."""
+    import numpy as np
+    import torch
+
+    from aerof_dev_amd import FlowParams, init_state
+    from aerof_dev_amd import mesh as M
+    from aerof_dev_amd.operators import SpaceOperator, direct_solve, steady_solve
+
+    rng = np.random.default_rng(5)
+    centres = rng.uniform(0.25, 0.75, size=8)
+    amps = rng.uniform(0.004, 0.008, size=8)
+    mesh = M.kuhn_box(n, n, n, jitter=0.15, seed=1234, bumps=[(a, c, 0.1) for a, c in zip(amps, centres)])
+    mesh = M.renumber(mesh, M.rcm_gpu(mesh))
+    params = FlowParams(mach=0.8)
+    op = SpaceOperator(mesh, params)
+    N = op.num_nodes
+    U = torch.from_numpy(init_state(params, mesh.X, "freestream")).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = steady_solve(op, U, precision=1, max_steps=400, eps=1e-8, cfl0=10.0, cfl_max=1e4)
+    torch.cuda.synchronize()
+    t_steady = time.perf_counter() - t0
+    design = [(b, prm) for b in range(8) for prm in ("amplitude", "center")][:ndesign]
+    rhs = torch.empty((len(design), N, 5), dtype=torch.float64, device=dev)
+    for k, (b, prm) in enumerate(design):
+        dX = torch.from_numpy(M.bump_derivative(mesh, b, prm)).to(dev)
+        dR = torch.empty_like(U)
+        op.compute_derivative_of_residual(U, dR, None, dX)
+        rhs[k] = -dR
+    eps = 1e-10
+    x = torch.empty_like(rhs)
+    direct_solve(op, U, rhs[:1].contiguous(), x[:1].contiguous(), precision=1, eps=1e-2, max_its=2)  # set-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sb = direct_solve(op, U, rhs, x, precision=1, cfl=1e4, eps=eps, max_its=400)
+    torch.cuda.synchronize()
+    t_blk = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    worst, its = 0.0, 0
+    xs = []
+    for k in range(len(design)):
+        xk = torch.empty_like(U)
+        s1 = direct_solve(op, U, rhs[k].contiguous(), xk, precision=0, cfl=1e4, eps=eps, max_its=400)
+        its += s1["krylov_its"]
+        xs.append(xk)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    for k in range(len(design)):
+        worst = max(worst, float(torch.linalg.norm(x[k] - xs[k]) / torch.linalg.norm(xs[k])))
+    op.close()
+    res = {"workload": f"{len(design)} direct sensitivities (8 bump amplitudes and centres), {n}^3 Kuhn box with "
+                       f"jittered wall bumps ({N} nodes), second order, M=0.8, stop at "
+                       f"||rhs - dR/dU x|| <= {eps} ||rhs|| per RHS",
+           "steady_mixed_s": round(t_steady, 3), "steady_steps": r["steps"], "steady_converged": r["converged"],
+           "block_mixed_s": round(t_blk, 3), "block_outer_its": sb["its"], "block_inner_its": sb["krylov_its"],
+           "block_converged": sb["converged"], "single_fp64_s": round(t_single, 3),
+           "single_fp64_krylov_its": its, "speedup_block_vs_single": round(t_single / t_blk, 3),
+           "max_rel_solution_diff_vs_single_fp64": worst}
+    log(f"[bench] sensitivity: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +433,8 @@ def main():
                     help="skip timing the adjoint kernels on the 192x64x64 channel")
     ap.add_argument("--shape", action="store_true",
                     help="also run the adjoint shape gradient (8 bump amplitudes) on the 192x64x64 channel")
+    ap.add_argument("--sensitivity", action="store_true",
+                    help="also run 16 batched direct sensitivities on the 128^3 bump box")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
@@ -479,7 +549,7 @@ def main():
     }
     roofline["copy_peak_GBs"] = copy_peak(dev)
     roofline["frac_of_copy_peak"] = round(achieved / roofline["copy_peak_GBs"], 4)
-    if not args.no_newton or args.batched or not args.no_adjoint or args.shape:
+    if not args.no_newton or args.batched or not args.no_adjoint or args.shape or args.sensitivity:
         op.close()
     if not args.no_newton:
         out["newton"] = newton_bench(dev, args.newton_steps)
@@ -489,6 +559,8 @@ def main():
         out["adjoint"] = adjoint_bench(dev)
     if args.shape:
         out["shape"] = shape_bench(dev)
+    if args.sensitivity:
+        out["sensitivity"] = sensitivity_bench(dev, args.n)
     if not args.no_cpu:
         log("[bench] timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(mesh, params, U, args.cpu_budget)

@@ -387,6 +387,34 @@ int aerofx_steady_solve(aerofx_ctx *ctx, double *U, const aerofx_newton_opts *op
                         aerofx_newton_stats *stats, double *history,
                         int hist_cap);
 
+/* ---- physics routines, one case per thread ----------------------------- */
+/* No counterpart in the reference: evaluates the device routines that the
+ * residual, Jacobian and preconditioner kernels inline, on n cases in device
+ * arrays, so that tests can compare them case by case with stored outputs of
+ * the reference's routines.  Runs on the null stream and synchronises.
+ *   AEROFX_PHYS_ROEFLUX7     ROEFLUX7 (roeflux7.f), scalar = gamma:
+ *                            a normal [n][3], b V_L [n][5], c V_R [n][5] -> out [n][5]
+ *   AEROFX_PHYS_BOUNDFLUX5   BOUNDFLUX5 (boundflux5.f), scalar = gamma:
+ *                            a normal, b V [n][5], c U_b [n][5] (conservative) -> out [n][5]
+ *   AEROFX_PHYS_VANALBADA    RecFcn::vanalbada per variable (RecFcn.h:433-446),
+ *                            scalar = beta: a V_i, b ddV_ij, c V_j, d ddV_ji
+ *                            [n][5] -> out [n][10] (V_ij, V_ji)
+ *   AEROFX_PHYS_ROEJAC6_SW1/2  ROEJAC6 type 1 or 2 (roejac6.f), scalar = gamma:
+ *                            a normal, b U_g [n][5], c U_d [n][5] -> out [n][25] row-major
+ *   AEROFX_PHYS_INVERT5_F64/F32  LINPACK dgefa + dgedi (DenseMatrixOps.h:1006-1181)
+ *                            in FP64 or FP32: a blocks [n][25] -> out [n][25]
+ *                            (FP32 results widened to FP64)
+ * Returns the number of singular blocks (inversions), 0, or < 0. */
+#define AEROFX_PHYS_ROEFLUX7 0
+#define AEROFX_PHYS_BOUNDFLUX5 1
+#define AEROFX_PHYS_VANALBADA 2
+#define AEROFX_PHYS_ROEJAC6_SW1 3
+#define AEROFX_PHYS_ROEJAC6_SW2 4
+#define AEROFX_PHYS_INVERT5_F64 5
+#define AEROFX_PHYS_INVERT5_F32 6
+int aerofx_physics_eval(int kind, int n, double scalar, const double *a, const double *b,
+                        const double *c, const double *d, double *out);
+
 #ifdef __cplusplus
 }
 #endif

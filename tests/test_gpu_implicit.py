@@ -244,13 +244,22 @@ def test_gmres_fd_matches_oracle(gpu, fd_order):
     """GMRES with MatVecProdFD products to a Newton-like tolerance (1e-3): the
     same iteration count as the oracle's restatement within 2, the same
     solution within 1e-2, and a true residual of the finite-difference system
-    (the oracle's products) within a few times the tolerance."""
+    (the oracle's products) within twice the tolerance.
+
+    The state carries noise on every component: a velocity component that is
+    exactly uniform makes the Van Albada arguments of that component exactly
+    zero, where the limiter (eps0 = 1e-16) is far from linear at the
+    finite-difference step, so the products stop being linear in the
+    direction (5.7 % on two random directions here) and no Krylov method
+    converges on them; with the noise they are linear to 1e-6."""
     import torch
     from aerof_dev_amd.operators import GmresSolver, MatVecProdFD, MatVecProdH1
     m, p, U, op, om = _setup(gpu, channel=True)
+    rng = np.random.default_rng(18)
+    U[:, 1:4] += 0.02 * p.mach * U[:, :1] * rng.standard_normal((U.shape[0], 3))
     ref = om.residual(U)
     V = ref["V"]
-    dt = om.timestep(V, 100.0)
+    dt = om.timestep(V, 10.0)
     b = -ref["R"]
     x_ref, its_ref, _ = om.gmres(V, dt, b, precision=0, m=30, eps=1e-3, maxits=60, U=U, fd_order=fd_order)
     Ud = torch.from_numpy(U).to(gpu)
@@ -262,10 +271,10 @@ def test_gmres_fd_matches_oracle(gpu, fd_order):
     its, res = GmresSolver(op, 30, 1e-3, 60).solve(torch.from_numpy(b).to(gpu), xd)
     x = xd.cpu().numpy()
     fd.release()
-    assert abs(its - its_ref) <= 2 and its < 60
+    assert its_ref < 60 and abs(its - its_ref) <= 2
     assert _rel(x, x_ref) <= 1e-2
     Ax = om.jvp_fd(U, x, fd_order, dt)
-    assert np.linalg.norm(Ax - b) <= 5e-3 * np.linalg.norm(b)
+    assert np.linalg.norm(Ax - b) <= 2e-3 * np.linalg.norm(b)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
