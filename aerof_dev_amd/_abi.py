@@ -118,6 +118,7 @@ SIGNATURES = {
     "aerofx_ctx_tables": (C.c_int, [C.c_void_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p]),
     "aerofx_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_residual_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "aerofx_residual_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "aerofx_error_count": (C.c_int, [C.c_void_p, c_int_p]),
     "aerofx_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_primitive": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

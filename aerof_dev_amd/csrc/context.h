@@ -137,6 +137,11 @@ struct aerofx_ctx {
   double *d_P = nullptr;     // [N][5] per-node sums over upper edges
   int *d_err = nullptr;      // [0] reconstruction errors, [1] singular blocks, [2] bad states
   int residual_kernels = 0;  // AEROFX_RESIDUAL_* (auto: LDS-staged when the mesh fits)
+  // device address of this GPU's static scratch array of
+  // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
+  // with the reason in noalloc_err)
+  char *noalloc_scratch = nullptr;
+  std::string noalloc_err;
   int *h_err = nullptr;
   // implicit operators (jacobian.hip) and Krylov workspace (krylov.hip)
   int jac_precision = -1;    // 0: FP64 blocks, 1: FP32 blocks
@@ -199,6 +204,14 @@ __global__ void k_zero_int(int *p, int n, const int *done);
 // returns at once when done != nullptr and *done != 0 (device-resident
 // Krylov loops with a matrix-free operator)
 int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done = nullptr);
+// the same passes with their scratch (primitive state V [N][5], gradients
+// [N][16], upper-edge sums P [N][5], edge fluxes Fm [E][5]) given explicitly;
+// the error count still goes to c->d_err[0]
+struct ResidualBufs {
+  double *V, *grad, *Fm, *P;
+};
+int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale,
+                         const int *done = nullptr);
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz);
 int launch_primitive(aerofx_ctx *c, const double *U, double *V);
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,47 @@ int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out) {
 }
 
 namespace {
+
+// ---------------------------------------------------------------------------
+// Scratch of aerofx_residual_noalloc: one static array per GPU, holding the
+// residual passes' V [N][5], gradients [N][16], upper-edge sums P [N][5] and
+// edge fluxes Fm [E][5], each part at a 256-byte boundary.  Host code never
+// passes the array's name to a kernel: it uses the device address that
+// hipGetSymbolAddress returns for the current GPU, looked up once per GPU.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// The current GPU's scratch address (nullptr + message on failure).
+char *scratch_address(std::string &err) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = "device ordinal " + std::to_string(dev) + " beyond the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<char *>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 template <class T>
 int upload(aerofx_ctx *c, const std::vector<T> &v, const T **dst) {
@@ -324,6 +366,9 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) { destroy(c); return set_hip_error(e, "hipStreamCreate"); }
     c->stream = c->own_stream;
+    // the no-allocation residual's scratch on this GPU (a failure here only
+    // disables that entry point, which then reports it)
+    c->noalloc_scratch = scratch_address(c->noalloc_err);
   } catch (std::exception &ex) {
     destroy(c);
     return set_error(AEROFX_ERR_MESH, ex.what());
@@ -419,6 +464,30 @@ int aerofx_residual_ex(aerofx_ctx *c, const double *U, double *R, int flags) {
 }
 
 int aerofx_residual(aerofx_ctx *c, const double *U, double *R) { return aerofx_residual_ex(c, U, R, 0); }
+
+int aerofx_residual_noalloc(aerofx_ctx *c, const double *U, double *R, int flags) {
+  if (!c || !U || !R) return set_error(AEROFX_ERR_ARG, "residual_noalloc: null argument");
+  if (!c->noalloc_scratch)
+    return set_error(AEROFX_ERR_STATE, "residual_noalloc: no scratch address on this GPU: " + c->noalloc_err);
+  const size_t oV = 0, oG = align256(5 * sizeof(double) * (size_t)c->N);
+  const size_t oP = oG + align256(16 * sizeof(double) * (size_t)c->N);
+  const size_t oF = oP + align256(5 * sizeof(double) * (size_t)c->N);
+  const size_t need = oF + align256(5 * sizeof(double) * (size_t)c->E);
+  if (need > kNoAllocScratchBytes)
+    return set_error(AEROFX_ERR_ARG, "residual_noalloc: the mesh (" + std::to_string(c->N) + " nodes, " +
+                                         std::to_string(c->E) + " edges) needs " + std::to_string(need) +
+                                         " bytes of scratch; the static scratch holds " +
+                                         std::to_string(kNoAllocScratchBytes) + " (use aerofx_residual_ex)");
+  char *base = c->noalloc_scratch;
+  const ResidualBufs b{reinterpret_cast<double *>(base + oV), reinterpret_cast<double *>(base + oG),
+                       reinterpret_cast<double *>(base + oF), reinterpret_cast<double *>(base + oP)};
+  int rc = launch_residual_with(c, b, U, R, (flags & 1) == 0);
+  if (rc) return rc;
+  if (flags & 2) return AEROFX_OK;
+  int cnt = 0;
+  rc = aerofx_error_count(c, &cnt);
+  return rc ? rc : cnt;
+}
 
 int aerofx_gradients(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz) {
   if (!c || !V || !ddx || !ddy || !ddz) return set_error(AEROFX_ERR_ARG, "gradients: null argument");

@@ -505,50 +505,56 @@ __global__ void k_zero_int(int *p, int n, const int *done) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done) {
+int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale,
+                         const int *done) {
   const DevTables &t = c->tab;
   const bool staged = t.ngb > 0 && c->residual_kernels != AEROFX_RESIDUAL_PER_LANE;
   if (c->p.second_order) {
     ProfScope ps(c, AEROFX_TIMER_NODAL_GRAD);
     if (staged)
-      hipLaunchKernelGGL(k_gradients_staged, dim3(t.ngb), dim3(kGradStrip), 0, c->stream, t, U, c->gas, c->d_V,
-                         c->d_grad, c->d_err, done);
+      hipLaunchKernelGGL(k_gradients_staged, dim3(t.ngb), dim3(kGradStrip), 0, c->stream, t, U, c->gas, b.V,
+                         b.grad, c->d_err, done);
     else
       hipLaunchKernelGGL((k_gradients<true, false>), dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream,
-                         t, U, c->gas, c->d_V, c->d_grad, nullptr, nullptr, nullptr, c->d_err, done);
+                         t, U, c->gas, b.V, b.grad, nullptr, nullptr, nullptr, c->d_err, done);
   } else {
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err, 1, done);
-    hipLaunchKernelGGL(k_primitive, dim3((t.N + 255) / 256), dim3(256), 0, c->stream, t.N, U, c->gas, c->d_V, done);
+    hipLaunchKernelGGL(k_primitive, dim3((t.N + 255) / 256), dim3(256), 0, c->stream, t.N, U, c->gas, b.V, done);
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_TERM);
     const dim3 g(t.nfchunks);
     if (staged && c->p.second_order)
-      hipLaunchKernelGGL(k_flux_staged<true>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, c->d_V, c->d_grad,
-                         c->fc, c->d_Fm, c->d_P, c->d_err, done);
+      hipLaunchKernelGGL(k_flux_staged<true>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, b.V, b.grad,
+                         c->fc, b.Fm, b.P, c->d_err, done);
     else if (staged)
-      hipLaunchKernelGGL(k_flux_staged<false>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, c->d_V, c->d_grad,
-                         c->fc, c->d_Fm, c->d_P, c->d_err, done);
+      hipLaunchKernelGGL(k_flux_staged<false>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, b.V, b.grad,
+                         c->fc, b.Fm, b.P, c->d_err, done);
     else if (c->p.second_order && t.fblock == kFluxBlockSmall)
-      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t, c->d_V,
-                         c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err, done);
+      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t, b.V,
+                         b.grad, c->fc, b.Fm, b.P, c->d_err, done);
     else if (c->p.second_order)
-      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, c->d_V, c->d_grad,
-                         c->fc, c->d_Fm, c->d_P, c->d_err, done);
+      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, b.V, b.grad,
+                         c->fc, b.Fm, b.P, c->d_err, done);
     else if (t.fblock == kFluxBlockSmall)
       hipLaunchKernelGGL((k_flux_edges<false, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t,
-                         c->d_V, c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err, done);
+                         b.V, b.grad, c->fc, b.Fm, b.P, c->d_err, done);
     else
-      hipLaunchKernelGGL((k_flux_edges<false, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, c->d_V,
-                         c->d_grad, c->fc, c->d_Fm, c->d_P, c->d_err, done);
+      hipLaunchKernelGGL((k_flux_edges<false, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, b.V,
+                         b.grad, c->fc, b.Fm, b.P, c->d_err, done);
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_NODES);
-    hipLaunchKernelGGL(k_flux_nodes, dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, t, c->d_V,
-                       c->d_Fm, c->d_P, c->fc, scale ? 1 : 0, R, done);
+    hipLaunchKernelGGL(k_flux_nodes, dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, t, b.V,
+                       b.Fm, b.P, c->fc, scale ? 1 : 0, R, done);
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "residual launch");
+}
+
+int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done) {
+  const ResidualBufs b{c->d_V, c->d_grad, c->d_Fm, c->d_P};
+  return launch_residual_with(c, b, U, R, scale, done);
 }
 
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz) {

@@ -179,6 +179,18 @@ int aerofx_residual(aerofx_ctx *ctx, const double *U, double *R);
  * bit 1: do not synchronise or read the error count (returns 0; use
  * aerofx_error_count afterwards). */
 int aerofx_residual_ex(aerofx_ctx *ctx, const double *U, double *R, int flags);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  As aerofx_residual_ex (same flags, same kernels, results
+ * bit-identical), but the passes' scratch (V, gradients, upper-edge sums,
+ * edge fluxes: about 8 (26 N + 5 E) bytes) lives in one static device array
+ * of 64 MiB per GPU instead of the context's buffers, and the call allocates
+ * nothing.  A mesh that needs more than 64 MiB is rejected with
+ * AEROFX_ERR_ARG (about 137k nodes of a Kuhn mesh).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several contexts,
+ * several streams) race, so the caller serialises them.  The error count
+ * goes to the context as for aerofx_residual_ex. */
+int aerofx_residual_noalloc(aerofx_ctx *ctx, const double *U, double *R, int flags);
 int aerofx_error_count(aerofx_ctx *ctx, int *count);
 /* DistNodalGrad::compute with LeastSquares (DistNodalGrad.C:790-806,
  * SubDomain.C:297-374) on primitive V [N][5]: ddx, ddy, ddz [N][5]. */
