@@ -136,7 +136,7 @@ struct aerofx_ctx {
   double *d_Fm = nullptr;    // [E][5] edge fluxes in edge order
   double *d_P = nullptr;     // [N][5] per-node sums over upper edges
   int *d_err = nullptr;      // [0] reconstruction errors, [1] singular blocks, [2] bad states
-  int residual_kernels = 0;  // AEROFX_RESIDUAL_* (auto: LDS-staged when the mesh fits)
+  int residual_kernels = AEROFX_RESIDUAL_PER_LANE;  // AEROFX_RESIDUAL_PER_LANE or _STAGED
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
   // with the reason in noalloc_err)

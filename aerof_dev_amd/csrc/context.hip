@@ -400,10 +400,12 @@ int aerofx_ctx_set_stream(aerofx_ctx *c, void *stream, int use_context_stream) {
 
 int aerofx_ctx_set_residual_kernels(aerofx_ctx *c, int family) {
   if (!c) return set_error(AEROFX_ERR_ARG, "set_residual_kernels: null ctx");
-  if (family != AEROFX_RESIDUAL_AUTO && family != AEROFX_RESIDUAL_PER_LANE)
-    return set_error(AEROFX_ERR_ARG, "set_residual_kernels: AEROFX_RESIDUAL_AUTO or AEROFX_RESIDUAL_PER_LANE");
-  c->residual_kernels = family;
-  return family == AEROFX_RESIDUAL_AUTO && c->tab.ngb > 0 ? AEROFX_RESIDUAL_STAGED : AEROFX_RESIDUAL_PER_LANE;
+  if (family != AEROFX_RESIDUAL_AUTO && family != AEROFX_RESIDUAL_PER_LANE && family != AEROFX_RESIDUAL_STAGED)
+    return set_error(AEROFX_ERR_ARG,
+                     "set_residual_kernels: AEROFX_RESIDUAL_AUTO, AEROFX_RESIDUAL_PER_LANE or AEROFX_RESIDUAL_STAGED");
+  c->residual_kernels = family == AEROFX_RESIDUAL_STAGED && c->tab.ngb > 0 ? AEROFX_RESIDUAL_STAGED
+                                                                            : AEROFX_RESIDUAL_PER_LANE;
+  return c->residual_kernels;
 }
 
 int aerofx_ctx_sync(aerofx_ctx *c) {

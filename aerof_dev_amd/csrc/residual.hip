@@ -508,7 +508,7 @@ __global__ void k_zero_int(int *p, int n, const int *done) {
 int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale,
                          const int *done) {
   const DevTables &t = c->tab;
-  const bool staged = t.ngb > 0 && c->residual_kernels != AEROFX_RESIDUAL_PER_LANE;
+  const bool staged = t.ngb > 0 && c->residual_kernels == AEROFX_RESIDUAL_STAGED;
   if (c->p.second_order) {
     ProfScope ps(c, AEROFX_TIMER_NODAL_GRAD);
     if (staged)

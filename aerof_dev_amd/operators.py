@@ -103,9 +103,10 @@ class SpaceOperator:
         check(lib().aerofx_ctx_sync(self._h), "sync")
 
     def set_residual_kernels(self, family: str = "auto") -> str:
-        """'auto' (LDS-staged strips when the mesh fits) or 'per_lane';
-        returns the family the next residual uses ('staged' or 'per_lane')."""
-        code = {"auto": 0, "per_lane": 1}[family]
+        """'per_lane' (the default; 'auto' selects it) or 'staged' (LDS-staged
+        strips when the mesh fits them); returns the family the next residual
+        uses ('staged' or 'per_lane')."""
+        code = {"auto": 0, "per_lane": 1, "staged": 2}[family]
         rc = check(lib().aerofx_ctx_set_residual_kernels(self._h, code), "set_residual_kernels")
         return {1: "per_lane", 2: "staged"}[rc]
 

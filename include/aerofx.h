@@ -155,12 +155,14 @@ int aerofx_ctx_timers(aerofx_ctx *ctx, double *ms, long long *counts);
 
 /* Kernel family of the residual passes (no counterpart in the reference; the
  * results are bit-identical either way):
- *   AEROFX_RESIDUAL_AUTO      LDS-staged strips when the mesh fits them
+ *   AEROFX_RESIDUAL_PER_LANE  one lane per node or edge gathering from HBM
+ *                             (the default, and what AEROFX_RESIDUAL_AUTO
+ *                             selects: the faster family on the 128^3 box)
+ *   AEROFX_RESIDUAL_STAGED    LDS-staged strips, when the mesh fits them
  *                             (each node with its neighbours within 960
  *                             staged nodes; each node with at most 256 upper
  *                             edges, touching at most 191 other nodes through
- *                             them), otherwise per-lane gathers
- *   AEROFX_RESIDUAL_PER_LANE  one lane per node or edge gathering from HBM
+ *                             them); otherwise per-lane
  * Returns the family the next residual will use (AEROFX_RESIDUAL_STAGED or
  * AEROFX_RESIDUAL_PER_LANE), or < 0. */
 #define AEROFX_RESIDUAL_AUTO 0

@@ -23,7 +23,7 @@ def _both(gpu, mesh, params, U):
     op = SpaceOperator(mesh, params)
     Ud = torch.from_numpy(U).to(gpu)
     out = {}
-    for fam in ("per_lane", "auto"):
+    for fam in ("per_lane", "staged"):
         used = op.set_residual_kernels(fam)
         Rd = torch.full_like(Ud, float("nan"))
         ierr = op.compute_residual(Ud, Rd)
