@@ -18,6 +18,9 @@ AEROFX_BC_INLET = 4
 AEROFX_BC_OUTLET = 5
 AEROFX_MVP_H1 = 0
 AEROFX_MVP_FD = 1
+AEROFX_MVP_H2 = 2
+AEROFX_PREC_JACOBI = 0
+AEROFX_PREC_RAS_ILU0 = 1
 AEROFX_ERR_ARG = -1
 AEROFX_ERR_HIP = -2
 AEROFX_ERR_MESH = -3
@@ -30,6 +33,9 @@ AEROFX_PHYS_ROEJAC6_SW1 = 3
 AEROFX_PHYS_ROEJAC6_SW2 = 4
 AEROFX_PHYS_INVERT5_F64 = 5
 AEROFX_PHYS_INVERT5_F32 = 6
+AEROFX_PHYS_ROEFLUX5 = 7
+AEROFX_PHYS_ROEJAC6_LM_SW1 = 8
+AEROFX_PHYS_ROEJAC6_LM_SW2 = 9
 TIMER_NAMES = ("nodalGrad", "fvTerm", "fvTermNodes", "fvJac", "fluidPrecSetup", "jacApply",
                "precApply", "gramSchmidt")
 
@@ -65,12 +71,23 @@ class Mesh(C.Structure):
     ]
 
 
+class LowMachC(C.Structure):
+    _fields_ = [("min_mach", C.c_double), ("slope", C.c_double), ("cutoff", C.c_double),
+                ("shock_reducer", C.c_double), ("irey", C.c_double)]
+
+
+class SowerHeader(C.Structure):
+    _fields_ = [("num_nodes", C.c_int), ("num_tets", C.c_int), ("num_faces", C.c_int),
+                ("num_clus_nodes", C.c_int), ("num_sub_cluster", C.c_int), ("num_neighbours", C.c_int),
+                ("num_bcs", C.c_int), ("version", C.c_double)]
+
+
 class NewtonOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("max_steps", C.c_int), ("eps", C.c_double),
                 ("cfl0", C.c_double), ("cfl_max", C.c_double), ("ser", C.c_double),
                 ("cfl_coef1", C.c_double), ("cfl_coef2", C.c_double),
                 ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double),
-                ("mvp", C.c_int), ("fd_order", C.c_int)]
+                ("mvp", C.c_int), ("fd_order", C.c_int), ("prec", C.c_int), ("prec_subdomain", C.c_int)]
 
 
 class BlockStats(C.Structure):
@@ -113,6 +130,7 @@ SIGNATURES = {
     "aerofx_ctx_set_stream": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "aerofx_ctx_sync": (C.c_int, [C.c_void_p]),
     "aerofx_ctx_set_residual_kernels": (C.c_int, [C.c_void_p, C.c_int]),
+    "aerofx_ctx_set_low_mach": (C.c_int, [C.c_void_p, C.POINTER(LowMachC)]),
     "aerofx_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_timers": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_longlong)]),
     "aerofx_ctx_tables": (C.c_int, [C.c_void_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p]),
@@ -132,6 +150,7 @@ SIGNATURES = {
     "aerofx_jvp_fd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "aerofx_mvp_select": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "aerofx_mvp_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_prec_select": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "aerofx_jvp_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_prec_apply_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_gmres_transpose": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
@@ -149,6 +168,14 @@ SIGNATURES = {
     "aerofx_residual_derivative": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "aerofx_steady_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(NewtonOpts), C.POINTER(NewtonStats),
                                       c_double_p, C.c_int]),
+    "aerofx_sower_info": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(SowerHeader)]),
+    "aerofx_sower_read": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, c_double_p, c_int_p, c_int_p, c_int_p,
+                                    c_int_p, c_int_p]),
+    "aerofx_sower_write": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, c_double_p, c_int_p,
+                                     c_int_p, c_int_p]),
+    "aerofx_solution_info": (C.c_int, [C.c_char_p, c_int_p, c_int_p, c_int_p]),
+    "aerofx_solution_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_int, C.c_int, c_double_p]),
+    "aerofx_solution_read": (C.c_int, [C.c_char_p, C.c_int, c_double_p, C.c_int, C.c_int, c_int_p, c_double_p]),
     "aerofx_physics_eval": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p]),
 }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -94,6 +95,8 @@ struct FluxConsts {
   Gas gas;
   double beta;
   double ub_in[5], ub_out[5];
+  int low_mach;  // 1: spatial low-Mach preconditioning (ROEFLUX5 and its ROEJAC6)
+  LowMach lm;
 };
 
 int set_hip_error(hipError_t e, const char *what);
@@ -101,6 +104,8 @@ struct Krylov;
 struct Adjoint;
 struct BlockSolver;
 struct Tangent;
+struct Ilu;
+struct FgWork;
 
 // Element tables of the geometry derivatives (device copies, built on first
 // use): the adjoint and the tangent passes sum per-element contributions per
@@ -161,14 +166,17 @@ struct aerofx_ctx {
   int mvp_type = AEROFX_MVP_H1, mvp_order = 1;
   const double *mvp_Q = nullptr, *mvp_F0 = nullptr, *mvp_dt = nullptr;
   double *d_fd = nullptr;
-  // direct sensitivity solves: residuals and corrections of up to 16
-  // right-hand sides plus one product vector (allocated on first use)
-  double *d_direct = nullptr;
   double *d_dt = nullptr;
   aerofx::Krylov *ksp = nullptr;
   aerofx::Adjoint *adj = nullptr;
   aerofx::ElemTables *elem = nullptr;
   aerofx::Tangent *tan = nullptr;
+  // preconditioner of the Krylov solves (AEROFX_PREC_*), taking effect at the
+  // next Jacobian assembly; the RAS / ILU(0) subdomain size in nodes
+  int prec_type = AEROFX_PREC_JACOBI;
+  int prec_subdomain = 256;
+  aerofx::Ilu *ilu = nullptr;
+  aerofx::FgWork *fgm = nullptr;  // flexible GMRES workspace (fgmres.hip)
   aerofx::BlockSolver *blk = nullptr;
   int blk_tune[2] = {0, 0};  // autotuned waves per workgroup of the block GEMMs
   // per-kernel timers (the reference's Timer buckets, Timer.h:12-15)
@@ -195,8 +203,9 @@ struct ProfScope {
   ~ProfScope();
 };
 
-// device allocation owned by the context
+// device allocation owned by the context, and its early release
 int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out);
+void ctx_release(aerofx_ctx *c, void *p);
 
 __global__ void k_zero_int(int *p, int n, const int *done);
 
@@ -239,12 +248,41 @@ int launch_dcgs_update(aerofx_ctx *c, double *V, size_t ldv, int j, const double
 void free_krylov(aerofx_ctx *c);
 void free_adjoint(aerofx_ctx *c);
 void free_tangent(aerofx_ctx *c);
+// RAS / ILU(0) (ilu.hip): tables for subdomains of B nodes, numericILU of the
+// current H1 (zero pivots counted in d_err[1]), and the solve x = M^-1 y
+// (returning at once when done != nullptr and *done != 0)
+void free_ilu(aerofx_ctx *c);
+// flexible GMRES over nrhs right-hand sides rhs, x [nrhs][5N] (fgmres.hip):
+// apply(x, y, nq): y_q = A x_q for q < nq; prec(v, z, nq, &inner): z_q ~
+// M^-1 v_q (any, possibly varying, approximate inverse), adding its inner
+// iterations.  Stops when every ||rhs_q - A x_q|| <= eps ||rhs_q|| or after
+// max_outer steps; outer restart mo (<= 31).
+struct FgOps {
+  std::function<int(const double *, double *, int)> apply;
+  std::function<int(const double *, double *, int, int *)> prec;
+};
+struct FgResult {
+  int converged = 0, outer_its = 0, inner_its = 0;
+  double res0 = 0.0, res = 0.0;  // the worst right-hand side: ||rhs||, ||rhs - A x||
+};
+int fgmres_solve(aerofx_ctx *c, int nrhs, const double *rhs, double *x, int mo, double eps, int max_outer,
+                 const FgOps &ops, FgResult *res);
+void free_fgmres(aerofx_ctx *c);
+int ensure_ilu(aerofx_ctx *c, int B);
+int launch_ilu_factor(aerofx_ctx *c);
+int launch_ilu_solve(aerofx_ctx *c, const double *y, double *x, const int *done);
 // c->elem (adjoint.hip)
 int ensure_elem_tables(aerofx_ctx *c);
 // dR = (dR/dU) dU + (dR/dX) dX at U (tangent.hip; dU, dX may be NULL),
 // enqueued on c->stream; the forward residual it runs leaves R (raw flux
 // sums) in the tangent workspace
 int launch_tangent(aerofx_ctx *c, const double *U, const double *dU, const double *dX, double *dR);
+// the exact product operator (tangent.hip): select keeps the forward pass at
+// Q; apply gives y = (dR/dU) p + p / dt (dt may be NULL), every kernel
+// returning at once when done != nullptr and *done != 0
+int launch_h2_select(aerofx_ctx *c, const double *Q);
+int launch_h2_apply(aerofx_ctx *c, const double *Q, const double *p, const double *dt, double *y,
+                    const int *done);
 // (dR/dU)^T lam -> Ub and/or (dR/dX)^T lam -> Xout (adjoint.hip), enqueued on c->stream
 int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout);
 void free_block(aerofx_ctx *c);

@@ -32,6 +32,13 @@ int ctx_alloc(aerofx_ctx *c, size_t bytes, void **out) {
   return 0;
 }
 
+void ctx_release(aerofx_ctx *c, void *p) {
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it == c->allocs.end()) return;
+  (void)hipFree(p);
+  c->allocs.erase(it);
+}
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -117,6 +124,8 @@ void destroy(aerofx_ctx *c) {
   free_krylov(c);
   free_adjoint(c);
   free_tangent(c);
+  free_ilu(c);
+  free_fgmres(c);
   delete c->elem;  // device memory is in c->allocs
   free_block(c);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -406,6 +415,21 @@ int aerofx_ctx_set_residual_kernels(aerofx_ctx *c, int family) {
   c->residual_kernels = family == AEROFX_RESIDUAL_STAGED && c->tab.ngb > 0 ? AEROFX_RESIDUAL_STAGED
                                                                             : AEROFX_RESIDUAL_PER_LANE;
   return c->residual_kernels;
+}
+
+int aerofx_ctx_set_low_mach(aerofx_ctx *c, const aerofx_low_mach *lm) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "set_low_mach: null ctx");
+  if (!lm) {
+    c->fc.low_mach = 0;
+    return AEROFX_OK;
+  }
+  if (!(lm->min_mach > 0.0) || !(lm->cutoff >= lm->min_mach) || !(lm->slope >= 0.0) ||
+      !(lm->shock_reducer >= 0.0) || !(lm->irey >= 0.0))
+    return set_error(AEROFX_ERR_ARG,
+                     "set_low_mach: need 0 < min_mach <= cutoff, slope >= 0, shock_reducer >= 0, irey >= 0");
+  c->fc.low_mach = 1;
+  c->fc.lm = LowMach{lm->min_mach, lm->slope, lm->cutoff, lm->shock_reducer, lm->irey};
+  return AEROFX_OK;
 }
 
 int aerofx_ctx_sync(aerofx_ctx *c) {

@@ -1,6 +1,6 @@
 // device_jacobian.h -- flux Jacobians as gfx950 device code.
 //
-//   roejac6      ROEJAC6, type 0, prec 0 (f77src/roejac6.f:1-316):
+//   roejac6      ROEJAC6, type 0, prec 0 or 1 (f77src/roejac6.f:1-316):
 //                0.5 |n| (A(Ug) +/- |A_roe(Ug,Ud)|), row-major 5x5
 //   post_dVdU    VarFcnSGEuler::computedVdU + VarFcnBase::postMultiplyBydVdU
 //                (VarFcnSGEuler.h:588-603, VarFcnBase.h:383-411)
@@ -14,8 +14,11 @@
 
 namespace aerofx {
 
+// With prec, Turkel's beta of the edge (turkel_beta, the law of
+// roejac6.f:135-143) replaces 1: the Jacobian of the low-Mach flux ROEFLUX5.
 __device__ __forceinline__ void roejac6(double gamma_diss, const Gas &g, const double en[3], double evitno,
-                                        const double Ug[5], const double Ud[5], int sw, double jac[25]) {
+                                        const double Ug[5], const double Ud[5], int sw, double jac[25],
+                                        bool prec = false, const LowMach &lm = LowMach{}, double length = 1.0) {
   const double gam = g.gam, gam1 = g.gam1, pstiff = g.pstiff;
   double H[25];
   double rnorm = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
@@ -63,7 +66,11 @@ __device__ __forceinline__ void roejac6(double gamma_diss, const Gas &g, const d
   double cr = sqrt(cr2);
   double gc = gam1 / (cr * cr);
   double usc2 = 0.5 / (cr * cr);
-  const double betaRT2 = 1.0;  // beta = 1 without low-Mach preconditioning
+  double betaRT2 = 1.0;  // beta = 1 without low-Mach preconditioning
+  if (prec) {
+    const double beta = turkel_beta(lm, Ug[4], Ud[4], length, qir, 1.0 / cr2);
+    betaRT2 = beta * beta;
+  }
   double vp1 = VdotN;
   double tt = (1.0 - betaRT2) * VdotN;
   double root = sqrt(tt * tt + 4.0 * betaRT2 * (cr * cr));

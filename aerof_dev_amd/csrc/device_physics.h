@@ -8,6 +8,10 @@
 //   roeflux7        ROEFLUX7, type 0 (f77src/roeflux7.f:26-187)
 //   boundflux5      BOUNDFLUX5, type 0 (f77src/boundflux5.f:23-98)
 //   wallflux        FluxFcnSGWallEuler3D::compute (FluxFcnDescSG.C:779-789)
+//   roe_turkel_flux ROEFLUX5, type 0, gamma = 1, Ugr = Ug, Udr = Ud
+//                   (f77src/roeflux5.f; called by FluxFcnSGApprJacRoeEuler3D::
+//                   compute, FluxFcnDescSG.C:125-139, when the spatial
+//                   low-Mach preconditioner is on)
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -183,6 +187,118 @@ __device__ __forceinline__ void boundflux5(const Gas &g, const T nv[3], double s
   flux[2] = fgp2 + (vp1 * ub[2] + v * ff + capd2 * gg);
   flux[3] = fgp3 + (vp1 * ub[3] + w * ff + capd3 * gg);
   flux[4] = fgp4 + (vp1 * ub[4] + cc * ff + unx * gg);
+}
+
+// Spatial low-Mach preconditioner constants (SpatialLowMachPrec,
+// LowMachPrec.h): minimum Mach, slope k1 on the local Mach number, cut-off
+// Mach, shock reducer, and the 1/Re of the edge (0 for inviscid flow).
+struct LowMach {
+  double min_mach, slope, cutoff, shock, irey;
+};
+
+// Turkel's preconditioning parameter for the edge: beta = min((1 + sqrt(1/Re))
+// max(k1 M_roe, M_min) + c_shock |p_g - p_d| / ((p_g + p_d) L), M_cut).
+__host__ __device__ __forceinline__ double turkel_beta(const LowMach &lm, double pg, double pd, double length,
+                                                       double q2, double inv_c2) {
+  const double shock = fabs(pg - pd) / (pg + pd) / length;
+  const double mroe = sqrt(2.0 * q2 * inv_c2);
+  double b = fmax(lm.slope * mroe, lm.min_mach);
+  b = (1.0 + sqrt(lm.irey)) * b + lm.shock * shock;
+  return fmin(b, lm.cutoff);
+}
+
+// Roe flux with Turkel-preconditioned dissipation through area vector en
+// (not normalised), static face, for primitive states Ug (i side), Ud (j
+// side):  F = 1/2 |n| [F(Ug).n + F(Ud).n - P^-1 |P A_roe| (Ud - Ug)].
+// The preconditioned system has one convective eigenvalue u_n (entropy and
+// two shear waves) and two acoustic ones
+//   lambda_+- = 1/2 ((1 + beta^2) u_n +- sqrt(((1 - beta^2) u_n)^2 + 4 beta^2 c^2)),
+// with r, s = lambda_+- - beta^2 u_n and t = 1/2 (lambda_- - lambda_+); with
+// beta = 1 it is Roe's flux.  Each intermediate is evaluated in the order
+// of roeflux5.f so that, built without FMA contraction, the result matches
+// the reference's bit for bit.
+__host__ __device__ __forceinline__ void roe_turkel_flux(const Gas &g, const LowMach &lm, bool prec,
+                                                         const double en[3], double length, const double Ug[5],
+                                                         const double Ud[5], double phi[5]) {
+  const double gam = g.gam, gam1 = g.gam1, pstiff = g.pstiff;
+  const double rnorm = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+  const double invnorm = 1.0 / rnorm;
+  const double n[3] = {en[0] * invnorm, en[1] * invnorm, en[2] * invnorm};
+  // skew[d][k] = (n x e_k)_d style rotation terms of the shear waves
+  const double skew[3][3] = {{0.0, n[2], -n[1]}, {-n[2], 0.0, n[0]}, {n[1], -n[0], 0.0}};
+  // centred part: F(Ug).n + F(Ud).n
+  double q2g = Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3];
+  double q2d = Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3];
+  {
+    const double ung = Ug[1] * n[0] + Ug[2] * n[1] + Ug[3] * n[2];
+    const double Hg = (gam * (Ug[4] + pstiff) + 0.5 * gam1 * Ug[0] * q2g) / (gam1 * Ug[0]);
+    const double und = Ud[1] * n[0] + Ud[2] * n[1] + Ud[3] * n[2];
+    const double Hd = (gam * (Ud[4] + pstiff) + 0.5 * gam1 * Ud[0] * q2d) / (gam1 * Ud[0]);
+    const double mg = Ug[0] * ung;
+    phi[0] = mg + Ud[0] * und;
+    for (int d = 0; d < 3; ++d) phi[1 + d] = (mg * Ug[1 + d] + Ug[4] * n[d]) + Ud[0] * Ud[1 + d] * und + Ud[4] * n[d];
+    phi[4] = mg * Hg + Ud[0] * und * Hd;
+  }
+  // Roe average
+  const double sg = sqrt(Ug[0]), sd = sqrt(Ud[0]);
+  const double Eg = (Ug[4] + gam * pstiff) / gam1 + 0.5 * Ug[0] * q2g;
+  const double Ed = (Ud[4] + gam * pstiff) / gam1 + 0.5 * Ud[0] * q2d;
+  const double inv_s = 1.0 / (sg + sd);
+  double ua[3];
+  for (int d = 0; d < 3; ++d) ua[d] = (sg * Ug[1 + d] + sd * Ud[1 + d]) * inv_s;
+  const double Ha = ((Eg + Ug[4]) / sg + (Ed + Ud[4]) / sd) * inv_s;
+  const double un = n[0] * ua[0] + n[1] * ua[1] + n[2] * ua[2];
+  const double q2 = 0.5 * (ua[0] * ua[0] + ua[1] * ua[1] + ua[2] * ua[2]);
+  const double tet[3] = {n[2] * ua[1] - n[1] * ua[2], n[0] * ua[2] - n[2] * ua[0], n[1] * ua[0] - n[0] * ua[1]};
+  const double c2 = gam1 * (Ha - q2);
+  const double c = sqrt(c2);
+  const double ic2 = 1.0 / c2;
+  const double beta = prec ? turkel_beta(lm, Ug[4], Ud[4], length, q2, ic2) : 1.0;
+  const double b2 = beta * beta;
+  // jump in the conservative variables
+  double dU[5];
+  dU[0] = -Ug[0] + Ud[0];
+  for (int d = 0; d < 3; ++d) dU[1 + d] = -Ug[0] * Ug[1 + d] + Ud[0] * Ud[1 + d];
+  {
+    const double eg = (Ug[4] + gam * pstiff) / gam1 + 0.5 * Ug[0] * (Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3]);
+    const double ed = (Ud[4] + gam * pstiff) / gam1 + 0.5 * Ud[0] * (Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3]);
+    dU[4] = -eg + ed;
+  }
+  // eigenvalues of the preconditioned system
+  const double sq = ((1.0 - b2) * un) * ((1.0 - b2) * un);
+  const double lp = 0.5 * ((1.0 + b2) * un + sqrt(sq + 4.0 * b2 * (c * c)));
+  const double lm_ = 0.5 * ((1.0 + b2) * un - sqrt(sq + 4.0 * b2 * (c * c)));
+  const double r = lp - un * b2, s = lm_ - un * b2, t = 0.5 * (lm_ - lp);
+  const double a0 = fabs(un), ap = fabs(lp), am = fabs(lm_);
+  // wave strengths times |eigenvalue|: entropy + shear (3), acoustic (2)
+  double w[3];
+  for (int d = 0; d < 3; ++d) {
+    double acc = (n[d] * (1.0 - gam1 * q2 * ic2) - tet[d]) * dU[0];
+    for (int k = 0; k < 3; ++k) {
+      const double base = n[d] * gam1 * ua[k] * ic2;
+      acc = acc + (k == d ? base : skew[d][k] + base) * dU[1 + k];
+    }
+    w[d] = a0 * (acc - (n[d] * gam1 * ic2) * dU[4]);
+  }
+  const double cc2 = c * c;
+  double wp = (cc2 * un / t + gam1 * q2 * s / (t * b2)) * dU[0];
+  for (int k = 0; k < 3; ++k) wp = wp - (cc2 * n[k] / t + gam1 * ua[k] * s / (t * b2)) * dU[1 + k];
+  wp = ap * (wp + gam1 * s / (t * b2) * dU[4]);
+  double wm = -(cc2 * un / t + gam1 * q2 * r / (b2 * t)) * dU[0];
+  for (int k = 0; k < 3; ++k) wm = wm + (cc2 * n[k] / t + gam1 * ua[k] * r / (b2 * t)) * dU[1 + k];
+  wm = am * (wm - gam1 * r / (b2 * t) * dU[4]);
+  // dissipation: right eigenvectors times the wave strengths
+  const double sum = wp + wm, rs = r * wp + s * wm;
+  phi[0] = phi[0] - (n[0] * w[0] + n[1] * w[1] + n[2] * w[2] + 0.5 * sum * ic2);
+  for (int m = 0; m < 3; ++m) {
+    double acc = (m == 0 ? ua[m] * n[0] : ua[m] * n[0] + skew[0][m]) * w[0];
+    acc = acc + (m == 1 ? ua[m] * n[1] : ua[m] * n[1] + skew[1][m]) * w[1];
+    acc = acc + (m == 2 ? ua[m] * n[2] : ua[m] * n[2] + skew[2][m]) * w[2];
+    phi[1 + m] = phi[1 + m] - (acc + (n[m] * rs + ua[m] * sum) * 0.5 * ic2);
+  }
+  phi[4] = phi[4] - ((q2 * n[0] + tet[0]) * w[0] + (q2 * n[1] + tet[1]) * w[1] + (q2 * n[2] + tet[2]) * w[2] +
+                     (un * rs + Ha * sum) * 0.5 * ic2);
+  for (int k = 0; k < 5; ++k) phi[k] = phi[k] * 0.5 * rnorm;
 }
 
 template <class T>

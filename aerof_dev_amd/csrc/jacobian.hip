@@ -96,7 +96,8 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *
 // SIMD; asking for two waves spills ~18 registers to scratch and is faster
 // (assembly 50 -> 38 ms over a 96x32x32 FP64 steady solve; three waves spill
 // 116 and are slower).
-template <class T>
+// LM: the low-Mach (Roe-Turkel) edge blocks.
+template <class T, bool LM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
                                                      const double *__restrict__ dt, double *__restrict__ fjac,
                                                      T *__restrict__ D, T *__restrict__ B) {
@@ -135,21 +136,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         Vm[k] = V[5 * (size_t)m + k];
       }
       const double voln = t.invcv[n];
+      // low-Mach: the edge length |X_j - X_i| of Turkel's beta (Edge.C:4493-4494)
+      double length = 1.0;
+      if (LM) {
+        const double4 xn = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)n);
+        const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
+        const double d0 = n < m ? xm.x - xn.x : xn.x - xm.x, d1 = n < m ? xm.y - xn.y : xn.y - xm.y,
+                     d2 = n < m ? xm.z - xn.z : xn.z - xm.z;
+        length = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      }
+      const bool lm = LM;
       double blk[25];
       if (n < m) {
         // n = i: A_ii += dF/dU_i, A_ij = dF/dU_j / V_i
-        roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk);
+        roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk, lm, fc.lm, length);
 #pragma unroll
         for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = blk[k];
-        roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk);
+        roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk, lm, fc.lm, length);
 #pragma unroll
         for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 + blk[k] * voln);
       } else {
         // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
-        roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk);
+        roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk, lm, fc.lm, length);
 #pragma unroll
         for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = -blk[k];
-        roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk);
+        roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk, lm, fc.lm, length);
 #pragma unroll
         for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 - blk[k] * voln);
       }
@@ -480,12 +491,19 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   const int nchunks = c->tab.ngchunks;
   {
     ProfScope ps(c, AEROFX_TIMER_FV_JAC);
-    if (precision)
-      hipLaunchKernelGGL((k_jacobian<float>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
-                         c->d_fjac, (float *)c->d_jacD, (float *)c->d_jacB);
+    const bool lm = c->fc.low_mach != 0;
+    if (precision && lm)
+      hipLaunchKernelGGL((k_jacobian<float, true>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
+                         c->fc, dt, c->d_fjac, (float *)c->d_jacD, (float *)c->d_jacB);
+    else if (precision)
+      hipLaunchKernelGGL((k_jacobian<float, false>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
+                         c->fc, dt, c->d_fjac, (float *)c->d_jacD, (float *)c->d_jacB);
+    else if (lm)
+      hipLaunchKernelGGL((k_jacobian<double, true>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
+                         c->fc, dt, c->d_fjac, (double *)c->d_jacD, (double *)c->d_jacB);
     else
-      hipLaunchKernelGGL((k_jacobian<double>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc,
-                         dt, c->d_fjac, (double *)c->d_jacD, (double *)c->d_jacB);
+      hipLaunchKernelGGL((k_jacobian<double, false>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
+                         c->fc, dt, c->d_fjac, (double *)c->d_jacD, (double *)c->d_jacB);
   }
   if ((rc = check_launch("jacobian launch"))) return rc;
   {
@@ -497,8 +515,13 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
     else
       hipLaunchKernelGGL((k_invert<double>), dim3(nb), dim3(kBlock), 0, c->stream, c->N,
                          (const double *)c->d_jacD, (double *)c->d_jacDi, c->d_err + 1);
+    if ((rc = check_launch("invert launch"))) return rc;
+    // RAS / ILU(0): factors of the same assembly, in its precision
+    if (c->prec_type == AEROFX_PREC_RAS_ILU0) {
+      if ((rc = ensure_ilu(c, c->prec_subdomain)) || (rc = launch_ilu_factor(c))) return rc;
+    }
   }
-  return check_launch("invert launch");
+  return 0;
 }
 
 int launch_invert_f32(aerofx_ctx *c, hipStream_t stream, const float *D, float *Di) {
@@ -633,6 +656,7 @@ int launch_dcgs_update(aerofx_ctx *c, double *V, size_t ldv, int j, const double
 
 int launch_prec_apply(aerofx_ctx *c, const double *y, double *x, bool trans) {
   ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
+  if (!trans && c->prec_type == AEROFX_PREC_RAS_ILU0) return launch_ilu_solve(c, y, x, nullptr);
   return launch_prec_apply_flag(c, y, x, nullptr, trans);
 }
 

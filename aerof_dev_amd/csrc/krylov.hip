@@ -467,7 +467,8 @@ __global__ __launch_bounds__(kReduceThreads) void k_dcgs_reduce(const double *__
   if (sbreak) givens_col(st, j, col, gstage);  // sets the stop flag (zero subdiagonal)
 }
 
-// y = a x + b y, a possibly scaled by a device scalar (*as)
+// y = a x + b y, a possibly scaled by a device scalar (*as).  With b = 0, y
+// is not read (it may be fresh memory holding NaN, and 0 * NaN is NaN).
 __global__ __launch_bounds__(kBlock) void k_axpby(size_t n, double a, const double *__restrict__ as,
                                                   const double *__restrict__ x, double b, double *__restrict__ y,
                                                   const int *__restrict__ done) {
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_axpby(size_t n, double a, const doub
   const size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x;
   if (i >= n) return;
   const double aa = as ? a * (*as) : a;
-  y[i] = aa * x[i] + b * y[i];
+  y[i] = b == 0.0 ? aa * x[i] : aa * x[i] + b * y[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(size_t n, double v, double *__restrict__ y) {
@@ -772,6 +773,7 @@ static double *fd_own_F0(aerofx_ctx *c) { return c->d_fd + 3 * 5 * (size_t)c->N;
 // y = A p with the selected MatVecProd (transposed solves: always H1^T).
 static int launch_operator(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
   if (trans || c->mvp_type == AEROFX_MVP_H1) return launch_h1_apply_flag(c, p, y, done, trans);
+  if (c->mvp_type == AEROFX_MVP_H2) return launch_h2_apply(c, c->mvp_Q, p, c->mvp_dt, y, done);
   FdScratch s;
   int rc = ensure_fd(c, &s);
   return rc ? rc
@@ -802,6 +804,19 @@ static int select_fd(aerofx_ctx *c, const double *Q, const double *F0, const dou
   return e == hipSuccess ? 0 : set_hip_error(e, "mvp_select launch");
 }
 
+// MatVecProdH2::evaluate: selects the exact product about Q (dt NULL: no
+// time term).
+static int select_h2(aerofx_ctx *c, const double *Q, const double *dt) {
+  int rc = launch_h2_select(c, Q);
+  if (rc) return rc;
+  c->mvp_type = AEROFX_MVP_H2;
+  c->mvp_Q = Q;
+  c->mvp_F0 = nullptr;
+  c->mvp_dt = dt;
+  c->mvp_order = 1;
+  return 0;
+}
+
 static int fetch_state(aerofx_ctx *c) {
   hipError_t e = hipMemcpyAsync(c->ksp->h_st, c->ksp->st, sizeof(GmresState), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -825,6 +840,8 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
     ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
     if ((rc = launch_prec_norm_flag(c, k->r, &k->st->invhn, k->V, k->ldv, 0, nullptr, k->V, k->z, done, trans)))
       return rc;
+    // RAS / ILU(0) replaces the block-Jacobi result (forward solves only)
+    if (!trans && c->prec_type == AEROFX_PREC_RAS_ILU0 && (rc = launch_ilu_solve(c, k->V, k->z, done))) return rc;
   }
   for (int j = 0; j < m; ++j) {
     {
@@ -842,6 +859,9 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
       ProfScope ps(c, AEROFX_TIMER_PREC_APPLY);
       if ((rc = launch_dcgs_update(c, k->V, k->ldv, j, k->w, k->st->coef, &k->st->done, &k->st->jlast, k->z,
                                    trans)))
+        return rc;
+      if (!trans && c->prec_type == AEROFX_PREC_RAS_ILU0 &&
+          (rc = launch_ilu_solve(c, k->V + (size_t)(j + 1) * k->ldv, k->z, done)))
         return rc;
     }
   }
@@ -862,10 +882,20 @@ static int enqueue_arnoldi(aerofx_ctx *c, int m, bool trans) {
 static int arnoldi_cycle(aerofx_ctx *c, int m, bool trans) {
   Krylov *k = c->ksp;
   if (c->prof) return enqueue_arnoldi(c, m, trans);
-  const bool fd = !trans && c->mvp_type == AEROFX_MVP_FD;
-  const long long key = (((((long long)m * 2 + (trans ? 1 : 0)) * 2 + c->jac_precision) * 2 + (fd ? 1 : 0)) * 3 +
-                         (fd ? c->mvp_order : 0)) * 2 + c->residual_kernels;
-  const void *fdp[3] = {fd ? c->mvp_Q : nullptr, fd ? c->mvp_F0 : nullptr, fd ? c->mvp_dt : nullptr};
+  const int mvp = trans ? AEROFX_MVP_H1 : c->mvp_type;
+  const bool fd = mvp == AEROFX_MVP_FD, mf = mvp != AEROFX_MVP_H1;
+  const long long key =
+      ((((((((long long)m * 2 + (trans ? 1 : 0)) * 2 + c->jac_precision) * 3 + mvp) * 3 + (fd ? c->mvp_order : 0)) *
+             3 +
+         c->residual_kernels) *
+            2 +
+        c->prec_type) *
+           2 +
+       c->fc.low_mach) *
+          2048 +
+      (c->prec_type == AEROFX_PREC_RAS_ILU0 ? c->prec_subdomain : 0);
+  // the captured products read these buffers: a new selection re-captures
+  const void *fdp[3] = {mf ? c->mvp_Q : nullptr, fd ? c->mvp_F0 : nullptr, mf ? c->mvp_dt : nullptr};
   const bool same_fd = k->graph_fd[0] == fdp[0] && k->graph_fd[1] == fdp[1] && k->graph_fd[2] == fdp[2];
   hipError_t e = hipSuccess;
   if (!k->cap_stream) e = hipStreamCreateWithFlags(&k->cap_stream, hipStreamNonBlocking);
@@ -979,8 +1009,19 @@ int aerofx_mvp_select(aerofx_ctx *c, int type, const double *Q, const double *F0
     c->mvp_Q = c->mvp_F0 = c->mvp_dt = nullptr;
     return AEROFX_OK;
   }
-  if (type != AEROFX_MVP_FD) return set_error(AEROFX_ERR_ARG, "mvp_select: AEROFX_MVP_H1 or AEROFX_MVP_FD");
+  if (type != AEROFX_MVP_FD && type != AEROFX_MVP_H2)
+    return set_error(AEROFX_ERR_ARG, "mvp_select: AEROFX_MVP_H1, AEROFX_MVP_FD or AEROFX_MVP_H2");
   if (!Q) return set_error(AEROFX_ERR_ARG, "mvp_select: null state");
+  if (type == AEROFX_MVP_H2 && c->fc.low_mach)
+    return set_error(AEROFX_ERR_STATE, "mvp_select: exact products not available with low-Mach preconditioning");
+  if (type == AEROFX_MVP_H2) {
+    int rc = select_h2(c, Q, dt);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "mvp_select sync");
+    return c->h_err[0];
+  }
   if (order != 1 && order != 2) return set_error(AEROFX_ERR_ARG, "mvp_select: order 1 or 2");
   int rc = select_fd(c, Q, F0, dt, order);
   if (rc) return rc;
@@ -988,6 +1029,17 @@ int aerofx_mvp_select(aerofx_ctx *c, int type, const double *Q, const double *F0
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "mvp_select sync");
   return c->h_err[3];
+}
+
+int aerofx_prec_select(aerofx_ctx *c, int type, int subdomain_nodes) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "prec_select: null ctx");
+  if (type != AEROFX_PREC_JACOBI && type != AEROFX_PREC_RAS_ILU0)
+    return set_error(AEROFX_ERR_ARG, "prec_select: AEROFX_PREC_JACOBI or AEROFX_PREC_RAS_ILU0");
+  if (type == AEROFX_PREC_RAS_ILU0 && (subdomain_nodes < 1 || subdomain_nodes > 1024))
+    return set_error(AEROFX_ERR_ARG, "prec_select: subdomain size in [1, 1024]");
+  c->prec_type = type;
+  if (type == AEROFX_PREC_RAS_ILU0) c->prec_subdomain = subdomain_nodes;
+  return AEROFX_OK;
 }
 
 int aerofx_mvp_apply(aerofx_ctx *c, const double *p, double *prod) {
@@ -1008,67 +1060,63 @@ int aerofx_mvp_apply(aerofx_ctx *c, const double *p, double *prod) {
   return c->h_err[3];
 }
 
+// Outer basis size of the flexible GMRES of the derivative solves.
+constexpr int kDerivRestart = 20;
+
+// The inner operator of the derivative solves: H1 + diag(1/dt) at U (the
+// pseudo-time step of the given CFL; cfl <= 0: none) and its block-Jacobi
+// inverse in `precision`; the zero-pivot count is checked.
+static int derivative_operators(aerofx_ctx *c, const double *U, double cfl, int precision, const char *who) {
+  int rc;
+  if ((rc = launch_primitive(c, U, c->d_V))) return rc;
+  if (cfl > 0.0 && (rc = launch_timestep(c, cfl, c->d_dt))) return rc;
+  if ((rc = launch_jacobian(c, cfl > 0.0 ? c->d_dt : nullptr, precision))) return rc;
+  hipError_t e = hipMemcpyAsync(c->h_err + 1, c->d_err + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, who);
+  if (c->h_err[1]) return set_error(AEROFX_ERR_STATE, std::string(who) + ": zero pivot in a diagonal block");
+  return 0;
+}
+
 int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, double *psi,
                          const aerofx_adjoint_opts *o, aerofx_adjoint_stats *st) {
   if (!c || !U || !rhs || !psi || !o || !st) return set_error(AEROFX_ERR_ARG, "adjoint_solve: null argument");
+  if (c->fc.low_mach)
+    return set_error(AEROFX_ERR_STATE, "adjoint_solve: not available with low-Mach preconditioning");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "adjoint_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
-  Krylov *k = c->ksp;
-  const size_t n = k->n;
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_adjoint_stats{};
-  // A = H1 + diag(1/dt) at U (pseudo-time step of the given CFL), its
-  // block-Jacobi inverse in the requested precision
-  if ((rc = launch_primitive(c, U, c->d_V))) return rc;
-  if (o->cfl > 0.0) {
-    if ((rc = launch_timestep(c, o->cfl, c->d_dt))) return rc;
-  }
-  if ((rc = launch_jacobian(c, o->cfl > 0.0 ? c->d_dt : nullptr, o->precision))) return rc;
-  // ||rhs|| and the block inversion's zero-pivot count
-  norm2(c, rhs, k->scal);
-  hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(c->h_err + 1, c->d_err + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
-  if (c->h_err[1]) return set_error(AEROFX_ERR_STATE, "adjoint_solve: zero pivot in a diagonal block");
-  const double rnorm0 = std::sqrt(k->h_scal[0]);
-  hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, c->stream, n, 0.0, psi);
-  double res = rnorm0;
-  int it;
-  for (it = 0;; ++it) {
-    // r = rhs - (dR/dU)^T psi (FP64, exact second-order adjoint)
-    if (it == 0) {
-      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, rhs, 0.0, k->b,
-                         nullptr);
-    } else {
-      if ((rc = launch_adjoint(c, U, psi, k->R, nullptr))) return rc;
-      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, rhs, 0.0, k->b,
-                         nullptr);
-      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 1.0, k->b,
-                         nullptr);
+  if ((rc = derivative_operators(c, U, o->cfl, o->precision, "adjoint_solve"))) return rc;
+  // (dR/dU)^T psi = rhs: flexible GMRES on the exact adjoint products, each
+  // step preconditioned by transposed GMRES on H1^T with block-Jacobi^T
+  FgOps ops;
+  ops.apply = [&](const double *x, double *y, int nq) -> int {
+    for (int q = 0; q < nq; ++q) {
+      const size_t off = (size_t)q * c->ksp->n;
+      int r = launch_adjoint(c, U, x + off, y + off, nullptr);
+      if (r) return r;
     }
-    norm2(c, k->b, k->scal);
-    e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
-    res = std::sqrt(k->h_scal[0]);
-    if (res <= o->eps * rnorm0 || rnorm0 == 0.0) {
-      st->converged = 1;
-      break;
+    return 0;
+  };
+  ops.prec = [&](const double *v, double *z, int nq, int *inner) -> int {
+    for (int q = 0; q < nq; ++q) {
+      const size_t off = (size_t)q * c->ksp->n;
+      GmresOut go;
+      int r = gmres(c, v + off, z + off, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go, true);
+      if (r) return r;
+      *inner += go.its;
     }
-    if (it >= o->max_its) break;
-    // A^T d = r with the transposed H1 and block-Jacobi
-    GmresOut go;
-    if ((rc = gmres(c, k->b, k->x, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go, true))) return rc;
-    st->krylov_its += go.its;
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->x, 1.0, psi, nullptr);
-  }
-  e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return set_hip_error(e, "adjoint_solve sync");
-  st->its = it;
-  st->res0 = rnorm0;
-  st->res = res;
+    return 0;
+  };
+  FgResult fr;
+  if ((rc = fgmres_solve(c, 1, rhs, psi, kDerivRestart, o->eps, o->max_its, ops, &fr))) return rc;
+  st->its = fr.outer_its;
+  st->krylov_its = fr.inner_its;
+  st->converged = fr.converged;
+  st->res0 = fr.res0;
+  st->res = fr.res;
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return AEROFX_OK;
 }
@@ -1076,18 +1124,12 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
 int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *rhs, double *x,
                         const aerofx_adjoint_opts *o, aerofx_adjoint_stats *st) {
   if (!c || !U || !rhs || !x || !o || !st) return set_error(AEROFX_ERR_ARG, "direct_solve: null argument");
+  if (c->fc.low_mach)
+    return set_error(AEROFX_ERR_STATE, "direct_solve: not available with low-Mach preconditioning");
   if (nrhs < 1 || nrhs > 16) return set_error(AEROFX_ERR_ARG, "direct_solve: 1 <= nrhs <= 16");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "direct_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
-  Krylov *k = c->ksp;
-  const size_t n = k->n;
-  if (!c->d_direct) {
-    void *p = nullptr;
-    if ((rc = ctx_alloc(c, sizeof(double) * (2 * 16 + 1) * n, &p))) return rc;
-    c->d_direct = static_cast<double *>(p);
-  }
-  double *Rb = c->d_direct, *Db = Rb + 16 * n, *Ax = Db + 16 * n;
   // the inner solves use the assembled H1, whatever operator is selected
   struct Restore {
     aerofx_ctx *c;
@@ -1097,77 +1139,47 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
   c->mvp_type = AEROFX_MVP_H1;
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_adjoint_stats{};
-  // A = H1 + diag(1/dt) at U and its block-Jacobi inverse (block GMRES takes
-  // FP64 blocks and makes its own FP32 copies for precision 1)
-  const int jac_prec = nrhs > 1 ? 0 : o->precision;
-  if ((rc = launch_primitive(c, U, c->d_V))) return rc;
-  if (o->cfl > 0.0 && (rc = launch_timestep(c, o->cfl, c->d_dt))) return rc;
-  if ((rc = launch_jacobian(c, o->cfl > 0.0 ? c->d_dt : nullptr, jac_prec))) return rc;
-  std::vector<double> bn(nrhs), rn(nrhs);
-  auto norms = [&](const double *base, std::vector<double> &out) -> int {
-    for (int q = 0; q < nrhs; ++q) {
-      norm2(c, base + (size_t)q * n, k->scal + q);
+  // block GMRES takes FP64 blocks and makes its own FP32 copies (precision 1)
+  if ((rc = derivative_operators(c, U, o->cfl, nrhs > 1 ? 0 : o->precision, "direct_solve"))) return rc;
+  const int bm = std::min(o->gmres_m, 16);
+  FgOps ops;
+  ops.apply = [&](const double *xv, double *y, int nq) -> int {
+    for (int q = 0; q < nq; ++q) {
+      const size_t off = (size_t)q * c->ksp->n;
+      int r = launch_tangent(c, U, xv + off, nullptr, y + off);
+      if (r) return r;
     }
-    hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double) * nrhs, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return set_hip_error(e, "direct_solve sync");
-    for (int q = 0; q < nrhs; ++q) out[q] = std::sqrt(k->h_scal[q]);
     return 0;
   };
-  if ((rc = norms(rhs, bn))) return rc;
-  if (c->h_err[1]) return set_error(AEROFX_ERR_STATE, "direct_solve: zero pivot in a diagonal block");
-  hipLaunchKernelGGL(k_fill, grid_for(n * nrhs), dim3(kBlock), 0, c->stream, n * nrhs, 0.0, x);
-  double worst = 0.0, worst_abs = 0.0, worst_b = 0.0;
-  int it;
-  for (it = 0;; ++it) {
-    // r_k = rhs_k - (dR/dU) x_k: exact second-order products, FP64
-    for (int q = 0; q < nrhs; ++q) {
-      double *r = Rb + (size_t)q * n;
-      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, rhs + (size_t)q * n, 0.0,
-                         r, nullptr);
-      if (it > 0) {
-        if ((rc = launch_tangent(c, U, x + (size_t)q * n, nullptr, Ax))) return rc;
-        hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, Ax, 1.0, r, nullptr);
-      }
-    }
-    if ((rc = norms(Rb, rn))) return rc;
-    if (it > 0 && c->h_err[0]) return set_error(AEROFX_ERR_STATE, "direct_solve: non-physical reconstructed state");
-    worst = 0.0;
-    for (int q = 0; q < nrhs; ++q) {
-      const double rel = bn[q] > 0.0 ? rn[q] / bn[q] : 0.0;
-      if (rel >= worst) {
-        worst = rel;
-        worst_abs = rn[q];
-        worst_b = bn[q];
-      }
-    }
-    if (worst <= o->eps) {
-      st->converged = 1;
-      break;
-    }
-    if (it >= o->max_its) break;
-    // A d_k = r_k with H1 + 1/dt and block-Jacobi
-    if (nrhs == 1) {
+  ops.prec = [&](const double *v, double *z, int nq, int *inner) -> int {
+    if (nq == 1) {
       GmresOut go;
-      if ((rc = gmres(c, Rb, Db, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go))) return rc;
-      st->krylov_its += go.its;
-    } else {
-      aerofx_block_stats bs;
-      const int bm = std::min(o->gmres_m, 16);
-      if ((rc = aerofx_block_gmres(c, nrhs, Rb, Db, bm, o->gmres_eps, std::max(1, o->gmres_maxits / bm), o->precision,
-                                   1, &bs)) < 0)
-        return rc;
-      st->krylov_its += bs.inner_its;
+      int r = gmres(c, v, z, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go);
+      if (r) return r;
+      *inner += go.its;
+      return 0;
     }
-    hipLaunchKernelGGL(k_axpby, grid_for(n * nrhs), dim3(kBlock), 0, c->stream, n * nrhs, 1.0, nullptr, Db, 1.0, x,
-                       nullptr);
-  }
-  hipError_t e = hipStreamSynchronize(c->stream);
+    aerofx_block_stats bs;
+    // as many block steps per right-hand side as the single-vector inner
+    // solve may take (gmres_maxits), in cycles of bm
+    int r = aerofx_block_gmres(c, nq, v, z, bm, o->gmres_eps, std::max(1, (o->gmres_maxits + bm - 1) / bm),
+                               o->precision, 1, &bs);
+    if (r < 0) return r;
+    *inner += bs.inner_its;
+    return 0;
+  };
+  FgResult fr;
+  if ((rc = fgmres_solve(c, nrhs, rhs, x, kDerivRestart, o->eps, o->max_its, ops, &fr))) return rc;
+  // a non-physical reconstruction in the exact products (state U) voids x
+  hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "direct_solve sync");
-  st->its = it;
-  st->res0 = worst_b;
-  st->res = worst_abs;
+  if (c->h_err[0]) return set_error(AEROFX_ERR_STATE, "direct_solve: non-physical reconstructed state");
+  st->its = fr.outer_its;
+  st->krylov_its = fr.inner_its;
+  st->converged = fr.converged;
+  st->res0 = fr.res0;
+  st->res = fr.res;
   st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return AEROFX_OK;
 }
@@ -1276,10 +1288,16 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
                         double *history, int hist_cap) {
   if (!c || !U || !o || !st) return set_error(AEROFX_ERR_ARG, "steady_solve: null argument");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "steady_solve: precision 0 or 1");
-  if (o->mvp != AEROFX_MVP_H1 && o->mvp != AEROFX_MVP_FD)
-    return set_error(AEROFX_ERR_ARG, "steady_solve: mvp AEROFX_MVP_H1 or AEROFX_MVP_FD");
+  if (o->mvp != AEROFX_MVP_H1 && o->mvp != AEROFX_MVP_FD && o->mvp != AEROFX_MVP_H2)
+    return set_error(AEROFX_ERR_ARG, "steady_solve: mvp AEROFX_MVP_H1, AEROFX_MVP_FD or AEROFX_MVP_H2");
+  if (o->mvp == AEROFX_MVP_H2 && c->fc.low_mach)
+    return set_error(AEROFX_ERR_STATE, "steady_solve: exact products not available with low-Mach preconditioning");
   if (o->mvp == AEROFX_MVP_FD && o->fd_order != 1 && o->fd_order != 2)
     return set_error(AEROFX_ERR_ARG, "steady_solve: fd_order 1 or 2");
+  if (o->prec != AEROFX_PREC_JACOBI && o->prec != AEROFX_PREC_RAS_ILU0)
+    return set_error(AEROFX_ERR_ARG, "steady_solve: prec AEROFX_PREC_JACOBI or AEROFX_PREC_RAS_ILU0");
+  if (o->prec == AEROFX_PREC_RAS_ILU0 && (o->prec_subdomain < 1 || o->prec_subdomain > 1024))
+    return set_error(AEROFX_ERR_ARG, "steady_solve: prec_subdomain in [1, 1024]");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
   Krylov *k = c->ksp;
@@ -1298,6 +1316,16 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     }
   } restore{c, c->mvp_type, c->mvp_order, c->mvp_Q, c->mvp_F0, c->mvp_dt};
   c->mvp_type = AEROFX_MVP_H1;
+  struct RestorePrec {
+    aerofx_ctx *c;
+    int type, sub;
+    ~RestorePrec() {
+      c->prec_type = type;
+      c->prec_subdomain = sub;
+    }
+  } restore_prec{c, c->prec_type, c->prec_subdomain};
+  c->prec_type = o->prec;
+  if (o->prec == AEROFX_PREC_RAS_ILU0) c->prec_subdomain = o->prec_subdomain;
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_newton_stats{};
   double res0 = 0.0, res = 0.0, cfl = o->cfl0;
@@ -1349,6 +1377,10 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
       // MatVecProdFD::evaluate at the Newton state: F(U) = R(U) (the time
       // term vanishes at Q = Un); one residual per Krylov product
       if ((rc = select_fd(c, U, k->R, c->d_dt, o->fd_order))) return rc;
+    } else if (o->mvp == AEROFX_MVP_H2) {
+      // MatVecProdH2::evaluate at the Newton state: exact products of the
+      // second-order residual plus the time term
+      if ((rc = select_h2(c, U, c->d_dt))) return rc;
     }
     // b = -F
     hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 0.0, k->b,

@@ -67,6 +67,20 @@ __global__ __launch_bounds__(kEvalBlock) void k_physics_eval(int kind, int n, Ga
       for (int k = 0; k < 25; ++k) out[25 * s + k] = jac[k];
       break;
     }
+    case AEROFX_PHYS_ROEJAC6_LM_SW1:
+    case AEROFX_PHYS_ROEJAC6_LM_SW2: {
+      double nv[3], ug[5], ud[5], jac[25];
+      for (int k = 0; k < 3; ++k) nv[k] = a[3 * s + k];
+      for (int k = 0; k < 5; ++k) {
+        ug[k] = b[5 * s + k];
+        ud[k] = c[5 * s + k];
+      }
+      const double *pr = d + 7 * s;
+      const LowMach lm{pr[0], pr[1], pr[2], pr[3], pr[4]};
+      roejac6(1.0, g, nv, 0.0, ug, ud, kind == AEROFX_PHYS_ROEJAC6_LM_SW1 ? 1 : 2, jac, pr[6] != 0.0, lm, pr[5]);
+      for (int k = 0; k < 25; ++k) out[25 * s + k] = jac[k];
+      break;
+    }
     case AEROFX_PHYS_INVERT5_F64: {
       double blk[25];
       for (int k = 0; k < 25; ++k) blk[k] = a[25 * s + k];
@@ -81,6 +95,19 @@ __global__ __launch_bounds__(kEvalBlock) void k_physics_eval(int kind, int n, Ga
       for (int k = 0; k < 25; ++k) out[25 * s + k] = (double)blk[k];
       break;
     }
+    case AEROFX_PHYS_ROEFLUX5: {
+      double nv[3], vl[5], vr[5], f[5];
+      for (int k = 0; k < 3; ++k) nv[k] = a[3 * s + k];
+      for (int k = 0; k < 5; ++k) {
+        vl[k] = b[5 * s + k];
+        vr[k] = c[5 * s + k];
+      }
+      const double *pr = d + 7 * s;
+      const LowMach lm{pr[0], pr[1], pr[2], pr[3], pr[4]};
+      roe_turkel_flux(g, lm, pr[6] != 0.0, nv, pr[5], vl, vr, f);
+      for (int k = 0; k < 5; ++k) out[5 * s + k] = f[k];
+      break;
+    }
     default:
       break;
   }
@@ -93,11 +120,11 @@ using namespace aerofx;
 
 extern "C" int aerofx_physics_eval(int kind, int n, double scalar, const double *a, const double *b, const double *c,
                                    const double *d, double *out) {
-  if (kind < AEROFX_PHYS_ROEFLUX7 || kind > AEROFX_PHYS_INVERT5_F32)
+  if (kind < AEROFX_PHYS_ROEFLUX7 || kind > AEROFX_PHYS_ROEJAC6_LM_SW2)
     return set_error(AEROFX_ERR_ARG, "physics_eval: unknown kind");
   if (n < 0) return set_error(AEROFX_ERR_ARG, "physics_eval: n < 0");
   const bool need_bc = kind != AEROFX_PHYS_INVERT5_F64 && kind != AEROFX_PHYS_INVERT5_F32;
-  if (!a || !out || (need_bc && (!b || !c)) || (kind == AEROFX_PHYS_VANALBADA && !d))
+  if (!a || !out || (need_bc && (!b || !c)) || ((kind == AEROFX_PHYS_VANALBADA || kind >= AEROFX_PHYS_ROEFLUX5) && !d))
     return set_error(AEROFX_ERR_ARG, "physics_eval: null argument");
   if (n == 0) return AEROFX_OK;
   Gas g;

@@ -137,7 +137,9 @@ __global__ __launch_bounds__(kBlock) void k_gradients(DevTables t, const double 
 //    a row hit the same lines); only the fluxes go through LDS, for the
 //    per-row sums P_i in edge order.
 // ---------------------------------------------------------------------------
-template <bool SECOND, int FB>
+//    LM: the spatial low-Mach flux (ROEFLUX5) with the edge length
+//    |X_j - X_i| in Turkel's beta.
+template <bool SECOND, int FB, bool LM>
 __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ grad, FluxConsts fc,
                                                        double *__restrict__ Fm, double *__restrict__ P,
@@ -159,6 +161,13 @@ __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__
       Vj[k] = V[5 * (size_t)j + k];
     }
     double Vij[5], Vji[5];
+    double length = 1.0;
+    if (LM) {
+      const double4 xi = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)i);
+      const double4 xj = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)j);
+      const double dx0 = xj.x - xi.x, dx1 = xj.y - xi.y, dx2 = xj.z - xi.z;
+      length = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
+    }
     if (SECOND) {
       const double4 xi = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)i);
       const double4 xj = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)j);
@@ -195,6 +204,8 @@ __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__
     if (sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) < 1e-18) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) f[k] = 0.0;
+    } else if (LM) {
+      roe_turkel_flux(g, fc.lm, true, nv, length, Vij, Vji, f);
     } else {
       roeflux7(g, nv, 0.0, Vij, Vji, f);
     }
@@ -505,10 +516,22 @@ __global__ void k_zero_int(int *p, int n, const int *done) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <bool SECOND, int FB>
+static void launch_flux_edges(aerofx_ctx *c, const ResidualBufs &b, const int *done) {
+  const DevTables &t = c->tab;
+  if (c->fc.low_mach)
+    hipLaunchKernelGGL((k_flux_edges<SECOND, FB, true>), dim3(t.nfchunks), dim3(FB), 0, c->stream, t, b.V, b.grad,
+                       c->fc, b.Fm, b.P, c->d_err, done);
+  else
+    hipLaunchKernelGGL((k_flux_edges<SECOND, FB, false>), dim3(t.nfchunks), dim3(FB), 0, c->stream, t, b.V, b.grad,
+                       c->fc, b.Fm, b.P, c->d_err, done);
+}
+
 int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale,
                          const int *done) {
   const DevTables &t = c->tab;
-  const bool staged = t.ngb > 0 && c->residual_kernels == AEROFX_RESIDUAL_STAGED;
+  // the staged kernels evaluate ROEFLUX7 only
+  const bool staged = t.ngb > 0 && c->residual_kernels == AEROFX_RESIDUAL_STAGED && !c->fc.low_mach;
   if (c->p.second_order) {
     ProfScope ps(c, AEROFX_TIMER_NODAL_GRAD);
     if (staged)
@@ -523,7 +546,6 @@ int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, 
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_TERM);
-    const dim3 g(t.nfchunks);
     if (staged && c->p.second_order)
       hipLaunchKernelGGL(k_flux_staged<true>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, b.V, b.grad,
                          c->fc, b.Fm, b.P, c->d_err, done);
@@ -531,17 +553,13 @@ int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, 
       hipLaunchKernelGGL(k_flux_staged<false>, dim3(t.nfb), dim3(kFluxStrip), 0, c->stream, t, b.V, b.grad,
                          c->fc, b.Fm, b.P, c->d_err, done);
     else if (c->p.second_order && t.fblock == kFluxBlockSmall)
-      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t, b.V,
-                         b.grad, c->fc, b.Fm, b.P, c->d_err, done);
+      launch_flux_edges<true, kFluxBlockSmall>(c, b, done);
     else if (c->p.second_order)
-      hipLaunchKernelGGL((k_flux_edges<true, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, b.V, b.grad,
-                         c->fc, b.Fm, b.P, c->d_err, done);
+      launch_flux_edges<true, kFluxBlock>(c, b, done);
     else if (t.fblock == kFluxBlockSmall)
-      hipLaunchKernelGGL((k_flux_edges<false, kFluxBlockSmall>), g, dim3(kFluxBlockSmall), 0, c->stream, t,
-                         b.V, b.grad, c->fc, b.Fm, b.P, c->d_err, done);
+      launch_flux_edges<false, kFluxBlockSmall>(c, b, done);
     else
-      hipLaunchKernelGGL((k_flux_edges<false, kFluxBlock>), g, dim3(kFluxBlock), 0, c->stream, t, b.V,
-                         b.grad, c->fc, b.Fm, b.P, c->d_err, done);
+      launch_flux_edges<false, kFluxBlock>(c, b, done);
   }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_NODES);

@@ -39,6 +39,9 @@ struct Tangent {
   double *dcv = nullptr;  // [N]
   double *dfn = nullptr;  // [F][3] full face normal
   double *dF = nullptr;   // [E][5]
+  // the state part of the exact product operator (MatVecProdH2), kept from
+  // its selection: primitive state [N][5] and LSQ gradients [N][16]
+  double *V0 = nullptr, *G0 = nullptr;
 };
 
 void free_tangent(aerofx_ctx *c) {
@@ -198,7 +201,9 @@ template <bool SECOND>
 __global__ __launch_bounds__(kTanBlock) void k_tan_grad(DevTables t, const double *__restrict__ U,
                                                         const double *__restrict__ dU,
                                                         const double *__restrict__ dX, Gas gas,
-                                                        double *__restrict__ dV, double *__restrict__ dG) {
+                                                        double *__restrict__ dV, double *__restrict__ dG,
+                                                        const int *__restrict__ done) {
+  if (done && *done) return;
   const int n = blockIdx.x * kTanBlock + threadIdx.x;
   if (n >= t.N) return;
   Dual Vn[5];
@@ -252,7 +257,8 @@ __global__ __launch_bounds__(kTanBlock) void k_tan_flux(DevTables t, const doubl
                                                         const double *__restrict__ dG,
                                                         const double *__restrict__ dX,
                                                         const double *__restrict__ dn, FluxConsts fc,
-                                                        double *__restrict__ dF) {
+                                                        double *__restrict__ dF, const int *__restrict__ done) {
+  if (done && *done) return;
   const int l = blockIdx.x * kTanBlock + threadIdx.x;
   if (l >= t.E) return;
   const int2 ij = t.eij[l];
@@ -302,7 +308,8 @@ __global__ __launch_bounds__(kTanBlock) void k_tan_nodes(DevTables t, const doub
                                                          const double *__restrict__ dcv,
                                                          const double *__restrict__ dfn,
                                                          const double *__restrict__ dF, FluxConsts fc,
-                                                         double *__restrict__ dR) {
+                                                         double *__restrict__ dR, const int *__restrict__ done) {
+  if (done && *done) return;
   const int n = blockIdx.x * kTanBlock + threadIdx.x;
   if (n >= t.N) return;
   double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -334,6 +341,16 @@ __global__ __launch_bounds__(kTanBlock) void k_tan_nodes(DevTables t, const doub
   const double ic = t.invcv[n], dic = -dcv[n] * ic * ic;
 #pragma unroll
   for (int k = 0; k < 5; ++k) dR[5 * (size_t)n + k] = s[k] * ic + S[5 * (size_t)n + k] * dic;
+}
+
+// y += p / dt (the backward-Euler time term of the exact product, dt fixed)
+__global__ __launch_bounds__(kTanBlock) void k_tan_time(size_t n, const double *__restrict__ p,
+                                                        const double *__restrict__ dt, double *__restrict__ y,
+                                                        const int *__restrict__ done) {
+  if (done && *done) return;
+  const size_t i = blockIdx.x * (size_t)kTanBlock + threadIdx.x;
+  if (i >= n) return;
+  y[i] += p[i] * (1.0 / dt[i / 5]);
 }
 
 template <class T>
@@ -390,22 +407,76 @@ int launch_tangent(aerofx_ctx *c, const double *U, const double *dU, const doubl
   const bool second = c->p.second_order != 0;
   if (second)
     hipLaunchKernelGGL(k_tan_grad<true>, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, U, dU, dX, c->gas, g->dV,
-                       g->dG);
+                       g->dG, nullptr);
   else
     hipLaunchKernelGGL(k_tan_grad<false>, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, U, dU, dX, c->gas, g->dV,
-                       g->dG);
+                       g->dG, nullptr);
   if (t.E > 0) {
     if (second)
       hipLaunchKernelGGL(k_tan_flux<true>, grid_for(t.E), dim3(kTanBlock), 0, c->stream, t, c->d_V, c->d_grad, g->dV,
-                         g->dG, dX, g->dn, c->fc, g->dF);
+                         g->dG, dX, g->dn, c->fc, g->dF, nullptr);
     else
       hipLaunchKernelGGL(k_tan_flux<false>, grid_for(t.E), dim3(kTanBlock), 0, c->stream, t, c->d_V, c->d_grad,
-                         g->dV, g->dG, dX, g->dn, c->fc, g->dF);
+                         g->dV, g->dG, dX, g->dn, c->fc, g->dF, nullptr);
   }
   hipLaunchKernelGGL(k_tan_nodes, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, c->d_V, g->dV, g->S, g->dcv, g->dfn,
-                     g->dF, c->fc, dR);
+                     g->dF, c->fc, dR, nullptr);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "tangent launch");
+}
+
+// MatVecProdH2::evaluate (MatVecProd.C:1560-1609) as a matrix-free exact
+// product: the forward pass at Q (primitive state, LSQ gradients, raw flux
+// sums) is kept in the tangent workspace, and the geometry derivatives are
+// zero (the mesh does not move).
+int launch_h2_select(aerofx_ctx *c, const double *Q) {
+  int rc = ensure_tangent(c);
+  if (rc) return rc;
+  Tangent *g = c->tan;
+  const DevTables &t = c->tab;
+  if (!g->V0 && ((rc = alloc_n(c, 5 * (size_t)t.N, &g->V0)) || (rc = alloc_n(c, 16 * (size_t)t.N, &g->G0))))
+    return rc;
+  if ((rc = launch_residual(c, Q, g->S, false))) return rc;
+  hipError_t e = hipMemcpyAsync(g->V0, c->d_V, 5 * sizeof(double) * (size_t)t.N, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(g->G0, c->d_grad, 16 * sizeof(double) * (size_t)t.N, hipMemcpyDeviceToDevice, c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "h2 select copy");
+  hipLaunchKernelGGL(k_fill0, grid_for(3LL * t.E), dim3(kTanBlock), 0, c->stream, 3 * (size_t)t.E, g->dn);
+  hipLaunchKernelGGL(k_fill0, grid_for(t.N), dim3(kTanBlock), 0, c->stream, (size_t)t.N, g->dcv);
+  hipLaunchKernelGGL(k_fill0, grid_for(3LL * t.F), dim3(kTanBlock), 0, c->stream, 3 * (size_t)t.F, g->dfn);
+  e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "h2 select launch");
+}
+
+// MatVecProdH2::apply (MatVecProd.C:1647-1670): y = (dR/dU) p + p / dt at the
+// selected Q (dt NULL: no time term); every kernel returns at once when
+// *done != 0 (device-resident Krylov loops).
+int launch_h2_apply(aerofx_ctx *c, const double *Q, const double *p, const double *dt, double *y,
+                    const int *done) {
+  Tangent *g = c->tan;
+  if (!g || !g->V0) return set_error(AEROFX_ERR_STATE, "h2 apply: operator not selected");
+  const DevTables &t = c->tab;
+  if (c->p.second_order)
+    hipLaunchKernelGGL(k_tan_grad<true>, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, Q, p, nullptr, c->gas,
+                       g->dV, g->dG, done);
+  else
+    hipLaunchKernelGGL(k_tan_grad<false>, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, Q, p, nullptr, c->gas,
+                       g->dV, g->dG, done);
+  if (t.E > 0) {
+    if (c->p.second_order)
+      hipLaunchKernelGGL(k_tan_flux<true>, grid_for(t.E), dim3(kTanBlock), 0, c->stream, t, g->V0, g->G0, g->dV,
+                         g->dG, nullptr, g->dn, c->fc, g->dF, done);
+    else
+      hipLaunchKernelGGL(k_tan_flux<false>, grid_for(t.E), dim3(kTanBlock), 0, c->stream, t, g->V0, g->G0, g->dV,
+                         g->dG, nullptr, g->dn, c->fc, g->dF, done);
+  }
+  hipLaunchKernelGGL(k_tan_nodes, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, g->V0, g->dV, g->S, g->dcv,
+                     g->dfn, g->dF, c->fc, y, done);
+  if (dt)
+    hipLaunchKernelGGL(k_tan_time, grid_for(5LL * t.N), dim3(kTanBlock), 0, c->stream, 5 * (size_t)t.N, p, dt, y,
+                       done);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "h2 apply launch");
 }
 
 }  // namespace aerofx
@@ -415,6 +486,8 @@ using namespace aerofx;
 extern "C" int aerofx_residual_derivative(aerofx_ctx *c, const double *U, const double *dU, const double *dX,
                                           double *dR) {
   if (!c || !U || !dR) return set_error(AEROFX_ERR_ARG, "residual_derivative: null argument");
+  if (c->fc.low_mach)
+    return set_error(AEROFX_ERR_STATE, "residual_derivative: not available with low-Mach preconditioning");
   int rc = launch_tangent(c, U, dU, dX, dR);
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);

@@ -25,6 +25,18 @@ def _ip(a: np.ndarray):
 
 
 @dataclass
+class LowMachPrec:
+    """Spatial low-Mach preconditioner constants (SpatialLowMachPrec,
+    LowMachPrec.h; Preconditioner.Mach / k / CutOffMach / ShockReducer) and
+    the edges' 1/Re (0 for inviscid flow)."""
+    min_mach: float = 0.1
+    slope: float = 1.0
+    cutoff: float = 1.0
+    shock_reducer: float = 0.0
+    irey: float = 0.0
+
+
+@dataclass
 class FlowParams:
     """Perfect-gas Euler parameters (IoDataCore.C:2418-2429 defaults)."""
     gamma: float = 1.4
@@ -36,6 +48,9 @@ class FlowParams:
     mach: float = 0.5
     alpha: float = 0.0
     yaw: float = 0.0
+    # spatial low-Mach preconditioning (Problem.Prec = Preconditioned): the
+    # Roe-Turkel flux ROEFLUX5 and its Jacobian; None: Roe's flux
+    low_mach: LowMachPrec | None = None
 
     def c(self) -> Params:
         return Params(self.gamma, self.pstiff, self.beta, int(self.second_order), self.rho_inf,

@@ -65,6 +65,16 @@ class SpaceOperator:
         self.num_nodes, self.num_edges, self.num_faces = N.value, E.value, F.value
         self.n5, self.n3 = 5 * self.num_nodes, 3 * self.num_nodes  # element counts of [N][5], [N][3]
         self.set_stream(stream)
+        self.set_low_mach(params.low_mach)
+
+    def set_low_mach(self, lm=None):
+        """Spatial low-Mach preconditioning (LowMachPrec) on, or off (None)."""
+        from ._abi import LowMachC
+        if lm is None:
+            check(lib().aerofx_ctx_set_low_mach(self._h, None), "set_low_mach")
+        else:
+            c = LowMachC(lm.min_mach, lm.slope, lm.cutoff, lm.shock_reducer, lm.irey)
+            check(lib().aerofx_ctx_set_low_mach(self._h, C.byref(c)), "set_low_mach")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -178,6 +188,12 @@ class MatVecProdH1:
     def __init__(self, op: SpaceOperator):
         self.op = op
 
+    def select_preconditioner(self, kind: str = "jacobi", subdomain: int = 256):
+        """'jacobi' (JacobiPrec) or 'ras_ilu0' (IluPrec, fill 0, RAS over
+        subdomains of `subdomain` consecutive nodes); effective at the next
+        evaluate()."""
+        check(lib().aerofx_prec_select(self.op._h, {"jacobi": 0, "ras_ilu0": 1}[kind], int(subdomain)), "prec_select")
+
     def evaluate(self, U, dt=None, precision: int = 0):
         check(lib().aerofx_jacobian_setup(self.op._h, _ptr(U, self.op.n5), _optr(dt, self.op.num_nodes),
                                           int(precision)), "jacobian_setup")
@@ -241,6 +257,33 @@ class MatVecProdFD:
         return check(lib().aerofx_jvp_fd(self.op._h, _ptr(U, self.op.n5), _ptr(p, self.op.n5),
                                          _ptr(prod, self.op.n5), int(self.order),
                                          _optr(dt, self.op.num_nodes)), "jvp_fd")
+
+
+class MatVecProdH2:
+    """MatVecProdH2<5,double,5> (MatVecProd.h, MatVecProd.C:1560-1670): the
+    exact derivative of the second-order residual plus the time term, about
+    U, as the operator of the context's GMRES.  Formed matrix-free by the
+    forward-mode kernels; evaluate() keeps the forward pass at U.  U and dt
+    must stay alive and unchanged while it is selected."""
+
+    def __init__(self, op: SpaceOperator):
+        self.op = op
+        self.U = self.dt = None
+
+    def evaluate(self, U, dt=None) -> int:
+        from ._abi import AEROFX_MVP_H2
+        self.U, self.dt = U, dt
+        return check(lib().aerofx_mvp_select(self.op._h, AEROFX_MVP_H2, _ptr(U, self.op.n5), None,
+                                             _optr(dt, self.op.num_nodes), 1), "mvp_select")
+
+    def release(self):
+        """Selects the assembled H1 again as the GMRES operator."""
+        from ._abi import AEROFX_MVP_H1
+        check(lib().aerofx_mvp_select(self.op._h, AEROFX_MVP_H1, None, None, None, 1), "mvp_select")
+        self.U = self.dt = None
+
+    def apply(self, p, prod) -> int:
+        return check(lib().aerofx_mvp_apply(self.op._h, _ptr(p, self.op.n5), _ptr(prod, self.op.n5)), "mvp_apply")
 
 
 class GmresSolver:
@@ -326,15 +369,20 @@ def direct_solve(op: SpaceOperator, U, rhs, x, precision: int = 1, cfl: float = 
 
 def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200, eps: float = 1e-8,
                  cfl0: float = 10.0, cfl_max: float = 1e4, ser: float = 0.7, gmres_m: int = 30,
-                 gmres_maxits: int = 30, gmres_eps: float = 1e-2, mvp: str = "h1", fd_order: int = 1):
+                 gmres_maxits: int = 30, gmres_eps: float = 1e-2, mvp: str = "h1", fd_order: int = 1,
+                 prec: str = "jacobi", prec_subdomain: int = 256):
     """Pseudo-transient Newton-GMRES to ||R|| <= eps ||R0|| (U updated in place).
 
     mvp "h1": Krylov products with the assembled first-order Jacobian (FP32
     blocks when precision=1); "fd": matrix-free MatVecProdFD products of the
-    FP64 residual (one residual per product), block-Jacobi in `precision`."""
-    code = {"h1": 0, "fd": 1}[mvp]
+    FP64 residual (one residual per product); "h2": exact products of the
+    second-order residual (MatVecProdH2, forward-mode kernels); block-Jacobi
+    in `precision` for all three.  prec "jacobi" (block-Jacobi of H1) or
+    "ras_ilu0" (RAS with ILU(0) on subdomains of prec_subdomain nodes)."""
+    code = {"h1": 0, "fd": 1, "h2": 2}[mvp]
+    pcode = {"jacobi": 0, "ras_ilu0": 1}[prec]
     o = NewtonOpts(int(precision), int(max_steps), float(eps), float(cfl0), float(cfl_max), float(ser), 0.0, 0.0,
-                   int(gmres_m), int(gmres_maxits), float(gmres_eps), code, int(fd_order))
+                   int(gmres_m), int(gmres_maxits), float(gmres_eps), code, int(fd_order), pcode, int(prec_subdomain))
     st = NewtonStats()
     hist = np.zeros(max_steps + 1)
     check(lib().aerofx_steady_solve(op._h, _ptr(U, op.n5), C.byref(o), C.byref(st), hist.ctypes.data_as(c_double_p),

@@ -37,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 NX = 128
 
 
@@ -44,13 +45,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_traffic(name):
-    """HBM bytes per launch for a kernel from profiles/traffic_<name>.json."""
+def load_traffic(name, key="bytes_per_launch"):
+    """Per-launch counter figure (HBM bytes, FP64 lane-operations) from
+    profiles/traffic_<name>.json (rocprofv3 FETCH_SIZE/WRITE_SIZE and
+    SQ_INSTS_VALU_*_F64 of the same workload)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get("bytes_per_launch")
+        return d.get(key)
     except (OSError, ValueError):
         return None
 
@@ -159,6 +162,22 @@ def newton_bench(dev, max_steps):
                                                              np.linalg.norm(states["fp64"]))}
     log(f"[bench] newton matrix-free: {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its, "
         f"converged={r['converged']}")
+    # exact second-order products (MatVecProdH2, forward-mode kernels), FP32
+    # block-Jacobi of H1
+    steady_solve(op, torch.from_numpy(U0).to(dev), precision=1, max_steps=2, mvp="h2")
+    Ud = torch.from_numpy(U0).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = steady_solve(op, Ud, precision=1, max_steps=max_steps, eps=1e-8, cfl0=10.0, cfl_max=1e4, mvp="h2")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out["exact_products"] = {"seconds": round(wall, 4), "steps": r["steps"], "krylov_its": r["krylov_its"],
+                             "converged": r["converged"], "rel_res": float(r["res"] / r["res0"]),
+                             "preconditioner": "FP32 block-Jacobi of H1",
+                             "speedup_vs_h1_fp64": round(out["fp64"]["seconds"] / wall, 3),
+                             "state_rel_diff_vs_h1_fp64": float(np.linalg.norm(Ud.cpu().numpy() - states["fp64"]) /
+                                                                np.linalg.norm(states["fp64"]))}
+    log(f"[bench] newton exact products: {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its")
     # per-kernel-bucket breakdown from a separate profiled run of each precision
     for name, prec in (("fp64", 0), ("mixed", 1)):
         Ud = torch.from_numpy(U0).to(dev)
@@ -171,6 +190,27 @@ def newton_bench(dev, max_steps):
     out["state_rel_diff_mixed_vs_fp64"] = float(np.linalg.norm(states["mixed"] - states["fp64"]) /
                                                 np.linalg.norm(states["fp64"]))
     op.close()
+    # RAS with ILU(0) subdomain solves on the same channel numbered by RCM (the
+    # reference's IluPrec renumbering), against block-Jacobi on that numbering
+    mesh_r = M.renumber(mesh, M.rcm_gpu(mesh))
+    opr = SpaceOperator(mesh_r, params)
+    U0r = init_state(params, mesh_r.X, "freestream")
+    ras = {"workload": "the same solve on the channel numbered by RCM, subdomains of 512 nodes"}
+    for name, prec, pc in (("jacobi_mixed", 1, "jacobi"), ("ras_ilu0_mixed", 1, "ras_ilu0"),
+                           ("ras_ilu0_fp64", 0, "ras_ilu0")):
+        steady_solve(opr, torch.from_numpy(U0r).to(dev), precision=prec, max_steps=2, prec=pc, prec_subdomain=512)
+        Ud = torch.from_numpy(U0r).to(dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = steady_solve(opr, Ud, precision=prec, max_steps=max_steps, eps=1e-8, cfl0=10.0, cfl_max=1e4, prec=pc,
+                         prec_subdomain=512)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ras[name] = {"seconds": round(wall, 4), "steps": r["steps"], "krylov_its": r["krylov_its"],
+                     "converged": r["converged"], "rel_res": float(r["res"] / r["res0"])}
+        log(f"[bench] newton {name} (RCM numbering): {wall:.3f} s, {r['steps']} steps, {r['krylov_its']} Krylov its")
+    opr.close()
+    out["ras_ilu0"] = ras
     return out
 
 
@@ -549,6 +589,14 @@ def main():
     }
     roofline["copy_peak_GBs"] = copy_peak(dev)
     roofline["frac_of_copy_peak"] = round(achieved / roofline["copy_peak_GBs"], 4)
+    # FP64 throughput: the counted FP64 lane-operations of one evaluation
+    # (profiles/) over this run's kernel time, against the FP64 vector peak
+    ops = load_traffic("residual_box128", "fp64_lane_ops_per_launch")
+    if ops:
+        roofline["fp64_lane_ops"] = ops
+        roofline["fp64_tflops"] = round(ops / (kern_ms * 1e-3) / 1e12, 2)
+        roofline["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
+        roofline["fp64_frac"] = round(roofline["fp64_tflops"] / FP64_PEAK_TFLOPS, 4)
     if not args.no_newton or args.batched or not args.no_adjoint or args.shape or args.sensitivity:
         op.close()
     if not args.no_newton:

@@ -170,6 +170,29 @@ int aerofx_ctx_timers(aerofx_ctx *ctx, double *ms, long long *counts);
 #define AEROFX_RESIDUAL_STAGED 2
 int aerofx_ctx_set_residual_kernels(aerofx_ctx *ctx, int family);
 
+/* Spatial low-Mach preconditioning: SpatialLowMachPrec (LowMachPrec.h) with
+ * Problem.Prec = Preconditioned and no time preconditioner (prec tag 1).
+ * The interior edge flux becomes the Roe-Turkel flux ROEFLUX5
+ * (FluxFcnSGApprJacRoeEuler3D::compute, FluxFcnDescSG.C:125-139) with
+ * beta = min((1 + sqrt(irey)) max(slope M_roe, min_mach)
+ *            + shock_reducer |p_i - p_j| / ((p_i + p_j) |X_j - X_i|), cutoff)
+ * per edge (length |X_j - X_i|, Edge.C:400; irey the edge's 1/Re, 0 for
+ * Euler flow), and the H1 edge blocks its ROEJAC6 (roejacappr3Dgas,
+ * FluxFcnDescSG.C:157-175, Edge.C:4472-4526).  The time step keeps beta = 1
+ * (TimeLowMachPrec without Ts.Prec).  lm = NULL turns it off.  The residual
+ * runs the per-lane kernels while it is on; the forward and reverse
+ * derivative passes (aerofx_residual_derivative, aerofx_adjoint_residual,
+ * AEROFX_MVP_H2, aerofx_adjoint_solve, aerofx_direct_solve) return
+ * AEROFX_ERR_STATE: they differentiate ROEFLUX7. */
+typedef struct {
+  double min_mach;       /* Preconditioner.Mach (beta floor) */
+  double slope;          /* Preconditioner.k (slope on the local Mach number) */
+  double cutoff;         /* Preconditioner.CutOffMach (beta ceiling) */
+  double shock_reducer;  /* Preconditioner.ShockReducer */
+  double irey;           /* 1/Re of the edges (0: inviscid) */
+} aerofx_low_mach;
+int aerofx_ctx_set_low_mach(aerofx_ctx *ctx, const aerofx_low_mach *lm);
+
 /* ---- SpaceOperator<5> -------------------------------------------------- */
 /* SpaceOperator<5>::computeResidual (SpaceOperator.C:524-634), NONDESCRIPTOR
  * form: R = (sum of edge and face fluxes)/V.  U, R: device [N][5].
@@ -255,13 +278,42 @@ int aerofx_jvp_fd(aerofx_ctx *ctx, const double *U, const double *p, double *Jp,
  *                  (device [N][5], [N][5], [N]) must stay valid and unchanged
  *                  while the operator is selected.  One residual per product
  *                  at fd_order 1, two at fd_order 2.
+ *   AEROFX_MVP_H2  MatVecProdH2::evaluate/apply (MatVecProd.C:1560-1670)
+ *                  about Q: the exact derivative of the second-order
+ *                  residual, (dR/dU) p + p / dt (dt NULL: no time term),
+ *                  formed matrix-free by the forward-mode kernels of
+ *                  aerofx_residual_derivative (the forward pass at Q is
+ *                  kept at selection; F0 and fd_order are ignored).  Q and
+ *                  dt must stay valid and unchanged while it is selected.
  * The preconditioner stays the block-Jacobi of the last aerofx_jacobian_setup;
  * the transposed solves always use H1^T.  Returns the count of non-physical
- * reconstructions in the evaluation of F0 (0 when valid), or < 0. */
+ * reconstructions in the evaluation of F0 (AEROFX_MVP_FD) or of the
+ * residual at Q (AEROFX_MVP_H2), 0 when valid, or < 0. */
 #define AEROFX_MVP_H1 0
 #define AEROFX_MVP_FD 1
+#define AEROFX_MVP_H2 2
 int aerofx_mvp_select(aerofx_ctx *ctx, int type, const double *Q, const double *F0,
                       const double *dt, int fd_order);
+/* The preconditioner of aerofx_prec_apply, aerofx_gmres and the Newton
+ * driver (PcData::type, IoDataCore.C:2744), taking effect at the next
+ * aerofx_jacobian_setup (or Newton step):
+ *   AEROFX_PREC_JACOBI    JacobiPrec: the inverted 5x5 diagonal blocks of
+ *                         H1 (the default)
+ *   AEROFX_PREC_RAS_ILU0  IluPrec with fill 0, type RAS (KspPrec.C:127-320):
+ *                         the ILU(0) (SparseMat::numericILU / lusol,
+ *                         SparseMatrix.C:353-466) of each subdomain of
+ *                         `subdomain_nodes` consecutive nodes (1..1024), the
+ *                         solution kept on the subdomain's own nodes.  The
+ *                         node numbering is the subdomains' order: number the
+ *                         mesh by RCM (aerofx_mesh_rcm) as the reference's
+ *                         IluPrec renumbers its subdomains.  Factors in the
+ *                         Jacobian's precision.  Transposed solves, the
+ *                         adjoint and direct solves and block GMRES keep
+ *                         block-Jacobi.
+ * Returns 0 or < 0. */
+#define AEROFX_PREC_JACOBI 0
+#define AEROFX_PREC_RAS_ILU0 1
+int aerofx_prec_select(aerofx_ctx *ctx, int type, int subdomain_nodes);
 /* MatVecProd::apply (MatVecProd.h:54-80) with the selected operator:
  * prod = A p.  Synchronises; returns the count of non-physical
  * reconstructions in its residual evaluations (0 when valid), or < 0. */
@@ -386,8 +438,11 @@ typedef struct {
   double cfl0, cfl_max, ser, cfl_coef1, cfl_coef2;  /* CflLaw Old */
   int gmres_m, gmres_maxits;
   double gmres_eps;     /* relative Krylov tolerance */
-  int mvp;              /* AEROFX_MVP_H1 (assembled H1) or AEROFX_MVP_FD (matrix-free) */
+  int mvp;              /* AEROFX_MVP_H1 (assembled H1), AEROFX_MVP_FD (matrix-free
+                           finite differences) or AEROFX_MVP_H2 (exact products) */
   int fd_order;         /* 1 or 2 with AEROFX_MVP_FD */
+  int prec;             /* AEROFX_PREC_JACOBI or AEROFX_PREC_RAS_ILU0 (for the solve) */
+  int prec_subdomain;   /* RAS / ILU(0) subdomain size in nodes */
 } aerofx_newton_opts;
 typedef struct {
   int steps, converged, krylov_its;
@@ -400,6 +455,48 @@ typedef struct {
 int aerofx_steady_solve(aerofx_ctx *ctx, double *U, const aerofx_newton_opts *opts,
                         aerofx_newton_stats *stats, double *history,
                         int hist_cap);
+
+/* ---- SOWER mesh and solution files (host memory, no GPU) --------------- */
+/* GeoSource::getSubDomain (GeoSource.C:246-356): one subdomain of a SOWER
+ * cluster, from its geometry (.geo) and decomposition (.dec) files in the
+ * BinFileHandler format (BinFileHandler.h; either byte order): node, element
+ * and face ranges (getRangeInfo, GeoSource.C:204-240), NodeSet::read
+ * (Node.C:7-50), ElemSet::read (ElemCore.C:135-200, tets only), FaceSet::read
+ * (FaceCore.C:270-347, triangles; code 6 is read as BC_SYMMETRY = 11), with
+ * element and face nodes renumbered from cluster to subdomain numbering
+ * (GeoSource.C:360-379). */
+typedef struct {
+  int num_nodes, num_tets, num_faces;  /* of the subdomain */
+  int num_clus_nodes;                  /* of the cluster (solution files hold these) */
+  int num_sub_cluster;                 /* subdomains in the cluster */
+  int num_neighbours;                  /* subdomains sharing nodes with this one */
+  int num_bcs;                         /* BCondSet entries of the cluster */
+  double version;                      /* BinFileHandler header version */
+} aerofx_sower_header;
+int aerofx_sower_info(const char *geo, const char *dec, int sub, aerofx_sower_header *info);
+/* X [N][3], tets [T][4], faces [F][3], codes [F] (host); node_glob [N]
+ * (global node numbers) and node_clus [N] (cluster node numbers, the rows of
+ * the cluster's solution files) may be NULL. */
+int aerofx_sower_read(const char *geo, const char *dec, int sub, double *X, int *tets, int *faces, int *codes,
+                      int *node_glob, int *node_clus);
+/* No counterpart in the reference (SOWER writes these files): a cluster of
+ * one subdomain holding the whole mesh, in the layout aerofx_sower_read and
+ * GeoSource::getSubDomain read (global numbers = indices, ids 0, no
+ * neighbours, no boundary-condition records). */
+int aerofx_sower_write(const char *geo, const char *dec, int N, int T, int F, const double *X, const int *tets,
+                       const int *faces, const int *codes);
+/* Solution / restart files (SubDomain.C:5609-5810): {cluster nodes, dim,
+ * solutions} then per solution a double tag and [cluster nodes][dim]
+ * values.  aerofx_solution_write: step 0 creates the file, a later step
+ * updates it in place and sets the count to step + 1 (openFileForWriting,
+ * writeTagToFile, writeVectorToFile with a cluster of one subdomain).
+ * aerofx_solution_read: the rows node_clus[0..n) (NULL: all n, which must be
+ * the cluster size) of solution `step`, min(dim, file dim) values per row
+ * (readVectorFromFile), and its tag. */
+int aerofx_solution_info(const char *path, int *num_clus_nodes, int *dim, int *nsol);
+int aerofx_solution_write(const char *path, int step, double tag, int n, int dim, const double *U);
+int aerofx_solution_read(const char *path, int step, double *tag, int n, int dim, const int *node_clus,
+                         double *U);
 
 /* ---- physics routines, one case per thread ----------------------------- */
 /* No counterpart in the reference: evaluates the device routines that the
@@ -418,6 +515,13 @@ int aerofx_steady_solve(aerofx_ctx *ctx, double *U, const aerofx_newton_opts *op
  *   AEROFX_PHYS_INVERT5_F64/F32  LINPACK dgefa + dgedi (DenseMatrixOps.h:1006-1181)
  *                            in FP64 or FP32: a blocks [n][25] -> out [n][25]
  *                            (FP32 results widened to FP64)
+ *   AEROFX_PHYS_ROEFLUX5     ROEFLUX5 (roeflux5.f, Roe-Turkel low-Mach dissipation,
+ *                            static face), scalar = gamma: a normal, b V_L, c V_R,
+ *                            d [n][7] = (min Mach, slope, cut-off Mach, shock
+ *                            reducer, 1/Re, edge length, prec flag) -> out [n][5]
+ *   AEROFX_PHYS_ROEJAC6_LM_SW1/2  ROEJAC6 type 1 or 2 with the low-Mach constants
+ *                            (roejac6.f:130-160), scalar = gamma: a normal,
+ *                            b U_g, c U_d, d [n][7] as for ROEFLUX5 -> out [n][25]
  * Returns the number of singular blocks (inversions), 0, or < 0. */
 #define AEROFX_PHYS_ROEFLUX7 0
 #define AEROFX_PHYS_BOUNDFLUX5 1
@@ -426,6 +530,9 @@ int aerofx_steady_solve(aerofx_ctx *ctx, double *U, const aerofx_newton_opts *op
 #define AEROFX_PHYS_ROEJAC6_SW2 4
 #define AEROFX_PHYS_INVERT5_F64 5
 #define AEROFX_PHYS_INVERT5_F32 6
+#define AEROFX_PHYS_ROEFLUX5 7
+#define AEROFX_PHYS_ROEJAC6_LM_SW1 8
+#define AEROFX_PHYS_ROEJAC6_LM_SW2 9
 int aerofx_physics_eval(int kind, int n, double scalar, const double *a, const double *b,
                         const double *c, const double *d, double *out);
 

@@ -30,7 +30,9 @@ def _i(a):
 class OrParams(C.Structure):
     _fields_ = [("gamma", C.c_double), ("pstiff", C.c_double), ("beta", C.c_double),
                 ("second_order", C.c_int), ("rho_inf", C.c_double), ("p_inf", C.c_double),
-                ("mach", C.c_double), ("alpha", C.c_double), ("yaw", C.c_double)]
+                ("mach", C.c_double), ("alpha", C.c_double), ("yaw", C.c_double),
+                ("low_mach", C.c_int), ("lm_min_mach", C.c_double), ("lm_slope", C.c_double),
+                ("lm_cutoff", C.c_double), ("lm_shock", C.c_double), ("lm_irey", C.c_double)]
 
 
 class OrMesh(C.Structure):
@@ -96,8 +98,12 @@ def ref():
 
 def params(p) -> OrParams:
     """OrParams from an aerof_dev_amd.FlowParams-like object."""
+    lm = getattr(p, "low_mach", None)
+    if lm is None:
+        return OrParams(p.gamma, p.pstiff, p.beta, int(p.second_order), p.rho_inf, p.p_inf, p.mach,
+                        p.alpha, p.yaw, 0, 1.0, 0.0, 1.0, 0.0, 0.0)
     return OrParams(p.gamma, p.pstiff, p.beta, int(p.second_order), p.rho_inf, p.p_inf, p.mach,
-                    p.alpha, p.yaw)
+                    p.alpha, p.yaw, 1, lm.min_mach, lm.slope, lm.cutoff, lm.shock_reducer, lm.irey)
 
 
 class OracleMesh:
@@ -240,6 +246,23 @@ def rcm(N: int, ptr: np.ndarray) -> np.ndarray:
     return renum
 
 
+def ras_ilu0(N: int, ptr: np.ndarray, D: np.ndarray, A: np.ndarray, starts, y: np.ndarray, single=False):
+    """IluPrec, fill 0, RAS over the node ranges starts[s]..starts[s+1]:
+    x = M^-1 y with M the per-range ILU(0) of the H1 blocks D [N,5,5],
+    A [E,2,5,5] (or_jacobian_h1 layout) on the edge table ptr [E,2]."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    st = np.ascontiguousarray(starts, dtype=np.int32)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.empty_like(y)
+    rc = lib().or_ras_ilu0_apply(int(N), int(ptr.shape[0]), _i(ptr), _d(D), _d(A), int(st.size - 1), _i(st),
+                                 1 if single else 0, _d(y), _d(x))
+    if rc:
+        raise RuntimeError(f"zero pivot in ILU(0) at node {rc - 1}")
+    return x
+
+
 def color_edges(N: int, ptr: np.ndarray):
     ptr = np.ascontiguousarray(ptr, dtype=np.int32)
     color = np.empty(ptr.shape[0], np.int32)
@@ -256,6 +279,19 @@ def roeflux7(gam, pstiff, n, vn, Vl, Vr, use_ref=False):
         ref().refh_roeflux7(C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn), _d(Vl), _d(Vr), _d(out))
     else:
         lib().or_roeflux7(C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn), _d(Vl), _d(Vr), _d(out))
+    return out
+
+
+def roeflux5(gamma, gam, pstiff, n, vn, Vl, Vr, mach, k1, cmach, shock, irey, length, prec, use_ref=False):
+    """ROEFLUX5 (Roe-Turkel low-Mach dissipation), type 0, Ugr = Ug, Udr = Ud."""
+    out = np.empty(5)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    Vl = np.ascontiguousarray(Vl, dtype=np.float64)
+    Vr = np.ascontiguousarray(Vr, dtype=np.float64)
+    fn = ref().refh_roeflux5 if use_ref else lib().or_roeflux5
+    fn(C.c_double(gamma), C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn), _d(Vl), _d(Vr),
+       C.c_double(mach), C.c_double(k1), C.c_double(cmach), C.c_double(shock), C.c_double(irey), C.c_double(length),
+       C.c_int(prec), _d(out))
     return out
 
 
@@ -285,12 +321,19 @@ def wallflux(n, nvel, V):
     return out
 
 
-def roejac6(gam, pstiff, n, vn, Vl, Vr, sw, gamma_diss=1.0, use_ref=False):
-    """ROEJAC6 5x5 block (row-major)."""
+def roejac6(gam, pstiff, n, vn, Vl, Vr, sw, gamma_diss=1.0, use_ref=False, low_mach=None):
+    """ROEJAC6 5x5 block (row-major).  low_mach = (min Mach, slope, cut-off,
+    shock reducer, 1/Re, edge length) turns the Roe-Turkel preconditioning
+    on (prec 1)."""
     jac = np.empty(25)
     n = np.ascontiguousarray(n, dtype=np.float64)
     Vl = np.ascontiguousarray(Vl, dtype=np.float64)
     Vr = np.ascontiguousarray(Vr, dtype=np.float64)
+    if low_mach is not None:
+        fn = ref().refh_roejac6_lm if use_ref else lib().or_roejac6_lm
+        fn(C.c_double(gamma_diss), C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn), _d(Vl), _d(Vr),
+           C.c_int(sw), *[C.c_double(float(x)) for x in low_mach], C.c_int(1), _d(jac))
+        return jac.reshape(5, 5)
     if use_ref:
         ref().refh_roejac6(C.c_double(gamma_diss), C.c_double(gam), C.c_double(pstiff), _d(n), C.c_double(vn),
                            _d(Vl), _d(Vr), C.c_int(sw), _d(jac))

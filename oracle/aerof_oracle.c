@@ -529,7 +529,16 @@ int or_residual_ex(const or_mesh *m, const or_params *p, const double *U, double
     if (ierr) continue;
     const double *nl = m->normals + 3 * l;
     if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) continue;
-    or_roeflux7(gam, pstiff, nl, m->normal_vel[l], Vi, Vj, flux);
+    if (p->low_mach) {
+      /* FluxFcnSGApprJacRoeEuler3D::compute (FluxFcnDescSG.C:125-139) with
+       * the spatial preconditioner on; length = |X_j - X_i| (Edge.C:400),
+       * edge 1/Re = mean of the nodes' (Edge.C:412) */
+      double length = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+      or_roeflux5(1.0, gam, pstiff, nl, m->normal_vel[l], Vi, Vj, p->lm_min_mach, p->lm_slope, p->lm_cutoff,
+                  p->lm_shock, p->lm_irey, length, 1, flux);
+    } else {
+      or_roeflux7(gam, pstiff, nl, m->normal_vel[l], Vi, Vj, flux);
+    }
     for (int k = 0; k < 5; ++k) {
       R[5 * i + k] += flux[k];
       R[5 * j + k] -= flux[k];
@@ -788,6 +797,15 @@ int or_color_edges(int N, int E, const int *ptr, int *color) {
  * Schemes.NavierStokes.Gamma dissipation factor, gam the heat ratio. */
 void or_roejac6(double gamma_diss, double gam, double pstiff, const double *enormal,
                 double evitno, const double *Ug, const double *Ud, int sw, double *jac) {
+  or_roejac6_lm(gamma_diss, gam, pstiff, enormal, evitno, Ug, Ud, sw, 1.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0, jac);
+}
+
+/* ROEJAC6 with the spatial low-Mach constants (roejac6.f:130-160): with prec
+ * the Roe-Turkel beta of the edge replaces 1 (min Mach "mach", slope k1,
+ * cut-off cmach, shock reducer, 1/Re irey, edge length). */
+void or_roejac6_lm(double gamma_diss, double gam, double pstiff, const double *enormal, double evitno,
+                   const double *Ug, const double *Ud, int sw, double mach, double k1, double cmach,
+                   double shockreducer, double irey, double length, int prec, double *jac) {
   double H[5][5], tmw[5][5], dmat[5][5];
   double gam1 = gam - 1.0;
   double rnorm = sqrt(enormal[0] * enormal[0] + enormal[1] * enormal[1] + enormal[2] * enormal[2]);
@@ -837,6 +855,13 @@ void or_roejac6(double gamma_diss, double gam, double pstiff, const double *enor
   double gc = gam1 / (cr * cr);
   double usc2 = 0.5 / (cr * cr);
   double beta = 1.0;
+  if (prec) {
+    double shock = fabs(Ug[4] - Ud[4]) / (Ug[4] + Ud[4]) / length;
+    double locMach = sqrt(2.0 * qir * cr2);
+    beta = fmax(k1 * locMach, mach);
+    beta = (1.0 + sqrt(irey)) * beta + shockreducer * shock;
+    beta = fmin(beta, cmach);
+  }
   double betaRT2 = beta * beta;
   double vp1 = VdotN;
   double t = (1.0 - betaRT2) * VdotN;
@@ -1022,9 +1047,18 @@ void or_jacobian_h1(const or_mesh *m, const or_params *p, const double *V, const
     int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
     const double *nl = m->normals + 3 * l;
     if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) continue;
-    /* roejacappr3Dgas (FluxFcnDescSG.C:157-175), conservative Jacobians */
-    or_roejac6(1.0, gam, pstiff, nl, m->normal_vel[l], V + 5 * i, V + 5 * j, 1, dfdUi);
-    or_roejac6(1.0, gam, pstiff, nl, m->normal_vel[l], V + 5 * j, V + 5 * i, 2, dfdUj);
+    /* roejacappr3Dgas (FluxFcnDescSG.C:157-175), conservative Jacobians;
+     * length = |X_j - X_i| (Edge.C:4493-4494) */
+    const double *X = m->X;
+    double dx[3] = {X[3 * j] - X[3 * i], X[3 * j + 1] - X[3 * i + 1], X[3 * j + 2] - X[3 * i + 2]};
+    double length = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    int lm = p->low_mach;
+    or_roejac6_lm(1.0, gam, pstiff, nl, m->normal_vel[l], V + 5 * i, V + 5 * j, 1, lm ? p->lm_min_mach : 1.0,
+                  lm ? p->lm_slope : 0.0, lm ? p->lm_cutoff : 1.0, lm ? p->lm_shock : 0.0, lm ? p->lm_irey : 0.0,
+                  length, lm, dfdUi);
+    or_roejac6_lm(1.0, gam, pstiff, nl, m->normal_vel[l], V + 5 * j, V + 5 * i, 2, lm ? p->lm_min_mach : 1.0,
+                  lm ? p->lm_slope : 0.0, lm ? p->lm_cutoff : 1.0, lm ? p->lm_shock : 0.0, lm ? p->lm_irey : 0.0,
+                  length, lm, dfdUj);
     double *Aii = Adiag + 25 * (size_t)i, *Ajj = Adiag + 25 * (size_t)j;
     for (int k = 0; k < 25; ++k) { Aii[k] += dfdUi[k]; Ajj[k] -= dfdUj[k]; }
     double *Aij = Aedge + 50 * (size_t)l, *Aji = Aij + 25;
@@ -1607,7 +1641,14 @@ int or_residual_par(const or_mesh *m, const or_params *p, const double *U, doubl
       if (Vj[0] <= 0.0 || Vj[4] + pstiff <= 0.0) ++ierr;
       const double *nl = m->normals + 3 * (size_t)l;
       if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) continue;
-      or_roeflux7(gam, pstiff, nl, m->normal_vel[l], Vi, Vj, flux);
+      if (p->low_mach) {
+        double ex[3] = {X[3 * j] - X[3 * i], X[3 * j + 1] - X[3 * i + 1], X[3 * j + 2] - X[3 * i + 2]};
+        double length = sqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
+        or_roeflux5(1.0, gam, pstiff, nl, m->normal_vel[l], Vi, Vj, p->lm_min_mach, p->lm_slope, p->lm_cutoff,
+                    p->lm_shock, p->lm_irey, length, 1, flux);
+      } else {
+        or_roeflux7(gam, pstiff, nl, m->normal_vel[l], Vi, Vj, flux);
+      }
       double *rj = j < b ? R + 5 * (size_t)j : H + 5 * (size_t)(j - b);
       for (int k = 0; k < 5; ++k) {
         R[5 * (size_t)i + k] += flux[k];
@@ -1643,4 +1684,270 @@ int or_residual_par(const or_mesh *m, const or_params *p, const double *U, doubl
   for (int q = 0; q < P; ++q) free(hb[q]);
   free(hb);
   return ierr_total;
+}
+
+/* ---- RAS / ILU(0) ---------------------------------------------------------
+ * IluPrec<Scalar,5,double> with fill 0 and type RAS (KspPrec.C:127-320):
+ * SparseMat::numericILU (SparseMatrix.C:353-396) factors each subdomain's
+ * matrix in place (the L part scaled by the inverted pivot block, the pivot
+ * blocks inverted by invertDenseMatrix), SparseMat::lusol (SparseMatrix.C:
+ * 406-466) solves; RAS keeps each subdomain's solution on the subdomain's own
+ * nodes (killSlave).  A subdomain here is the node range [start[s],
+ * start[s+1]); its matrix holds the H1 blocks that couple two of its nodes,
+ * rows in node order and, within a row, columns in ascending order (the
+ * order of the reference's RCM-renumbered subdomain matrix when the mesh is
+ * numbered by RCM).  Blocks products are DENSEMATRIXTIMESDENSEMATRIX5 and
+ * DENSEMATRIXTIMESVECTOR5 (DenseMatrixOps.h:302-331, 441-446): each entry a
+ * left-to-right sum of five products, then assigned or subtracted. */
+#define OR_DEFINE_ILU(NAME, T, INV)                                                                         \
+  static int NAME(int N, int E, const int *ptr, const double *D, const double *A, int nsub,                 \
+                  const int *start, const double *y, double *x) {                                           \
+    int *cnt = (int *)calloc((size_t)N + 1, sizeof(int));                                                   \
+    for (int l = 0; l < E; ++l) {                                                                           \
+      int i = ptr[2 * l], j = ptr[2 * l + 1];                                                               \
+      cnt[i + 1]++;                                                                                         \
+      cnt[j + 1]++;                                                                                         \
+    }                                                                                                       \
+    for (int i = 0; i < N; ++i) cnt[i + 1] += cnt[i] + 0;                                                   \
+    /* full adjacency with the block of each entry (row i, column j) */                                     \
+    int *adj = (int *)malloc(sizeof(int) * 2 * (size_t)(E > 0 ? E : 1));                                   \
+    const double **blk = (const double **)malloc(sizeof(double *) * 2 * (size_t)(E > 0 ? E : 1));          \
+    int *fill = (int *)malloc(sizeof(int) * (size_t)(N > 0 ? N : 1));                                       \
+    for (int i = 0; i < N; ++i) fill[i] = cnt[i];                                                           \
+    for (int l = 0; l < E; ++l) {                                                                           \
+      int i = ptr[2 * l], j = ptr[2 * l + 1];                                                               \
+      adj[fill[i]] = j; blk[fill[i]++] = A + 50 * (size_t)l;                                                \
+      adj[fill[j]] = i; blk[fill[j]++] = A + 50 * (size_t)l + 25;                                           \
+    }                                                                                                       \
+    int rc = 0;                                                                                             \
+    for (int s = 0; s < nsub && !rc; ++s) {                                                                 \
+      const int a0 = start[s], n = start[s + 1] - start[s];                                                 \
+      /* local CSR: diagonal + couplings inside the subdomain, columns ascending */                         \
+      int *ia = (int *)malloc(sizeof(int) * ((size_t)n + 1)), *ju = (int *)malloc(sizeof(int) * (size_t)n); \
+      int nnz = 0;                                                                                          \
+      for (int r = 0; r < n; ++r) {                                                                         \
+        int i = a0 + r;                                                                                     \
+        nnz += 1;                                                                                           \
+        for (int q = cnt[i]; q < cnt[i + 1]; ++q)                                                           \
+          if (adj[q] >= a0 && adj[q] < a0 + n) nnz++;                                                       \
+      }                                                                                                     \
+      int *ja = (int *)malloc(sizeof(int) * (size_t)nnz);                                                   \
+      T *a = (T *)malloc(sizeof(T) * 25 * (size_t)nnz);                                                     \
+      int k = 0;                                                                                            \
+      for (int r = 0; r < n; ++r) {                                                                         \
+        int i = a0 + r;                                                                                     \
+        ia[r] = k;                                                                                          \
+        /* columns ascending: the neighbours of a row are not sorted here, so insert in order */           \
+        int cols[512];                                                                                      \
+        const double *src[512];                                                                             \
+        int nc = 0;                                                                                         \
+        cols[nc] = r; src[nc++] = D + 25 * (size_t)i;                                                       \
+        for (int q = cnt[i]; q < cnt[i + 1]; ++q)                                                           \
+          if (adj[q] >= a0 && adj[q] < a0 + n && nc < 512) { cols[nc] = adj[q] - a0; src[nc++] = blk[q]; }  \
+        for (int u = 1; u < nc; ++u)                                                                        \
+          for (int v = u; v > 0 && cols[v - 1] > cols[v]; --v) {                                            \
+            int tc = cols[v]; cols[v] = cols[v - 1]; cols[v - 1] = tc;                                      \
+            const double *ts = src[v]; src[v] = src[v - 1]; src[v - 1] = ts;                                \
+          }                                                                                                 \
+        for (int u = 0; u < nc; ++u, ++k) {                                                                 \
+          ja[k] = cols[u];                                                                                  \
+          if (cols[u] == r) ju[r] = k;                                                                      \
+          for (int e = 0; e < 25; ++e) a[25 * (size_t)k + e] = (T)src[u][e];                                \
+        }                                                                                                   \
+      }                                                                                                     \
+      ia[n] = k;                                                                                            \
+      /* numericILU */                                                                                      \
+      int *marker = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));                                   \
+      for (int r = 0; r < n; ++r) marker[r] = -1;                                                           \
+      for (int i = 0; i < n && !rc; ++i) {                                                                  \
+        for (int q = ia[i]; q < ia[i + 1]; ++q) marker[ja[q]] = q;                                          \
+        for (int q = ia[i]; q < ju[i]; ++q) {                                                               \
+          int l = ja[q];                                                                                    \
+          T mult[25], res[25];                                                                              \
+          const T *pa = a + 25 * (size_t)q, *pb = a + 25 * (size_t)ju[l];                                   \
+          for (int r5 = 0; r5 < 5; ++r5)                                                                    \
+            for (int c5 = 0; c5 < 5; ++c5)                                                                  \
+              mult[5 * r5 + c5] = pa[5 * r5] * pb[c5] + pa[5 * r5 + 1] * pb[5 + c5] +                       \
+                                  pa[5 * r5 + 2] * pb[10 + c5] + pa[5 * r5 + 3] * pb[15 + c5] +             \
+                                  pa[5 * r5 + 4] * pb[20 + c5];                                             \
+          for (int e = 0; e < 25; ++e) a[25 * (size_t)q + e] = mult[e];                                     \
+          for (int jj = ju[l] + 1; jj < ia[l + 1]; ++jj) {                                                  \
+            int mm = marker[ja[jj]];                                                                        \
+            if (mm == -1) continue;                                                                         \
+            const T *pj = a + 25 * (size_t)jj;                                                              \
+            for (int r5 = 0; r5 < 5; ++r5)                                                                  \
+              for (int c5 = 0; c5 < 5; ++c5)                                                                \
+                res[5 * r5 + c5] = mult[5 * r5] * pj[c5] + mult[5 * r5 + 1] * pj[5 + c5] +                  \
+                                   mult[5 * r5 + 2] * pj[10 + c5] + mult[5 * r5 + 3] * pj[15 + c5] +        \
+                                   mult[5 * r5 + 4] * pj[20 + c5];                                          \
+            for (int e = 0; e < 25; ++e) a[25 * (size_t)mm + e] -= res[e];                                  \
+          }                                                                                                 \
+        }                                                                                                   \
+        for (int q = ia[i]; q < ia[i + 1]; ++q) marker[ja[q]] = -1;                                         \
+        if (INV(a + 25 * (size_t)ju[i])) rc = a0 + i + 1;                                                   \
+      }                                                                                                     \
+      /* lusol: forward (unit L), backward with the inverted pivots */                                      \
+      if (!rc) {                                                                                            \
+        double *xs = x + 5 * (size_t)a0;                                                                    \
+        for (int i = 0; i < n; ++i) {                                                                       \
+          for (int c = 0; c < 5; ++c) xs[5 * i + c] = y[5 * ((size_t)a0 + i) + c];                          \
+          for (int q = ia[i]; q < ju[i]; ++q) {                                                             \
+            const T *pa = a + 25 * (size_t)q;                                                               \
+            const double *xb = xs + 5 * (size_t)ja[q];                                                      \
+            for (int r5 = 0; r5 < 5; ++r5)                                                                  \
+              xs[5 * i + r5] -= (double)pa[5 * r5] * xb[0] + (double)pa[5 * r5 + 1] * xb[1] +               \
+                                (double)pa[5 * r5 + 2] * xb[2] + (double)pa[5 * r5 + 3] * xb[3] +           \
+                                (double)pa[5 * r5 + 4] * xb[4];                                             \
+          }                                                                                                 \
+        }                                                                                                   \
+        for (int i = n - 1; i >= 0; --i) {                                                                  \
+          for (int q = ju[i] + 1; q < ia[i + 1]; ++q) {                                                     \
+            const T *pa = a + 25 * (size_t)q;                                                               \
+            const double *xb = xs + 5 * (size_t)ja[q];                                                      \
+            for (int r5 = 0; r5 < 5; ++r5)                                                                  \
+              xs[5 * i + r5] -= (double)pa[5 * r5] * xb[0] + (double)pa[5 * r5 + 1] * xb[1] +               \
+                                (double)pa[5 * r5 + 2] * xb[2] + (double)pa[5 * r5 + 3] * xb[3] +           \
+                                (double)pa[5 * r5 + 4] * xb[4];                                             \
+          }                                                                                                 \
+          const T *pd = a + 25 * (size_t)ju[i];                                                             \
+          double res[5];                                                                                    \
+          for (int r5 = 0; r5 < 5; ++r5)                                                                    \
+            res[r5] = (double)pd[5 * r5] * xs[5 * i] + (double)pd[5 * r5 + 1] * xs[5 * i + 1] +             \
+                      (double)pd[5 * r5 + 2] * xs[5 * i + 2] + (double)pd[5 * r5 + 3] * xs[5 * i + 3] +     \
+                      (double)pd[5 * r5 + 4] * xs[5 * i + 4];                                               \
+          for (int r5 = 0; r5 < 5; ++r5) xs[5 * i + r5] = res[r5];                                          \
+        }                                                                                                   \
+      }                                                                                                     \
+      free(marker); free(ia); free(ju); free(ja); free(a);                                                  \
+    }                                                                                                       \
+    free(cnt); free(adj); free(blk); free(fill);                                                            \
+    return rc;                                                                                              \
+  }
+OR_DEFINE_ILU(or_ras_ilu0_f64, double, or_invert5)
+OR_DEFINE_ILU(or_ras_ilu0_f32, float, or_invert5f)
+
+int or_ras_ilu0_apply(int N, int E, const int *ptr, const double *D, const double *A, int nsub, const int *start,
+                      int single, const double *y, double *x) {
+  return single ? or_ras_ilu0_f32(N, E, ptr, D, A, nsub, start, y, x)
+                : or_ras_ilu0_f64(N, E, ptr, D, A, nsub, start, y, x);
+}
+
+/* ROEFLUX5, type 0 (f77src/roeflux5.f): the Roe flux with the Roe-Turkel
+ * low-Mach preconditioning of the dissipation (prec != 0), with Ugr = Ug,
+ * Udr = Ud (FluxFcnSGApprJacRoeEuler3D without a Galerkin shift,
+ * FluxFcnDescSG.C:127-131).  gamma is the dissipation coefficient, gam the
+ * specific-heat ratio; mach, k1, cmach, shockreducer are the spatial
+ * preconditioner's minimum Mach, slope, cut-off Mach and shock parameter
+ * (LowMachPrec.h:119-124); irey the inverse Reynolds number; length the edge
+ * length. */
+void or_roeflux5(double gamma, double gam, double pstiff, const double *enormal, double evitno, const double *Ug,
+                 const double *Ud, double mach, double k1, double cmach, double shockreducer, double irey,
+                 double length, int prec, double *phi) {
+  const double *Ugr = Ug, *Udr = Ud;
+  double gam1 = gam - 1.0;
+  double rnorm = sqrt(enormal[0] * enormal[0] + enormal[1] * enormal[1] + enormal[2] * enormal[2]);
+  double invnorm = 1.0 / rnorm;
+  double normal[3] = {enormal[0] * invnorm, enormal[1] * invnorm, enormal[2] * invnorm};
+  double vitno = evitno * invnorm;
+  /* centred terms */
+  double VdotN = Ug[1] * normal[0] + Ug[2] * normal[1] + Ug[3] * normal[2];
+  double vitg2 = Ug[1] * Ug[1] + Ug[2] * Ug[2] + Ug[3] * Ug[3];
+  double H = gam * (Ug[4] + pstiff) + 0.5 * gam1 * Ug[0] * vitg2;
+  H = H / (gam1 * Ug[0]);
+  phi[0] = Ug[0] * (VdotN - vitno);
+  phi[1] = phi[0] * Ug[1] + Ug[4] * normal[0];
+  phi[2] = phi[0] * Ug[2] + Ug[4] * normal[1];
+  phi[3] = phi[0] * Ug[3] + Ug[4] * normal[2];
+  phi[4] = phi[0] * H + Ug[4] * vitno;
+  VdotN = Ud[1] * normal[0] + Ud[2] * normal[1] + Ud[3] * normal[2];
+  VdotN = VdotN - vitno;
+  double vitd2 = Ud[1] * Ud[1] + Ud[2] * Ud[2] + Ud[3] * Ud[3];
+  H = gam * (Ud[4] + pstiff) + 0.5 * gam1 * Ud[0] * vitd2;
+  H = H / (gam1 * Ud[0]);
+  phi[0] = phi[0] + Ud[0] * VdotN;
+  phi[1] = phi[1] + Ud[0] * Ud[1] * VdotN + Ud[4] * normal[0];
+  phi[2] = phi[2] + Ud[0] * Ud[2] * VdotN + Ud[4] * normal[1];
+  phi[3] = phi[3] + Ud[0] * Ud[3] * VdotN + Ud[4] * normal[2];
+  phi[4] = phi[4] + Ud[0] * VdotN * H + Ud[4] * vitno;
+  /* Roe-averaged state */
+  double squsr1 = sqrt(Ug[0]);
+  double squsr2 = sqrt(Ud[0]);
+  double ener1 = (Ug[4] + gam * pstiff) / gam1 + 0.5 * Ug[0] * vitg2;
+  double ener2 = (Ud[4] + gam * pstiff) / gam1 + 0.5 * Ud[0] * vitd2;
+  double usro = 1.0 / (squsr1 + squsr2);
+  double uar2 = (squsr1 * Ug[1] + squsr2 * Ud[1]) * usro;
+  double uar3 = (squsr1 * Ug[2] + squsr2 * Ud[2]) * usro;
+  double uar4 = (squsr1 * Ug[3] + squsr2 * Ud[3]) * usro;
+  double uar5 = ((ener1 + Ug[4]) / squsr1 + (ener2 + Ud[4]) / squsr2) * usro;
+  /* dissipation */
+  VdotN = normal[0] * uar2 + normal[1] * uar3 + normal[2] * uar4;
+  double qir = 0.5 * (uar2 * uar2 + uar3 * uar3 + uar4 * uar4);
+  double tet1 = normal[2] * uar3 - normal[1] * uar4;
+  double tet2 = normal[0] * uar4 - normal[2] * uar2;
+  double tet3 = normal[1] * uar2 - normal[0] * uar3;
+  double cr2 = gam1 * (uar5 - qir);
+  double cr = sqrt(cr2);
+  cr2 = 1.0 / cr2;
+  double beta;
+  if (prec == 0) {
+    beta = 1.0;
+  } else {
+    double shock = fabs(Ugr[4] - Udr[4]) / (Ugr[4] + Udr[4]) / length;
+    double locMach = sqrt(2.0 * qir * cr2);
+    beta = fmax(k1 * locMach, mach);
+    beta = (1.0 + sqrt(irey)) * beta + shockreducer * shock;
+    beta = fmin(beta, cmach);
+  }
+  double beta2 = beta * beta;
+  double energ = (Ugr[4] + gam * pstiff) / gam1 + 0.5 * Ugr[0] * (Ugr[1] * Ugr[1] + Ugr[2] * Ugr[2] + Ugr[3] * Ugr[3]);
+  double enerd = (Udr[4] + gam * pstiff) / gam1 + 0.5 * Udr[0] * (Udr[1] * Udr[1] + Udr[2] * Udr[2] + Udr[3] * Udr[3]);
+  double dif1 = -Ugr[0] + Udr[0];
+  double dif2 = -Ugr[0] * Ugr[1] + Udr[0] * Udr[1];
+  double dif3 = -Ugr[0] * Ugr[2] + Udr[0] * Udr[2];
+  double dif4 = -Ugr[0] * Ugr[3] + Udr[0] * Udr[3];
+  double dif5 = -energ + enerd;
+  double vp1 = VdotN;
+  double sq = ((1.0 - beta2) * VdotN) * ((1.0 - beta2) * VdotN);
+  double vp4 = 0.5 * ((1.0 + beta2) * VdotN + sqrt(sq + 4.0 * beta2 * (cr * cr)));
+  double vp5 = 0.5 * ((1.0 + beta2) * VdotN - sqrt(sq + 4.0 * beta2 * (cr * cr)));
+  /* Roe-Turkel coefficients */
+  double r = vp4 - vp1 * beta2;
+  double s = vp5 - vp1 * beta2;
+  double t = 0.5 * (vp5 - vp4);
+  vp1 = vp1 - vitno;
+  vp4 = vp4 - vitno;
+  vp5 = vp5 - vitno;
+  double flur1 = fabs(vp1) * ((normal[0] * (1.0 - gam1 * qir * cr2) - tet1) * dif1 + (normal[0] * gam1 * uar2 * cr2) * dif2 +
+                              (normal[2] + (normal[0] * gam1 * uar3 * cr2)) * dif3 +
+                              (-normal[1] + (normal[0] * gam1 * uar4 * cr2)) * dif4 - (normal[0] * gam1 * cr2) * dif5);
+  double flur2 = fabs(vp1) * ((normal[1] * (1.0 - gam1 * qir * cr2) - tet2) * dif1 +
+                              (-normal[2] + (normal[1] * gam1 * uar2 * cr2)) * dif2 + (normal[1] * gam1 * uar3 * cr2) * dif3 +
+                              (normal[0] + (normal[1] * gam1 * uar4 * cr2)) * dif4 - (normal[1] * gam1 * cr2) * dif5);
+  double flur3 = fabs(vp1) * ((normal[2] * (1.0 - gam1 * qir * cr2) - tet3) * dif1 +
+                              (normal[1] + (normal[2] * gam1 * uar2 * cr2)) * dif2 +
+                              (-normal[0] + (normal[2] * gam1 * uar3 * cr2)) * dif3 + (normal[2] * gam1 * uar4 * cr2) * dif4 -
+                              (normal[2] * gam1 * cr2) * dif5);
+  double c2 = cr * cr;
+  double flur4 = fabs(vp4) * ((c2 * VdotN / t + gam1 * qir * s / (t * beta2)) * dif1 -
+                              (c2 * normal[0] / t + gam1 * uar2 * s / (t * beta2)) * dif2 -
+                              (c2 * normal[1] / t + gam1 * uar3 * s / (t * beta2)) * dif3 -
+                              (c2 * normal[2] / t + gam1 * uar4 * s / (t * beta2)) * dif4 + gam1 * s / (t * beta2) * dif5);
+  double flur5 = fabs(vp5) * (-(c2 * VdotN / t + gam1 * qir * r / (beta2 * t)) * dif1 +
+                              (c2 * normal[0] / t + gam1 * uar2 * r / (beta2 * t)) * dif2 +
+                              (c2 * normal[1] / t + gam1 * uar3 * r / (beta2 * t)) * dif3 +
+                              (c2 * normal[2] / t + gam1 * uar4 * r / (beta2 * t)) * dif4 - gam1 * r / (beta2 * t) * dif5);
+  phi[0] = phi[0] - gamma * (normal[0] * flur1 + normal[1] * flur2 + normal[2] * flur3 + 0.5 * (flur4 + flur5) * cr2);
+  phi[1] = phi[1] - gamma * ((uar2 * normal[0]) * flur1 + (uar2 * normal[1] - normal[2]) * flur2 +
+                             (uar2 * normal[2] + normal[1]) * flur3 +
+                             (normal[0] * (r * flur4 + s * flur5) + uar2 * (flur4 + flur5)) * 0.5 * cr2);
+  phi[2] = phi[2] - gamma * ((uar3 * normal[0] + normal[2]) * flur1 + (uar3 * normal[1]) * flur2 +
+                             (uar3 * normal[2] - normal[0]) * flur3 +
+                             (normal[1] * (r * flur4 + s * flur5) + uar3 * (flur4 + flur5)) * 0.5 * cr2);
+  phi[3] = phi[3] - gamma * ((uar4 * normal[0] - normal[1]) * flur1 + (uar4 * normal[1] + normal[0]) * flur2 +
+                             (uar4 * normal[2]) * flur3 +
+                             (normal[2] * (r * flur4 + s * flur5) + uar4 * (flur4 + flur5)) * 0.5 * cr2);
+  phi[4] = phi[4] - gamma * ((qir * normal[0] + tet1) * flur1 + (qir * normal[1] + tet2) * flur2 +
+                             (qir * normal[2] + tet3) * flur3 +
+                             (VdotN * (r * flur4 + s * flur5) + uar5 * (flur4 + flur5)) * 0.5 * cr2);
+  for (int k = 0; k < 5; ++k) phi[k] = phi[k] * 0.5 * rnorm;
 }

@@ -67,6 +67,11 @@ void or_vanalbada5(double beta, const double *Vi, const double *ddVij,
                    double *Vji);
 
 /* ROEFLUX7 (f77src/roeflux7.f:1-202), type 0 (5 equations). */
+/* ROEFLUX5, type 0 (f77src/roeflux5.f), Ugr = Ug, Udr = Ud: the Roe flux
+ * with the Roe-Turkel low-Mach preconditioning of the dissipation. */
+void or_roeflux5(double gamma, double gam, double pstiff, const double *enormal, double evitno, const double *Ug,
+                 const double *Ud, double mach, double k1, double cmach, double shockreducer, double irey,
+                 double length, int prec, double *phi);
 void or_roeflux7(double gam, double pstiff, const double *enormal,
                  double evitno, const double *Ug, const double *Ud,
                  double *phi);
@@ -74,6 +79,14 @@ void or_roeflux7(double gam, double pstiff, const double *enormal,
 /* BOUNDFLUX5 (f77src/boundflux5.f:1-111), type 0. */
 void or_boundflux5(double gam, const double *normal, double sig,
                    const double *ua, const double *ub, double *flux);
+
+/* ROEJAC6 (f77src/roejac6.f), type 0: prec 0, and with the spatial
+ * low-Mach constants (prec 1: Roe-Turkel beta of the edge). */
+void or_roejac6(double gamma_diss, double gam, double pstiff, const double *enormal,
+                double evitno, const double *Ug, const double *Ud, int sw, double *jac);
+void or_roejac6_lm(double gamma_diss, double gam, double pstiff, const double *enormal, double evitno,
+                   const double *Ug, const double *Ud, int sw, double mach, double k1, double cmach,
+                   double shockreducer, double irey, double length, int prec, double *jac);
 
 /* FluxFcnSGWallEuler3D::compute (FluxFcnDescSG.C:779-789). */
 void or_wallflux(const double *normal, double normal_vel, const double *V,
@@ -87,6 +100,10 @@ typedef struct {
   double beta;       /* MUSCL beta (Schemes.NavierStokes.Beta) */
   int second_order;  /* 1: linear reconstruction + Van Albada; 0: constant */
   double rho_inf, p_inf, mach, alpha, yaw;  /* freestream (radians) */
+  /* spatial low-Mach preconditioner (SpatialLowMachPrec, LowMachPrec.h):
+   * 1 = Roe-Turkel flux ROEFLUX5 and its ROEJAC6 Jacobian */
+  int low_mach;
+  double lm_min_mach, lm_slope, lm_cutoff, lm_shock, lm_irey;
 } or_params;
 
 typedef struct {
@@ -171,6 +188,16 @@ int or_gmres_mvp(const or_mesh *m, const or_params *p, const double *V, const do
 int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
                         int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
                         double geps, double *history, int *converged, int *krylov_its);
+
+/* IluPrec<Scalar,5,double>, fill 0, RAS (KspPrec.C:127-320): per node range
+ * [start[s], start[s+1]) the ILU(0) of the H1 blocks coupling two of its
+ * nodes (SparseMat::numericILU, SparseMatrix.C:353-396; factors in FP32 when
+ * single != 0) and its solve (SparseMat::lusol, SparseMatrix.C:406-466); x
+ * keeps each subdomain's solution on its own nodes.  D [N][25] and A
+ * [E][2][25] as or_jacobian_h1 writes them.  Returns 0, or 1 + the first
+ * node with a zero pivot. */
+int or_ras_ilu0_apply(int N, int E, const int *ptr, const double *D, const double *A, int nsub, const int *start,
+                      int single, const double *y, double *x);
 
 #ifdef __cplusplus
 }

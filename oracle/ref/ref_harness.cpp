@@ -22,6 +22,25 @@ void roejac6_(const int *type, const double *gamma, const double *gam,
               const double *shockreducer, const double *irey,
               const double *length, const int *prec);
 
+void roeflux5_(const int *type, const double *gamma, const double *gam, const double *pstiff, double *enormal,
+               const double *evitno, double *Ugr, double *Ug, double *Udr, double *Ud, double *phi,
+               const double *mach, const double *k1, const double *cmach, const double *shockreducer,
+               const double *irey, const double *length, const int *prec);
+
+// ROEFLUX5 (f77src/roeflux5.f) with type = 0 and the reconstructed states
+// as both argument pairs (Ugr = Ug, Udr = Ud), as FluxFcnSGApprJacRoeEuler3D
+// calls it without a Galerkin shift (FluxFcnDescSG.C:127-131).
+void refh_roeflux5(double gamma, double gam, double pstiff, const double *n, double vn, const double *Vl,
+                   const double *Vr, double mach, double k1, double cmach, double shock, double irey, double length,
+                   int prec, double *phi) {
+  int type = 0;
+  double nn[3] = {n[0], n[1], n[2]};
+  double a[7] = {0}, b[7] = {0}, f[7] = {0};
+  for (int k = 0; k < 5; ++k) { a[k] = Vl[k]; b[k] = Vr[k]; }
+  roeflux5_(&type, &gamma, &gam, &pstiff, nn, &vn, a, a, b, b, f, &mach, &k1, &cmach, &shock, &irey, &length, &prec);
+  for (int k = 0; k < 5; ++k) phi[k] = f[k];
+}
+
 // ROEFLUX7 (f77src/roeflux7.f) with type = 0.
 void refh_roeflux7(double gam, double pstiff, const double *n, double vn,
                    const double *Vl, const double *Vr, double *phi) {
@@ -46,6 +65,17 @@ void refh_boundflux5(double gam, const double *n, double sig, const double *ua,
 
 // ROEJAC6 (f77src/roejac6.f) with type = 0, prec = 0, as called by
 // roejacappr3Dgas (FluxFcnDescSG.C:171-172).
+void refh_roejac6_lm(double gamma, double gam, double pstiff, const double *n, double vn, const double *Vl,
+                     const double *Vr, int sw, double mach, double k1, double cmach, double shock, double irey,
+                     double length, int prec, double *jac) {
+  int type = 0;
+  double nn[3] = {n[0], n[1], n[2]};
+  double a[7] = {0}, b[7] = {0}, j[49] = {0};
+  for (int k = 0; k < 5; ++k) { a[k] = Vl[k]; b[k] = Vr[k]; }
+  roejac6_(&type, &gamma, &gam, &pstiff, nn, &vn, a, b, j, &sw, &mach, &k1, &cmach, &shock, &irey, &length, &prec);
+  for (int k = 0; k < 25; ++k) jac[k] = j[k];
+}
+
 void refh_roejac6(double gamma, double gam, double pstiff, const double *n,
                   double vn, const double *Vl, const double *Vr, int sw,
                   double *jac) {

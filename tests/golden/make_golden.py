@@ -52,6 +52,38 @@ def main():
                         roejac6_sw1=jac1, roejac6_sw2=jac2, blocks=A, inv_f64=inv64,
                         inv_f32=inv32.astype(np.float32))
     print("wrote", os.path.join(HERE, "ref_kernels.npz"))
+    write_lowmach()
+
+
+def write_lowmach():
+    """tests/golden/ref_lowmach.npz: ROEFLUX5 (Roe-Turkel low-Mach
+    dissipation) and ROEJAC6 with the preconditioning on, from the compiled
+    reference, on seeded states, normals, edge lengths and preconditioner
+    parameters (ROEFLUX5 also with the preconditioning off)."""
+    rng = np.random.default_rng(20241203)
+    n = 400
+    gam = 1.4
+    nrm = rng.normal(size=(n, 3)) * rng.uniform(1e-3, 1.0, (n, 1))
+    VL, VR = states(rng, n), states(rng, n)
+    VL[:, 1:4] *= 0.1  # low-Mach states: |u| well below the speed of sound
+    VR[:, 1:4] *= 0.1
+    par = np.column_stack([rng.uniform(0.005, 0.2, n),   # minimum Mach
+                           rng.uniform(0.5, 2.0, n),     # slope k1
+                           rng.uniform(0.5, 1.0, n),     # cut-off Mach
+                           rng.uniform(0.0, 0.5, n),     # shock parameter
+                           np.where(np.arange(n) % 2 == 0, 0.0, rng.uniform(0.0, 1e-3, n)),  # 1/Re
+                           rng.uniform(0.01, 0.2, n)])   # edge length
+    prec = (np.arange(n) % 4 != 0).astype(np.int32)
+    out = np.array([O.roeflux5(1.0, gam, 0.0, nrm[k], 0.0, VL[k], VR[k], *par[k], int(prec[k]), use_ref=True)
+                    for k in range(n)])
+    # ROEJAC6 with the preconditioning on (roejacappr3Dgas, FluxFcnDescSG.C:157-175)
+    j1 = np.array([O.roejac6(gam, 0.0, nrm[k], 0.0, VL[k], VR[k], 1, low_mach=par[k], use_ref=True)
+                   for k in range(n)])
+    j2 = np.array([O.roejac6(gam, 0.0, nrm[k], 0.0, VR[k], VL[k], 2, low_mach=par[k], use_ref=True)
+                   for k in range(n)])
+    np.savez_compressed(os.path.join(HERE, "ref_lowmach.npz"), gamma=gam, normal=nrm, VL=VL, VR=VR, params=par,
+                        prec=prec, roeflux5=out, roejac6_sw1=j1, roejac6_sw2=j2)
+    print("wrote", os.path.join(HERE, "ref_lowmach.npz"))
 
 
 if __name__ == "__main__":

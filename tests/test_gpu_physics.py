@@ -83,3 +83,26 @@ def test_linpack_inverse_bit_exact(gpu, golden):
     bad, inv32 = _eval(gpu, AEROFX_PHYS_INVERT5_F32, 0.0, [blocks], 25)
     assert bad == 0
     _same(inv32.astype(np.float32).reshape(-1, 5, 5), g["inv_f32"], "LINPACK inverse FP32")
+
+
+def test_roeflux5_bit_exact(gpu):
+    """ROEFLUX5 (roeflux5.f, Roe-Turkel low-Mach dissipation) on 400 seeded
+    low-Mach cases, three in four with the preconditioning on
+    (tests/golden/ref_lowmach.npz)."""
+    from aerof_dev_amd._abi import AEROFX_PHYS_ROEFLUX5
+    g = dict(np.load(os.path.join(os.path.dirname(GOLDEN), "ref_lowmach.npz")))
+    par = np.column_stack([g["params"], g["prec"].astype(np.float64)])
+    _, out = _eval(gpu, AEROFX_PHYS_ROEFLUX5, float(g["gamma"]), [g["normal"], g["VL"], g["VR"], par], 5)
+    _same(out, g["roeflux5"], "roeflux5")
+
+
+def test_roejac6_low_mach_bit_exact(gpu):
+    """ROEJAC6 with the Roe-Turkel preconditioning on, both sides, on the
+    400 low-Mach cases of tests/golden/ref_lowmach.npz."""
+    from aerof_dev_amd._abi import AEROFX_PHYS_ROEJAC6_LM_SW1, AEROFX_PHYS_ROEJAC6_LM_SW2
+    g = dict(np.load(os.path.join(os.path.dirname(GOLDEN), "ref_lowmach.npz")))
+    par = np.column_stack([g["params"], np.ones(len(g["VL"]))])
+    _, j1 = _eval(gpu, AEROFX_PHYS_ROEJAC6_LM_SW1, float(g["gamma"]), [g["normal"], g["VL"], g["VR"], par], 25)
+    _, j2 = _eval(gpu, AEROFX_PHYS_ROEJAC6_LM_SW2, float(g["gamma"]), [g["normal"], g["VR"], g["VL"], par], 25)
+    _same(j1, g["roejac6_sw1"].reshape(-1, 25), "roejac6 low-Mach sw1")
+    _same(j2, g["roejac6_sw2"].reshape(-1, 25), "roejac6 low-Mach sw2")
