@@ -57,7 +57,9 @@ struct DevTables {
   const double *cv;      // [N] ctrlVol
   // gradient pass: node adjacency, rows sorted ascending (= edge-id order)
   int ngchunks;
-  const int *gchunk;     // [ngchunks+1] row starts
+  const int *gchunk;     // [ngchunks+1] row starts; a row of more than kBlock entries is a chunk alone
+  int nglong;
+  const int *glong;      // [nglong] the rows of more than kBlock entries
   const int *rowptr;     // [N+1]
   const int *adj;        // [2E]
   const int *eidx;       // [2E] edge id of each adjacency entry
@@ -65,7 +67,8 @@ struct DevTables {
   // flux pass A: edges grouped by their lower node i (edge order)
   int nfchunks;
   int fblock;            // edges per flux block: kFluxBlockSmall or kFluxBlock
-  const int4 *fchunk4;   // [nfchunks] (first row, end row, first edge, end edge) of each chunk
+  const int4 *fchunk4;   // [nfchunks] (first row, end row, first edge, end edge) of each chunk;
+                         // a row of more than fblock upper edges is cut into row-less pieces
   const int *eup;        // [N+1] first edge whose lower node is i
   const int2 *eij;       // [E] edge endpoints (i, j), i < j
 
@@ -142,6 +145,7 @@ struct aerofx_ctx {
   double *d_P = nullptr;     // [N][5] per-node sums over upper edges
   int *d_err = nullptr;      // [0] reconstruction errors, [1] singular blocks, [2] bad states
   int residual_kernels = AEROFX_RESIDUAL_PER_LANE;  // AEROFX_RESIDUAL_PER_LANE or _STAGED
+  int max_degree = 0;        // most neighbours of a node
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
   // with the reason in noalloc_err)

@@ -135,15 +135,21 @@ void destroy(aerofx_ctx *c) {
   delete c;
 }
 
-// Whole-row chunks with at most `limit` items and kMaxRows rows.
-std::vector<int> chunk_rows(const std::vector<int> &rowptr, int N, int limit, const char **why) {
+// Whole-row chunks with at most `limit` items and kMaxRows rows; a row with
+// more than `limit` items is a chunk of its own (the kernels hand such
+// chunks to their long-row variants) and is listed in *longrows.
+std::vector<int> chunk_rows(const std::vector<int> &rowptr, int N, int limit, std::vector<int> *longrows) {
   std::vector<int> ch(1, 0);
   int acc = 0, rows = 0;
   for (int n = 0; n < N; ++n) {
     int cnt = rowptr[n + 1] - rowptr[n];
     if (cnt > limit) {
-      *why = "node degree exceeds the rows a block can hold";
-      return {};
+      if (rows) ch.push_back(n);
+      ch.push_back(n + 1);
+      longrows->push_back(n);
+      acc = 0;
+      rows = 0;
+      continue;
     }
     if (acc + cnt > limit || rows == kMaxRows) {
       ch.push_back(n);
@@ -153,8 +159,35 @@ std::vector<int> chunk_rows(const std::vector<int> &rowptr, int N, int limit, co
     acc += cnt;
     rows += 1;
   }
-  ch.push_back(N);
+  if (rows || ch.size() == 1) ch.push_back(N);
   return ch;
+}
+
+// Flux chunks (first row, end row, first edge, end edge): whole rows with at
+// most `limit` upper edges and kMaxRows rows; a row with more upper edges is
+// cut into pieces of `limit` edges with no rows (first row = end row): its
+// flux sum is then formed by the node pass, in the same edge order.
+std::vector<int4> flux_chunks(const std::vector<int> &eup, int N, int limit) {
+  std::vector<int4> out;
+  int r0 = 0, acc = 0;
+  auto close = [&](int r1) {
+    if (r1 > r0) out.push_back(make_int4(r0, r1, eup[r0], eup[r1]));
+    r0 = r1;
+    acc = 0;
+  };
+  for (int n = 0; n < N; ++n) {
+    const int cnt = eup[n + 1] - eup[n];
+    if (cnt > limit) {
+      close(n);
+      for (int l = eup[n]; l < eup[n + 1]; l += limit) out.push_back(make_int4(n, n, l, std::min(l + limit, eup[n + 1])));
+      r0 = n + 1;
+      continue;
+    }
+    if (acc + cnt > limit || n - r0 == kMaxRows) close(n);
+    acc += cnt;
+  }
+  close(N);
+  return out;
 }
 
 }  // namespace
@@ -273,20 +306,16 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
         for (int a = 0; a < 3; ++a) fidx[fill[m->faces[3 * f + a]]++] = f;
       }
     }
-    const char *why = nullptr;
-    std::vector<int> gchunk = chunk_rows(rowptr, N, kBlock, &why);
+    // rows with more than kBlock neighbours: chunks of their own, served by
+    // the long-row kernels of the Jacobian and H1 passes
+    std::vector<int> glong;
+    std::vector<int> gchunk = chunk_rows(rowptr, N, kBlock, &glong);
+    for (int n = 0; n < N; ++n) c->max_degree = std::max(c->max_degree, rowptr[n + 1] - rowptr[n]);
     int maxup = 0;
     for (int n = 0; n < N; ++n) maxup = std::max(maxup, eup[n + 1] - eup[n]);
     const int fblock = maxup <= kFluxBlockSmall ? kFluxBlockSmall : kFluxBlock;
-    std::vector<int> fchunk = why ? std::vector<int>() : chunk_rows(eup, N, fblock, &why);
-    if (why) {
-      destroy(c);
-      return set_error(AEROFX_ERR_MESH, std::string("ctx_create: ") + why);
-    }
     // one 16-B record per flux chunk: a block reads its rows and edges at once
-    std::vector<int4> fchunk4(fchunk.empty() ? 0 : fchunk.size() - 1);
-    for (size_t k = 0; k + 1 < fchunk.size(); ++k)
-      fchunk4[k] = make_int4(fchunk[k], fchunk[k + 1], eup[fchunk[k]], eup[fchunk[k + 1]]);
+    std::vector<int4> fchunk4 = flux_chunks(eup, N, fblock);
     // node-derived tables
     std::vector<double> X4(4 * (size_t)N), lsq6(6 * (size_t)N), invcv(N), fnorm3(3 * (size_t)F);
     for (int n = 0; n < N; ++n) {
@@ -311,7 +340,7 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     t.E = E;
     t.F = F;
     if ((rc = upload(c, X4, &t.X4)) || (rc = upload(c, lsq6, &t.lsq6)) || (rc = upload(c, invcv, &t.invcv)) || (rc = upload(c, c->cv, &t.cv)) ||
-        (rc = upload(c, gchunk, &t.gchunk)) || (rc = upload(c, rowptr, &t.rowptr)) ||
+        (rc = upload(c, gchunk, &t.gchunk)) || (rc = upload(c, glong, &t.glong)) || (rc = upload(c, rowptr, &t.rowptr)) ||
         (rc = upload(c, adj, &t.adj)) || (rc = upload(c, eid, &t.eidx)) || (rc = upload(c, mirror, &t.mirror)) || (rc = upload(c, fchunk4, &t.fchunk4)) || (rc = upload(c, eup, &t.eup)) ||
         (rc = upload(c, eij, &t.eij)) || (rc = upload(c, ledge, &t.ledge)) ||
         (rc = upload(c, c->normals, &t.normal)) || (rc = upload(c, lrow, &t.lrow)) ||
@@ -321,7 +350,8 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
       return rc;
     }
     t.ngchunks = (int)gchunk.size() - 1;
-    t.nfchunks = (int)fchunk.size() - 1;
+    t.nglong = (int)glong.size();
+    t.nfchunks = (int)fchunk4.size();
     t.fblock = fblock;
     // LDS-staged strips (both passes or neither)
     {

@@ -96,7 +96,84 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *
 // SIMD; asking for two waves spills ~18 registers to scratch and is faster
 // (assembly 50 -> 38 ms over a 96x32x32 FP64 steady solve; three waves spill
 // 116 and are slower).
-// LM: the low-Mach (Roe-Turkel) edge blocks.
+// Adjacency entry e = (n, m) of H1: its off-diagonal block A_nm (scaled by
+// 1/V_n) into B, and its contribution to the diagonal block of n into dg
+// (dF/dU_i for n = i, -dF/dU_j for n = j).  LM: the low-Mach (Roe-Turkel)
+// edge blocks, with the edge length |X_j - X_i| (Edge.C:4493-4494).
+template <class T, bool LM>
+__device__ __forceinline__ void jac_entry(const DevTables &t, const double *__restrict__ V, const FluxConsts &fc,
+                                          int n, int e, T *__restrict__ B, double dg[25]) {
+  const Gas &g = fc.gas;
+  const int m = t.adj[e], l = t.eidx[e];
+  const double nl[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
+  T *Be = B + blk64_base(e);
+  if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) {
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+      Be[k * kBlkStride] = (T)0;
+      dg[k] = 0.0;
+    }
+    return;
+  }
+  double Vn[5], Vm[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    Vn[k] = V[5 * (size_t)n + k];
+    Vm[k] = V[5 * (size_t)m + k];
+  }
+  const double voln = t.invcv[n];
+  double length = 1.0;
+  if (LM) {
+    const double4 xn = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)n);
+    const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
+    const double d0 = n < m ? xm.x - xn.x : xn.x - xm.x, d1 = n < m ? xm.y - xn.y : xn.y - xm.y,
+                 d2 = n < m ? xm.z - xn.z : xn.z - xm.z;
+    length = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  }
+  double blk[25];
+  if (n < m) {
+    // n = i: A_ii += dF/dU_i, A_ij = dF/dU_j / V_i
+    roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk, LM, fc.lm, length);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) dg[k] = blk[k];
+    roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk, LM, fc.lm, length);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 + blk[k] * voln);
+  } else {
+    // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
+    roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk, LM, fc.lm, length);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) dg[k] = -blk[k];
+    roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk, LM, fc.lm, length);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 - blk[k] * voln);
+  }
+}
+
+// Boundary-face Jacobian of face slot q (a face of node n) into fjac.
+__device__ __forceinline__ void jac_face(const DevTables &t, const double *__restrict__ V, const FluxConsts &fc,
+                                         int n, int q, double *__restrict__ fjac) {
+  const int face = t.fidx[q];
+  const int code = t.fcode[face];
+  const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1], t.fnorm3[3 * (size_t)face + 2]};
+  double Vn[5], jac[25];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
+  if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+    wall_jac(fc.gas, nv, 0.0, Vn, jac);
+  } else {
+    const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
+    farfield_jac(fc.gas, nv, 0.0, Vn, ub, jac);
+  }
+#pragma unroll
+  for (int k = 0; k < 25; ++k) fjac[25 * (size_t)q + k] = jac[k];
+}
+
+// Two roejac6 evaluations inline take 256 VGPRs + 18 AGPRs, one wave per
+// SIMD; asking for two waves spills ~18 registers to scratch and is faster
+// (assembly 50 -> 38 ms over a 96x32x32 FP64 steady solve; three waves spill
+// 116 and are slower).  A chunk that is one row of more than kBlock entries
+// is left to k_jacobian_long.
 template <class T, bool LM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_jacobian(DevTables t, const double *__restrict__ V, FluxConsts fc,
                                                      const double *__restrict__ dt, double *__restrict__ fjac,
@@ -106,8 +183,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
   const int c = xcd_chunk(blockIdx.x, t.ngchunks);
   const int r0 = t.gchunk[c], r1 = t.gchunk[c + 1];
   const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
+  if (e1 - e0 > kBlock) return;
   const int tid = threadIdx.x, nr = r1 - r0;
-  const Gas &g = fc.gas;
   if (tid <= nr) srow[tid] = t.rowptr[r0 + tid] - e0;
   __syncthreads();
   if (tid < e1 - e0) {
@@ -118,53 +195,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       const int mid = (lo + hi) >> 1;
       if (srow[mid] <= tid) lo = mid; else hi = mid;
     }
-    const int n = r0 + lo;
-    const int m = t.adj[e], l = t.eidx[e];
-    const double nl[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
-    T *Be = B + blk64_base(e);
-    if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) {
+    double dg[25];
+    jac_entry<T, LM>(t, V, fc, r0 + lo, e, B, dg);
 #pragma unroll
-      for (int k = 0; k < 25; ++k) {
-        Be[k * kBlkStride] = (T)0;
-        sd[k * kPad + tid] = 0.0;
-      }
-    } else {
-      double Vn[5], Vm[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        Vn[k] = V[5 * (size_t)n + k];
-        Vm[k] = V[5 * (size_t)m + k];
-      }
-      const double voln = t.invcv[n];
-      // low-Mach: the edge length |X_j - X_i| of Turkel's beta (Edge.C:4493-4494)
-      double length = 1.0;
-      if (LM) {
-        const double4 xn = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)n);
-        const double4 xm = *reinterpret_cast<const double4 *>(t.X4 + 4 * (size_t)m);
-        const double d0 = n < m ? xm.x - xn.x : xn.x - xm.x, d1 = n < m ? xm.y - xn.y : xn.y - xm.y,
-                     d2 = n < m ? xm.z - xn.z : xn.z - xm.z;
-        length = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-      }
-      const bool lm = LM;
-      double blk[25];
-      if (n < m) {
-        // n = i: A_ii += dF/dU_i, A_ij = dF/dU_j / V_i
-        roejac6(1.0, g, nl, 0.0, Vn, Vm, 1, blk, lm, fc.lm, length);
-#pragma unroll
-        for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = blk[k];
-        roejac6(1.0, g, nl, 0.0, Vm, Vn, 2, blk, lm, fc.lm, length);
-#pragma unroll
-        for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 + blk[k] * voln);
-      } else {
-        // n = j: A_jj -= dF/dU_j, A_ji = -dF/dU_i / V_j
-        roejac6(1.0, g, nl, 0.0, Vn, Vm, 2, blk, lm, fc.lm, length);
-#pragma unroll
-        for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = -blk[k];
-        roejac6(1.0, g, nl, 0.0, Vm, Vn, 1, blk, lm, fc.lm, length);
-#pragma unroll
-        for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)(0.0 - blk[k] * voln);
-      }
-    }
+    for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = dg[k];
   }
   // boundary-face Jacobians of the block's rows, one per face slot
   const int q0 = t.frow[r0], q1 = t.frow[r1];
@@ -174,22 +208,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       const int mid = (lo + hi) >> 1;
       if (t.frow[r0 + mid] <= q) lo = mid; else hi = mid;
     }
-    const int n = r0 + lo;
-    const int face = t.fidx[q];
-    const int code = t.fcode[face];
-    const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
-                          t.fnorm3[3 * (size_t)face + 2]};
-    double Vn[5], jac[25];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
-    if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
-      wall_jac(g, nv, 0.0, Vn, jac);
-    } else {
-      const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
-      farfield_jac(g, nv, 0.0, Vn, ub, jac);
-    }
-#pragma unroll
-    for (int k = 0; k < 25; ++k) fjac[25 * (size_t)q + k] = jac[k];
+    jac_face(t, V, fc, r0 + lo, q, fjac);
   }
   __syncthreads();
   for (int idx = tid; idx < nr * 25; idx += kBlock) {
@@ -201,6 +220,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     s *= t.invcv[n];
     if (dt && k % 6 == 0) s += 1.0 / dt[n];
     D[blk64_base(n) + k * kBlkStride] = (T)s;
+  }
+}
+
+// One row of more than kBlock entries per block: its entries in passes of
+// kBlock, the diagonal sums carried across passes in the entries' order --
+// the sums of k_jacobian, term for term.
+template <class T, bool LM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_jacobian_long(DevTables t, const double *__restrict__ V, FluxConsts fc,
+                                                          const double *__restrict__ dt, double *__restrict__ fjac,
+                                                          T *__restrict__ D, T *__restrict__ B) {
+  __shared__ double sd[25 * kPad];
+  const int n = t.glong[blockIdx.x];
+  const int e0 = t.rowptr[n], e1 = t.rowptr[n + 1];
+  const int tid = threadIdx.x;
+  double s = 0.0;  // lanes 0..24: element tid of the diagonal block
+  for (int base = e0; base < e1; base += kBlock) {
+    if (base + tid < e1) {
+      double dg[25];
+      jac_entry<T, LM>(t, V, fc, n, base + tid, B, dg);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) sd[k * kPad + tid] = dg[k];
+    }
+    __syncthreads();
+    const int cnt = min(kBlock, e1 - base);
+    if (tid < 25)
+      for (int q = 0; q < cnt; ++q) s += sd[tid * kPad + q];
+    __syncthreads();
+  }
+  for (int q = t.frow[n] + tid; q < t.frow[n + 1]; q += kBlock) jac_face(t, V, fc, n, q, fjac);
+  __syncthreads();
+  if (tid < 25) {
+    for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) s += fjac[25 * (size_t)q + tid];
+    s *= t.invcv[n];
+    if (dt && tid % 6 == 0) s += 1.0 / dt[n];
+    D[blk64_base(n) + tid * kBlkStride] = (T)s;
   }
 }
 
@@ -264,6 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__res
   const int c = xcd_chunk(blockIdx.x, t.ngchunks);
   const int r0 = t.gchunk[c], r1 = t.gchunk[c + 1];
   const int e0 = t.rowptr[r0], e1 = t.rowptr[r1];
+  if (e1 - e0 > kBlock) return;  // a long row: k_h1_apply_long
   const int tid = threadIdx.x;
   if (tid < e1 - e0) {
     const int e = e0 + tid;
@@ -300,6 +355,54 @@ __global__ __launch_bounds__(kBlock) void k_h1_apply(DevTables t, const T *__res
     for (int e = b; e < eend; ++e) s += sc[q * kPad + e];
     y[5 * (size_t)n + q] = s;
   }
+}
+
+// k_h1_apply for one row of more than kBlock entries per block: the entries'
+// products in passes of kBlock through LDS, summed in entry order after the
+// diagonal product.
+template <class T, bool TRANS, bool MIRRORED = false>
+__global__ __launch_bounds__(kBlock) void k_h1_apply_long(DevTables t, const T *__restrict__ D,
+                                                          const T *__restrict__ B, const double *__restrict__ p,
+                                                          double *__restrict__ y, const int *__restrict__ done) {
+  if (done && *done) return;
+  __shared__ double sc[5 * kPad];
+  const int n = t.glong[blockIdx.x];
+  const int e0 = t.rowptr[n], e1 = t.rowptr[n + 1];
+  const int tid = threadIdx.x;
+  double s = 0.0;  // lanes 0..4: component tid
+  if (tid < 5) {
+    double pn[5], dd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pn[k] = p[5 * (size_t)n + k];
+    if (TRANS)
+      amtv5_add(D + blk64_base(n), pn, dd);
+    else
+      amv5_add(D + blk64_base(n), pn, dd);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (k == tid) s = dd[k];
+  }
+  for (int base = e0; base < e1; base += kBlock) {
+    const int e = base + tid;
+    if (e < e1) {
+      const int m = t.adj[e];
+      double pm[5], cc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) pm[k] = p[5 * (size_t)m + k];
+      if (TRANS)
+        amtv5_add(B + blk64_base(MIRRORED ? e : t.mirror[e]), pm, cc);
+      else
+        amv5_add(B + blk64_base(e), pm, cc);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) sc[k * kPad + tid] = cc[k];
+    }
+    __syncthreads();
+    const int cnt = min(kBlock, e1 - base);
+    if (tid < 5)
+      for (int q = 0; q < cnt; ++q) s += sc[tid * kPad + q];
+    __syncthreads();
+  }
+  if (tid < 5) y[5 * (size_t)n + tid] = s;
 }
 
 // x = Di y (DiagMat::apply, DiagMatrix.C:86-100), or x = Di^T y
@@ -483,6 +586,15 @@ int launch_timestep(aerofx_ctx *c, double cfl, double *dt) {
   return check_launch("timestep launch");
 }
 
+template <class T, bool LM>
+static void jacobian_launch(aerofx_ctx *c, int nchunks, const double *dt) {
+  hipLaunchKernelGGL((k_jacobian<T, LM>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->fc, dt,
+                     c->d_fjac, (T *)c->d_jacD, (T *)c->d_jacB);
+  if (c->tab.nglong)
+    hipLaunchKernelGGL((k_jacobian_long<T, LM>), dim3(c->tab.nglong), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
+                       c->fc, dt, c->d_fjac, (T *)c->d_jacD, (T *)c->d_jacB);
+}
+
 int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
   int rc = ensure_jacobian_storage(c, precision);
   if (rc) return rc;
@@ -493,17 +605,13 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
     ProfScope ps(c, AEROFX_TIMER_FV_JAC);
     const bool lm = c->fc.low_mach != 0;
     if (precision && lm)
-      hipLaunchKernelGGL((k_jacobian<float, true>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
-                         c->fc, dt, c->d_fjac, (float *)c->d_jacD, (float *)c->d_jacB);
+      jacobian_launch<float, true>(c, nchunks, dt);
     else if (precision)
-      hipLaunchKernelGGL((k_jacobian<float, false>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
-                         c->fc, dt, c->d_fjac, (float *)c->d_jacD, (float *)c->d_jacB);
+      jacobian_launch<float, false>(c, nchunks, dt);
     else if (lm)
-      hipLaunchKernelGGL((k_jacobian<double, true>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
-                         c->fc, dt, c->d_fjac, (double *)c->d_jacD, (double *)c->d_jacB);
+      jacobian_launch<double, true>(c, nchunks, dt);
     else
-      hipLaunchKernelGGL((k_jacobian<double, false>), dim3(nchunks), dim3(kBlock), 0, c->stream, c->tab, c->d_V,
-                         c->fc, dt, c->d_fjac, (double *)c->d_jacD, (double *)c->d_jacB);
+      jacobian_launch<double, false>(c, nchunks, dt);
   }
   if ((rc = check_launch("jacobian launch"))) return rc;
   {
@@ -564,15 +672,26 @@ template <class T>
 static void h1_launch(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {
   const int nb = c->tab.ngchunks;
   const bool mirrored = c->mirror_version == c->jac_version && c->mirror_precision == c->jac_precision;
-  if (trans && mirrored)
+  const int nl = c->tab.nglong;
+  if (trans && mirrored) {
     hipLaunchKernelGGL((k_h1_apply<T, true, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
                        (const T *)c->d_jacD, (const T *)c->jac_mirror[c->jac_precision], p, y, done);
-  else if (trans)
+    if (nl)
+      hipLaunchKernelGGL((k_h1_apply_long<T, true, true>), dim3(nl), dim3(kBlock), 0, c->stream, c->tab,
+                         (const T *)c->d_jacD, (const T *)c->jac_mirror[c->jac_precision], p, y, done);
+  } else if (trans) {
     hipLaunchKernelGGL((k_h1_apply<T, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
                        (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
-  else
+    if (nl)
+      hipLaunchKernelGGL((k_h1_apply_long<T, true>), dim3(nl), dim3(kBlock), 0, c->stream, c->tab,
+                         (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
+  } else {
     hipLaunchKernelGGL((k_h1_apply<T, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->tab,
                        (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
+    if (nl)
+      hipLaunchKernelGGL((k_h1_apply_long<T, false>), dim3(nl), dim3(kBlock), 0, c->stream, c->tab,
+                         (const T *)c->d_jacD, (const T *)c->d_jacB, p, y, done);
+  }
 }
 
 int launch_h1_apply_flag(aerofx_ctx *c, const double *p, double *y, const int *done, bool trans) {

@@ -1140,7 +1140,8 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_adjoint_stats{};
   // block GMRES takes FP64 blocks and makes its own FP32 copies (precision 1)
-  if ((rc = derivative_operators(c, U, o->cfl, nrhs > 1 ? 0 : o->precision, "direct_solve"))) return rc;
+  const bool blocked = nrhs > 1 && c->max_degree <= 64;
+  if ((rc = derivative_operators(c, U, o->cfl, blocked ? 0 : o->precision, "direct_solve"))) return rc;
   const int bm = std::min(o->gmres_m, 16);
   FgOps ops;
   ops.apply = [&](const double *xv, double *y, int nq) -> int {
@@ -1152,11 +1153,16 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
     return 0;
   };
   ops.prec = [&](const double *v, double *z, int nq, int *inner) -> int {
-    if (nq == 1) {
-      GmresOut go;
-      int r = gmres(c, v, z, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go);
-      if (r) return r;
-      *inner += go.its;
+    // one GMRES per right-hand side, or one block GMRES for all of them
+    // (whose batched H1 apply takes nodes of at most 64 neighbours)
+    if (nq == 1 || !blocked) {
+      for (int q = 0; q < nq; ++q) {
+        const size_t off = (size_t)q * c->ksp->n;
+        GmresOut go;
+        int r = gmres(c, v + off, z + off, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go);
+        if (r) return r;
+        *inner += go.its;
+      }
       return 0;
     }
     aerofx_block_stats bs;

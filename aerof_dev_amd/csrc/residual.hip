@@ -448,7 +448,8 @@ __global__ __launch_bounds__(kFluxStrip) void k_flux_staged(DevTables t, const d
 }
 
 // ---------------------------------------------------------------------------
-// 3. node assembly: lower edges, upper-edge partial sum, faces, 1/V
+// 3. node assembly: lower edges, upper-edge partial sum (formed here from the
+//    edge fluxes for a row of more than fblock upper edges), faces, 1/V
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ Fm,
@@ -466,8 +467,19 @@ __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double
 #pragma unroll
     for (int k = 0; k < 5; ++k) s[k] -= Fm[base + k];
   }
+  const int u0 = t.eup[n], u1 = t.eup[n + 1];
+  if (u1 - u0 > t.fblock) {
+    // a row cut over several flux blocks: its upper-edge sum, in edge order
+    double p[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int l = u0; l < u1; ++l)
 #pragma unroll
-  for (int k = 0; k < 5; ++k) s[k] += P[5 * (size_t)n + k];
+      for (int k = 0; k < 5; ++k) p[k] += Fm[5 * (size_t)l + k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] += p[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] += P[5 * (size_t)n + k];
+  }
   const int f0 = t.frow[n], f1 = t.frow[n + 1];
   if (f0 < f1) {
     double Vn[5];
