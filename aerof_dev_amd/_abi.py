@@ -87,7 +87,8 @@ class NewtonOpts(C.Structure):
                 ("cfl0", C.c_double), ("cfl_max", C.c_double), ("ser", C.c_double),
                 ("cfl_coef1", C.c_double), ("cfl_coef2", C.c_double),
                 ("gmres_m", C.c_int), ("gmres_maxits", C.c_int), ("gmres_eps", C.c_double),
-                ("mvp", C.c_int), ("fd_order", C.c_int), ("prec", C.c_int), ("prec_subdomain", C.c_int)]
+                ("mvp", C.c_int), ("fd_order", C.c_int), ("prec", C.c_int), ("prec_subdomain", C.c_int),
+                ("eps_formula", C.c_int), ("failsafe", C.c_int), ("check_final_res", C.c_int)]
 
 
 class BlockStats(C.Structure):
@@ -108,7 +109,8 @@ class AdjointStats(C.Structure):
 class NewtonStats(C.Structure):
     _fields_ = [("steps", C.c_int), ("converged", C.c_int), ("krylov_its", C.c_int),
                 ("res0", C.c_double), ("res", C.c_double), ("final_cfl", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("failsafe_redos", C.c_int), ("saturated", C.c_int),
+                ("final_res_failures", C.c_int)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/aerofx.h
@@ -131,6 +133,10 @@ SIGNATURES = {
     "aerofx_ctx_sync": (C.c_int, [C.c_void_p]),
     "aerofx_ctx_set_residual_kernels": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_set_low_mach": (C.c_int, [C.c_void_p, C.POINTER(LowMachC)]),
+    "aerofx_ctx_set_failsafe": (C.c_int, [C.c_void_p, C.c_int]),
+    "aerofx_ctx_set_krylov_check": (C.c_int, [C.c_void_p, C.c_int]),
+    "aerofx_ctx_reset_failsafe": (C.c_int, [C.c_void_p]),
+    "aerofx_ctx_failsafe_nodes": (C.c_int, [C.c_void_p, c_int_p]),
     "aerofx_ctx_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_timers": (C.c_int, [C.c_void_p, c_double_p, C.POINTER(C.c_longlong)]),
     "aerofx_ctx_tables": (C.c_int, [C.c_void_p, c_int_p, c_double_p, c_double_p, c_double_p, c_double_p]),

@@ -1144,6 +1144,7 @@ extern "C" int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const d
   if (!ctx || !U || !lambda) return set_error(AEROFX_ERR_ARG, "adjoint_residual: null argument");
   if (ctx->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_residual: not available with low-Mach preconditioning");
+  if (int rf = refuse_failsafe_nodes(ctx, "adjoint_residual")) return rf;
   if (!dRdU_T_lambda && !dRdX_T_lambda) return AEROFX_OK;
   int rc = launch_adjoint(ctx, U, lambda, dRdU_T_lambda, dRdX_T_lambda);
   if (rc) return rc;

@@ -52,6 +52,7 @@ inline size_t blk64_count(size_t nblocks) { return (nblocks + 63) / 64 * 64 * 25
 struct DevTables {
   int N, E, F;
   const double *X4;      // [N][4] coordinates padded to 32 B
+  const int *fs_mask;    // [N] failsafe first-order nodes (gradients zeroed), or null
   const double *lsq6;    // [N][6] (1/r11, 1/r22, 1/r33, r12/r11, r23/r22, psi)
   const double *invcv;   // [N] 1/ctrlVol
   const double *cv;      // [N] ctrlVol
@@ -145,6 +146,12 @@ struct aerofx_ctx {
   double *d_P = nullptr;     // [N][5] per-node sums over upper edges
   int *d_err = nullptr;      // [0] reconstruction errors, [1] singular blocks, [2] bad states
   int residual_kernels = AEROFX_RESIDUAL_PER_LANE;  // AEROFX_RESIDUAL_PER_LANE or _STAGED
+  // Newton.FailSafe: 0 off, 1 on, 2 always on; d_fs [3N] = tag0, tag1, mask
+  // (DistSVec<int,2> tag of Domain and the node tags of DistNodalGrad)
+  int failsafe = 0;
+  int *d_fs = nullptr;
+  bool fs_any = false;  // some node is first order by the failsafe
+  int check_final_res = 0;  // KspData checkFinalRes: GMRES reports -999 above its target
   int max_degree = 0;        // most neighbours of a node
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
@@ -222,7 +229,17 @@ int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const
 // the error count still goes to c->d_err[0]
 struct ResidualBufs {
   double *V, *grad, *Fm, *P;
+  int *tag0 = nullptr;  // failsafe: sides of non-physical reconstructions are tagged here
 };
+// The residual with Newton.FailSafe (Domain.C:864-917): with the failsafe on
+// and a non-physical reconstruction in the pass, the nodes are tagged
+// (finalizeTags), their gradients zeroed from then on (DistNodalGrad::fix and
+// limit) and the pass re-run.  Synchronises; *count = the non-physical
+// reconstructions of the pass that stands.
+int residual_failsafe(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale, int *count);
+// AEROFX_ERR_STATE while failsafe first-order nodes are set (the derivative
+// passes differentiate the full second-order residual)
+int refuse_failsafe_nodes(aerofx_ctx *c, const char *who);
 int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale,
                          const int *done = nullptr);
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz);

@@ -462,6 +462,50 @@ int aerofx_ctx_set_low_mach(aerofx_ctx *c, const aerofx_low_mach *lm) {
   return AEROFX_OK;
 }
 
+int aerofx_ctx_set_failsafe(aerofx_ctx *c, int mode) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "set_failsafe: null ctx");
+  if (mode < 0 || mode > 2) return set_error(AEROFX_ERR_ARG, "set_failsafe: 0 (off), 1 (on) or 2 (always on)");
+  if (mode && !c->d_fs) {
+    void *p = nullptr;
+    int rc = ctx_alloc(c, 3 * sizeof(int) * (size_t)c->N, &p);
+    if (rc) return rc;
+    c->d_fs = static_cast<int *>(p);
+    hipError_t e = hipMemsetAsync(c->d_fs, 0, 3 * sizeof(int) * (size_t)c->N, c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "set_failsafe");
+  }
+  c->failsafe = mode;
+  c->tab.fs_mask = mode ? c->d_fs + 2 * (size_t)c->N : nullptr;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_set_krylov_check(aerofx_ctx *c, int check_final_res) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "set_krylov_check: null ctx");
+  c->check_final_res = check_final_res ? 1 : 0;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_reset_failsafe(aerofx_ctx *c) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "reset_failsafe: null ctx");
+  if (c->d_fs) {
+    hipError_t e = hipMemsetAsync(c->d_fs, 0, 3 * sizeof(int) * (size_t)c->N, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "reset_failsafe");
+  }
+  c->fs_any = false;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_failsafe_nodes(aerofx_ctx *c, int *tags) {
+  if (!c || !tags) return set_error(AEROFX_ERR_ARG, "failsafe_nodes: null argument");
+  if (!c->d_fs) {
+    std::fill(tags, tags + 3 * (size_t)c->N, 0);
+    return AEROFX_OK;
+  }
+  hipError_t e = hipMemcpyAsync(tags, c->d_fs, 3 * sizeof(int) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "failsafe_nodes");
+}
+
 int aerofx_ctx_sync(aerofx_ctx *c) {
   if (!c) return set_error(AEROFX_ERR_ARG, "sync: null ctx");
   hipError_t e = hipStreamSynchronize(c->stream);
@@ -511,11 +555,10 @@ int aerofx_error_count(aerofx_ctx *c, int *count) {
 
 int aerofx_residual_ex(aerofx_ctx *c, const double *U, double *R, int flags) {
   if (!c || !U || !R) return set_error(AEROFX_ERR_ARG, "residual: null argument");
-  int rc = launch_residual(c, U, R, (flags & 1) == 0);
-  if (rc) return rc;
-  if (flags & 2) return AEROFX_OK;
+  const ResidualBufs b{c->d_V, c->d_grad, c->d_Fm, c->d_P};
+  if (flags & 2) return launch_residual_with(c, b, U, R, (flags & 1) == 0);
   int cnt = 0;
-  rc = aerofx_error_count(c, &cnt);
+  int rc = residual_failsafe(c, b, U, R, (flags & 1) == 0, &cnt);
   return rc ? rc : cnt;
 }
 
@@ -537,11 +580,9 @@ int aerofx_residual_noalloc(aerofx_ctx *c, const double *U, double *R, int flags
   char *base = c->noalloc_scratch;
   const ResidualBufs b{reinterpret_cast<double *>(base + oV), reinterpret_cast<double *>(base + oG),
                        reinterpret_cast<double *>(base + oF), reinterpret_cast<double *>(base + oP)};
-  int rc = launch_residual_with(c, b, U, R, (flags & 1) == 0);
-  if (rc) return rc;
-  if (flags & 2) return AEROFX_OK;
+  if (flags & 2) return launch_residual_with(c, b, U, R, (flags & 1) == 0);
   int cnt = 0;
-  rc = aerofx_error_count(c, &cnt);
+  int rc = residual_failsafe(c, b, U, R, (flags & 1) == 0, &cnt);
   return rc ? rc : cnt;
 }
 

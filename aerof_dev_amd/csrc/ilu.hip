@@ -31,7 +31,6 @@ namespace {
 
 constexpr int kIluThreads = 256;
 constexpr int kIluGroup = 32;                            // lanes per row in the factorization
-constexpr int kIluGroups = kIluThreads / kIluGroup;      // rows factored at once
 constexpr int kIluMaxRow = 32;                           // entries per row (diagonal included)
 constexpr int kIluMaxB = 1024;                           // nodes per subdomain
 
@@ -48,6 +47,13 @@ struct Ilu {
   int *flevptr = nullptr;  // [nsub+1] into flev
   int *frow = nullptr;     // [N] rows (global) in forward level order
   int *blev = nullptr, *blevptr = nullptr, *brow = nullptr;  // the same for the backward solve
+  // per position of the forward / backward schedule: (row - subdomain start,
+  // first entry, end entry, diagonal entry) of the solve's sweep over the row
+  int4 *fdesc = nullptr, *bdesc = nullptr;
+  // the factorization's per-entry tables: pivot block of the column's row
+  // (lower entries), and the updates [pptr[k], pptr[k+1]) of lower entry k:
+  // upper block pjj of the column's row into entry pmm of the row
+  int *ldiag = nullptr, *pptr = nullptr, *pjj = nullptr, *pmm = nullptr;
   void *LU[2] = {nullptr, nullptr};  // [nnz][25] factors, FP64 / FP32
   int lu_precision = -1;
   long long lu_version = -1;
@@ -78,30 +84,40 @@ int upload_vec(aerofx_ctx *c, const std::vector<T> &v, T **dst) {
 // blocks are staged in LDS (converted to T, as IluPrec::getData copies the
 // H1 blocks into the Scalar matrix), the lower entries are processed in
 // ascending column order (25 lanes, one block entry each), the pivot block is
-// inverted by one lane, and the row goes back to global memory.
+// inverted by one lane, and the row goes back to global memory.  For each
+// lower entry k the position of the pivot block of its column's row
+// (ldiag[k]) and the updates it makes -- the upper blocks jj of that row
+// whose column is present further along row i, and where (pmm) -- are
+// listed on the host, so a step reads them directly instead of searching.
+// FP32 factors take 16 rows per pass, FP64 8 (the LDS of the row stages).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kIluThreads) void k_ilu_factor(int N, int B, const int *__restrict__ rowptr,
-                                                            const int *__restrict__ ju, const int *__restrict__ col,
-                                                            const int *__restrict__ src, const int *__restrict__ flev,
-                                                            const int *__restrict__ flevptr,
-                                                            const int *__restrict__ frow, const T *__restrict__ D,
-                                                            const T *__restrict__ Boff, T *__restrict__ LU,
-                                                            int *__restrict__ err) {
-  __shared__ T rowb[kIluGroups][kIluMaxRow * 25];
-  __shared__ T mult[kIluGroups][25];
-  __shared__ int cols[kIluGroups][kIluMaxRow];
+struct IluFactorShape {
+  static constexpr int groups = sizeof(T) == 4 ? 16 : 8;
+  static constexpr int threads = groups * kIluGroup;
+};
+
+template <class T>
+__global__ __launch_bounds__(IluFactorShape<T>::threads) void k_ilu_factor(
+    int B, const int *__restrict__ rowptr, const int *__restrict__ ju, const int *__restrict__ src,
+    const int *__restrict__ ldiag, const int *__restrict__ pptr, const int *__restrict__ pjj,
+    const int *__restrict__ pmm, const int *__restrict__ flev, const int *__restrict__ flevptr,
+    const int *__restrict__ frow, const T *__restrict__ D, const T *__restrict__ Boff, T *__restrict__ LU,
+    int *__restrict__ err) {
+  constexpr int G = IluFactorShape<T>::groups;
+  __shared__ T rowb[G][kIluMaxRow * 25];
+  __shared__ T mult[G][25];
   __shared__ int maxd;
   const int s = blockIdx.x;
   const int g = threadIdx.x / kIluGroup, lane = threadIdx.x % kIluGroup;
   const int r5 = lane / 5, c5 = lane - 5 * (lane / 5);
-  const int a0 = s * B;
   const int l0 = flevptr[s], l1 = flevptr[s + 1];
+  (void)B;
   for (int lev = l0; lev + 1 < l1; ++lev) {
     const int q0 = flev[lev], q1 = flev[lev + 1];
-    // rows of this level, kIluGroups at a time; all loop bounds below are
-    // uniform over the workgroup, so every thread meets every barrier
-    for (int base = q0; base < q1; base += kIluGroups) {
+    // rows of this level, G at a time; all loop bounds below are uniform over
+    // the workgroup, so every thread meets every barrier
+    for (int base = q0; base < q1; base += G) {
       const int q = base + g;
       const bool active = q < q1;
       int gi = 0, r0 = 0, nr = 0, dpos = 0;
@@ -118,23 +134,19 @@ __global__ __launch_bounds__(kIluThreads) void k_ilu_factor(int N, int B, const 
           rowb[g][idx] = sq < 0 ? D[blk64_base(gi) + (size_t)kk * kBlkStride]
                                 : Boff[blk64_base(sq) + (size_t)kk * kBlkStride];
         }
-        for (int e = lane; e < nr; e += kIluGroup) cols[g][e] = col[r0 + e];
         if (lane == 0) atomicMax(&maxd, dpos);
       }
       __syncthreads();
       const int nd = maxd;
       for (int e = 0; e < nd; ++e) {
         const bool go = active && e < dpos;
-        int gl = 0;
+        const int k = r0 + e;
         T m = T(0);
-        if (go) {
+        if (go && lane < 25) {
           // mult = a_k inv(U_ll)  (applyToDenseMatrix, DENSEMATRIXTIMESDENSEMATRIX5)
-          gl = a0 + cols[g][e];
-          if (lane < 25) {
-            const T *Ul = LU + 25 * (size_t)ju[gl];
-            const T *pa = rowb[g] + 25 * e + 5 * r5;
-            m = pa[0] * Ul[c5] + pa[1] * Ul[5 + c5] + pa[2] * Ul[10 + c5] + pa[3] * Ul[15 + c5] + pa[4] * Ul[20 + c5];
-          }
+          const T *Ul = LU + 25 * (size_t)ldiag[k];
+          const T *pa = rowb[g] + 25 * e + 5 * r5;
+          m = pa[0] * Ul[c5] + pa[1] * Ul[5 + c5] + pa[2] * Ul[10 + c5] + pa[3] * Ul[15 + c5] + pa[4] * Ul[20 + c5];
         }
         __syncthreads();
         if (go && lane < 25) {
@@ -142,20 +154,15 @@ __global__ __launch_bounds__(kIluThreads) void k_ilu_factor(int N, int B, const 
           mult[g][lane] = m;
         }
         __syncthreads();
-        if (go) {
+        if (go && lane < 25) {
           // a_m -= mult a_j over the upper entries j of row l present in row i
-          const int j1 = rowptr[gl + 1];
-          for (int jj = ju[gl] + 1; jj < j1; ++jj) {
-            const int cj = col[jj];
-            int mm = -1;
-            for (int t = e + 1; t < nr; ++t)
-              if (cols[g][t] == cj) mm = t;
-            if (mm < 0 || lane >= 25) continue;
-            const T *pj = LU + 25 * (size_t)jj;
-            const T *pm = mult[g] + 5 * r5;
+          const T *pm = mult[g] + 5 * r5;
+          const int p1 = pptr[k + 1];
+          for (int pp = pptr[k]; pp < p1; ++pp) {
+            const T *pj = LU + 25 * (size_t)pjj[pp];
             const T res = pm[0] * pj[c5] + pm[1] * pj[5 + c5] + pm[2] * pj[10 + c5] + pm[3] * pj[15 + c5] +
                           pm[4] * pj[20 + c5];
-            rowb[g][25 * mm + lane] -= res;
+            rowb[g][25 * pmm[pp] + lane] -= res;
           }
         }
         __syncthreads();
@@ -180,17 +187,48 @@ __global__ __launch_bounds__(kIluThreads) void k_ilu_factor(int N, int B, const 
 // lusol, level by level: five lanes per row (one per component), the
 // subdomain's x in LDS.  Forward: x_i = y_i, then x_i -= a_k x_col(k) over the
 // lower entries in order; backward: x_i -= a_k x_col(k) over the upper
-// entries in order, then x_i = inv(U_ii) x_i.
+// entries in order, then x_i = inv(U_ii) x_i.  A row's schedule record gives
+// its entry range in one load, and its entries' columns and block rows are
+// fetched eight at a time before any is used, so a level costs about two
+// memory latencies rather than one per entry; the subtractions still run one
+// entry at a time in ascending order.
 // ---------------------------------------------------------------------------
+constexpr int kIluBatch = 8;
+
 template <class T>
-__global__ __launch_bounds__(kIluThreads) void k_ilu_solve(int N, int B, const int *__restrict__ rowptr,
-                                                           const int *__restrict__ ju, const int *__restrict__ col,
+__device__ __forceinline__ void ilu_row_sweep(int k0, int k1, int r5, const int *__restrict__ col,
+                                              const T *__restrict__ LU, const double *xs, double &v) {
+  for (int kb = k0; kb < k1; kb += kIluBatch) {
+    int cl[kIluBatch];
+    T a[kIluBatch][5];
+#pragma unroll
+    for (int u = 0; u < kIluBatch; ++u) {
+      const int k = kb + u < k1 ? kb + u : kb;
+      cl[u] = col[k];
+      const T *pa = LU + 25 * (size_t)k + 5 * r5;
+#pragma unroll
+      for (int cc = 0; cc < 5; ++cc) a[u][cc] = pa[cc];
+    }
+    // past the row's end the batch repeats its first entry and subtracts
+    // 0 (v - 0 is v): the loads stay ahead of every use
+#pragma unroll
+    for (int u = 0; u < kIluBatch; ++u) {
+      const double *xb = xs + 5 * cl[u];
+      const double t = (double)a[u][0] * xb[0] + (double)a[u][1] * xb[1] + (double)a[u][2] * xb[2] +
+                       (double)a[u][3] * xb[3] + (double)a[u][4] * xb[4];
+      v -= kb + u < k1 ? t : 0.0;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kIluThreads) void k_ilu_solve(int N, int B, const int *__restrict__ col,
                                                            const int *__restrict__ flev,
                                                            const int *__restrict__ flevptr,
-                                                           const int *__restrict__ frow,
+                                                           const int4 *__restrict__ fdesc,
                                                            const int *__restrict__ blev,
                                                            const int *__restrict__ blevptr,
-                                                           const int *__restrict__ brow, const T *__restrict__ LU,
+                                                           const int4 *__restrict__ bdesc, const T *__restrict__ LU,
                                                            const double *__restrict__ y, double *__restrict__ x,
                                                            const int *__restrict__ done) {
   if (done && *done) return;
@@ -209,16 +247,10 @@ __global__ __launch_bounds__(kIluThreads) void k_ilu_solve(int N, int B, const i
     for (int base = q0; base < q1; base += kRows) {
       const int q = base + rr;
       if (rr < kRows && q < q1) {
-        const int gi = frow[q];
-        const int li = gi - a0;
-        double v = xs[5 * li + r5];
-        for (int k = rowptr[gi]; k < ju[gi]; ++k) {
-          const T *pa = LU + 25 * (size_t)k + 5 * r5;
-          const double *xb = xs + 5 * col[k];
-          v -= (double)pa[0] * xb[0] + (double)pa[1] * xb[1] + (double)pa[2] * xb[2] + (double)pa[3] * xb[3] +
-               (double)pa[4] * xb[4];
-        }
-        xs[5 * li + r5] = v;
+        const int4 d = fdesc[q];
+        double v = xs[5 * d.x + r5];
+        ilu_row_sweep<T>(d.y, d.z, r5, col, LU, xs, v);
+        xs[5 * d.x + r5] = v;
       }
       __syncthreads();
     }
@@ -229,25 +261,22 @@ __global__ __launch_bounds__(kIluThreads) void k_ilu_solve(int N, int B, const i
     for (int base = q0; base < q1; base += kRows) {
       const int q = base + rr;
       const bool act = rr < kRows && q < q1;
-      int li = 0, gi = 0;
+      int4 d = make_int4(0, 0, 0, 0);
+      T piv[5];
       if (act) {
-        gi = brow[q];
-        li = gi - a0;
-        double v = xs[5 * li + r5];
-        for (int k = ju[gi] + 1; k < rowptr[gi + 1]; ++k) {
-          const T *pa = LU + 25 * (size_t)k + 5 * r5;
-          const double *xb = xs + 5 * col[k];
-          v -= (double)pa[0] * xb[0] + (double)pa[1] * xb[1] + (double)pa[2] * xb[2] + (double)pa[3] * xb[3] +
-               (double)pa[4] * xb[4];
-        }
+        d = bdesc[q];
+        const T *pd = LU + 25 * (size_t)d.w + 5 * r5;
+#pragma unroll
+        for (int cc = 0; cc < 5; ++cc) piv[cc] = pd[cc];
+        double v = xs[5 * d.x + r5];
+        ilu_row_sweep<T>(d.y, d.z, r5, col, LU, xs, v);
         tmp[tid] = v;
       }
       __syncthreads();
       if (act) {
-        const T *pd = LU + 25 * (size_t)ju[gi] + 5 * r5;
         const double *tv = tmp + 5 * rr;
-        xs[5 * li + r5] = (double)pd[0] * tv[0] + (double)pd[1] * tv[1] + (double)pd[2] * tv[2] +
-                          (double)pd[3] * tv[3] + (double)pd[4] * tv[4];
+        xs[5 * d.x + r5] = (double)piv[0] * tv[0] + (double)piv[1] * tv[1] + (double)piv[2] * tv[2] +
+                           (double)piv[3] * tv[3] + (double)piv[4] * tv[4];
       }
       __syncthreads();
     }
@@ -361,8 +390,36 @@ int ensure_ilu(aerofx_ctx *c, int B) {
       return set_error(AEROFX_ERR_STATE, "ilu: inconsistent level schedule");
     }
   }
+  // numericILU's updates, listed once: for lower entry k = (i, l) in order,
+  // the upper entries jj of row l whose column is also a later entry of row i
+  std::vector<int> ldiag(u->nnz, -1), pptr(u->nnz + 1, 0), pjj, pmm;
+  for (int i = 0; i < N; ++i) {
+    const int a0 = (i / B) * B;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      pptr[k] = (int)pjj.size();
+      if (k >= ju[i]) continue;
+      const int l = a0 + col[k];
+      ldiag[k] = ju[l];
+      for (int jj = ju[l] + 1; jj < rowptr[l + 1]; ++jj)
+        for (int t = k + 1; t < rowptr[i + 1]; ++t)
+          if (col[t] == col[jj]) {
+            pjj.push_back(jj);
+            pmm.push_back(t - rowptr[i]);
+          }
+    }
+  }
+  pptr[u->nnz] = (int)pjj.size();
+  std::vector<int4> fdesc(N), bdesc(N);
+  for (int q = 0; q < N; ++q) {
+    const int fi = frow[q], bi = brow[q];
+    fdesc[q] = make_int4(fi - (fi / B) * B, rowptr[fi], ju[fi], ju[fi]);
+    bdesc[q] = make_int4(bi - (bi / B) * B, ju[bi] + 1, rowptr[bi + 1], ju[bi]);
+  }
   int rc;
-  if ((rc = upload_vec(c, rowptr, &u->rowptr)) || (rc = upload_vec(c, ju, &u->ju)) || (rc = upload_vec(c, col, &u->col)) ||
+  if ((rc = upload_vec(c, fdesc, &u->fdesc)) || (rc = upload_vec(c, bdesc, &u->bdesc)) ||
+      (rc = upload_vec(c, ldiag, &u->ldiag)) || (rc = upload_vec(c, pptr, &u->pptr)) ||
+      (rc = upload_vec(c, pjj, &u->pjj)) || (rc = upload_vec(c, pmm, &u->pmm)) ||
+      (rc = upload_vec(c, rowptr, &u->rowptr)) || (rc = upload_vec(c, ju, &u->ju)) || (rc = upload_vec(c, col, &u->col)) ||
       (rc = upload_vec(c, src, &u->src)) || (rc = upload_vec(c, flev, &u->flev)) ||
       (rc = upload_vec(c, flevptr, &u->flevptr)) || (rc = upload_vec(c, frow, &u->frow)) ||
       (rc = upload_vec(c, blev, &u->blev)) || (rc = upload_vec(c, blevptr, &u->blevptr)) ||
@@ -385,13 +442,14 @@ int launch_ilu_factor(aerofx_ctx *c) {
     if (rc) return rc;
   }
   if (prec)
-    hipLaunchKernelGGL(k_ilu_factor<float>, dim3(u->nsub), dim3(kIluThreads), 0, c->stream, c->N, u->B, u->rowptr,
-                       u->ju, u->col, u->src, u->flev, u->flevptr, u->frow, (const float *)c->d_jacD,
-                       (const float *)c->d_jacB, (float *)u->LU[1], c->d_err + 1);
+    hipLaunchKernelGGL(k_ilu_factor<float>, dim3(u->nsub), dim3(IluFactorShape<float>::threads), 0, c->stream, u->B,
+                       u->rowptr, u->ju, u->src, u->ldiag, u->pptr, u->pjj, u->pmm, u->flev, u->flevptr, u->frow,
+                       (const float *)c->d_jacD, (const float *)c->d_jacB, (float *)u->LU[1], c->d_err + 1);
   else
-    hipLaunchKernelGGL(k_ilu_factor<double>, dim3(u->nsub), dim3(kIluThreads), 0, c->stream, c->N, u->B, u->rowptr,
-                       u->ju, u->col, u->src, u->flev, u->flevptr, u->frow, (const double *)c->d_jacD,
-                       (const double *)c->d_jacB, (double *)u->LU[0], c->d_err + 1);
+    hipLaunchKernelGGL(k_ilu_factor<double>, dim3(u->nsub), dim3(IluFactorShape<double>::threads), 0, c->stream,
+                       u->B, u->rowptr, u->ju, u->src, u->ldiag, u->pptr, u->pjj, u->pmm, u->flev, u->flevptr,
+                       u->frow, (const double *)c->d_jacD, (const double *)c->d_jacB, (double *)u->LU[0],
+                       c->d_err + 1);
   u->lu_precision = prec;
   u->lu_version = c->jac_version;
   hipError_t e = hipGetLastError();
@@ -403,13 +461,13 @@ int launch_ilu_solve(aerofx_ctx *c, const double *y, double *x, const int *done)
   if (!u || u->lu_version != c->jac_version || u->lu_precision != c->jac_precision)
     return set_error(AEROFX_ERR_STATE, "ilu: factors do not belong to the current Jacobian");
   if (u->lu_precision)
-    hipLaunchKernelGGL(k_ilu_solve<float>, dim3(u->nsub), dim3(kIluThreads), 0, c->stream, c->N, u->B, u->rowptr,
-                       u->ju, u->col, u->flev, u->flevptr, u->frow, u->blev, u->blevptr, u->brow,
-                       (const float *)u->LU[1], y, x, done);
+    hipLaunchKernelGGL(k_ilu_solve<float>, dim3(u->nsub), dim3(kIluThreads), 0, c->stream, c->N, u->B, u->col,
+                       u->flev, u->flevptr, u->fdesc, u->blev, u->blevptr, u->bdesc, (const float *)u->LU[1], y, x,
+                       done);
   else
-    hipLaunchKernelGGL(k_ilu_solve<double>, dim3(u->nsub), dim3(kIluThreads), 0, c->stream, c->N, u->B, u->rowptr,
-                       u->ju, u->col, u->flev, u->flevptr, u->frow, u->blev, u->blevptr, u->brow,
-                       (const double *)u->LU[0], y, x, done);
+    hipLaunchKernelGGL(k_ilu_solve<double>, dim3(u->nsub), dim3(kIluThreads), 0, c->stream, c->N, u->B, u->col,
+                       u->flev, u->flevptr, u->fdesc, u->blev, u->blevptr, u->bdesc, (const double *)u->LU[0], y, x,
+                       done);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "ilu solve launch");
 }

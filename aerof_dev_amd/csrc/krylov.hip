@@ -544,6 +544,29 @@ __global__ __launch_bounds__(kBlock) void k_check_states(int N, const double *__
   if (!(rho > 0.0) || !(p + g.pstiff > 0.0)) atomicAdd(bad, 1);
 }
 
+// ImplicitTsDesc::checkFailSafe -> SubDomain::checkFailSafe (SubDomain.C:7123-7153):
+// tag0 = node non-physical; the first-order nodes gain every node with a
+// non-physical neighbour (tag1 = OR of the neighbours' tag0, SpaceOperator::fix)
+__global__ __launch_bounds__(kBlock) void k_fs_state_tags(int N, const double *__restrict__ U, Gas g,
+                                                          int *__restrict__ tag0) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  const double *u = U + 5 * (size_t)n;
+  double rho = u[0];
+  double vel2 = (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / (rho * rho);
+  double p = (g.gam - 1.0) * (u[4] - 0.5 * rho * vel2) - g.gam * g.pstiff;
+  tag0[n] = (!(rho > 0.0) || !(p + g.pstiff > 0.0)) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fs_state_fix(DevTables t, const int *__restrict__ tag0,
+                                                         int *__restrict__ mask) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= t.N) return;
+  int tag1 = 0;
+  for (int e = t.rowptr[n]; e < t.rowptr[n + 1]; ++e) tag1 |= tag0[t.adj[e]];
+  if (tag1) mask[n] = 1;
+}
+
 // ---------------------------------------------------------------------------
 // MatVecProdFD (MatVecProd.C:88-118, 262-372, 879-929)
 // ---------------------------------------------------------------------------
@@ -930,7 +953,7 @@ static int arnoldi_cycle(aerofx_ctx *c, int m, bool trans) {
 
 struct GmresOut {
   int its = 0;
-  double res = 0, target = 0;
+  double res = 0, target = 0, true_res = -1.0;
 };
 
 // Right-preconditioned GMRES(m) on H1 with block-Jacobi: A x = b, x0 = 0
@@ -980,6 +1003,18 @@ static int gmres(aerofx_ctx *c, const double *b, double *x, int m, double eps, i
   out->its = k->h_st->iter;
   out->res = k->h_st->l2res;
   out->target = k->h_st->target;
+  if (c->check_final_res) {
+    // GmresSolver::solve with checkFinalRes (KspSolver.C:455-469): the true
+    // residual ||b - A x|| against the target; above it the solve reports -999
+    if ((rc = launch_operator(c, x, k->w, nullptr, trans))) return rc;
+    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, b, -1.0, k->w, nullptr);
+    norm2(c, k->w, k->scal);
+    e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "gmres final residual");
+    out->true_res = std::sqrt(k->h_scal[0]);
+    if (out->true_res > out->target) out->its = -999;
+  }
   return 0;
 }
 
@@ -1014,6 +1049,8 @@ int aerofx_mvp_select(aerofx_ctx *c, int type, const double *Q, const double *F0
   if (!Q) return set_error(AEROFX_ERR_ARG, "mvp_select: null state");
   if (type == AEROFX_MVP_H2 && c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "mvp_select: exact products not available with low-Mach preconditioning");
+  if (type == AEROFX_MVP_H2)
+    if (int rf = refuse_failsafe_nodes(c, "mvp_select")) return rf;
   if (type == AEROFX_MVP_H2) {
     int rc = select_h2(c, Q, dt);
     if (rc) return rc;
@@ -1083,6 +1120,7 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
   if (!c || !U || !rhs || !psi || !o || !st) return set_error(AEROFX_ERR_ARG, "adjoint_solve: null argument");
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_solve: not available with low-Mach preconditioning");
+  if (int rf = refuse_failsafe_nodes(c, "adjoint_solve")) return rf;
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "adjoint_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
@@ -1126,6 +1164,7 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
   if (!c || !U || !rhs || !x || !o || !st) return set_error(AEROFX_ERR_ARG, "direct_solve: null argument");
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "direct_solve: not available with low-Mach preconditioning");
+  if (int rf = refuse_failsafe_nodes(c, "direct_solve")) return rf;
   if (nrhs < 1 || nrhs > 16) return set_error(AEROFX_ERR_ARG, "direct_solve: 1 <= nrhs <= 16");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "direct_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
@@ -1332,9 +1371,40 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
   } restore_prec{c, c->prec_type, c->prec_subdomain};
   c->prec_type = o->prec;
   if (o->prec == AEROFX_PREC_RAS_ILU0) c->prec_subdomain = o->prec_subdomain;
+  // Newton.FailSafe and checkFinalRes for the solve (the caller's restored)
+  if (o->failsafe < 0 || o->failsafe > 2) return set_error(AEROFX_ERR_ARG, "steady_solve: failsafe 0, 1 or 2");
+  if (o->eps_formula != 0 && o->eps_formula != 1) return set_error(AEROFX_ERR_ARG, "steady_solve: eps_formula 0 or 1");
+  struct RestoreChecks {
+    aerofx_ctx *c;
+    int failsafe, check;
+    ~RestoreChecks() {
+      (void)aerofx_ctx_set_failsafe(c, failsafe);
+      c->check_final_res = check;
+    }
+  } restore_checks{c, c->failsafe, c->check_final_res};
+  if ((rc = aerofx_ctx_set_failsafe(c, o->failsafe))) return rc;
+  c->check_final_res = o->check_final_res ? 1 : 0;
+  int *fs_tmp = nullptr;  // [N] state tags of the Newton failsafe
+  if (c->failsafe) {
+    void *p = nullptr;
+    if ((rc = ctx_alloc(c, sizeof(int) * (size_t)c->N, &p))) return rc;
+    fs_tmp = static_cast<int *>(p);
+  }
+  struct FreeTmp {
+    aerofx_ctx *c;
+    int *p;
+    ~FreeTmp() {
+      if (p) ctx_release(c, p);
+    }
+  } free_tmp{c, fs_tmp};
+  const ResidualBufs rbuf{c->d_V, c->d_grad, c->d_Fm, c->d_P};
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_newton_stats{};
   double res0 = 0.0, res = 0.0, cfl = o->cfl0;
+  // Eisenstat (KspConvCriterion::computeEpsilonEisenstadt): one Newton
+  // iteration per pseudo-time step, so its history runs over the steps
+  int ew_its = 0;
+  double ew_fprev = 0.0, ew_eprev = 0.0;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err, 4, nullptr);
   int step;
   for (step = 0; step <= o->max_steps; ++step) {
@@ -1342,7 +1412,12 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     // synchronisation per step reads the norm and the error counters of the
     // previous step's block inversion [1], update [2] and finite-difference
     // products [3]
-    if ((rc = launch_residual(c, U, k->R, true))) return rc;
+    if (c->failsafe) {
+      int cnt = 0;
+      if ((rc = residual_failsafe(c, rbuf, U, k->R, true, &cnt))) return rc;
+    } else if ((rc = launch_residual(c, U, k->R, true))) {
+      return rc;
+    }
     norm2(c, k->R, k->scal);
     hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
@@ -1379,26 +1454,67 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
                    o->cfl_max);
     if ((rc = launch_timestep(c, cfl, c->d_dt))) return rc;
     if ((rc = launch_jacobian(c, c->d_dt, o->precision))) return rc;
-    if (o->mvp == AEROFX_MVP_FD) {
-      // MatVecProdFD::evaluate at the Newton state: F(U) = R(U) (the time
-      // term vanishes at Q = Un); one residual per Krylov product
-      if ((rc = select_fd(c, U, k->R, c->d_dt, o->fd_order))) return rc;
-    } else if (o->mvp == AEROFX_MVP_H2) {
-      // MatVecProdH2::evaluate at the Newton state: exact products of the
-      // second-order residual plus the time term
-      if ((rc = select_h2(c, U, c->d_dt))) return rc;
+    // the Krylov tolerance of this step, from ||b|| = ||R||
+    double geps = o->gmres_eps;
+    if (o->eps_formula == 1) {
+      if (ew_its > 0) {
+        const double eg = 0.1 * std::pow(res / ew_fprev, 2.0), e1 = 0.1 * std::pow(ew_eprev, 2.0);
+        geps = e1 < 0.1 ? std::min(o->gmres_eps, eg) : std::min(o->gmres_eps, std::max(eg, e1));
+      }
+      ew_fprev = res;
+      ew_eprev = geps;
+      ++ew_its;
     }
-    // b = -F
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 0.0, k->b,
-                       nullptr);
-    GmresOut go;
-    if ((rc = gmres(c, k->b, k->x, o->gmres_m, o->gmres_eps, o->gmres_maxits, &go))) return rc;
-    st->krylov_its += go.its;
-    // Q += dQ; the state check is read at the next step's synchronisation
-    hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->x, 1.0, U, nullptr);
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 2, 1, nullptr);
-    hipLaunchKernelGGL(k_check_states, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N, U,
-                       c->gas, c->d_err + 2);
+    for (int redo = 0;; ++redo) {
+      if (o->mvp == AEROFX_MVP_FD) {
+        // MatVecProdFD::evaluate at the Newton state: F(U) = R(U) (the time
+        // term vanishes at Q = Un); one residual per Krylov product
+        if ((rc = select_fd(c, U, k->R, c->d_dt, o->fd_order))) return rc;
+      } else if (o->mvp == AEROFX_MVP_H2) {
+        // MatVecProdH2::evaluate at the Newton state: exact products of the
+        // second-order residual plus the time term
+        if ((rc = select_h2(c, U, c->d_dt))) return rc;
+      }
+      // b = -F
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, -1.0, nullptr, k->R, 0.0, k->b,
+                         nullptr);
+      GmresOut go;
+      if ((rc = gmres(c, k->b, k->x, o->gmres_m, geps, o->gmres_maxits, &go))) return rc;
+      if (go.its == -999) {
+        ++st->final_res_failures;
+        go.its = k->h_st->iter;
+      }
+      st->krylov_its += go.its;
+      if (go.its == o->gmres_maxits) ++st->saturated;
+      // Q += dQ (the failsafe keeps Q for a redo); the state check is read
+      // at the next step's synchronisation
+      if (c->failsafe)
+        hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, U, 0.0, k->z, nullptr);
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->x, 1.0, U, nullptr);
+      hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 2, 1, nullptr);
+      hipLaunchKernelGGL(k_check_states, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N, U,
+                         c->gas, c->d_err + 2);
+      if (!c->failsafe) break;
+      e = hipMemcpyAsync(c->h_err + 2, c->d_err + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) return set_hip_error(e, "steady_solve failsafe sync");
+      if (c->h_err[2] == 0 || redo == 5) {
+        // NewtonSolver.h:216: after a redo, FailSafe = On resets the fixes
+        if (redo > 0 && c->failsafe == 1 && (rc = aerofx_ctx_reset_failsafe(c))) return rc;
+        break;
+      }
+      // NewtonSolver.h:193-205: tag the neighbours of the non-physical
+      // nodes first order, restore Q and redo the iteration
+      const int nb = (c->N + kBlock - 1) / kBlock;
+      hipLaunchKernelGGL(k_fs_state_tags, dim3(nb), dim3(kBlock), 0, c->stream, c->N, U, c->gas, fs_tmp);
+      hipLaunchKernelGGL(k_fs_state_fix, dim3(nb), dim3(kBlock), 0, c->stream, c->tab, fs_tmp,
+                         c->d_fs + 2 * (size_t)c->N);
+      c->fs_any = true;
+      hipLaunchKernelGGL(k_axpby, grid_for(n), dim3(kBlock), 0, c->stream, n, 1.0, nullptr, k->z, 0.0, U, nullptr);
+      ++st->failsafe_redos;
+      int cnt = 0;
+      if ((rc = residual_failsafe(c, rbuf, U, k->R, true, &cnt))) return rc;
+    }
     st->final_cfl = cfl;
   }
   hipError_t e = hipStreamSynchronize(c->stream);

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "context.h"
+#include "host_internal.h"
 
 namespace aerofx {
 
@@ -115,6 +116,11 @@ __global__ __launch_bounds__(kBlock) void k_gradients(DevTables t, const double 
       }
     }
   }
+  if (!OUT3 && t.fs_mask && t.fs_mask[n]) {
+    // DistNodalGrad::limit with failsafe tags (DistNodalGrad.C:1083-1104)
+#pragma unroll
+    for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+  }
   if (!OUT3) {
     double2 *g2 = reinterpret_cast<double2 *>(grad + 16 * (size_t)n);
 #pragma unroll
@@ -143,7 +149,8 @@ template <bool SECOND, int FB, bool LM>
 __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ grad, FluxConsts fc,
                                                        double *__restrict__ Fm, double *__restrict__ P,
-                                                       int *__restrict__ err, const int *__restrict__ done) {
+                                                       int *__restrict__ err, int *__restrict__ tag0,
+                                                       const int *__restrict__ done) {
   if (done && *done) return;
   __shared__ double sf[5 * (FB + 1)];
   const int4 ch = t.fchunk4[xcd_chunk(blockIdx.x, t.nfchunks)];
@@ -196,9 +203,16 @@ __global__ __launch_bounds__(FB) void k_flux_edges(DevTables t, const double *__
         Vji[k] = Vj[k];
       }
     }
-    // checkReconstructedValues (EdgeCore.C:137-166, VarFcnSGEuler.h:90-110)
-    const int bad = (Vij[0] <= 0.0 || Vij[4] + g.pstiff <= 0.0) + (Vji[0] <= 0.0 || Vji[4] + g.pstiff <= 0.0);
-    if (bad) atomicAdd(err, bad);
+    // checkReconstructedValues (EdgeCore.C:137-166, VarFcnSGEuler.h:90-110);
+    // with the failsafe the failing sides are tagged
+    const bool bi = Vij[0] <= 0.0 || Vij[4] + g.pstiff <= 0.0, bj = Vji[0] <= 0.0 || Vji[4] + g.pstiff <= 0.0;
+    if (bi || bj) {
+      atomicAdd(err, (int)bi + (int)bj);
+      if (tag0) {
+        if (bi) tag0[i] = 1;
+        if (bj) tag0[j] = 1;
+      }
+    }
     const double nv[3] = {t.normal[3 * (size_t)l], t.normal[3 * (size_t)l + 1], t.normal[3 * (size_t)l + 2]};
     double f[5];
     if (sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) < 1e-18) {
@@ -533,17 +547,17 @@ static void launch_flux_edges(aerofx_ctx *c, const ResidualBufs &b, const int *d
   const DevTables &t = c->tab;
   if (c->fc.low_mach)
     hipLaunchKernelGGL((k_flux_edges<SECOND, FB, true>), dim3(t.nfchunks), dim3(FB), 0, c->stream, t, b.V, b.grad,
-                       c->fc, b.Fm, b.P, c->d_err, done);
+                       c->fc, b.Fm, b.P, c->d_err, b.tag0, done);
   else
     hipLaunchKernelGGL((k_flux_edges<SECOND, FB, false>), dim3(t.nfchunks), dim3(FB), 0, c->stream, t, b.V, b.grad,
-                       c->fc, b.Fm, b.P, c->d_err, done);
+                       c->fc, b.Fm, b.P, c->d_err, b.tag0, done);
 }
 
 int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale,
                          const int *done) {
   const DevTables &t = c->tab;
-  // the staged kernels evaluate ROEFLUX7 only
-  const bool staged = t.ngb > 0 && c->residual_kernels == AEROFX_RESIDUAL_STAGED && !c->fc.low_mach;
+  // the staged kernels evaluate ROEFLUX7 only, without the failsafe
+  const bool staged = t.ngb > 0 && c->residual_kernels == AEROFX_RESIDUAL_STAGED && !c->fc.low_mach && !c->failsafe;
   if (c->p.second_order) {
     ProfScope ps(c, AEROFX_TIMER_NODAL_GRAD);
     if (staged)
@@ -585,6 +599,54 @@ int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, 
 int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done) {
   const ResidualBufs b{c->d_V, c->d_grad, c->d_Fm, c->d_P};
   return launch_residual_with(c, b, U, R, scale, done);
+}
+
+// SubDomain::finalizeTags (SubDomainCore.C:5177-5188) in its edge order: for
+// each edge (i, j) tag1[i] = tag1[j] = tag0[i] + tag0[j], so a node keeps the
+// value of its last edge (the last entry of its adjacency row, which is in
+// edge order); then DistNodalGrad::fix (DistNodalGrad.C:1265-1284): the node
+// is first order from now on if tag1 is set.
+__global__ void k_fs_finalize(DevTables t, const int *__restrict__ tag0, int *__restrict__ tag1,
+                              int *__restrict__ mask) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= t.N) return;
+  const int e0 = t.rowptr[n], e1 = t.rowptr[n + 1];
+  if (e1 > e0) tag1[n] = tag0[n] + tag0[t.adj[e1 - 1]];
+  if (tag1[n]) mask[n] = 1;
+}
+
+static int read_error_count(aerofx_ctx *c, int *count) {
+  hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_hip_error(e, "residual error count");
+  *count = c->h_err[0];
+  return 0;
+}
+
+int residual_failsafe(aerofx_ctx *c, const ResidualBufs &b, const double *U, double *R, bool scale, int *count) {
+  const bool fs = c->failsafe && c->p.second_order;
+  ResidualBufs b1 = b;
+  const int N = c->N;
+  b1.tag0 = fs ? c->d_fs : nullptr;
+  int rc = launch_residual_with(c, b1, U, R, scale);
+  if (rc || (rc = read_error_count(c, count))) return rc;
+  if (!fs || *count == 0) return 0;
+  // Domain::computeFiniteVolumeTerm with failsafe (Domain.C:884-917)
+  hipLaunchKernelGGL(k_fs_finalize, dim3((N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->tab, c->d_fs,
+                     c->d_fs + N, c->d_fs + 2 * (size_t)N);
+  c->fs_any = true;
+  if ((rc = launch_residual_with(c, b, U, R, scale)) || (rc = read_error_count(c, count))) return rc;
+  // FailSafe = On clears the tags after the fix; AlwaysOn keeps them
+  if (c->failsafe == 1) hipLaunchKernelGGL(k_zero_int, dim3((2 * N + 63) / 64), dim3(64), 0, c->stream, c->d_fs,
+                                           2 * N, nullptr);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, "failsafe launch");
+}
+
+int refuse_failsafe_nodes(aerofx_ctx *c, const char *who) {
+  if (!c->fs_any) return 0;
+  return set_error(AEROFX_ERR_STATE, std::string(who) +
+                                         ": failsafe first-order nodes are set (aerofx_ctx_reset_failsafe clears them)");
 }
 
 int launch_gradients_v(aerofx_ctx *c, const double *V, double *ddx, double *ddy, double *ddz) {

@@ -67,6 +67,22 @@ class SpaceOperator:
         self.set_stream(stream)
         self.set_low_mach(params.low_mach)
 
+    def set_failsafe(self, mode: int):
+        """Newton.FailSafe for the synchronous residual: 0 off, 1 on, 2 always on."""
+        check(lib().aerofx_ctx_set_failsafe(self._h, int(mode)), "set_failsafe")
+
+    def reset_failsafe(self):
+        check(lib().aerofx_ctx_reset_failsafe(self._h), "reset_failsafe")
+
+    def failsafe_nodes(self):
+        """(tag0, tag1, first-order mask), host int arrays [N]."""
+        t = np.zeros((3, self.num_nodes), dtype=np.int32)
+        check(lib().aerofx_ctx_failsafe_nodes(self._h, t.ctypes.data_as(c_int_p)), "failsafe_nodes")
+        return t[0], t[1], t[2]
+
+    def set_krylov_check(self, on: bool):
+        check(lib().aerofx_ctx_set_krylov_check(self._h, int(bool(on))), "set_krylov_check")
+
     def set_low_mach(self, lm=None):
         """Spatial low-Mach preconditioning (LowMachPrec) on, or off (None)."""
         from ._abi import LowMachC
@@ -370,7 +386,8 @@ def direct_solve(op: SpaceOperator, U, rhs, x, precision: int = 1, cfl: float = 
 def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200, eps: float = 1e-8,
                  cfl0: float = 10.0, cfl_max: float = 1e4, ser: float = 0.7, gmres_m: int = 30,
                  gmres_maxits: int = 30, gmres_eps: float = 1e-2, mvp: str = "h1", fd_order: int = 1,
-                 prec: str = "jacobi", prec_subdomain: int = 256):
+                 prec: str = "jacobi", prec_subdomain: int = 256, eps_formula: str = "constant",
+                 failsafe: int = 0, check_final_res: bool = False):
     """Pseudo-transient Newton-GMRES to ||R|| <= eps ||R0|| (U updated in place).
 
     mvp "h1": Krylov products with the assembled first-order Jacobian (FP32
@@ -378,15 +395,21 @@ def steady_solve(op: SpaceOperator, U, precision: int = 0, max_steps: int = 200,
     FP64 residual (one residual per product); "h2": exact products of the
     second-order residual (MatVecProdH2, forward-mode kernels); block-Jacobi
     in `precision` for all three.  prec "jacobi" (block-Jacobi of H1) or
-    "ras_ilu0" (RAS with ILU(0) on subdomains of prec_subdomain nodes)."""
+    "ras_ilu0" (RAS with ILU(0) on subdomains of prec_subdomain nodes).
+    eps_formula "constant" (gmres_eps) or "eisenstat" (KspConvCriterion);
+    failsafe 0/1/2 (Newton.FailSafe Off/On/AlwaysOn); check_final_res
+    counts the Krylov solves whose true residual misses the target."""
     code = {"h1": 0, "fd": 1, "h2": 2}[mvp]
     pcode = {"jacobi": 0, "ras_ilu0": 1}[prec]
+    ecode = {"constant": 0, "eisenstat": 1}[eps_formula]
     o = NewtonOpts(int(precision), int(max_steps), float(eps), float(cfl0), float(cfl_max), float(ser), 0.0, 0.0,
-                   int(gmres_m), int(gmres_maxits), float(gmres_eps), code, int(fd_order), pcode, int(prec_subdomain))
+                   int(gmres_m), int(gmres_maxits), float(gmres_eps), code, int(fd_order), pcode, int(prec_subdomain),
+                   ecode, int(failsafe), int(bool(check_final_res)))
     st = NewtonStats()
     hist = np.zeros(max_steps + 1)
     check(lib().aerofx_steady_solve(op._h, _ptr(U, op.n5), C.byref(o), C.byref(st), hist.ctypes.data_as(c_double_p),
                                     max_steps + 1), "steady_solve")
     n = min(st.steps + 1, max_steps + 1)
     return dict(steps=st.steps, converged=bool(st.converged), krylov_its=st.krylov_its, res0=st.res0,
-                res=st.res, final_cfl=st.final_cfl, seconds=st.seconds, history=hist[:n].copy())
+                res=st.res, final_cfl=st.final_cfl, seconds=st.seconds, history=hist[:n].copy(),
+                failsafe_redos=st.failsafe_redos, saturated=st.saturated, final_res_failures=st.final_res_failures)

@@ -193,6 +193,31 @@ typedef struct {
 } aerofx_low_mach;
 int aerofx_ctx_set_low_mach(aerofx_ctx *ctx, const aerofx_low_mach *lm);
 
+/* Newton.FailSafe (NewtonData::failsafe, IoData.C:10-35): 0 Off (default),
+ * 1 On, 2 AlwaysOn; second-order schemes only, as the reference turns it off
+ * for constant reconstruction (IoDataCore.C:6010-6015).  With it on, a
+ * synchronous residual whose pass meets a non-physical reconstruction tags
+ * the failing sides, finalizes the tags over the edges (finalizeTags: each
+ * node takes tag0[i] + tag0[j] of its last edge), makes the tagged nodes
+ * first order from then on (their gradients zeroed, DistNodalGrad::fix and
+ * limit) and re-runs the pass (Domain::computeFiniteVolumeTerm,
+ * Domain.C:864-917); On clears the tags after each fix, AlwaysOn keeps them.
+ * aerofx_steady_solve also redoes a Newton step whose update leaves
+ * non-physical states (NewtonSolver.h:193-215, up to 5 redos per step).
+ * First-order nodes stay until aerofx_ctx_reset_failsafe (the Newton driver
+ * resets them after a step that redid, with FailSafe = On); while any is set,
+ * the derivative passes return AEROFX_ERR_STATE.  The asynchronous residual
+ * (aerofx_residual_ex flag 2) and the residuals inside captured Krylov cycles
+ * use the first-order nodes but do not tag. */
+int aerofx_ctx_set_failsafe(aerofx_ctx *ctx, int mode);
+/* KspData checkFinalRes for aerofx_gmres / aerofx_gmres_transpose: after
+ * the solve, ||b - A x|| is recomputed and an iteration count of -999 is
+ * returned when it misses the target (KspSolver.C:455-469). */
+int aerofx_ctx_set_krylov_check(aerofx_ctx *ctx, int check_final_res);
+int aerofx_ctx_reset_failsafe(aerofx_ctx *ctx);
+/* tags: host [3][N] = tag0 (failing sides), tag1 (finalized), first-order mask */
+int aerofx_ctx_failsafe_nodes(aerofx_ctx *ctx, int *tags);
+
 /* ---- SpaceOperator<5> -------------------------------------------------- */
 /* SpaceOperator<5>::computeResidual (SpaceOperator.C:524-634), NONDESCRIPTOR
  * form: R = (sum of edge and face fluxes)/V.  U, R: device [N][5].
@@ -443,10 +468,21 @@ typedef struct {
   int fd_order;         /* 1 or 2 with AEROFX_MVP_FD */
   int prec;             /* AEROFX_PREC_JACOBI or AEROFX_PREC_RAS_ILU0 (for the solve) */
   int prec_subdomain;   /* RAS / ILU(0) subdomain size in nodes */
+  int eps_formula;      /* Krylov tolerance: 0 constant gmres_eps, 1 Eisenstat
+                           (KspConvCriterion.C:28-74: gamma (|F|/|F_prev|)^alpha,
+                           gamma 0.1, alpha 2, safeguard delta 0.1, capped at gmres_eps) */
+  int failsafe;         /* Newton.FailSafe for the solve: 0 off, 1 on, 2 always on
+                           (aerofx_ctx_set_failsafe) */
+  int check_final_res;  /* KspData checkFinalRes: recompute ||b - A x|| after each
+                           Krylov solve and count the solves above their target
+                           (GmresSolver returns -999 for them, KspSolver.C:455-469) */
 } aerofx_newton_opts;
 typedef struct {
   int steps, converged, krylov_its;
   double res0, res, final_cfl, seconds;
+  int failsafe_redos;      /* Newton steps redone by the failsafe */
+  int saturated;           /* Krylov solves that hit gmres_maxits (ErrorHandler::SATURATED_LS) */
+  int final_res_failures;  /* Krylov solves whose true residual missed the target */
 } aerofx_newton_stats;
 /* TsSolver::resolve + NewtonSolver::solve with one Newton iteration per
  * pseudo-time step (NewtonSolver.h:70-231, TsDesc.C:697-730, 1178-1278):

@@ -157,6 +157,19 @@ class OracleMesh:
             out["ddx"], out["ddy"], out["ddz"] = g
         return out
 
+    def residual_fs(self, U, failsafe, tags):
+        """The residual with Newton.FailSafe; tags int32 [3, N] (tag0, tag1,
+        first-order mask) is updated in place.  Returns (R, count of the
+        pass that stands, count of the first pass)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        R = np.empty((self.N, 5))
+        first = C.c_int()
+        L = lib()
+        L.or_residual_fs.restype = C.c_int
+        ierr = L.or_residual_fs(C.byref(self.m), C.byref(self.p), _d(U), _d(R), int(failsafe), _i(tags),
+                                C.byref(first))
+        return R, ierr, first.value
+
     def residual_par(self, U, nthreads):
         """The residual on nthreads OpenMP threads over node partitions with
         the shared-node assembly (CPU timing baseline)."""
@@ -219,16 +232,16 @@ class OracleMesh:
         return x, its, fr.value
 
     def steady_solve(self, U, precision=0, max_steps=50, eps=1e-8, cfl0=10.0, cfl_max=1e4, ser=0.7, m=30,
-                     maxits=30, geps=1e-2, mvp=0, fd_order=1):
+                     maxits=30, geps=1e-2, mvp=0, fd_order=1, eisenstat=False):
         U = np.ascontiguousarray(U, dtype=np.float64).copy()
         hist = np.zeros(max_steps + 1)
         conv, kits = C.c_int(), C.c_int()
         L = lib()
-        L.or_steady_solve_mvp.restype = C.c_int
-        steps = L.or_steady_solve_mvp(C.byref(self.m), C.byref(self.p), _d(U), int(precision), int(mvp),
-                                      int(fd_order), int(max_steps), C.c_double(eps), C.c_double(cfl0),
-                                      C.c_double(cfl_max), C.c_double(ser), int(m), int(maxits), C.c_double(geps),
-                                      _d(hist), C.byref(conv), C.byref(kits))
+        L.or_steady_solve_ew.restype = C.c_int
+        steps = L.or_steady_solve_ew(C.byref(self.m), C.byref(self.p), _d(U), int(precision), int(mvp),
+                                     int(fd_order), int(max_steps), C.c_double(eps), C.c_double(cfl0),
+                                     C.c_double(cfl_max), C.c_double(ser), int(m), int(maxits), C.c_double(geps),
+                                     int(bool(eisenstat)), _d(hist), C.byref(conv), C.byref(kits))
         return dict(U=U, steps=steps, converged=bool(conv.value), krylov_its=kits.value,
                     history=hist[:steps + 1].copy())
 

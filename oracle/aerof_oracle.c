@@ -488,9 +488,49 @@ int or_residual(const or_mesh *m, const or_params *p, const double *U, double *R
   return or_residual_ex(m, p, U, R, 0, fluxes_only, V_out, ddx_out, ddy_out, ddz_out);
 }
 
+static int residual_pass(const or_mesh *m, const or_params *p, const double *U, double *R, double *Rabs,
+                         int fluxes_only, double *V_out, double *ddx_out, double *ddy_out, double *ddz_out,
+                         int *tag0, const int *mask);
+
 int or_residual_ex(const or_mesh *m, const or_params *p, const double *U, double *R,
                    double *Rabs, int fluxes_only, double *V_out, double *ddx_out,
                    double *ddy_out, double *ddz_out) {
+  return residual_pass(m, p, U, R, Rabs, fluxes_only, V_out, ddx_out, ddy_out, ddz_out, 0, 0);
+}
+
+/* SpaceOperator::computeResidual with Newton.FailSafe (failsafe 1 On, 2
+ * AlwaysOn) and Domain::computeFiniteVolumeTerm (Domain.C:847-917).  tags
+ * [3][N] carries Domain's tag (tag0, tag1) and DistNodalGrad's node tags
+ * (first order: gradients zeroed by limit, DistNodalGrad.C:1083-1104) from
+ * call to call.  A pass with non-physical reconstructions tags their sides
+ * (EdgeSet::checkReconstructedValues, EdgeCore.C:137-166), finalizeTags
+ * (SubDomainCore.C:5177-5188) sets tag1 over the edges in order, the nodes
+ * with tag1 become first order (DistNodalGrad::fix) and the pass is
+ * re-run; On clears Domain's tag afterwards.  Returns the re-run's count
+ * (the first pass's in *ierr_first). */
+int or_residual_fs(const or_mesh *m, const or_params *p, const double *U, double *R, int failsafe, int *tags,
+                   int *ierr_first) {
+  int N = m->N, E = m->E;
+  int *tag0 = tags, *tag1 = tags + N, *mask = tags + 2 * N;
+  int on = failsafe && p->second_order;
+  int ierr = residual_pass(m, p, U, R, 0, 0, 0, 0, 0, 0, on ? tag0 : 0, mask);
+  *ierr_first = ierr;
+  if (!ierr || !on) return ierr;
+  for (int l = 0; l < E; ++l) {
+    int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
+    tag1[i] = tag0[i] + tag0[j];
+    tag1[j] = tag1[i];
+  }
+  for (int i = 0; i < N; ++i) mask[i] = mask[i] || tag1[i];
+  ierr = residual_pass(m, p, U, R, 0, 0, 0, 0, 0, 0, 0, mask);
+  if (failsafe == 1)
+    for (int i = 0; i < 2 * N; ++i) tags[i] = 0;
+  return ierr;
+}
+
+static int residual_pass(const or_mesh *m, const or_params *p, const double *U, double *R, double *Rabs,
+                         int fluxes_only, double *V_out, double *ddx_out, double *ddy_out, double *ddz_out,
+                         int *tag0, const int *mask) {
   int N = m->N, E = m->E, F = m->F;
   double gam = p->gamma, pstiff = p->pstiff;
   double *V = V_out ? V_out : (double *)malloc(sizeof(double) * 5 * (size_t)N);
@@ -503,6 +543,10 @@ int or_residual_ex(const or_mesh *m, const or_params *p, const double *U, double
     ddy = ddy_out ? ddy_out : (double *)malloc(sizeof(double) * 5 * (size_t)N);
     ddz = ddz_out ? ddz_out : (double *)malloc(sizeof(double) * 5 * (size_t)N);
     or_gradients(N, E, m->X, m->ptr, m->R, V, ddx, ddy, ddz);
+    if (mask)
+      for (int i = 0; i < N; ++i)
+        if (mask[i])
+          for (int k = 0; k < 5; ++k) ddx[5 * i + k] = ddy[5 * i + k] = ddz[5 * i + k] = 0.0;
   }
   /* EdgeSet::computeFiniteVolumeTerm (Edge.C:372-439) */
   int ierr = 0;
@@ -524,8 +568,8 @@ int or_residual_ex(const or_mesh *m, const or_params *p, const double *U, double
       for (int k = 0; k < 5; ++k) { Vi[k] = vi[k]; Vj[k] = vj[k]; }
     }
     /* checkReconstructedValues (EdgeCore.C:137-166, VarFcnSGEuler.h:90-110) */
-    if (Vi[0] <= 0.0 || Vi[4] + pstiff <= 0.0) ++ierr;
-    if (Vj[0] <= 0.0 || Vj[4] + pstiff <= 0.0) ++ierr;
+    if (Vi[0] <= 0.0 || Vi[4] + pstiff <= 0.0) { ++ierr; if (tag0) tag0[i] = 1; }
+    if (Vj[0] <= 0.0 || Vj[4] + pstiff <= 0.0) { ++ierr; if (tag0) tag0[j] = 1; }
     if (ierr) continue;
     const double *nl = m->normals + 3 * l;
     if (sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]) < 1e-18) continue;
@@ -1430,10 +1474,40 @@ double or_cfl_old(int its, double res, double cfl, double cfl0, double cflMax, d
  * products with H1; 1: MatVecProdFD of order fd_order about the step's state
  * (ImplicitCoupledTsDesc.C:33-44).  Stops early (not converged) when a
  * residual or a product meets a non-physical reconstruction. */
+/* KspConvCriterion::computeEpsilonEisenstadt (KspConvCriterion.C:44-74):
+ * gamma 0.1, alpha 2, delta 0.1; *its counts the calls (nlits). */
+double or_eps_eisenstat(double eps0, double fnorm, int *its, double *fprev, double *eprev) {
+  double eps;
+  if (*its == 0) {
+    eps = eps0;
+  } else {
+    eps = 0.1 * pow(fnorm / *fprev, 2.0);
+    double eps1 = 0.1 * pow(*eprev, 2.0);
+    if (eps1 < 0.1) eps = fmin(eps0, eps);
+    else eps = fmin(eps0, fmax(eps, eps1));
+  }
+  *fprev = fnorm;
+  *eprev = eps;
+  ++*its;
+  return eps;
+}
+
 int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
                         int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
                         double geps, double *history, int *converged, int *krylov_its) {
+  return or_steady_solve_ew(m, p, U, precision, mvp, fd_order, max_steps, eps, cfl0, cfl_max, ser, gm, gmaxits, geps,
+                            0, history, converged, krylov_its);
+}
+
+/* As or_steady_solve_mvp, with the Krylov tolerance of KspData::EISENSTADT
+ * when eisenstat (the criterion's history runs over the pseudo-time steps:
+ * one Newton iteration each, so nlits never resets, KspConvCriterion.C:31). */
+int or_steady_solve_ew(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
+                       int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
+                       double geps, int eisenstat, double *history, int *converged, int *krylov_its) {
   int N = m->N;
+  int ew_its = 0;
+  double ew_f = 0.0, ew_e = 0.0;
   size_t n = 5 * (size_t)N;
   double *R = (double *)malloc(sizeof(double) * n), *V = (double *)malloc(sizeof(double) * n);
   double *dt = (double *)malloc(sizeof(double) * N), *b = (double *)malloc(sizeof(double) * n);
@@ -1456,7 +1530,8 @@ int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int pre
     for (size_t k = 0; k < n; ++k) b[k] = -1.0 * R[k];
     double fr;
     int perr = 0;
-    *krylov_its += or_gmres_mvp(m, p, V, U, dt, precision, mvp, fd_order, b, dQ, gm, geps, gmaxits, &fr, &perr);
+    double ge = eisenstat ? or_eps_eisenstat(geps, res, &ew_its, &ew_f, &ew_e) : geps;
+    *krylov_its += or_gmres_mvp(m, p, V, U, dt, precision, mvp, fd_order, b, dQ, gm, ge, gmaxits, &fr, &perr);
     if (perr) break;
     for (size_t k = 0; k < n; ++k) U[k] += dQ[k];
   }

@@ -145,6 +145,10 @@ int or_residual(const or_mesh *m, const or_params *p, const double *U,
 int or_residual_ex(const or_mesh *m, const or_params *p, const double *U,
                    double *R, double *Rabs, int fluxes_only, double *V_out,
                    double *ddx, double *ddy, double *ddz);
+/* The residual with Newton.FailSafe; tags [3][N] = tag0, tag1, first-order
+ * mask, carried from call to call. */
+int or_residual_fs(const or_mesh *m, const or_params *p, const double *U, double *R, int failsafe, int *tags,
+                   int *ierr_first);
 
 /* The residual of or_residual as the reference runs it on P OpenMP threads
  * over contiguous node partitions with the shared-node assembly of
@@ -185,6 +189,10 @@ int or_gmres_mvp(const or_mesh *m, const or_params *p, const double *V, const do
 
 /* Pseudo-transient Newton (NewtonSolver.h:70-231, CflLaw Old) with the
  * Krylov products of mvp (0: H1, 1: MatVecProdFD). */
+double or_eps_eisenstat(double eps0, double fnorm, int *its, double *fprev, double *eprev);
+int or_steady_solve_ew(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
+                       int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
+                       double geps, int eisenstat, double *history, int *converged, int *krylov_its);
 int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int precision, int mvp, int fd_order,
                         int max_steps, double eps, double cfl0, double cfl_max, double ser, int gm, int gmaxits,
                         double geps, double *history, int *converged, int *krylov_its);

@@ -7,10 +7,10 @@ amplitudes a_k with seeded centres on the bottom wall.
 
 psi comes from the mixed-precision adjoint solve (FP32 transposed H1 and
 block-Jacobi inside, exact FP64 adjoint residual outside).  Checked against
-central finite differences of J with fully re-converged states, step 1e-6 on
-the amplitudes:
-  first-order scheme   to 3e-6 relative (J(a) is smooth; the differences
-    carry the steady solves' round-off floor, |dJ| / 2h ~ 1e-6 relative);
+central finite differences of J with fully re-converged states (each
+perturbed solve polished by FP64 steps at CFL 1e4 down to the round-off floor
+of its residual), step 1e-6 on the amplitudes:
+  first-order scheme   to 1e-6 relative (J(a) is smooth);
   second-order scheme  to 5e-5 relative.  With the Van Albada limiter J(a) is
     only piecewise smooth (an edge's limited slope switches branch when its
     two slopes change sign), so central differences carry an O(h) error: on
@@ -39,10 +39,13 @@ def _mesh(amps, centres, width=0.2):
     return M.kuhn_box(NX, NY, NZ, lo=(0.0, 0.0, 0.0), hi=(3.0, 1.0, 1.0), face_codes=M.CHANNEL, bumps=bumps)
 
 
-def _converge(op, U, eps):
+def _converge(op, U, eps, polish=False):
     from aerof_dev_amd.operators import steady_solve
     r = steady_solve(op, U, precision=0, max_steps=3000, eps=eps, cfl0=10.0, cfl_max=1e4)
     assert r["converged"], r
+    if polish:
+        # Newton steps at the final CFL until the residual stops falling
+        steady_solve(op, U, precision=0, max_steps=40, eps=1e-6, cfl0=1e4, cfl_max=1e4)
     return r
 
 
@@ -56,7 +59,7 @@ def base(gpu, request):
     op = SpaceOperator(m, p)
     U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
     _converge(op, U, 1e-12)
-    return dict(p=p, m=m, op=op, U=U, centres=centres, tol=5e-5 if request.param == "second" else 3e-6)
+    return dict(p=p, m=m, op=op, U=U, centres=centres, tol=5e-5 if request.param == "second" else 1e-6)
 
 
 def test_wall_force_derivatives(gpu, base):
@@ -110,7 +113,7 @@ def test_adjoint_shape_gradient_vs_fd(gpu, base):
             mk = _mesh(amps, centres)
             opk = SpaceOperator(mk, p)
             Uk = U.clone()
-            _converge(opk, Uk, 1e-9)
+            _converge(opk, Uk, 1e-9, polish=True)
             Js.append(opk.wall_force(Uk, ZDIR))
         fd = (Js[0] - Js[1]) / (2 * H)
         assert abs(grad - fd) <= base["tol"] * abs(fd), (k, grad, fd)
