@@ -13,7 +13,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # with different compiler flags); the default is the in-tree build.
 LIB_PATH = os.environ.get("AEROFX_LIB") or os.path.join(_HERE, "_lib", "libaerofx.so")
 
+AEROFX_BC_ISOTHERMAL_WALL = 1
 AEROFX_BC_SLIP_WALL = 2
+AEROFX_BC_ADIABATIC_WALL = 3
 AEROFX_BC_INLET = 4
 AEROFX_BC_OUTLET = 5
 AEROFX_MVP_H1 = 0
@@ -37,7 +39,7 @@ AEROFX_PHYS_ROEFLUX5 = 7
 AEROFX_PHYS_ROEJAC6_LM_SW1 = 8
 AEROFX_PHYS_ROEJAC6_LM_SW2 = 9
 TIMER_NAMES = ("nodalGrad", "fvTerm", "fvTermNodes", "fvJac", "fluidPrecSetup", "jacApply",
-               "precApply", "gramSchmidt")
+               "precApply", "gramSchmidt", "feTerm", "feJac")
 
 c_double_p = C.POINTER(C.c_double)
 c_int_p = C.POINTER(C.c_int)
@@ -74,6 +76,12 @@ class Mesh(C.Structure):
 class LowMachC(C.Structure):
     _fields_ = [("min_mach", C.c_double), ("slope", C.c_double), ("cutoff", C.c_double),
                 ("shock_reducer", C.c_double), ("irey", C.c_double)]
+
+
+class ViscousC(C.Structure):
+    _fields_ = [("reynolds", C.c_double), ("viscosity_model", C.c_int), ("bulk_viscosity", C.c_double),
+                ("sutherland_ts", C.c_double), ("conductivity_model", C.c_int), ("prandtl", C.c_double),
+                ("conductivity", C.c_double), ("wall_temperature", C.c_double), ("viscous_cst", C.c_double)]
 
 
 class SowerHeader(C.Structure):
@@ -133,6 +141,10 @@ SIGNATURES = {
     "aerofx_ctx_sync": (C.c_int, [C.c_void_p]),
     "aerofx_ctx_set_residual_kernels": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_set_low_mach": (C.c_int, [C.c_void_p, C.POINTER(LowMachC)]),
+    "aerofx_ctx_set_viscous": (C.c_int, [C.c_void_p, C.POINTER(ViscousC)]),
+    "aerofx_ctx_node_types": (C.c_int, [C.c_void_p, c_int_p]),
+    "aerofx_apply_bcs_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "aerofx_apply_bcs_solution": (C.c_int, [C.c_void_p, C.c_void_p]),
     "aerofx_ctx_set_failsafe": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_set_krylov_check": (C.c_int, [C.c_void_p, C.c_int]),
     "aerofx_ctx_reset_failsafe": (C.c_int, [C.c_void_p]),

@@ -1042,7 +1042,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_lift_faces(int F, const int *__re
   const int f = blockIdx.x * kAdjBlock + threadIdx.x;
   if (f >= F) return;
   const int code = fcode[f];
-  const bool wall = code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL;
+  const bool wall = is_wall_code(code);
   int nd[3];
   double psum = 0.0;
 #pragma unroll
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_lift_nodes(DevTables t, const dou
   for (int q = t.frow[n]; q < t.frow[n + 1]; ++q) {
     const int f = t.fidx[q];
     const int code = t.fcode[f];
-    if (code != AEROFX_BC_SLIP_WALL && code != -AEROFX_BC_SLIP_WALL) continue;
+    if (!is_wall_code(code)) continue;
     const double *nv = t.fnorm3 + 3 * (size_t)f;
     w += nv[0] * d0 + nv[1] * d1 + nv[2] * d2;
     if (dJdX) {
@@ -1145,6 +1145,7 @@ extern "C" int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const d
   if (ctx->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_residual: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(ctx, "adjoint_residual")) return rf;
+  if (int rv = refuse_viscous(ctx, "adjoint_residual")) return rv;
   if (!dRdU_T_lambda && !dRdX_T_lambda) return AEROFX_OK;
   int rc = launch_adjoint(ctx, U, lambda, dRdU_T_lambda, dRdX_T_lambda);
   if (rc) return rc;

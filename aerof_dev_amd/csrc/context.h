@@ -8,6 +8,7 @@
 
 #include "aerofx.h"
 #include "device_physics.h"
+#include "device_viscous.h"
 
 namespace aerofx {
 
@@ -31,6 +32,13 @@ constexpr int kGradStrip = 128;
 constexpr int kGradHalo = 960;
 constexpr int kFluxStrip = 256;
 constexpr int kFluxHalo = 192;
+
+// Faces of every wall code take the wall flux (FluxFcn.h:270-276): slip,
+// adiabatic and isothermal walls, fixed or moving.
+__host__ __device__ inline bool is_wall_code(int code) {
+  const int a = code < 0 ? -code : code;
+  return a == AEROFX_BC_SLIP_WALL || a == AEROFX_BC_ADIABATIC_WALL || a == AEROFX_BC_ISOTHERMAL_WALL;
+}
 
 // Block b of a launch runs on XCD b % 8; remap so that each XCD sweeps one
 // contiguous range of chunks (its L2 then holds that range's neighbours).
@@ -137,6 +145,7 @@ struct aerofx_ctx {
   std::vector<int> ptr;
   std::vector<double> cv, normals, fnorm, lsq;
   std::vector<int> tets, faces, tet_edges;  // element tables for the adjoint geometry terms
+  std::vector<int> fcodes;                  // face codes
   // device
   aerofx::DevTables tab{};
   std::vector<void *> allocs;
@@ -153,6 +162,14 @@ struct aerofx_ctx {
   bool fs_any = false;  // some node is first order by the failsafe
   int check_final_res = 0;  // KspData checkFinalRes: GMRES reports -999 above its target
   int max_degree = 0;        // most neighbours of a node
+  // laminar Navier-Stokes terms (viscous.hip): constants, node types, the
+  // wall nodes (node, type) and the viscous time-step scratch
+  int viscous = 0;
+  aerofx::NsConsts ns{};
+  std::vector<int> node_type;
+  int nwall = 0, wall_cap = 0;
+  int2 *d_wall = nullptr;
+  double *d_idtv = nullptr;
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
   // with the reason in noalloc_err)
@@ -220,6 +237,17 @@ void ctx_release(aerofx_ctx *c, void *p);
 
 __global__ void k_zero_int(int *p, int n, const int *done);
 
+// laminar Navier-Stokes terms (viscous.hip), no-ops with the terms off:
+// R += Galerkin term (/ V if scale); H1 += its blocks, then the wall
+// conditions (U: the state, for isothermal walls); F's wall rows; the
+// viscous time-step term (idtv = NULL when not used)
+int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, const int *done);
+int launch_visc_jacobian(aerofx_ctx *c, const double *U, int precision);
+int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *done);
+int launch_visc_idtv(aerofx_ctx *c, double **idtv);
+// AEROFX_ERR_STATE when the Navier-Stokes terms are on
+int refuse_viscous(aerofx_ctx *c, const char *who);
+
 // residual passes (residual.hip); all enqueue on c->stream; every kernel
 // returns at once when done != nullptr and *done != 0 (device-resident
 // Krylov loops with a matrix-free operator)
@@ -247,7 +275,8 @@ int launch_primitive(aerofx_ctx *c, const double *U, double *V);
 
 // implicit operators (jacobian.hip)
 int launch_timestep(aerofx_ctx *c, double cfl, double *dt);
-int launch_jacobian(aerofx_ctx *c, const double *dt, int precision);
+// U: the state of the wall conditions (needed with the Navier-Stokes terms)
+int launch_jacobian(aerofx_ctx *c, const double *dt, int precision, const double *U = nullptr);
 // trans: apply the transposed operator (adjoint solves)
 int launch_h1_apply(aerofx_ctx *c, const double *p, double *y, bool trans = false);
 // builds the mirrored off-diagonal blocks of the current assembly if they are

@@ -224,8 +224,9 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     if (m->faces[k] < 0 || m->faces[k] >= N) return set_error(AEROFX_ERR_MESH, "ctx_create: face node out of range");
   for (int f = 0; f < F; ++f) {
     int cd = m->face_codes[f] < 0 ? -m->face_codes[f] : m->face_codes[f];
-    if (cd != AEROFX_BC_SLIP_WALL && cd != AEROFX_BC_INLET && cd != AEROFX_BC_OUTLET)
-      return set_error(AEROFX_ERR_MESH, "ctx_create: unsupported face code (2, 4, 5 only)");
+    if (cd != AEROFX_BC_SLIP_WALL && cd != AEROFX_BC_INLET && cd != AEROFX_BC_OUTLET &&
+        cd != AEROFX_BC_ADIABATIC_WALL && cd != AEROFX_BC_ISOTHERMAL_WALL)
+      return set_error(AEROFX_ERR_MESH, "ctx_create: unsupported face code (1, 2, 3, 4, 5 only)");
   }
   aerofx_ctx *c = new aerofx_ctx();
   c->N = N;
@@ -250,6 +251,7 @@ int aerofx_ctx_create(const aerofx_mesh *m, const aerofx_params *p, aerofx_ctx *
     build_edges(N, T, m->tets, c->ptr, &tet_edges);
     c->tets.assign(m->tets, m->tets + 4 * (size_t)T);
     c->faces.assign(m->faces, m->faces + 3 * (size_t)F);
+    c->fcodes.assign(m->face_codes, m->face_codes + (size_t)F);
     c->E = (int)(c->ptr.size() / 2);
     const int E = c->E;
     c->cv.resize(N);
@@ -453,6 +455,7 @@ int aerofx_ctx_set_low_mach(aerofx_ctx *c, const aerofx_low_mach *lm) {
     c->fc.low_mach = 0;
     return AEROFX_OK;
   }
+  if (c->viscous) return set_error(AEROFX_ERR_STATE, "set_low_mach: not with the Navier-Stokes terms");
   if (!(lm->min_mach > 0.0) || !(lm->cutoff >= lm->min_mach) || !(lm->slope >= 0.0) ||
       !(lm->shock_reducer >= 0.0) || !(lm->irey >= 0.0))
     return set_error(AEROFX_ERR_ARG,

@@ -47,7 +47,7 @@ __device__ __forceinline__ void lsq_weights_t(const T L[6], T dx0, T dx1, T dx2,
 // boundary flux of one face node; nv = (1/3) face normal
 template <class T>
 __device__ __forceinline__ void face_flux(const FluxConsts &fc, int code, const T nv[3], const T V[5], T fl[5]) {
-  if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+  if (is_wall_code(code)) {
     wallflux(nv, 0.0, V, fl);
   } else {
     const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;

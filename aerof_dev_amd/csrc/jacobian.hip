@@ -34,6 +34,7 @@ __device__ __forceinline__ double sound_speed(const Gas &g, const double V[5]) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *__restrict__ V, Gas g, double cfl,
+                                                     const double *__restrict__ idtv, double viscous,
                                                      double *__restrict__ dt) {
   const int n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= t.N) return;
@@ -77,7 +78,8 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevTables t, const double *
     double c = 0.5 * (coeff1 - coeff2);
     idti += (c < 0.0 ? c : 0.0) * S / 3;
   }
-  dt[n] = cfl * t.cv[n] / (-1.0 * idti + 0.0 * 0.0);
+  // Domain.C:91 with the viscous term weighted by Time.ViscosityParameter
+  dt[n] = cfl * t.cv[n] / (-1.0 * idti + (idtv ? viscous * idtv[n] : 0.0 * 0.0));
 }
 
 // ---------------------------------------------------------------------------
@@ -159,7 +161,7 @@ __device__ __forceinline__ void jac_face(const DevTables &t, const double *__res
   double Vn[5], jac[25];
 #pragma unroll
   for (int k = 0; k < 5; ++k) Vn[k] = V[5 * (size_t)n + k];
-  if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+  if (is_wall_code(code)) {
     wall_jac(fc.gas, nv, 0.0, Vn, jac);
   } else {
     const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
@@ -582,7 +584,11 @@ int ensure_jacobian_storage(aerofx_ctx *c, int precision) {
 
 int launch_timestep(aerofx_ctx *c, double cfl, double *dt) {
   const int nb = (c->N + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_timestep, dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->gas, cfl, dt);
+  double *idtv = nullptr;
+  int rc = launch_visc_idtv(c, &idtv);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_timestep, dim3(nb), dim3(kBlock), 0, c->stream, c->tab, c->d_V, c->gas, cfl, idtv,
+                     c->ns.viscous_cst, dt);
   return check_launch("timestep launch");
 }
 
@@ -595,7 +601,7 @@ static void jacobian_launch(aerofx_ctx *c, int nchunks, const double *dt) {
                        c->fc, dt, c->d_fjac, (T *)c->d_jacD, (T *)c->d_jacB);
 }
 
-int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
+int launch_jacobian(aerofx_ctx *c, const double *dt, int precision, const double *U) {
   int rc = ensure_jacobian_storage(c, precision);
   if (rc) return rc;
   c->jac_version++;
@@ -614,6 +620,9 @@ int launch_jacobian(aerofx_ctx *c, const double *dt, int precision) {
       jacobian_launch<double, false>(c, nchunks, dt);
   }
   if ((rc = check_launch("jacobian launch"))) return rc;
+  // the Navier-Stokes Galerkin blocks, then the wall conditions
+  // (MatVecProdH1::evaluate, MatVecProd.C:1080-1092)
+  if ((rc = launch_visc_jacobian(c, U, precision))) return rc;
   {
     ProfScope ps(c, AEROFX_TIMER_PREC_SETUP);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, c->d_err + 1, 1, nullptr);

@@ -674,6 +674,7 @@ static int enqueue_fd_product(aerofx_ctx *c, const FdScratch &s, int nparts, con
   if ((rc = launch_residual(c, s.Q, s.Fa, true, done))) return rc;
   hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, done);
   if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Q, U, dt, s.Fa, done);
+  if ((rc = launch_bcs_residual(c, s.Q, s.Fa, done))) return rc;
   if (order == 1) {
     hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Fa, F0, s.scal, 3, prod, done);
   } else {
@@ -681,6 +682,7 @@ static int enqueue_fd_product(aerofx_ctx *c, const FdScratch &s, int nparts, con
     if ((rc = launch_residual(c, s.Q, s.Fb, true, done))) return rc;
     hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, done);
     if (dt) hipLaunchKernelGGL(k_fd_time, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Q, U, dt, s.Fb, done);
+    if ((rc = launch_bcs_residual(c, s.Q, s.Fb, done))) return rc;
     hipLaunchKernelGGL(k_fd_diff, grid_for(n), dim3(kBlock), 0, c->stream, n, s.Fa, s.Fb, s.scal, 4, prod, done);
   }
   hipError_t e = hipGetLastError();
@@ -698,6 +700,7 @@ static int enqueue_jvp_fd(aerofx_ctx *c, const FdScratch &s, int nparts, const d
     // F(U): the time term vanishes at Q = Un
     if ((rc = launch_residual(c, U, s.Fb, true))) return rc;
     hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, nullptr);
+    if ((rc = launch_bcs_residual(c, U, s.Fb, nullptr))) return rc;
   }
   return enqueue_fd_product(c, s, nparts, U, dt, s.Fb, p, prod, order, nullptr);
 }
@@ -815,6 +818,7 @@ static int select_fd(aerofx_ctx *c, const double *Q, const double *F0, const dou
   if (!F0) {
     if ((rc = launch_residual(c, Q, fd_own_F0(c), true))) return rc;
     hipLaunchKernelGGL(k_err_accumulate, dim3(1), dim3(64), 0, c->stream, c->d_err, nullptr);
+    if ((rc = launch_bcs_residual(c, Q, fd_own_F0(c), nullptr))) return rc;
     F0 = fd_own_F0(c);
   }
   enqueue_fd_state_sum(c, s, Q);
@@ -1049,8 +1053,10 @@ int aerofx_mvp_select(aerofx_ctx *c, int type, const double *Q, const double *F0
   if (!Q) return set_error(AEROFX_ERR_ARG, "mvp_select: null state");
   if (type == AEROFX_MVP_H2 && c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "mvp_select: exact products not available with low-Mach preconditioning");
-  if (type == AEROFX_MVP_H2)
+  if (type == AEROFX_MVP_H2) {
     if (int rf = refuse_failsafe_nodes(c, "mvp_select")) return rf;
+    if (int rv = refuse_viscous(c, "mvp_select: exact products")) return rv;
+  }
   if (type == AEROFX_MVP_H2) {
     int rc = select_h2(c, Q, dt);
     if (rc) return rc;
@@ -1121,6 +1127,7 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_solve: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(c, "adjoint_solve")) return rf;
+  if (int rv = refuse_viscous(c, "adjoint_solve")) return rv;
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "adjoint_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
@@ -1165,6 +1172,7 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "direct_solve: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(c, "direct_solve")) return rf;
+  if (int rv = refuse_viscous(c, "direct_solve")) return rv;
   if (nrhs < 1 || nrhs > 16) return set_error(AEROFX_ERR_ARG, "direct_solve: 1 <= nrhs <= 16");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "direct_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
@@ -1264,7 +1272,7 @@ int aerofx_jacobian_setup(aerofx_ctx *c, const double *U, const double *dt, int 
   if (precision != 0 && precision != 1) return set_error(AEROFX_ERR_ARG, "jacobian_setup: precision 0 or 1");
   int rc = launch_primitive(c, U, c->d_V);
   if (rc) return rc;
-  if ((rc = launch_jacobian(c, dt, precision))) return rc;
+  if ((rc = launch_jacobian(c, dt, precision, U))) return rc;
   hipError_t e = hipMemcpyAsync(c->h_err + 1, c->d_err + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, "jacobian_setup sync");
@@ -1337,6 +1345,8 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     return set_error(AEROFX_ERR_ARG, "steady_solve: mvp AEROFX_MVP_H1, AEROFX_MVP_FD or AEROFX_MVP_H2");
   if (o->mvp == AEROFX_MVP_H2 && c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "steady_solve: exact products not available with low-Mach preconditioning");
+  if (o->mvp == AEROFX_MVP_H2)
+    if (int rv = refuse_viscous(c, "steady_solve: exact products")) return rv;
   if (o->mvp == AEROFX_MVP_FD && o->fd_order != 1 && o->fd_order != 2)
     return set_error(AEROFX_ERR_ARG, "steady_solve: fd_order 1 or 2");
   if (o->prec != AEROFX_PREC_JACOBI && o->prec != AEROFX_PREC_RAS_ILU0)
@@ -1418,6 +1428,8 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     } else if ((rc = launch_residual(c, U, k->R, true))) {
       return rc;
     }
+    // the wall conditions of the Navier-Stokes equations (ImplicitTsDesc.C:140)
+    if ((rc = launch_bcs_residual(c, U, k->R, nullptr))) return rc;
     norm2(c, k->R, k->scal);
     hipError_t e = hipMemcpyAsync(k->h_scal, k->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_err, c->d_err, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
@@ -1453,7 +1465,7 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
                             o->cfl0 / std::pow(rr, o->ser)),
                    o->cfl_max);
     if ((rc = launch_timestep(c, cfl, c->d_dt))) return rc;
-    if ((rc = launch_jacobian(c, c->d_dt, o->precision))) return rc;
+    if ((rc = launch_jacobian(c, c->d_dt, o->precision, U))) return rc;
     // the Krylov tolerance of this step, from ||b|| = ||R||
     double geps = o->gmres_eps;
     if (o->eps_formula == 1) {
@@ -1514,6 +1526,7 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
       ++st->failsafe_redos;
       int cnt = 0;
       if ((rc = residual_failsafe(c, rbuf, U, k->R, true, &cnt))) return rc;
+      if ((rc = launch_bcs_residual(c, U, k->R, nullptr))) return rc;
     }
     st->final_cfl = cfl;
   }

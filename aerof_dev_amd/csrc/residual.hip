@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double
       const double nv[3] = {t.fnorm3[3 * (size_t)face], t.fnorm3[3 * (size_t)face + 1],
                             t.fnorm3[3 * (size_t)face + 2]};
       double fl[5];
-      if (code == AEROFX_BC_SLIP_WALL || code == -AEROFX_BC_SLIP_WALL) {
+      if (is_wall_code(code)) {
         wallflux(nv, 0.0, Vn, fl);
       } else {
         const double *ub = (code == AEROFX_BC_INLET || code == -AEROFX_BC_INLET) ? fc.ub_in : fc.ub_out;
@@ -593,7 +593,9 @@ int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, 
                        b.Fm, b.P, c->fc, scale ? 1 : 0, R, done);
   }
   hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : set_hip_error(e, "residual launch");
+  if (e != hipSuccess) return set_hip_error(e, "residual launch");
+  // the Navier-Stokes Galerkin term (SpaceOperator.C:570-573)
+  return launch_visc_residual(c, b.V, R, scale, done);
 }
 
 int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done) {

@@ -489,6 +489,7 @@ extern "C" int aerofx_residual_derivative(aerofx_ctx *c, const double *U, const 
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "residual_derivative: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(c, "residual_derivative")) return rf;
+  if (int rv = refuse_viscous(c, "residual_derivative")) return rv;
   int rc = launch_tangent(c, U, dU, dX, dR);
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);

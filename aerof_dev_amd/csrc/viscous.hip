@@ -1,0 +1,443 @@
+// viscous.hip -- laminar Navier-Stokes terms on the GPU: the Galerkin term
+// of the residual (FemEquationTermNS, ElemTet::computeGalerkinTerm), its
+// Jacobian blocks in H1 (ElemTet::computeJacobianGalerkinTerm), the viscous
+// time-step term (ElemSet::computeTimeStep) and BcFcnNS's strong wall
+// conditions (SubDomain::applyBCsToResidual / applyBCsToJacobian /
+// applyBCsToSolutionVector).
+//
+// The element terms are gathered, not scattered: a node's residual sums its
+// elements' contributions in element order, an H1 entry (n, m) sums those of
+// the elements sharing edge (n, m), a diagonal block those of the node's
+// elements -- the reference's per-node (per-block) order, with no atomics.
+// Each element is evaluated once per vertex (residual) or per entry (H1):
+// the gathers stay in L2 and the sums deterministic.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.h"
+#include "device_viscous.h"
+#include "host_internal.h"
+
+namespace aerofx {
+
+namespace {
+
+constexpr int kLocalEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+__device__ __forceinline__ void load_element(const int4 *__restrict__ tets, const double *__restrict__ X4,
+                                             const double *__restrict__ V, int t, double x[4][3], double v[4][5]) {
+  const int4 nd = tets[t];
+  const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double4 p = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)id[k]);
+    x[k][0] = p.x;
+    x[k][1] = p.y;
+    x[k][2] = p.z;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) v[k][q] = V[5 * (size_t)id[k] + q];
+  }
+}
+
+// row k of a [4][W] register array without dynamic indexing (which would
+// send the array to scratch)
+template <int W>
+__device__ __forceinline__ void pick_row(const double a[4][W], int k, double out[W]) {
+#pragma unroll
+  for (int q = 0; q < W; ++q) out[q] = a[0][q];
+#pragma unroll
+  for (int r = 1; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < W; ++q) out[q] = k == r ? a[r][q] : out[q];
+}
+
+__device__ __forceinline__ double pick4(const double a[4], int k) {
+  return k == 0 ? a[0] : (k == 1 ? a[1] : (k == 2 ? a[2] : a[3]));
+}
+
+// R[n] += (sum over the node's elements of vol (r . g_n)) (/ V_n if scale)
+__global__ __launch_bounds__(kBlock) void k_visc_residual(int N, const int *__restrict__ ntrow,
+                                                          const int *__restrict__ ntslot,
+                                                          const int4 *__restrict__ tets, const double *__restrict__ X4,
+                                                          const double *__restrict__ V,
+                                                          const double *__restrict__ invcv, NsConsts ns, int scale,
+                                                          double *__restrict__ R, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int s = ntrow[n]; s < ntrow[n + 1]; ++s) {
+    const int slot = ntslot[s], t = slot >> 2, loc = slot & 3;
+    double x[4][3], v[4][5], g[4][3];
+    load_element(tets, X4, V, t, x, v);
+    const double vol = p1_gradients(x, g);
+    NsElem e;
+    ns_element(ns, g, v, e);
+    double c[5], gl[3];
+    pick_row<3>(g, loc, gl);
+    ns_node_residual(e, gl, vol, c);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[k] += c[k];
+  }
+  const double ic = invcv[n];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) R[5 * (size_t)n + k] += scale ? acc[k] * ic : acc[k];
+}
+
+// H1 entries: index q < 2E is adjacency entry q = (n, m), summed over the
+// elements of edge (n, m) and added (times 1/V_n per contribution) to its
+// off-diagonal block; q = 2E + n is the diagonal block of n, summed over the
+// node's elements and added times 1/V_n.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_visc_jacobian(DevTables tb, const int *__restrict__ ntrow,
+                                                          const int *__restrict__ ntslot,
+                                                          const int *__restrict__ etrow,
+                                                          const int *__restrict__ etslot,
+                                                          const int4 *__restrict__ tets, const double *__restrict__ V,
+                                                          NsConsts ns, T *__restrict__ D, T *__restrict__ B) {
+  const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int E2 = 2 * tb.E;
+  if (q >= (long long)E2 + tb.N) return;
+  double acc[25];
+#pragma unroll
+  for (int k = 0; k < 25; ++k) acc[k] = 0.0;
+  if (q < E2) {
+    const int e = (int)q;
+    // row of entry e
+    int lo = 0, hi = tb.N;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tb.rowptr[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int n = lo, l = tb.eidx[e];
+    const double cn = tb.invcv[n];
+    for (int s = etrow[l]; s < etrow[l + 1]; ++s) {
+      const int slot = etslot[s], t = slot / 6, le = slot - 6 * t;
+      const int4 nd = tets[t];
+      const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+      const int a = kLocalEdge[le][0], b = kLocalEdge[le][1];
+      const int ln = id[a] == n ? a : b, lm = id[a] == n ? b : a;
+      double x[4][3], v[4][5], g[4][3];
+      load_element(tets, tb.X4, V, t, x, v);
+      const double vol = p1_gradients(x, g);
+      NsElem el;
+      ns_element(ns, g, v, el);
+      double blk[25], gm[3], gn[3], vm[5];
+      pick_row<3>(g, lm, gm);
+      pick_row<3>(g, ln, gn);
+      pick_row<5>(v, lm, vm);
+      ns_node_jacobian(ns, el, gm, vm, pick4(el.T, lm), gn, blk);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) acc[k] += cn * blk[k] * vol;
+    }
+    T *Be = B + blk64_base(e);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Be[k * kBlkStride] = (T)((double)Be[k * kBlkStride] + acc[k]);
+  } else {
+    const int n = (int)(q - E2);
+    for (int s = ntrow[n]; s < ntrow[n + 1]; ++s) {
+      const int slot = ntslot[s], t = slot >> 2, loc = slot & 3;
+      double x[4][3], v[4][5], g[4][3];
+      load_element(tets, tb.X4, V, t, x, v);
+      const double vol = p1_gradients(x, g);
+      NsElem el;
+      ns_element(ns, g, v, el);
+      double blk[25], gl[3], vl[5];
+      pick_row<3>(g, loc, gl);
+      pick_row<5>(v, loc, vl);
+      ns_node_jacobian(ns, el, gl, vl, pick4(el.T, loc), gl, blk);
+#pragma unroll
+      for (int k = 0; k < 25; ++k) acc[k] += blk[k] * vol;
+    }
+    const double ic = tb.invcv[n];
+    T *Dn = D + blk64_base(n);
+#pragma unroll
+    for (int k = 0; k < 25; ++k) Dn[k * kBlkStride] = (T)((double)Dn[k * kBlkStride] + acc[k] * ic);
+  }
+}
+
+// idtv[n] = sum over the node's elements of 9 (mu/Re)/rho_mid |T| |g_n|^2,
+// the element's mean state (Vmid summed in vertex order, times 1/4)
+__global__ __launch_bounds__(kBlock) void k_visc_idtv(int N, const int *__restrict__ ntrow,
+                                                      const int *__restrict__ ntslot, const int4 *__restrict__ tets,
+                                                      const double *__restrict__ X4, const double *__restrict__ V,
+                                                      NsConsts ns, double *__restrict__ idtv) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double acc = 0.0;
+  for (int s = ntrow[n]; s < ntrow[n + 1]; ++s) {
+    const int slot = ntslot[s], t = slot >> 2, loc = slot & 3;
+    double x[4][3], v[4][5], g[4][3];
+    load_element(tets, X4, V, t, x, v);
+    double Vmid[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vmid[k] = (((0.0 + v[0][k]) + v[1][k]) + v[2][k]) + v[3][k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Vmid[k] *= 1.0 / 4.0;
+    const double mul = ns.ooRe * ns_mu(ns, ns_temperature(ns, Vmid));
+    double a[3], b[3], w[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      a[d] = x[1][d] - x[0][d];
+      b[d] = x[2][d] - x[0][d];
+      w[d] = x[3][d] - x[0][d];
+    }
+    const double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double volume = (1.0 / 6.0) * (w[0] * cr[0] + w[1] * cr[1] + w[2] * cr[2]);
+    const double localDt = 9.0 * (mul / Vmid[0]) * volume;
+    p1_gradients(x, g);
+    double gl[3];
+    pick_row<3>(g, loc, gl);
+    acc += localDt * (gl[0] * gl[0] + gl[1] * gl[1] + gl[2] * gl[2]);
+  }
+  idtv[n] = acc;
+}
+
+__device__ __forceinline__ bool wall_isothermal(int type) { return type == 1 || type == -1; }
+
+// BcFcnNS::applyToResidualTerm (BcFcnCore.C:230-245), fixed walls
+__global__ void k_bcs_residual(int nwall, const int2 *__restrict__ wall, double wall_T, const double *__restrict__ U,
+                               double *__restrict__ F, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwall) return;
+  const int n = wall[i].x;
+  const double Vw[5] = {0.0, 0.0, 0.0, 0.0, wall_T};
+  const double *u = U + 5 * (size_t)n;
+  double *f = F + 5 * (size_t)n;
+  f[1] = -Vw[1] * u[0] + u[1];
+  f[2] = -Vw[2] * u[0] + u[2];
+  f[3] = -Vw[3] * u[0] + u[3];
+  if (wall_isothermal(wall[i].y)) f[4] = (u[4] - u[0] * Vw[4]) * u[0] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
+}
+
+// BcFcnNS::applyToSolutionVectorTerm (BcFcnCore.C:212-226)
+__global__ void k_bcs_solution(int nwall, const int2 *__restrict__ wall, double wall_T, double *__restrict__ U) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwall) return;
+  const int n = wall[i].x;
+  const double Vw[5] = {0.0, 0.0, 0.0, 0.0, wall_T};
+  double *u = U + 5 * (size_t)n;
+  u[1] = u[0] * Vw[1];
+  u[2] = u[0] * Vw[2];
+  u[3] = u[0] * Vw[3];
+  if (wall_isothermal(wall[i].y)) u[4] = u[0] * (Vw[4] + 0.5 * (Vw[1] * Vw[1] + Vw[2] * Vw[2] + Vw[3] * Vw[3]));
+}
+
+// BcFcnNS::applyToOffDiagonalTerm / applyToDiagonalTerm (BcFcn.C:8-122) on
+// every block of the wall rows
+template <class T>
+__global__ void k_bcs_jacobian(int nwall, const int2 *__restrict__ wall, const int *__restrict__ rowptr, double wall_T,
+                               const double *__restrict__ U, T *__restrict__ D, T *__restrict__ B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwall) return;
+  const int n = wall[i].x;
+  const bool iso = wall_isothermal(wall[i].y);
+  const int r1 = iso ? 5 : 4;
+  for (int e = rowptr[n]; e < rowptr[n + 1]; ++e) {
+    T *Be = B + blk64_base(e);
+    for (int r = 1; r < r1; ++r)
+      for (int k = 0; k < 5; ++k) Be[(5 * r + k) * kBlkStride] = (T)0.0;
+  }
+  const double Vw[5] = {0.0, 0.0, 0.0, 0.0, wall_T};
+  const double *u = U + 5 * (size_t)n;
+  T *Dn = D + blk64_base(n);
+  for (int r = 1; r < r1; ++r)
+    for (int k = 0; k < 5; ++k) Dn[(5 * r + k) * kBlkStride] = (T)0.0;
+  Dn[5 * kBlkStride] = (T)(-Vw[1]);
+  Dn[6 * kBlkStride] = (T)1.0;
+  Dn[10 * kBlkStride] = (T)(-Vw[2]);
+  Dn[12 * kBlkStride] = (T)1.0;
+  Dn[15 * kBlkStride] = (T)(-Vw[3]);
+  Dn[18 * kBlkStride] = (T)1.0;
+  if (iso) {
+    Dn[20 * kBlkStride] = (T)(u[4] - 2.0 * Vw[4] * u[0]);
+    Dn[21 * kBlkStride] = (T)(-u[1]);
+    Dn[22 * kBlkStride] = (T)(-u[2]);
+    Dn[23 * kBlkStride] = (T)(-u[3]);
+    Dn[24 * kBlkStride] = (T)u[0];
+  }
+}
+
+int check(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_hip_error(e, what);
+}
+
+// Domain::setNodeType priorities (DomainCore.C:618-640)
+int bc_priority(int code) {
+  static const int pr[22] = {15, 16, 13, 14, 17, 11, 12, 20, 18, 19, -1, 9, 8, 10, 2, 1, 7, 4, 3, 6, 5, 0};
+  return (code < -10 || code > 11) ? -1 : pr[code + 10];
+}
+
+}  // namespace
+
+int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, const int *done) {
+  if (!c->viscous) return 0;
+  const ElemTables *el = c->elem;
+  ProfScope ps(c, AEROFX_TIMER_FE_TERM);
+  hipLaunchKernelGGL(k_visc_residual, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
+                     el->ntrow, el->ntslot, (const int4 *)el->tets, c->tab.X4, V, c->tab.invcv, c->ns, scale ? 1 : 0,
+                     R, done);
+  return check("viscous residual launch");
+}
+
+int launch_visc_jacobian(aerofx_ctx *c, const double *U, int precision) {
+  if (!c->viscous) return 0;
+  const ElemTables *el = c->elem;
+  const long long nq = 2LL * c->E + c->N;
+  const dim3 grid((unsigned)((nq + kBlock - 1) / kBlock));
+  {
+    ProfScope ps(c, AEROFX_TIMER_FE_JAC);
+    if (precision)
+      hipLaunchKernelGGL(k_visc_jacobian<float>, grid, dim3(kBlock), 0, c->stream, c->tab, el->ntrow, el->ntslot,
+                         el->etrow, el->etslot, (const int4 *)el->tets, c->d_V, c->ns, (float *)c->d_jacD,
+                         (float *)c->d_jacB);
+    else
+      hipLaunchKernelGGL(k_visc_jacobian<double>, grid, dim3(kBlock), 0, c->stream, c->tab, el->ntrow, el->ntslot,
+                         el->etrow, el->etslot, (const int4 *)el->tets, c->d_V, c->ns, (double *)c->d_jacD,
+                         (double *)c->d_jacB);
+  }
+  int rc = check("viscous jacobian launch");
+  if (rc || !c->nwall) return rc;
+  if (!U) return set_error(AEROFX_ERR_ARG, "jacobian: the wall conditions need the state");
+  const dim3 g((c->nwall + 63) / 64), b(64);
+  if (precision)
+    hipLaunchKernelGGL(k_bcs_jacobian<float>, g, b, 0, c->stream, c->nwall, c->d_wall, c->tab.rowptr, c->ns.wall_T,
+                       U, (float *)c->d_jacD, (float *)c->d_jacB);
+  else
+    hipLaunchKernelGGL(k_bcs_jacobian<double>, g, b, 0, c->stream, c->nwall, c->d_wall, c->tab.rowptr, c->ns.wall_T,
+                       U, (double *)c->d_jacD, (double *)c->d_jacB);
+  return check("wall jacobian launch");
+}
+
+int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *done) {
+  if (!c->viscous || !c->nwall) return 0;
+  hipLaunchKernelGGL(k_bcs_residual, dim3((c->nwall + 63) / 64), dim3(64), 0, c->stream, c->nwall, c->d_wall,
+                     c->ns.wall_T, U, F, done);
+  return check("wall residual launch");
+}
+
+int launch_visc_idtv(aerofx_ctx *c, double **idtv) {
+  *idtv = nullptr;
+  if (!c->viscous || c->ns.viscous_cst == 0.0) return 0;
+  if (!c->d_idtv) {
+    void *p = nullptr;
+    int rc = ctx_alloc(c, sizeof(double) * (size_t)c->N, &p);
+    if (rc) return rc;
+    c->d_idtv = (double *)p;
+  }
+  const ElemTables *el = c->elem;
+  hipLaunchKernelGGL(k_visc_idtv, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N, el->ntrow,
+                     el->ntslot, (const int4 *)el->tets, c->tab.X4, c->d_V, c->ns, c->d_idtv);
+  *idtv = c->d_idtv;
+  return check("viscous time-step launch");
+}
+
+int refuse_viscous(aerofx_ctx *c, const char *who) {
+  if (!c->viscous) return 0;
+  return set_error(AEROFX_ERR_STATE, std::string(who) + ": not available with the Navier-Stokes terms");
+}
+
+}  // namespace aerofx
+
+using namespace aerofx;
+
+extern "C" {
+
+int aerofx_ctx_set_viscous(aerofx_ctx *c, const aerofx_viscous *v) {
+  if (!c) return set_error(AEROFX_ERR_ARG, "set_viscous: null context");
+  if (!v) {
+    c->viscous = 0;
+    c->nwall = 0;
+    return AEROFX_OK;
+  }
+  if (c->fc.low_mach) return set_error(AEROFX_ERR_STATE, "set_viscous: not with the low-Mach preconditioner");
+  if (!(v->reynolds > 0.0)) return set_error(AEROFX_ERR_ARG, "set_viscous: Reynolds number must be > 0");
+  if (v->viscosity_model < 0 || v->viscosity_model > 2 || v->conductivity_model < 0 || v->conductivity_model > 1)
+    return set_error(AEROFX_ERR_ARG, "set_viscous: unknown viscosity or conductivity model");
+  if (v->conductivity_model == AEROFX_COND_CONSTANT_PRANDTL && !(v->prandtl > 0.0))
+    return set_error(AEROFX_ERR_ARG, "set_viscous: Prandtl number must be > 0");
+  if (v->viscosity_model == AEROFX_VISC_SUTHERLAND && !(v->sutherland_ts > 0.0))
+    return set_error(AEROFX_ERR_ARG, "set_viscous: Sutherland temperature ratio must be > 0");
+  int rc = ensure_elem_tables(c);
+  if (rc) return rc;
+  NsConsts ns{};
+  ns.visc = v->viscosity_model;
+  ns.cond = v->conductivity_model;
+  ns.ooRe = 1.0 / v->reynolds;
+  ns.gam = c->p.gamma;
+  ns.invgam1 = 1.0 / (c->p.gamma - 1.0);
+  ns.pstiff = c->p.pstiff;
+  ns.alpha = c->p.gamma * (c->p.gamma - 1.0) * c->p.mach * c->p.mach;
+  ns.dalpha = 2.0 * c->p.gamma * (c->p.gamma - 1.0) * c->p.mach;
+  ns.Ts = v->sutherland_ts;
+  ns.bulk = v->bulk_viscosity;
+  ns.ooPr = v->conductivity_model == AEROFX_COND_CONSTANT_PRANDTL ? 1.0 / v->prandtl : 0.0;
+  ns.kcond = v->conductivity;
+  ns.wall_T = v->wall_temperature;
+  ns.viscous_cst = v->viscous_cst;
+  // the wall nodes: node type = code of the highest-priority face touching
+  // the node (Domain::setNodeType, Face::setNodeType)
+  const int N = c->N, F = c->F;
+  std::vector<int> type(N, 0);
+  for (int f = 0; f < F; ++f) {
+    const int code = c->fcodes[f];
+    if (code < -10 || code > 11) continue;
+    for (int a = 0; a < 3; ++a) {
+      const int n = c->faces[3 * (size_t)f + a];
+      if (bc_priority(code) > bc_priority(type[n])) type[n] = code;
+    }
+  }
+  std::vector<int2> wall;
+  for (int n = 0; n < N; ++n)
+    if (type[n] == 1 || type[n] == -1 || type[n] == 3 || type[n] == -3) wall.push_back(make_int2(n, type[n]));
+  if (!wall.empty() && (int)wall.size() > c->wall_cap) {
+    void *p = nullptr;
+    if ((rc = ctx_alloc(c, sizeof(int2) * wall.size(), &p))) return rc;
+    if (c->d_wall) ctx_release(c, c->d_wall);
+    c->d_wall = (int2 *)p;
+    c->wall_cap = (int)wall.size();
+  }
+  if (!wall.empty()) {
+    hipError_t e = hipMemcpyAsync(c->d_wall, wall.data(), sizeof(int2) * wall.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "set_viscous upload");
+  }
+  c->node_type = type;
+  c->nwall = (int)wall.size();
+  c->ns = ns;
+  c->viscous = 1;
+  return AEROFX_OK;
+}
+
+int aerofx_ctx_node_types(const aerofx_ctx *c, int *types) {
+  if (!c || !types) return set_error(AEROFX_ERR_ARG, "node_types: null argument");
+  if (!c->viscous) return set_error(AEROFX_ERR_STATE, "node_types: the Navier-Stokes terms are off");
+  std::memcpy(types, c->node_type.data(), sizeof(int) * (size_t)c->N);
+  return AEROFX_OK;
+}
+
+int aerofx_apply_bcs_residual(aerofx_ctx *c, const double *U, double *F) {
+  if (!c || !U || !F) return set_error(AEROFX_ERR_ARG, "apply_bcs_residual: null argument");
+  int rc = launch_bcs_residual(c, U, F, nullptr);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "apply_bcs_residual sync");
+}
+
+int aerofx_apply_bcs_solution(aerofx_ctx *c, double *U) {
+  if (!c || !U) return set_error(AEROFX_ERR_ARG, "apply_bcs_solution: null argument");
+  if (c->viscous && c->nwall) {
+    hipLaunchKernelGGL(k_bcs_solution, dim3((c->nwall + 63) / 64), dim3(64), 0, c->stream, c->nwall, c->d_wall,
+                       c->ns.wall_T, U);
+    int rc = check("wall solution launch");
+    if (rc) return rc;
+  }
+  hipError_t e = hipStreamSynchronize(c->stream);
+  return e == hipSuccess ? AEROFX_OK : set_hip_error(e, "apply_bcs_solution sync");
+}
+
+}  // extern "C"

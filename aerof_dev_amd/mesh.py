@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import (AEROFX_BC_INLET, AEROFX_BC_OUTLET, AEROFX_BC_SLIP_WALL, BoxSpec, Params,
+from ._abi import (AEROFX_BC_ADIABATIC_WALL, AEROFX_BC_INLET, AEROFX_BC_OUTLET, AEROFX_BC_SLIP_WALL, BoxSpec, Params,
                    c_double_p, c_int_p, check, lib)
 
 
@@ -37,6 +37,33 @@ class LowMachPrec:
 
 
 @dataclass
+class ViscousParams:
+    """Laminar Navier-Stokes terms (Equations.Type = NavierStokes, no
+    turbulence model, full wall integration): Reynolds number (based on the
+    freestream of the AERO-F non-dimensional scaling, rho = 1, |u| = 1,
+    p = 1/(gamma M^2)), viscosity model ("constant", "sutherland",
+    "prandtl"; ViscoFcn.h), conductivity model ("prandtl": constant Prandtl
+    number, "constant"; ThermalCondFcn.h), the temperature of isothermal walls
+    (non-dimensional T = p / ((gamma - 1) rho)) and the weight of the viscous
+    time-step term (Time.ViscosityParameter, 0 by default)."""
+    reynolds: float = 100.0
+    viscosity: str = "sutherland"
+    bulk_viscosity: float = 0.0
+    sutherland_ts: float = 110.4 / 288.15
+    conductivity_model: str = "prandtl"
+    prandtl: float = 0.72
+    conductivity: float = 0.0
+    wall_temperature: float = 0.0
+    viscous_cst: float = 0.0
+
+    _VISC = {"constant": 0, "sutherland": 1, "prandtl": 2}
+    _COND = {"prandtl": 0, "constant": 1}
+
+    def codes(self):
+        return self._VISC[self.viscosity], self._COND[self.conductivity_model]
+
+
+@dataclass
 class FlowParams:
     """Perfect-gas Euler parameters (IoDataCore.C:2418-2429 defaults)."""
     gamma: float = 1.4
@@ -51,6 +78,8 @@ class FlowParams:
     # spatial low-Mach preconditioning (Problem.Prec = Preconditioned): the
     # Roe-Turkel flux ROEFLUX5 and its Jacobian; None: Roe's flux
     low_mach: LowMachPrec | None = None
+    # laminar Navier-Stokes terms; None: Euler
+    viscous: ViscousParams | None = None
 
     def c(self) -> Params:
         return Params(self.gamma, self.pstiff, self.beta, int(self.second_order), self.rho_inf,
@@ -82,6 +111,9 @@ FARFIELD_BOX = (AEROFX_BC_INLET, AEROFX_BC_OUTLET, AEROFX_BC_OUTLET, AEROFX_BC_O
                 AEROFX_BC_SLIP_WALL, AEROFX_BC_OUTLET)
 CHANNEL = (AEROFX_BC_INLET, AEROFX_BC_OUTLET, AEROFX_BC_SLIP_WALL, AEROFX_BC_SLIP_WALL,
            AEROFX_BC_SLIP_WALL, AEROFX_BC_SLIP_WALL)
+# channel with no-slip adiabatic walls top and bottom, slip side walls
+VISCOUS_CHANNEL = (AEROFX_BC_INLET, AEROFX_BC_OUTLET, AEROFX_BC_SLIP_WALL, AEROFX_BC_SLIP_WALL,
+                   AEROFX_BC_ADIABATIC_WALL, AEROFX_BC_ADIABATIC_WALL)
 
 
 def _box_spec(nx, ny, nz, lo, hi, jitter, seed, face_codes, bumps):
@@ -213,7 +245,8 @@ def box_mesh(n: int, jitter: float = 0.15, seed: int = 1234, rcm_order: bool = F
 
 
 def channel_mesh(nx: int, ny: int, nz: int, bump_height: float = 0.05, bump_width: float = 0.2,
-                 jitter: float = 0.0, seed: int = 0) -> Mesh:
-    """Channel [0,3]x[0,1]x[0,1] with a Gaussian bump on the bottom wall."""
+                 jitter: float = 0.0, seed: int = 0, face_codes=CHANNEL) -> Mesh:
+    """Channel [0,3]x[0,1]x[0,1] with a Gaussian bump on the bottom wall
+    (face_codes=VISCOUS_CHANNEL: no-slip adiabatic top and bottom walls)."""
     return kuhn_box(nx, ny, nz, lo=(0.0, 0.0, 0.0), hi=(3.0, 1.0, 1.0), jitter=jitter, seed=seed,
-                    face_codes=CHANNEL, bumps=((bump_height, 1.5, bump_width),))
+                    face_codes=face_codes, bumps=((bump_height, 1.5, bump_width),))

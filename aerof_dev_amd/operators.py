@@ -66,6 +66,7 @@ class SpaceOperator:
         self.n5, self.n3 = 5 * self.num_nodes, 3 * self.num_nodes  # element counts of [N][5], [N][3]
         self.set_stream(stream)
         self.set_low_mach(params.low_mach)
+        self.set_viscous(getattr(params, "viscous", None))
 
     def set_failsafe(self, mode: int):
         """Newton.FailSafe for the synchronous residual: 0 off, 1 on, 2 always on."""
@@ -91,6 +92,31 @@ class SpaceOperator:
         else:
             c = LowMachC(lm.min_mach, lm.slope, lm.cutoff, lm.shock_reducer, lm.irey)
             check(lib().aerofx_ctx_set_low_mach(self._h, C.byref(c)), "set_low_mach")
+
+    def set_viscous(self, v=None):
+        """Laminar Navier-Stokes terms (ViscousParams) on, or off (None)."""
+        from ._abi import ViscousC
+        if v is None:
+            check(lib().aerofx_ctx_set_viscous(self._h, None), "set_viscous")
+            return
+        vm, cm = v.codes()
+        c = ViscousC(v.reynolds, vm, v.bulk_viscosity, v.sutherland_ts, cm, v.prandtl, v.conductivity,
+                     v.wall_temperature, v.viscous_cst)
+        check(lib().aerofx_ctx_set_viscous(self._h, C.byref(c)), "set_viscous")
+
+    def node_types(self):
+        """Node types of the wall conditions (host int array [N])."""
+        t = np.empty(self.num_nodes, dtype=np.int32)
+        check(lib().aerofx_ctx_node_types(self._h, t.ctypes.data_as(c_int_p)), "node_types")
+        return t
+
+    def apply_bcs_residual(self, U, F):
+        """SpaceOperator::applyBCsToResidual: the wall rows of F (in place)."""
+        check(lib().aerofx_apply_bcs_residual(self._h, _ptr(U, self.n5), _ptr(F, self.n5)), "apply_bcs_residual")
+
+    def apply_bcs_solution(self, U):
+        """SpaceOperator::applyBCsToSolutionVector (in place)."""
+        check(lib().aerofx_apply_bcs_solution(self._h, _ptr(U, self.n5)), "apply_bcs_solution")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -120,10 +146,11 @@ class SpaceOperator:
     def timers(self) -> dict:
         """{bucket: (total ms, launches)} for buckets with launches."""
         from ._abi import TIMER_NAMES
-        ms = (C.c_double * 8)()
-        n = (C.c_longlong * 8)()
+        nt = len(TIMER_NAMES)
+        ms = (C.c_double * nt)()
+        n = (C.c_longlong * nt)()
         check(lib().aerofx_ctx_timers(self._h, ms, n), "ctx_timers")
-        return {TIMER_NAMES[k]: (ms[k], n[k]) for k in range(8) if n[k]}
+        return {TIMER_NAMES[k]: (ms[k], n[k]) for k in range(nt) if n[k]}
 
     def sync(self):
         check(lib().aerofx_ctx_sync(self._h), "sync")

@@ -313,6 +313,76 @@ def adjoint_bench(dev, reps=5):
     return res
 
 
+def viscous_bench(dev, n=NX, steps=20):
+    """Laminar Navier-Stokes terms on the 128^3 Kuhn box (no-slip adiabatic
+    z- wall, M 0.5, Re 1000, Sutherland): the residual with the Galerkin term,
+    the Galerkin kernel alone (feTerm), and one H1 assembly with its Galerkin
+    blocks (feJac), FP64 and FP32."""
+    import torch
+
+    from aerof_dev_amd import FlowParams, ViscousParams, init_state
+    from aerof_dev_amd import mesh as M
+    from aerof_dev_amd.operators import MatVecProdH1, SpaceOperator
+
+    codes = list(M.FARFIELD_BOX)
+    codes[4] = 3
+    mesh = M.kuhn_box(n, n, n, jitter=0.15, seed=1234, face_codes=tuple(codes))
+    mesh = M.renumber(mesh, M.rcm_gpu(mesh))
+    mach = 0.5
+    params = FlowParams(mach=mach, p_inf=1.0 / (1.4 * mach * mach), viscous=ViscousParams(reynolds=1000.0))
+    op = SpaceOperator(mesh, params)
+    N, E, T = op.num_nodes, op.num_edges, mesh.num_tets
+    U = torch.from_numpy(init_state(params, mesh.X, "sine", 0.02, seed=2024)).to(dev)
+    R = torch.empty_like(U)
+    if op.compute_residual(U, R) != 0:
+        raise RuntimeError("non-physical reconstruction in the viscous bench state")
+    for _ in range(3):
+        op.compute_residual_async(U, R)
+    torch.cuda.synchronize()
+    op.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.compute_residual_async(U, R)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) / steps * 1e3
+    tm = op.timers()
+    op.profile(False)
+    fe_ms = tm["feTerm"][0] / tm["feTerm"][1]
+    fv_ms = sum(tm[k][0] / tm[k][1] for k in ("nodalGrad", "fvTerm", "fvTermNodes") if k in tm)
+    # algorithmic bytes of the Galerkin pass: per node V, X (72 B), R read and
+    # written (80 B), its incidence row (4 B); per element its 4 node ids
+    # (16 B) and 4 incidence slots (16 B)
+    fe_bytes = 156 * N + 32 * T
+    h1 = MatVecProdH1(op)
+    dt = torch.empty(N, dtype=torch.float64, device=dev)
+    h1.timestep(U, 100.0, dt)
+    jac = {}
+    for precision in (0, 1):
+        h1.evaluate(U, dt, precision)
+        torch.cuda.synchronize()
+        op.profile(True)
+        for _ in range(3):
+            h1.evaluate(U, dt, precision)
+        torch.cuda.synchronize()
+        tj = op.timers()
+        op.profile(False)
+        esz = 8 if precision == 0 else 4
+        # per block read and written once (25 entries), plus the node data
+        jb = 2 * 25 * esz * (2 * E + N) + 72 * N + 56 * T
+        fj = tj["feJac"][0] / tj["feJac"][1]
+        jac["fp64" if precision == 0 else "fp32"] = {
+            "feJac_ms": round(fj, 4), "fvJac_ms": round(tj["fvJac"][0] / tj["fvJac"][1], 4),
+            "algorithmic_GBs": round(jb / (fj * 1e-3) / 1e9, 1)}
+    op.close()
+    res = {"workload": f"{n}^3 Kuhn box ({N} nodes, {T} elements, {E} edges), no-slip z- wall, M {mach}, Re 1000, "
+                       "Sutherland, second order",
+           "residual_ms": round(wall_ms, 4), "finite_volume_ms": round(fv_ms, 4), "feTerm_ms": round(fe_ms, 4),
+           "feTerm_algorithmic_GBs": round(fe_bytes / (fe_ms * 1e-3) / 1e9, 1),
+           "feTerm_Melements_per_s": round(T / (fe_ms * 1e-3) / 1e6, 1), "jacobian": jac}
+    log(f"[bench] viscous: {res}")
+    return res
+
+
 def shape_bench(dev, order=1, ncheck=1):
     """Adjoint shape gradient on the 192x64x64 bump channel (815k nodes):
     the wall lift J (pressure force on the slip walls along z) and its
@@ -467,6 +537,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=600)
+    ap.add_argument("--no-viscous", action="store_true", help="skip the Navier-Stokes leg")
     ap.add_argument("--batched", action="store_true",
                     help="also time 16-RHS block GMRES against 16 single-RHS FP64 solves (128^3 box)")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -597,7 +668,8 @@ def main():
         roofline["fp64_tflops"] = round(ops / (kern_ms * 1e-3) / 1e12, 2)
         roofline["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
         roofline["fp64_frac"] = round(roofline["fp64_tflops"] / FP64_PEAK_TFLOPS, 4)
-    if not args.no_newton or args.batched or not args.no_adjoint or args.shape or args.sensitivity:
+    if (not args.no_newton or args.batched or not args.no_adjoint or args.shape or args.sensitivity
+            or not args.no_viscous):
         op.close()
     if not args.no_newton:
         out["newton"] = newton_bench(dev, args.newton_steps)
@@ -605,6 +677,8 @@ def main():
         out["batched"] = batched_bench(dev, args.n)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(dev)
+    if not args.no_viscous:
+        out["viscous"] = viscous_bench(dev, args.n)
     if args.shape:
         out["shape"] = shape_bench(dev)
     if args.sensitivity:

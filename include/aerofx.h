@@ -33,7 +33,9 @@ extern "C" {
 
 /* Boundary codes (BcDef.h:4-28). */
 #define AEROFX_BC_INTERNAL 0
+#define AEROFX_BC_ISOTHERMAL_WALL 1 /* no-slip wall at a fixed temperature (viscous) */
 #define AEROFX_BC_SLIP_WALL 2
+#define AEROFX_BC_ADIABATIC_WALL 3  /* no-slip adiabatic wall (viscous) */
 #define AEROFX_BC_INLET 4
 #define AEROFX_BC_OUTLET 5
 
@@ -149,7 +151,9 @@ int aerofx_ctx_tables(const aerofx_ctx *ctx, int *ptr, double *ctrl_vol,
 #define AEROFX_TIMER_JAC_APPLY 5    /* Jacobian-vector product */
 #define AEROFX_TIMER_PREC_APPLY 6   /* preconditioner apply */
 #define AEROFX_TIMER_GRAM_SCHMIDT 7 /* Krylov orthogonalisation */
-#define AEROFX_NUM_TIMERS 8
+#define AEROFX_TIMER_FE_TERM 8      /* Navier-Stokes Galerkin term */
+#define AEROFX_TIMER_FE_JAC 9       /* its H1 blocks */
+#define AEROFX_NUM_TIMERS 10
 int aerofx_ctx_profile(aerofx_ctx *ctx, int enable);
 int aerofx_ctx_timers(aerofx_ctx *ctx, double *ms, long long *counts);
 
@@ -191,7 +195,66 @@ typedef struct {
   double shock_reducer;  /* Preconditioner.ShockReducer */
   double irey;           /* 1/Re of the edges (0: inviscid) */
 } aerofx_low_mach;
+/* (not with the Navier-Stokes terms of aerofx_ctx_set_viscous) */
 int aerofx_ctx_set_low_mach(aerofx_ctx *ctx, const aerofx_low_mach *lm);
+
+/* Laminar Navier-Stokes terms (Equations.Type = NavierStokes without a
+ * turbulence model, full wall integration; the reference's
+ * FemEquationTermNS + BcFcnNS, SpaceOperator.C:76-123, 246-265):
+ *   - the residual (aerofx_residual*) adds the P1 Galerkin term of every
+ *     tetrahedron, vol (r . grad phi_j) with r the viscous stress and heat
+ *     flux at the element's mean state (ElemTet::computeGalerkinTerm,
+ *     ElemTet.C:24-120; FemEquationTermNS::computeVolumeTerm,
+ *     FemEquationTermDesc.C:98-158), before the 1/V scaling;
+ *   - aerofx_jacobian_setup adds its exact Jacobian blocks to H1
+ *     (ElemTet::computeJacobianGalerkinTerm, ElemTet.C:1912-1957;
+ *     FemEquationTermNS::computeJacobianVolumeTerm, FemEquationTermDesc.C:
+ *     338-449) and then applies the wall conditions to H1
+ *     (SpaceOperator::applyBCsToJacobian, SpaceOperator.C:2421-2433);
+ *   - the Newton driver and the finite-difference products apply the wall
+ *     conditions to every F(Q) (ImplicitTsDesc::computeFunction,
+ *     ImplicitTsDesc.C:125-141; MatVecProd.C:88-118, 262-300);
+ *   - the time step adds viscous_cst times the elemental viscous term
+ *     (ElemSet::computeTimeStep, Elem.C:25-83; Domain.C:91).
+ * Walls are the faces of code AEROFX_BC_ADIABATIC_WALL (no-slip) and
+ * AEROFX_BC_ISOTHERMAL_WALL (no-slip, temperature wall_temperature); a node
+ * takes the code of the highest-priority face touching it
+ * (Domain::setNodeType, DomainCore.C:604-670).  Walls are fixed (zero wall
+ * velocity, so the adiabatic-wall face term vanishes).  The transport laws
+ * are ViscoFcn.h's and ThermalCondFcn.h's with the AERO-F non-dimensional
+ * scaling (alpha = gamma (gamma - 1) M^2).  v = NULL turns the terms off.
+ * Not with the low-Mach preconditioner; the derivative passes
+ * (aerofx_residual_derivative, aerofx_adjoint_residual, AEROFX_MVP_H2,
+ * aerofx_adjoint_solve, aerofx_direct_solve) return AEROFX_ERR_STATE. */
+#define AEROFX_VISC_CONSTANT 0
+#define AEROFX_VISC_SUTHERLAND 1
+#define AEROFX_VISC_PRANDTL 2
+#define AEROFX_COND_CONSTANT_PRANDTL 0
+#define AEROFX_COND_CONSTANT 1
+typedef struct {
+  double reynolds;          /* ReferenceState.Reynolds (ref.reynolds_mu) */
+  int viscosity_model;      /* AEROFX_VISC_* (ViscosityModel.Type) */
+  double bulk_viscosity;    /* constant model: lambda = bulk - 2/3 mu */
+  double sutherland_ts;     /* SutherlandReferenceTemperature / ReferenceState.Temperature */
+  int conductivity_model;   /* AEROFX_COND_* (ThermalConductivityModel.Type) */
+  double prandtl;           /* constant-Prandtl model */
+  double conductivity;      /* constant model (non-dimensional) */
+  double wall_temperature;  /* isothermal walls: non-dimensional T = p / ((gamma - 1) rho) */
+  double viscous_cst;       /* Time.ViscosityParameter (0) */
+} aerofx_viscous;
+int aerofx_ctx_set_viscous(aerofx_ctx *ctx, const aerofx_viscous *v);
+/* Node types of the wall conditions [N] (the face codes by priority, 0
+ * inside); needs the viscous terms on. */
+int aerofx_ctx_node_types(const aerofx_ctx *ctx, int *types);
+/* SpaceOperator::applyBCsToResidual (SpaceOperator.C:2389-2395;
+ * BcFcnNS::applyToResidualTerm, BcFcnCore.C:230-245): on the wall nodes
+ * F[1..3] = U[1..3] (isothermal walls also F[4] = (E - rho T_w) rho -
+ * |m|^2 / 2).  Device pointers [N][5]; nothing without the viscous terms. */
+int aerofx_apply_bcs_residual(aerofx_ctx *ctx, const double *U, double *F);
+/* SpaceOperator::applyBCsToSolutionVector (BcFcnCore.C:212-226): zero
+ * momentum on the wall nodes (and the wall temperature on isothermal
+ * walls).  Device pointer [N][5]. */
+int aerofx_apply_bcs_solution(aerofx_ctx *ctx, double *U);
 
 /* Newton.FailSafe (NewtonData::failsafe, IoData.C:10-35): 0 Off (default),
  * 1 On, 2 AlwaysOn; second-order schemes only, as the reference turns it off

@@ -538,6 +538,8 @@ static int residual_pass(const or_mesh *m, const or_params *p, const double *U, 
   for (int k = 0; k < 5 * N; ++k) R[k] = 0.0;
   if (Rabs) for (int k = 0; k < 5 * N; ++k) Rabs[k] = 0.0;
   or_cons_to_prim(N, U, V, gam, pstiff);
+  /* FemEquationTermNS: the Galerkin term comes first (SpaceOperator.C:570-573) */
+  if (p->viscous) or_galerkin_term(m, p, V, R, Rabs);
   if (p->second_order) {
     ddx = ddx_out ? ddx_out : (double *)malloc(sizeof(double) * 5 * (size_t)N);
     ddy = ddy_out ? ddy_out : (double *)malloc(sizeof(double) * 5 * (size_t)N);
@@ -604,7 +606,9 @@ static int residual_pass(const or_mesh *m, const or_params *p, const double *U, 
     for (int a = 0; a < 3; ++a) {
       int node = m->faces[3 * f + a];
       double flux[5];
-      if (code == 2 || code == -2) or_wallflux(n, nvel, V + 5 * node, flux);
+      /* every wall code takes the wall flux (FluxFcn.h:270-276) */
+      if (code == 1 || code == -1 || code == 2 || code == -2 || code == 3 || code == -3)
+        or_wallflux(n, nvel, V + 5 * node, flux);
       else if (code == 4 || code == -4) or_boundflux5(gam, n, nvel, V + 5 * node, m->Ub_in, flux);
       else if (code == 5 || code == -5) or_boundflux5(gam, n, nvel, V + 5 * node, m->Ub_out, flux);
       else continue;
@@ -1072,7 +1076,14 @@ void or_timestep(const or_mesh *m, const or_params *p, const double *V, double c
       idti[node] += (ci < 0.0 ? ci : 0.0) * S / 3;
     }
   }
-  for (int i = 0; i < N; ++i) dt[i] = cfl * m->cv[i] / (-1.0 * idti[i] + 0.0 * 0.0);
+  if (p->viscous) {
+    double *idtv = (double *)calloc((size_t)N, sizeof(double));
+    or_viscous_idtv(m, p, V, idtv);
+    for (int i = 0; i < N; ++i) dt[i] = cfl * m->cv[i] / (-1.0 * idti[i] + p->viscous_cst * idtv[i]);
+    free(idtv);
+  } else {
+    for (int i = 0; i < N; ++i) dt[i] = cfl * m->cv[i] / (-1.0 * idti[i] + 0.0 * 0.0);
+  }
   free(idti);
 }
 
@@ -1086,6 +1097,8 @@ void or_jacobian_h1(const or_mesh *m, const or_params *p, const double *V, const
   double gam = p->gamma, pstiff = p->pstiff;
   for (size_t k = 0; k < 25 * (size_t)N; ++k) Adiag[k] = 0.0;
   for (size_t k = 0; k < 50 * (size_t)E; ++k) Aedge[k] = 0.0;
+  /* the Galerkin Jacobian comes before the finite-volume one (SpaceOperator.C:2131-2136) */
+  if (p->viscous) or_galerkin_jacobian(m, p, V, Adiag, Aedge);
   double dfdUi[25], dfdUj[25];
   for (int l = 0; l < E; ++l) {
     int i = m->ptr[2 * l], j = m->ptr[2 * l + 1];
@@ -1117,7 +1130,8 @@ void or_jacobian_h1(const or_mesh *m, const or_params *p, const double *V, const
     for (int a = 0; a < 3; ++a) {
       int node = m->faces[3 * f + a];
       double jac[25];
-      if (code == 2 || code == -2) or_wall_jac(gam, n, nvel, V + 5 * node, jac);
+      if (code == 1 || code == -1 || code == 2 || code == -2 || code == 3 || code == -3)
+        or_wall_jac(gam, n, nvel, V + 5 * node, jac);
       else if (code == 4 || code == -4) or_farfield_jac(gam, n, nvel, V + 5 * node, m->Ub_in, jac);
       else if (code == 5 || code == -5) or_farfield_jac(gam, n, nvel, V + 5 * node, m->Ub_out, jac);
       else continue;
@@ -1276,6 +1290,17 @@ typedef struct {
 static int fd_function(const or_mesh *m, const or_params *p, const double *Q, const double *U,
                        const double *dt, double *F);
 
+/* SpaceOperator::applyBCsToResidual (SpaceOperator.C:2389-2395): the
+ * strong wall conditions of the Navier-Stokes equations; nothing for Euler
+ * (no BcFcn, SpaceOperator.C:246-265). */
+static void residual_bcs(const or_mesh *m, const or_params *p, const double *U, double *F) {
+  if (!p->viscous) return;
+  int *type = (int *)malloc(sizeof(int) * (size_t)(m->N ? m->N : 1));
+  or_node_types(m, type);
+  or_apply_bcs_residual(m, p, type, U, F);
+  free(type);
+}
+
 /* MatVecProdFD::apply with computeEpsilon (MatVecProd.C:262-372, 879-929). */
 static void ops_fd_product(or_ops *o, const double *pv, double *y) {
   size_t n = 5 * (size_t)o->m->N;
@@ -1420,6 +1445,13 @@ int or_gmres_mvp(const or_mesh *m, const or_params *p, const double *V, const do
   double *Ae = (double *)malloc(sizeof(double) * 50 * (size_t)E);
   double *F0 = NULL;
   or_jacobian_h1(m, p, V, dt, Ad, Ae);
+  /* MatVecProdH1::evaluate ends with applyBCsToJacobian (MatVecProd.C:1080-1092) */
+  if (p->viscous && U) {
+    int *type = (int *)malloc(sizeof(int) * (size_t)N);
+    or_node_types(m, type);
+    or_apply_bcs_jacobian(m, p, type, U, Ad, Ae);
+    free(type);
+  }
   or_ops o;
   memset(&o, 0, sizeof(o));
   o.m = m;
@@ -1428,6 +1460,7 @@ int or_gmres_mvp(const or_mesh *m, const or_params *p, const double *V, const do
   if (mvp == 1) {
     F0 = (double *)malloc(sizeof(double) * n);
     o.ierr += or_residual(m, p, U, F0, 0, NULL, NULL, NULL, NULL);
+    residual_bcs(m, p, U, F0);
     o.p = p;
     o.U = U;
     o.F0 = F0;
@@ -1519,6 +1552,7 @@ int or_steady_solve_ew(const or_mesh *m, const or_params *p, double *U, int prec
   for (step = 0; step <= max_steps; ++step) {
     int ierr = or_residual(m, p, U, R, 0, V, 0, 0, 0);
     if (ierr) break;
+    residual_bcs(m, p, U, R);
     double res = sqrt(vdot(n, R, R));
     if (step == 0) res0 = res;
     if (history) history[step] = res;
@@ -1573,6 +1607,7 @@ static int fd_function(const or_mesh *m, const or_params *p, const double *Q, co
       double invDt = 1.0 / dt[i];
       for (int k = 0; k < 5; ++k) F[5 * i + k] += invDt * (1.0 * Q[5 * i + k] + -1.0 * U[5 * i + k]);
     }
+  residual_bcs(m, p, Q, F);
   return ierr;
 }
 
@@ -1743,7 +1778,7 @@ int or_residual_par(const or_mesh *m, const or_params *p, const double *U, doubl
         int node = m->faces[3 * f + c];
         if (node < a || node >= b) continue;
         double flux[5];
-        if (code == 2 || code == -2) or_wallflux(nv, 0.0, V + 5 * (size_t)node, flux);
+        if (code == 1 || code == -1 || code == 2 || code == -2 || code == 3 || code == -3) or_wallflux(nv, 0.0, V + 5 * (size_t)node, flux);
         else if (code == 4 || code == -4) or_boundflux5(gam, nv, 0.0, V + 5 * (size_t)node, m->Ub_in, flux);
         else if (code == 5 || code == -5) or_boundflux5(gam, nv, 0.0, V + 5 * (size_t)node, m->Ub_out, flux);
         else continue;

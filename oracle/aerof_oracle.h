@@ -104,6 +104,13 @@ typedef struct {
    * 1 = Roe-Turkel flux ROEFLUX5 and its ROEJAC6 Jacobian */
   int low_mach;
   double lm_min_mach, lm_slope, lm_cutoff, lm_shock, lm_irey;
+  /* laminar Navier-Stokes (Equations.Type = NavierStokes, no turbulence
+   * model): viscosity model 0 constant, 1 Sutherland, 2 Prandtl
+   * (ViscoFcn.h); conductivity model 0 constant Prandtl, 1 constant
+   * (ThermalCondFcn.h); wall temperature of isothermal walls; weight of the
+   * viscous time-step term (Time.ViscosityParameter) */
+  int viscous, visc_model, cond_model;
+  double reynolds, prandtl, bulk_viscosity, sutherland_ts, conductivity, wall_temperature, viscous_cst;
 } or_params;
 
 typedef struct {
@@ -206,6 +213,31 @@ int or_steady_solve_mvp(const or_mesh *m, const or_params *p, double *U, int pre
  * node with a zero pivot. */
 int or_ras_ilu0_apply(int N, int E, const int *ptr, const double *D, const double *A, int nsub, const int *start,
                       int single, const double *y, double *x);
+
+/* ---- laminar Navier-Stokes terms (navier_stokes.c) ----------------------*/
+
+/* P1 gradients g[4][3] of tet (node ids into X), returns the signed volume
+ * (ElemTet::computeGradientP1Function, ElemTetCore.C:1961-2012). */
+double or_p1_gradients(const double *X, const int *tet, double g[4][3]);
+/* mu, lambda, kappa at the temperature T, scaled by 1/Re. */
+void or_ns_transport(const or_params *p, double T, double *mu, double *lambda, double *kappa);
+/* r[3][5] of one element with node primitive states v[4]. */
+void or_ns_volume_term(const or_params *p, double g[4][3], const double *v[4], double r[3][5]);
+/* dRdU[k][j][m][c] = d r[j][m] / d U_k[c] (conservative U). */
+void or_ns_volume_jacobian(const or_params *p, double g[4][3], const double *v[4], double dRdU[4][3][5][5]);
+/* R += the Galerkin term, element order (Rabs: sums of |contribution|). */
+void or_galerkin_term(const or_mesh *m, const or_params *p, const double *V, double *R, double *Rabs);
+/* Adiag += unscaled diagonal blocks, Aedge += off-diagonal blocks / V_row. */
+void or_galerkin_jacobian(const or_mesh *m, const or_params *p, const double *V, double *Adiag, double *Aedge);
+/* idtv += the elemental viscous time-step term (before ViscosityParameter). */
+void or_viscous_idtv(const or_mesh *m, const or_params *p, const double *V, double *idtv);
+/* node types from the face codes (Domain::setNodeType priorities) */
+void or_node_types(const or_mesh *m, int *type);
+/* BcFcnNS strong wall conditions on the wall nodes of type[] */
+void or_apply_bcs_solution(const or_mesh *m, const or_params *p, const int *type, double *U);
+void or_apply_bcs_residual(const or_mesh *m, const or_params *p, const int *type, const double *U, double *F);
+void or_apply_bcs_jacobian(const or_mesh *m, const or_params *p, const int *type, const double *U, double *Adiag,
+                           double *Aedge);
 
 #ifdef __cplusplus
 }

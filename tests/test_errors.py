@@ -47,7 +47,7 @@ def test_null_arguments():
     (lambda m: dataclasses.replace(m, tets=np.where(m.tets == 5, -1, m.tets)), ERR_MESH, "tet node out of range"),
     (lambda m: dataclasses.replace(m, faces=np.where(m.faces == 0, m.num_nodes + 7, m.faces)), ERR_MESH,
      "face node out of range"),
-    (lambda m: dataclasses.replace(m, codes=np.where(np.arange(m.codes.size) == 3, 3, m.codes).astype(np.int32)),
+    (lambda m: dataclasses.replace(m, codes=np.where(np.arange(m.codes.size) == 3, 6, m.codes).astype(np.int32)),
      ERR_MESH, "unsupported face code"),
     (lambda m: dataclasses.replace(m, tets=m.tets[:0]), ERR_ARG, "bad mesh"),
 ])

@@ -1,0 +1,256 @@
+"""Laminar Navier-Stokes terms on the GPU against the oracle.
+
+* residual (Galerkin term + finite-volume term, 1/V) for every viscosity and
+  conductivity model, first and second order: 1e-12 of the summed
+  contribution magnitudes per node (the sums run in another order);
+* H1 with the Galerkin blocks and the wall conditions: 1e-12 relative per
+  block (FP64), 1e-6 (FP32);
+* node types and the wall rows of F: bit-exact;
+* the time step with the viscous term: 1e-13;
+* Newton on a no-slip channel: the residual history of the oracle's Newton
+  (1e-6); H1 Newton (CFL up to 100) and matrix-free Newton converge, with
+  zero wall momentum;
+* at 128^3 (2.1M nodes): the Galerkin term sums to zero over the mesh and
+  vanishes at interior nodes for a linear velocity field;
+* the derivative passes refuse the viscous terms.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from aerof_dev_amd import FlowParams, ViscousParams, init_state
+from aerof_dev_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+GAM = 1.4
+
+
+def ns_params(mach=0.3, re=100.0, second_order=True, **kw):
+    return FlowParams(mach=mach, p_inf=1.0 / (GAM * mach * mach), second_order=second_order,
+                      viscous=ViscousParams(reynolds=re, **kw))
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _box(n=8, jitter=0.15):
+    return M.kuhn_box(n, n, n, jitter=jitter, seed=3, face_codes=M.VISCOUS_CHANNEL)
+
+
+@pytest.mark.parametrize("visc,cond,second", [("sutherland", "prandtl", True), ("constant", "prandtl", True),
+                                              ("prandtl", "constant", True), ("sutherland", "prandtl", False)])
+def test_viscous_residual_matches_oracle(gpu, visc, cond, second):
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _box(10)
+    p = ns_params(0.4, 50.0, second_order=second, viscosity=visc, conductivity_model=cond, bulk_viscosity=0.02,
+                  conductivity=0.3)
+    U = init_state(p, m.X, "noise", 0.02, seed=5)
+    print(f"[viscous] residual {visc}/{cond} second={second} N={m.num_nodes}", flush=True)
+    ref = O.OracleMesh(m, p).residual(U, want_abs=True)
+    assert ref["ierr"] == 0
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U).to(gpu)
+    Rd = torch.empty_like(Ud)
+    assert op.compute_residual(Ud, Rd) == 0
+    torch.cuda.synchronize()
+    Rg = Rd.cpu().numpy()
+    per = np.max(np.abs(Rg - ref["R"]) / np.maximum(ref["Rabs"], 1e-300))
+    nw = np.linalg.norm(Rg - ref["R"]) / np.linalg.norm(ref["R"])
+    print(f"[viscous] per-node {per:.2e}, normwise {nw:.2e}", flush=True)
+    assert per <= 1e-12 and nw <= 1e-12
+    # the Galerkin part is there: the Euler residual differs
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf, second_order=second)
+    Re = O.OracleMesh(m, pe).residual(U)["R"]
+    assert np.linalg.norm(Rg - Re) > 1e-3 * np.linalg.norm(Re)
+    op.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_viscous_jacobian_with_wall_conditions(gpu, precision):
+    import torch
+    from aerof_dev_amd.operators import MatVecProdH1, SpaceOperator
+    m = _box(8)
+    p = ns_params(0.3, 40.0)
+    U = init_state(p, m.X, "noise", 0.01, seed=9)
+    om = O.OracleMesh(m, p)
+    V = om.residual(U)["V"]
+    dt = om.timestep(V, 30.0)
+    Dr, Ar = om.jacobian(V, dt)
+    Dr, Ar = om.apply_bcs_jacobian(U, Dr, Ar)
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U).to(gpu)
+    h1 = MatVecProdH1(op)
+    h1.evaluate(Ud, torch.from_numpy(dt).to(gpu), precision)
+    D, A = h1.blocks()
+    tol = 1e-12 if precision == 0 else 1e-6
+    scale_d = np.abs(Dr).max(axis=(1, 2))[:, None, None]
+    scale_a = np.abs(Ar).max(axis=(2, 3))[:, :, None, None] + 1e-300
+    ed, ea = np.max(np.abs(D - Dr) / scale_d), np.max(np.abs(A - Ar) / scale_a)
+    print(f"[viscous] H1 precision {precision}: diag {ed:.2e}, off-diag {ea:.2e}", flush=True)
+    assert ed <= tol and ea <= tol
+    # the wall rows: identity on the momentum rows of the diagonal, zero off it
+    wall = om.node_types() == 3
+    assert wall.sum() > 0
+    assert np.array_equal(D[wall][:, 1:4], np.broadcast_to(np.eye(5)[1:4], (wall.sum(), 3, 5)))
+    op.close()
+
+
+def test_node_types_and_residual_wall_rows(gpu):
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    codes = list(M.VISCOUS_CHANNEL)
+    codes[4] = 1  # isothermal bottom wall
+    m = M.kuhn_box(6, 6, 6, jitter=0.1, seed=2, face_codes=tuple(codes))
+    p = ns_params(0.3, 40.0, wall_temperature=9.0)
+    om = O.OracleMesh(m, p)
+    op = SpaceOperator(m, p)
+    assert np.array_equal(op.node_types(), om.node_types())
+    rng = np.random.default_rng(3)
+    U = init_state(p, m.X, "noise", 0.05, seed=4)
+    F = rng.standard_normal(U.shape)
+    ref = om.apply_bcs_residual(U, F)
+    Ud, Fd = torch.from_numpy(U).to(gpu), torch.from_numpy(F).to(gpu)
+    op.apply_bcs_residual(Ud, Fd)
+    assert np.array_equal(Fd.cpu().numpy(), ref)
+    op.apply_bcs_solution(Ud)
+    assert np.array_equal(Ud.cpu().numpy(), om.apply_bcs_solution(U))
+    op.close()
+
+
+def test_viscous_timestep_term(gpu):
+    import torch
+    from aerof_dev_amd.operators import MatVecProdH1, SpaceOperator
+    m = _box(8)
+    p = ns_params(0.3, 20.0, viscous_cst=1.0)
+    U = init_state(p, m.X, "noise", 0.01, seed=1)
+    om = O.OracleMesh(m, p)
+    ref = om.timestep(om.residual(U)["V"], 25.0)
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf)
+    inviscid = O.OracleMesh(m, pe).timestep(om.residual(U)["V"], 25.0)
+    assert np.all(ref < inviscid)
+    op = SpaceOperator(m, p)
+    dt = torch.empty(op.num_nodes, dtype=torch.float64, device=gpu)
+    MatVecProdH1(op).timestep(torch.from_numpy(U).to(gpu), 25.0, dt)
+    torch.cuda.synchronize()
+    assert _rel(dt.cpu().numpy(), ref) <= 1e-13
+    op.close()
+
+
+def _channel(nx=24, n=8):
+    m = M.channel_mesh(nx, n, n, face_codes=M.VISCOUS_CHANNEL)
+    return M.renumber(m, M.rcm(m))
+
+
+def test_viscous_newton_tracks_oracle(gpu):
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator, steady_solve
+    m = _channel()
+    p = ns_params(0.3, 100.0)
+    U = init_state(p, m.X, "freestream")
+    print(f"[viscous] Newton channel N={m.num_nodes}", flush=True)
+    ref = O.OracleMesh(m, p).steady_solve(U, precision=0, max_steps=20)
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U).to(gpu)
+    out = steady_solve(op, Ud, precision=0, max_steps=20)
+    h, hr = out["history"], ref["history"]
+    n = min(len(h), len(hr))
+    dev = np.max(np.abs(h[:n] - hr[:n]) / hr[:n])
+    print(f"[viscous] history {hr[0]:.3e} -> {hr[n - 1]:.3e}, deviation {dev:.2e}", flush=True)
+    assert n >= 15 and dev <= 1e-6
+    op.close()
+
+
+@pytest.mark.parametrize("mvp,second,cfl_max", [("h1", False, 100.0), ("h1", True, 100.0), ("fd", True, 1e4)])
+def test_viscous_newton_converges(gpu, mvp, second, cfl_max):
+    """H1 Newton with the CFL bounded at 100, matrix-free Newton at 1e4.
+    (With H1 and CFL 1e3-1e4 the pseudo-transient iteration on this channel
+    settles on a plateau, 1.7e-3 first order, 2e-6 to 2e-4 second order, in
+    the oracle's restatement as well: H1's first-order convective blocks are
+    too far from the Jacobian once the time term is gone.)"""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator, steady_solve
+    m = _channel()
+    p = ns_params(0.3, 100.0, second_order=second)
+    U = init_state(p, m.X, "freestream")
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U).to(gpu)
+    out = steady_solve(op, Ud, precision=1, max_steps=300, eps=1e-8, mvp=mvp, cfl_max=cfl_max)
+    print(f"[viscous] Newton {mvp}: {out['steps']} steps, {out['krylov_its']} its, converged {out['converged']}",
+          flush=True)
+    assert out["converged"]
+    Us = Ud.cpu().numpy()
+    wall = op.node_types() == 3
+    # no-slip: the wall momentum is driven to zero with the residual
+    assert np.abs(Us[wall, 1:4]).max() <= 1e-6
+    # the converged state satisfies the oracle's residual with the wall rows
+    om = O.OracleMesh(m, p)
+    R = om.apply_bcs_residual(Us, om.residual(Us)["R"])
+    R0 = om.apply_bcs_residual(U, om.residual(U)["R"])
+    assert np.linalg.norm(R) <= 2e-8 * np.linalg.norm(R0)
+    op.close()
+
+
+def test_viscous_full_size_properties(gpu):
+    """128^3 Kuhn box (2.1M nodes, 12.6M elements): the Galerkin term (the
+    viscous residual minus the Euler one, times V) sums to zero, and a linear
+    velocity field at constant temperature gives zero interior momentum."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    n = 128
+    m = M.kuhn_box(n, n, n, jitter=0.1, seed=7, face_codes=M.VISCOUS_CHANNEL)
+    p = ns_params(0.3, 100.0, viscosity="constant")
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf)
+    print(f"[viscous] full size N={m.num_nodes}", flush=True)
+    opv, ope = SpaceOperator(m, p), SpaceOperator(m, pe)
+    cv = torch.from_numpy(opv.tables()["ctrl_vol"]).to(gpu)
+    Tinf = 1.0 / (GAM * (GAM - 1.0) * 0.09)
+    X = torch.from_numpy(m.X).to(gpu)
+    rng = np.random.default_rng(1)
+    for kind in ("smooth", "linear"):
+        B = torch.from_numpy(rng.standard_normal((3, 3))).to(gpu)
+        if kind == "smooth":
+            u = torch.stack([torch.sin(3 * X[:, 1]), X[:, 0] * X[:, 2], torch.cos(2 * X[:, 0])], dim=1)
+            T = Tinf * (1.0 + 0.1 * X[:, 2])
+        else:
+            u = X @ B.T
+            T = torch.full_like(X[:, 0], Tinf)
+        rho = torch.ones_like(T)
+        pr = (GAM - 1.0) * rho * T
+        U = torch.empty((m.num_nodes, 5), dtype=torch.float64, device=gpu)
+        U[:, 0] = rho
+        U[:, 1:4] = rho[:, None] * u
+        U[:, 4] = pr / (GAM - 1.0) + 0.5 * rho * (u * u).sum(dim=1)
+        Rv, Re = torch.empty_like(U), torch.empty_like(U)
+        assert opv.compute_residual(U, Rv) == 0 and ope.compute_residual(U, Re) == 0
+        G = (Rv - Re) * cv[:, None]
+        if kind == "smooth":
+            s, a = G.sum(dim=0).abs().max().item(), G.abs().sum().item()
+            print(f"[viscous] sum of the Galerkin term {s:.2e} (sum |.| {a:.2e})", flush=True)
+            assert s <= 1e-12 * a
+        else:
+            Xc = m.X
+            interior = torch.from_numpy(np.all((Xc > 1e-9) & (Xc < 1 - 1e-9), axis=1)).to(gpu)
+            gi = G[interior][:, 1:4].abs().max().item()
+            # scale: one element's stress contribution, mu |B| V_i
+            scale = (1.0 / 100.0) * B.abs().max().item() * cv.max().item()
+            print(f"[viscous] linear field: interior momentum {gi:.2e} (scale {scale:.2e})", flush=True)
+            assert gi <= 1e-10 * scale
+    opv.close()
+    ope.close()
+
+
+def test_viscous_refusals(gpu):
+    import torch
+    from aerof_dev_amd import AerofxError, LowMachPrec
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _box(4)
+    p = ns_params(0.3, 40.0)
+    op = SpaceOperator(m, p)
+    U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
+    with pytest.raises(AerofxError):
+        op.compute_derivative_of_residual(U, torch.empty_like(U), dU=torch.zeros_like(U))
+    with pytest.raises(AerofxError):
+        op.set_low_mach(LowMachPrec())
+    op.close()
