@@ -170,6 +170,8 @@ struct aerofx_ctx {
   int nwall = 0, wall_cap = 0;
   int2 *d_wall = nullptr;
   double *d_idtv = nullptr;
+  double *d_visc_c = nullptr;  // [4T][5] element contributions of the Galerkin term, in node-slot order
+  int *d_visc_pos = nullptr;   // [4T] node-slot position of element slot 4 t + j
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
   // with the reason in noalloc_err)

@@ -86,6 +86,60 @@ __global__ __launch_bounds__(kBlock) void k_visc_residual(int N, const int *__re
   for (int k = 0; k < 5; ++k) R[5 * (size_t)n + k] += scale ? acc[k] * ic : acc[k];
 }
 
+// Element pass of the residual: each element once, its four vertex
+// contributions vol (r . g_j) to C[pos[4 t + j]][5], pos the place of slot
+// (t, j) in the node->element slot list, so that the node pass below reads
+// each node's contributions as one contiguous run (in element order: the
+// sums of k_visc_residual, term for term) and consecutive nodes read
+// consecutive runs.
+__global__ void k_slot_positions(int n, const int *__restrict__ ntslot, int *__restrict__ pos) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) pos[ntslot[s]] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_visc_elements(int T, const int4 *__restrict__ tets,
+                                                          const double *__restrict__ X4, const double *__restrict__ V,
+                                                          NsConsts ns, const int4 *__restrict__ pos,
+                                                          double *__restrict__ C, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= T) return;
+  double x[4][3], v[4][5], g[4][3];
+  load_element(tets, X4, V, t, x, v);
+  const double vol = p1_gradients(x, g);
+  NsElem e;
+  ns_element(ns, g, v, e);
+  const int4 p4 = pos[t];
+  const int pj[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double c[5];
+    ns_node_residual(e, g[j], vol, c);
+    double *out = C + 5 * (size_t)pj[j];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out[k] = c[k];
+  }
+}
+
+// Node pass: R[n] += sum over the node's slots (element order) (/ V_n)
+__global__ __launch_bounds__(kBlock) void k_visc_gather(int N, const int *__restrict__ ntrow,
+                                                        const double *__restrict__ C,
+                                                        const double *__restrict__ invcv, int scale,
+                                                        double *__restrict__ R, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int s = ntrow[n]; s < ntrow[n + 1]; ++s) {
+    const double *c = C + 5 * (size_t)s;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) acc[k] += c[k];
+  }
+  const double ic = invcv[n];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) R[5 * (size_t)n + k] += scale ? acc[k] * ic : acc[k];
+}
+
 // H1 entries: index q < 2E is adjacency entry q = (n, m), summed over the
 // elements of edge (n, m) and added (times 1/V_n per contribution) to its
 // off-diagonal block; q = 2E + n is the diagonal block of n, summed over the
@@ -278,9 +332,16 @@ int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, 
   if (!c->viscous) return 0;
   const ElemTables *el = c->elem;
   ProfScope ps(c, AEROFX_TIMER_FE_TERM);
-  hipLaunchKernelGGL(k_visc_residual, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
-                     el->ntrow, el->ntslot, (const int4 *)el->tets, c->tab.X4, V, c->tab.invcv, c->ns, scale ? 1 : 0,
-                     R, done);
+  if (c->d_visc_c) {
+    hipLaunchKernelGGL(k_visc_elements, dim3((c->T + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->T,
+                       (const int4 *)el->tets, c->tab.X4, V, c->ns, (const int4 *)c->d_visc_pos, c->d_visc_c, done);
+    hipLaunchKernelGGL(k_visc_gather, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
+                       el->ntrow, c->d_visc_c, c->tab.invcv, scale ? 1 : 0, R, done);
+  } else {
+    hipLaunchKernelGGL(k_visc_residual, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
+                       el->ntrow, el->ntslot, (const int4 *)el->tets, c->tab.X4, V, c->tab.invcv, c->ns,
+                       scale ? 1 : 0, R, done);
+  }
   return check("viscous residual launch");
 }
 
@@ -364,6 +425,26 @@ int aerofx_ctx_set_viscous(aerofx_ctx *c, const aerofx_viscous *v) {
     return set_error(AEROFX_ERR_ARG, "set_viscous: Sutherland temperature ratio must be > 0");
   int rc = ensure_elem_tables(c);
   if (rc) return rc;
+  // the element contributions of the residual [4T][5] (allocated here, at
+  // set-up, so that no residual call allocates); without the memory the
+  // per-node kernel evaluates each element once per vertex instead
+  if (!c->d_visc_c) {
+    void *p = nullptr, *q = nullptr;
+    if (hipMalloc(&p, sizeof(double) * 20 * (size_t)c->T) == hipSuccess &&
+        hipMalloc(&q, sizeof(int) * 4 * (size_t)c->T) == hipSuccess) {
+      c->allocs.push_back(p);
+      c->allocs.push_back(q);
+      c->d_visc_c = (double *)p;
+      c->d_visc_pos = (int *)q;
+      const int ns4 = 4 * c->T;
+      hipLaunchKernelGGL(k_slot_positions, dim3((ns4 + 255) / 256), dim3(256), 0, c->stream, ns4, c->elem->ntslot,
+                         c->d_visc_pos);
+      if ((rc = check("slot positions launch"))) return rc;
+    } else {
+      (void)hipGetLastError();
+      if (p) (void)hipFree(p);
+    }
+  }
   NsConsts ns{};
   ns.visc = v->viscosity_model;
   ns.cond = v->conductivity_model;
