@@ -984,6 +984,8 @@ int ensure_elem_tables(aerofx_ctx *c) {
 }
 
 int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout) {
+  if (c->viscous && Xout)
+    return set_error(AEROFX_ERR_STATE, "adjoint: the shape derivative of the Navier-Stokes terms is not built");
   int rc = ensure_adjoint(c);
   if (rc) return rc;
   Adjoint *a = c->adj;
@@ -1010,6 +1012,8 @@ int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub
   // 4.
   hipLaunchKernelGGL(k_adj_lsq, grid_for(t.N), dim3(kAdjBlock), 0, c->stream, t, c->d_V, U, c->gas, a->Vb, a->Gb,
                      second, want_x, Ub, a->Xb, a->S);
+  // the Navier-Stokes Galerkin term (its element derivative, transposed)
+  if (c->viscous && Ub && (rc = launch_visc_adjoint(c, lam, Ub))) return rc;
   if (want_x) {
     // 5.
     if (c->T > 0)
@@ -1145,7 +1149,8 @@ extern "C" int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const d
   if (ctx->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_residual: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(ctx, "adjoint_residual")) return rf;
-  if (int rv = refuse_viscous(ctx, "adjoint_residual")) return rv;
+  if (dRdX_T_lambda)
+    if (int rv = refuse_viscous(ctx, "adjoint_residual: shape derivative")) return rv;
   if (!dRdU_T_lambda && !dRdX_T_lambda) return AEROFX_OK;
   int rc = launch_adjoint(ctx, U, lambda, dRdU_T_lambda, dRdX_T_lambda);
   if (rc) return rc;

@@ -171,6 +171,7 @@ struct aerofx_ctx {
   int2 *d_wall = nullptr;
   double *d_idtv = nullptr;
   double *d_visc_c = nullptr;  // [4T][5] element contributions of the Galerkin term, in node-slot order
+  double *d_adj_tmp = nullptr;  // [N][5] the adjoint with the wall rows masked
   int *d_visc_pos = nullptr;   // [4T] node-slot position of element slot 4 t + j
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
@@ -247,6 +248,11 @@ int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, 
 int launch_visc_jacobian(aerofx_ctx *c, const double *U, int precision);
 int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *done);
 int launch_visc_idtv(aerofx_ctx *c, double **idtv);
+// Ub += (d Galerkin / dU)^T lam (the residual's 1/V included), at c->d_V
+int launch_visc_adjoint(aerofx_ctx *c, const double *lam, double *Ub);
+// the wall rows of the transposed Jacobian A^T psi = (dR/dU)^T P psi + B^T psi:
+// pre: Ppsi = P psi; post: y += B^T psi
+int launch_bcs_adjoint(aerofx_ctx *c, const double *U, const double *psi, double *Ppsi, double *y, bool pre);
 // AEROFX_ERR_STATE when the Navier-Stokes terms are on
 int refuse_viscous(aerofx_ctx *c, const char *who);
 

@@ -148,67 +148,91 @@ __host__ __device__ inline void ns_node_residual(const NsElem &e, const double g
   for (int k = 0; k < 5; ++k) out[k] = vol * ((r[0][k] * gj[0] + r[1][k] * gj[1] + r[2][k] * gj[2]) - 0.25 * 0.0);
 }
 
-// out[5 m + c] = sum_j dRdU[k][j][m][c] gi[j] for the vertex k with
-// primitive state V (the other vertices enter through e).
+// Column cc of d r / d U_k for the vertex k with primitive state V (the
+// other vertices enter through e): col[j][m] = d r[j][m] / d U_k[cc].
+__host__ __device__ inline void ns_node_column(const NsConsts &c, const NsElem &e, const double gk[3],
+                                               const double V[5], const double Tk, int cc, double col[3][5]) {
+  const double rho = V[0];
+  const double usq = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
+  double dTcg, dTk;
+  if (cc == 0) {
+    dTcg = -0.25 / rho * (Tk - 0.5 * usq);
+    dTk = -1.0 / rho * (Tk - 0.5 * usq);
+  } else if (cc < 4) {
+    dTcg = -0.25 / rho * V[cc];
+    dTk = -1.0 / rho * V[cc];
+  } else {
+    dTcg = 0.25 / rho;
+    dTk = 1.0 / rho;
+  }
+  const double div = e.dudxj[0][0] + e.dudxj[1][1] + e.dudxj[2][2];
+  const double dkap = c.ooRe * ns_dkappa(c, e.Tcg, dTcg);
+  double dmu = ns_dmu(c, e.Tcg, dTcg);
+  const double dlam = c.ooRe * (-(2.0 / 3.0) * dmu);
+  dmu *= c.ooRe;
+  // d u_k[i] / d U_k[cc] and d u_cg[i] / d U_k[cc]
+  double du[3], ducg[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    du[i] = cc == 0 ? -V[i + 1] / rho : (cc == i + 1 ? 1.0 / rho : 0.0);
+    ducg[i] = cc == 0 ? -0.25 * V[i + 1] / rho : (cc == i + 1 ? 0.25 / rho : 0.0);
+  }
+  double dg[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dg[i][j] = gk[j] * du[i];
+  const double ddiv = dg[0][0] + dg[1][1] + dg[2][2];
+  double dt[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dt[i][i] = 2.0 * dmu * e.dudxj[i][i] + 2.0 * e.mu * dg[i][i] + dlam * div + e.lambda * ddiv;
+  dt[0][1] = e.mu * (dg[1][0] + dg[0][1]) + dmu * (e.dudxj[1][0] + e.dudxj[0][1]);
+  dt[0][2] = e.mu * (dg[2][0] + dg[0][2]) + dmu * (e.dudxj[2][0] + e.dudxj[0][2]);
+  dt[1][2] = e.mu * (dg[2][1] + dg[1][2]) + dmu * (e.dudxj[2][1] + e.dudxj[1][2]);
+  dt[1][0] = dt[0][1];
+  dt[2][0] = dt[0][2];
+  dt[2][1] = dt[1][2];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double dq = -dkap * e.dTdxj[j] - e.kappa * (gk[j] * dTk);
+    col[j][0] = 0.0;
+    col[j][1] = dt[0][j];
+    col[j][2] = dt[1][j];
+    col[j][3] = dt[2][j];
+    col[j][4] = ducg[0] * e.tij[0][j] + e.ucg[0] * dt[0][j] + ducg[1] * e.tij[1][j] + e.ucg[1] * dt[1][j] +
+                ducg[2] * e.tij[2][j] + e.ucg[2] * dt[2][j] - dq;
+  }
+}
+
+// out[5 m + c] = sum_j dRdU[k][j][m][c] gi[j] for the vertex k
 __host__ __device__ inline void ns_node_jacobian(const NsConsts &c, const NsElem &e, const double gk[3],
                                                  const double V[5], const double Tk, const double gi[3],
                                                  double out[25]) {
-  const double rho = V[0];
-  const double usq = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
-  double dTcg[5], dTk[5];
-  dTcg[0] = -0.25 / rho * (Tk - 0.5 * usq);
-  dTcg[1] = -0.25 / rho * V[1];
-  dTcg[2] = -0.25 / rho * V[2];
-  dTcg[3] = -0.25 / rho * V[3];
-  dTcg[4] = 0.25 / rho;
-  dTk[0] = -1.0 / rho * (Tk - 0.5 * usq);
-  dTk[1] = -1.0 / rho * V[1];
-  dTk[2] = -1.0 / rho * V[2];
-  dTk[3] = -1.0 / rho * V[3];
-  dTk[4] = 1.0 / rho;
-  const double div = e.dudxj[0][0] + e.dudxj[1][1] + e.dudxj[2][2];
 #pragma unroll
   for (int cc = 0; cc < 5; ++cc) {
-    const double dkap = c.ooRe * ns_dkappa(c, e.Tcg, dTcg[cc]);
-    double dmu = ns_dmu(c, e.Tcg, dTcg[cc]);
-    const double dlam = c.ooRe * (-(2.0 / 3.0) * dmu);
-    dmu *= c.ooRe;
-    // d u_k[i] / d U_k[cc] and d u_cg[i] / d U_k[cc]
-    double du[3], ducg[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      du[i] = cc == 0 ? -V[i + 1] / rho : (cc == i + 1 ? 1.0 / rho : 0.0);
-      ducg[i] = cc == 0 ? -0.25 * V[i + 1] / rho : (cc == i + 1 ? 0.25 / rho : 0.0);
-    }
-    double dg[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) dg[i][j] = gk[j] * du[i];
-    const double ddiv = dg[0][0] + dg[1][1] + dg[2][2];
-    double dt[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      dt[i][i] = 2.0 * dmu * e.dudxj[i][i] + 2.0 * e.mu * dg[i][i] + dlam * div + e.lambda * ddiv;
-    dt[0][1] = e.mu * (dg[1][0] + dg[0][1]) + dmu * (e.dudxj[1][0] + e.dudxj[0][1]);
-    dt[0][2] = e.mu * (dg[2][0] + dg[0][2]) + dmu * (e.dudxj[2][0] + e.dudxj[0][2]);
-    dt[1][2] = e.mu * (dg[2][1] + dg[1][2]) + dmu * (e.dudxj[2][1] + e.dudxj[1][2]);
-    dt[1][0] = dt[0][1];
-    dt[2][0] = dt[0][2];
-    dt[2][1] = dt[1][2];
     double col[3][5];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double dq = -dkap * e.dTdxj[j] - e.kappa * (gk[j] * dTk[cc]);
-      col[j][0] = 0.0;
-      col[j][1] = dt[0][j];
-      col[j][2] = dt[1][j];
-      col[j][3] = dt[2][j];
-      col[j][4] = ducg[0] * e.tij[0][j] + e.ucg[0] * dt[0][j] + ducg[1] * e.tij[1][j] + e.ucg[1] * dt[1][j] +
-                  ducg[2] * e.tij[2][j] + e.ucg[2] * dt[2][j] - dq;
-    }
+    ns_node_column(c, e, gk, V, Tk, cc, col);
 #pragma unroll
     for (int m = 0; m < 5; ++m) out[5 * m + cc] = col[0][m] * gi[0] + col[1][m] * gi[1] + col[2][m] * gi[2];
+  }
+}
+
+// out[c] = sum_j sum_m dRdU[k][j][m][c] w[j][m]: the transposed element
+// derivative applied to w[j][m] = sum_i g_i[j] lambda_i[m]
+__host__ __device__ inline void ns_node_adjoint(const NsConsts &c, const NsElem &e, const double gk[3],
+                                                const double V[5], const double Tk, const double w[3][5],
+                                                double out[5]) {
+#pragma unroll
+  for (int cc = 0; cc < 5; ++cc) {
+    double col[3][5];
+    ns_node_column(c, e, gk, V, Tk, cc, col);
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) s += col[j][m] * w[j][m];
+    out[cc] = s;
   }
 }
 

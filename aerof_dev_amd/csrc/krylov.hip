@@ -1113,7 +1113,7 @@ static int derivative_operators(aerofx_ctx *c, const double *U, double cfl, int 
   int rc;
   if ((rc = launch_primitive(c, U, c->d_V))) return rc;
   if (cfl > 0.0 && (rc = launch_timestep(c, cfl, c->d_dt))) return rc;
-  if ((rc = launch_jacobian(c, cfl > 0.0 ? c->d_dt : nullptr, precision))) return rc;
+  if ((rc = launch_jacobian(c, cfl > 0.0 ? c->d_dt : nullptr, precision, U))) return rc;
   hipError_t e = hipMemcpyAsync(c->h_err + 1, c->d_err + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_hip_error(e, who);
@@ -1127,21 +1127,35 @@ int aerofx_adjoint_solve(aerofx_ctx *c, const double *U, const double *rhs, doub
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_solve: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(c, "adjoint_solve")) return rf;
-  if (int rv = refuse_viscous(c, "adjoint_solve")) return rv;
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "adjoint_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
   if (rc) return rc;
   auto t0 = std::chrono::steady_clock::now();
   *st = aerofx_adjoint_stats{};
   if ((rc = derivative_operators(c, U, o->cfl, o->precision, "adjoint_solve"))) return rc;
+  // with the Navier-Stokes wall conditions the system is A^T psi = rhs with
+  // A = P dR/dU + B (the wall rows replaced): A^T psi = (dR/dU)^T P psi + B^T psi
+  const bool walls = c->viscous && c->nwall;
+  if (walls && !c->d_adj_tmp) {
+    void *p = nullptr;
+    if ((rc = ctx_alloc(c, sizeof(double) * 5 * (size_t)c->N, &p))) return rc;
+    c->d_adj_tmp = (double *)p;
+  }
   // (dR/dU)^T psi = rhs: flexible GMRES on the exact adjoint products, each
   // step preconditioned by transposed GMRES on H1^T with block-Jacobi^T
   FgOps ops;
   ops.apply = [&](const double *x, double *y, int nq) -> int {
     for (int q = 0; q < nq; ++q) {
       const size_t off = (size_t)q * c->ksp->n;
-      int r = launch_adjoint(c, U, x + off, y + off, nullptr);
-      if (r) return r;
+      int r;
+      if (walls) {
+        if ((r = launch_bcs_adjoint(c, U, x + off, c->d_adj_tmp, nullptr, true)) ||
+            (r = launch_adjoint(c, U, c->d_adj_tmp, y + off, nullptr)) ||
+            (r = launch_bcs_adjoint(c, U, x + off, nullptr, y + off, false)))
+          return r;
+      } else if ((r = launch_adjoint(c, U, x + off, y + off, nullptr))) {
+        return r;
+      }
     }
     return 0;
   };

@@ -41,6 +41,8 @@ __device__ __forceinline__ void load_element(const int4 *__restrict__ tets, cons
   }
 }
 
+__device__ __forceinline__ bool wall_isothermal(int type) { return type == 1 || type == -1; }
+
 // row k of a [4][W] register array without dynamic indexing (which would
 // send the array to scratch)
 template <int W>
@@ -118,6 +120,85 @@ __global__ __launch_bounds__(kBlock) void k_visc_elements(int T, const int4 *__r
     double *out = C + 5 * (size_t)pj[j];
 #pragma unroll
     for (int k = 0; k < 5; ++k) out[k] = c[k];
+  }
+}
+
+// Transposed element pass: each element once, C[pos[4 t + k]] = vol (d r /
+// d U_k)^T w with w[j][m] = sum_i g_i[j] lambda_i[m] / V_i (the residual of
+// vertex i carries 1/V_i); k_visc_gather then adds the node sums to the
+// transposed product.
+__global__ __launch_bounds__(kBlock) void k_visc_adj_elements(int T, const int4 *__restrict__ tets,
+                                                              const double *__restrict__ X4,
+                                                              const double *__restrict__ V,
+                                                              const double *__restrict__ lam,
+                                                              const double *__restrict__ invcv, NsConsts ns,
+                                                              const int4 *__restrict__ pos, double *__restrict__ C) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= T) return;
+  double x[4][3], v[4][5], g[4][3];
+  load_element(tets, X4, V, t, x, v);
+  const double vol = p1_gradients(x, g);
+  NsElem e;
+  ns_element(ns, g, v, e);
+  const int4 nd = tets[t];
+  const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+  double w[3][5];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) w[j][m] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double ic = invcv[id[i]];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      const double l = lam[5 * (size_t)id[i] + m] * ic;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) w[j][m] += g[i][j] * l;
+    }
+  }
+  const int4 p4 = pos[t];
+  const int pk[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double out[5];
+    ns_node_adjoint(ns, e, g[k], v[k], e.T[k], w, out);
+    double *o = C + 5 * (size_t)pk[k];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) o[q] = vol * out[q];
+  }
+}
+
+// The wall rows in a transposed product: P zeroes the rows that the wall
+// conditions replace (momentum; energy on isothermal walls) ...
+__global__ void k_bcs_adj_mask(int nwall, const int2 *__restrict__ wall, double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwall) return;
+  const int n = wall[i].x;
+  const int r1 = wall_isothermal(wall[i].y) ? 5 : 4;
+  for (int r = 1; r < r1; ++r) y[5 * (size_t)n + r] = 0.0;
+}
+
+// ... and y += B^T psi adds the transposed rows of the conditions
+// (d F_r / d U = e_r on the momentum rows; the isothermal energy row
+// (U4 - 2 Tw U0, -U1, -U2, -U3, U0))
+__global__ void k_bcs_adj_rows(int nwall, const int2 *__restrict__ wall, double wall_T, const double *__restrict__ U,
+                               const double *__restrict__ psi, double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwall) return;
+  const int n = wall[i].x;
+  const double *p = psi + 5 * (size_t)n;
+  double *o = y + 5 * (size_t)n;
+  o[1] += p[1];
+  o[2] += p[2];
+  o[3] += p[3];
+  if (wall_isothermal(wall[i].y)) {
+    const double *u = U + 5 * (size_t)n;
+    o[0] += (u[4] - 2.0 * wall_T * u[0]) * p[4];
+    o[1] += -u[1] * p[4];
+    o[2] += -u[2] * p[4];
+    o[3] += -u[3] * p[4];
+    o[4] += u[0] * p[4];
   }
 }
 
@@ -249,7 +330,6 @@ __global__ __launch_bounds__(kBlock) void k_visc_idtv(int N, const int *__restri
   idtv[n] = acc;
 }
 
-__device__ __forceinline__ bool wall_isothermal(int type) { return type == 1 || type == -1; }
 
 // BcFcnNS::applyToResidualTerm (BcFcnCore.C:230-245), fixed walls
 __global__ void k_bcs_residual(int nwall, const int2 *__restrict__ wall, double wall_T, const double *__restrict__ U,
@@ -372,6 +452,31 @@ int launch_visc_jacobian(aerofx_ctx *c, const double *U, int precision) {
     hipLaunchKernelGGL(k_bcs_jacobian<double>, g, b, 0, c->stream, c->nwall, c->d_wall, c->tab.rowptr, c->ns.wall_T,
                        U, (double *)c->d_jacD, (double *)c->d_jacB);
   return check("wall jacobian launch");
+}
+
+int launch_visc_adjoint(aerofx_ctx *c, const double *lam, double *Ub) {
+  if (!c->viscous) return 0;
+  if (!c->d_visc_c) return set_error(AEROFX_ERR_STATE, "adjoint: the viscous contribution buffer is missing");
+  const ElemTables *el = c->elem;
+  hipLaunchKernelGGL(k_visc_adj_elements, dim3((c->T + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->T,
+                     (const int4 *)el->tets, c->tab.X4, c->d_V, lam, c->tab.invcv, c->ns,
+                     (const int4 *)c->d_visc_pos, c->d_visc_c);
+  hipLaunchKernelGGL(k_visc_gather, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
+                     el->ntrow, c->d_visc_c, c->tab.invcv, 0, Ub, nullptr);
+  return check("viscous adjoint launch");
+}
+
+int launch_bcs_adjoint(aerofx_ctx *c, const double *U, const double *psi, double *Ppsi, double *y, bool pre) {
+  if (!c->viscous || !c->nwall) return 0;
+  const dim3 g((c->nwall + 63) / 64), b(64);
+  if (pre) {
+    hipError_t e = hipMemcpyAsync(Ppsi, psi, sizeof(double) * 5 * (size_t)c->N, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "adjoint wall rows copy");
+    hipLaunchKernelGGL(k_bcs_adj_mask, g, b, 0, c->stream, c->nwall, c->d_wall, Ppsi);
+  } else {
+    hipLaunchKernelGGL(k_bcs_adj_rows, g, b, 0, c->stream, c->nwall, c->d_wall, c->ns.wall_T, U, psi, y);
+  }
+  return check("adjoint wall rows launch");
 }
 
 int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *done) {

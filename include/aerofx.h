@@ -223,9 +223,13 @@ int aerofx_ctx_set_low_mach(aerofx_ctx *ctx, const aerofx_low_mach *lm);
  * velocity, so the adiabatic-wall face term vanishes).  The transport laws
  * are ViscoFcn.h's and ThermalCondFcn.h's with the AERO-F non-dimensional
  * scaling (alpha = gamma (gamma - 1) M^2).  v = NULL turns the terms off.
- * Not with the low-Mach preconditioner; the derivative passes
- * (aerofx_residual_derivative, aerofx_adjoint_residual, AEROFX_MVP_H2,
- * aerofx_adjoint_solve, aerofx_direct_solve) return AEROFX_ERR_STATE. */
+ * The transposed product aerofx_adjoint_residual(dRdU_T_lambda) includes
+ * the Galerkin term's transposed element derivative, and
+ * aerofx_adjoint_solve solves with the wall rows (A = P dR/dU + B, A^T psi =
+ * (dR/dU)^T P psi + B^T psi).  Not with the low-Mach preconditioner; the
+ * shape adjoint (dRdX_T_lambda), the forward derivative passes
+ * (aerofx_residual_derivative, AEROFX_MVP_H2, aerofx_direct_solve) return
+ * AEROFX_ERR_STATE. */
 #define AEROFX_VISC_CONSTANT 0
 #define AEROFX_VISC_SUTHERLAND 1
 #define AEROFX_VISC_PRANDTL 2

@@ -12,7 +12,9 @@
   zero wall momentum;
 * at 128^3 (2.1M nodes): the Galerkin term sums to zero over the mesh and
   vanishes at interior nodes for a linear velocity field;
-* the derivative passes refuse the viscous terms.
+* the transposed product (dR/dU)^T lam against central differences, and the
+  adjoint solve of the system with the wall rows through <psi, A d> = <rhs, d>;
+* the forward derivative passes and the shape adjoint refuse the viscous terms.
 """
 import numpy as np
 import pytest
@@ -252,5 +254,99 @@ def test_viscous_refusals(gpu):
     with pytest.raises(AerofxError):
         op.compute_derivative_of_residual(U, torch.empty_like(U), dU=torch.zeros_like(U))
     with pytest.raises(AerofxError):
+        op.compute_transpose_derivative_of_residual(U, torch.ones_like(U), None,
+                                                    torch.empty((op.num_nodes, 3), dtype=torch.float64, device=gpu))
+    with pytest.raises(AerofxError):
         op.set_low_mach(LowMachPrec())
+    op.close()
+
+
+def _fd_dot(op, U, lam, d, h, gpu, bcs=False):
+    """sum(lam * (F(U + h d) - F(U - h d)) / (2 h)), F the residual (with the
+    wall rows when bcs)."""
+    import torch
+    out = []
+    for s in (1.0, -1.0):
+        Ud = torch.from_numpy(U + s * h * d).to(gpu)
+        Rd = torch.empty_like(Ud)
+        assert op.compute_residual(Ud, Rd) == 0
+        if bcs:
+            op.apply_bcs_residual(Ud, Rd)
+        out.append(Rd.cpu().numpy())
+    return float(np.sum(lam * (out[0] - out[1]))) / (2 * h)
+
+
+@pytest.mark.parametrize("second", [True, False])
+def test_viscous_adjoint_product(gpu, second):
+    """The Galerkin part of (dR/dU)^T lam (the Navier-Stokes product minus the
+    Euler one) against central differences (Richardson-extrapolated) of
+    lam . (R - R_euler) along random directions."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _box(6)
+    p = ns_params(0.4, 30.0, second_order=second)
+    p.alpha, p.yaw = 0.05, 0.03
+    # momentum noise and flow angles keep every edge off the kinks of Van
+    # Albada and of |u.n| (a uniform velocity field sits on them)
+    U = init_state(p, m.X, "noise", 0.01, seed=12)
+    rng = np.random.default_rng(6)
+    U[:, 1:4] += 0.02 * U[:, :1] * rng.standard_normal((U.shape[0], 3))
+    lam = rng.standard_normal(U.shape)
+    op = SpaceOperator(m, p)
+    Ud, ld = torch.from_numpy(U).to(gpu), torch.from_numpy(lam).to(gpu)
+    gu = torch.empty_like(Ud)
+    op.compute_transpose_derivative_of_residual(Ud, ld, gu, None)
+    g = gu.cpu().numpy()
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf, second_order=second, alpha=p.alpha, yaw=p.yaw)
+    ope = SpaceOperator(m, pe)
+    ge = torch.empty_like(Ud)
+    ope.compute_transpose_derivative_of_residual(Ud, ld, ge, None)
+    assert np.linalg.norm(g - ge.cpu().numpy()) > 1e-3 * np.linalg.norm(g)
+    # the Galerkin part alone: (g - g_euler) . d against differences of
+    # lam . (R - R_euler), a smooth function (no limiter, no |u.n|)
+    H = 1e-5
+    gv = g - ge.cpu().numpy()
+    for _ in range(3):
+        d = rng.standard_normal(U.shape) * (np.abs(U) + 0.1)
+        f1 = _fd_dot(op, U, lam, d, H, gpu) - _fd_dot(ope, U, lam, d, H, gpu)
+        f2 = _fd_dot(op, U, lam, d, H / 2, gpu) - _fd_dot(ope, U, lam, d, H / 2, gpu)
+        fd = (4 * f2 - f1) / 3
+        a = float(np.sum(gv * d))
+        print(f"[viscous] Galerkin adjoint . d {a:.10e} vs FD {fd:.10e}", flush=True)
+        assert abs(a - fd) <= 1e-6 * abs(fd)
+    op.close()
+    ope.close()
+
+
+def test_viscous_adjoint_solve(gpu):
+    """A^T psi = rhs with A = dF/dU of the residual with the wall rows, at the
+    converged no-slip channel: A^T psi = (dR/dU)^T P psi + B^T psi formed from
+    the transposed product (checked against differences above) and the wall
+    rows (P zeroes the wall momentum rows, B^T adds them back) equals rhs."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator, adjoint_solve, steady_solve
+    m = _channel()
+    p = ns_params(0.3, 100.0)
+    U0 = init_state(p, m.X, "freestream")
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U0).to(gpu)
+    out = steady_solve(op, Ud, precision=1, max_steps=300, eps=1e-8, mvp="fd")
+    assert out["converged"]
+    rng = np.random.default_rng(4)
+    rhs = rng.standard_normal(U0.shape)
+    psi = torch.zeros_like(Ud)
+    st = adjoint_solve(op, Ud, torch.from_numpy(rhs).to(gpu), psi, precision=1, cfl=1e3, eps=1e-10, max_its=600)
+    print(f"[viscous] adjoint solve: {st}", flush=True)
+    assert st["converged"]
+    ps = psi.cpu().numpy()
+    wall = op.node_types() == 3
+    pm = ps.copy()
+    pm[wall, 1:4] = 0.0
+    g = torch.empty_like(Ud)
+    op.compute_transpose_derivative_of_residual(Ud, torch.from_numpy(pm).to(gpu), g, None)
+    y = g.cpu().numpy()
+    y[wall, 1:4] += ps[wall, 1:4]
+    err = np.linalg.norm(y - rhs) / np.linalg.norm(rhs)
+    print(f"[viscous] ||A^T psi - rhs|| / ||rhs|| = {err:.2e}", flush=True)
+    assert err <= 1e-8
     op.close()
