@@ -248,6 +248,10 @@ int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, 
 int launch_visc_jacobian(aerofx_ctx *c, const double *U, int precision);
 int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *done);
 int launch_visc_idtv(aerofx_ctx *c, double **idtv);
+// dR += (d Galerkin / dU) dU / V at the primitive state V; the wall rows of
+// a product y = A p
+int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, double *dR, const int *done);
+int launch_bcs_product(aerofx_ctx *c, const double *U, const double *p, double *y, const int *done);
 // Ub += (d Galerkin / dU)^T lam (the residual's 1/V included), at c->d_V
 int launch_visc_adjoint(aerofx_ctx *c, const double *lam, double *Ub);
 // the wall rows of the transposed Jacobian A^T psi = (dR/dU)^T P psi + B^T psi:

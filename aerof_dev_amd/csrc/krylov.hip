@@ -1053,10 +1053,8 @@ int aerofx_mvp_select(aerofx_ctx *c, int type, const double *Q, const double *F0
   if (!Q) return set_error(AEROFX_ERR_ARG, "mvp_select: null state");
   if (type == AEROFX_MVP_H2 && c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "mvp_select: exact products not available with low-Mach preconditioning");
-  if (type == AEROFX_MVP_H2) {
+  if (type == AEROFX_MVP_H2)
     if (int rf = refuse_failsafe_nodes(c, "mvp_select")) return rf;
-    if (int rv = refuse_viscous(c, "mvp_select: exact products")) return rv;
-  }
   if (type == AEROFX_MVP_H2) {
     int rc = select_h2(c, Q, dt);
     if (rc) return rc;
@@ -1359,8 +1357,6 @@ int aerofx_steady_solve(aerofx_ctx *c, double *U, const aerofx_newton_opts *o, a
     return set_error(AEROFX_ERR_ARG, "steady_solve: mvp AEROFX_MVP_H1, AEROFX_MVP_FD or AEROFX_MVP_H2");
   if (o->mvp == AEROFX_MVP_H2 && c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "steady_solve: exact products not available with low-Mach preconditioning");
-  if (o->mvp == AEROFX_MVP_H2)
-    if (int rv = refuse_viscous(c, "steady_solve: exact products")) return rv;
   if (o->mvp == AEROFX_MVP_FD && o->fd_order != 1 && o->fd_order != 2)
     return set_error(AEROFX_ERR_ARG, "steady_solve: fd_order 1 or 2");
   if (o->prec != AEROFX_PREC_JACOBI && o->prec != AEROFX_PREC_RAS_ILU0)

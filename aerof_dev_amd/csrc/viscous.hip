@@ -169,6 +169,72 @@ __global__ __launch_bounds__(kBlock) void k_visc_adj_elements(int T, const int4 
   }
 }
 
+// Forward element pass of the state derivative: each element once,
+// dr = sum_k (d r / d U_k) dU_k and its vertex contributions vol (dr . g_j)
+// to C[pos[4 t + j]]; k_visc_gather then adds the node sums (/ V).
+__global__ __launch_bounds__(kBlock) void k_visc_tan_elements(int T, const int4 *__restrict__ tets,
+                                                              const double *__restrict__ X4,
+                                                              const double *__restrict__ V,
+                                                              const double *__restrict__ dU, NsConsts ns,
+                                                              const int4 *__restrict__ pos, double *__restrict__ C,
+                                                              const int *__restrict__ done) {
+  if (done && *done) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= T) return;
+  double x[4][3], v[4][5], g[4][3];
+  load_element(tets, X4, V, t, x, v);
+  const double vol = p1_gradients(x, g);
+  NsElem e;
+  ns_element(ns, g, v, e);
+  const int4 nd = tets[t];
+  const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+  double dr[3][5];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) dr[j][m] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+#pragma unroll
+    for (int cc = 0; cc < 5; ++cc) {
+      const double du = dU[5 * (size_t)id[k] + cc];
+      double col[3][5];
+      ns_node_column(ns, e, g[k], v[k], e.T[k], cc, col);
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dr[j][m] += col[j][m] * du;
+    }
+  }
+  const int4 p4 = pos[t];
+  const int pj[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    double *o = C + 5 * (size_t)pj[j];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) o[m] = vol * (dr[0][m] * g[j][0] + dr[1][m] * g[j][1] + dr[2][m] * g[j][2]);
+  }
+}
+
+// The wall rows of a product y = A p (H2 products): y_r = p_r on the
+// momentum rows (the isothermal energy row too)
+__global__ void k_bcs_product(int nwall, const int2 *__restrict__ wall, double wall_T, const double *__restrict__ U,
+                              const double *__restrict__ p, double *__restrict__ y, const int *__restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwall) return;
+  const int n = wall[i].x;
+  const double *q = p + 5 * (size_t)n;
+  double *o = y + 5 * (size_t)n;
+  o[1] = q[1];
+  o[2] = q[2];
+  o[3] = q[3];
+  if (wall_isothermal(wall[i].y)) {
+    const double *u = U + 5 * (size_t)n;
+    o[4] = (u[4] - 2.0 * wall_T * u[0]) * q[0] - u[1] * q[1] - u[2] * q[2] - u[3] * q[3] + u[0] * q[4];
+  }
+}
+
 // The wall rows in a transposed product: P zeroes the rows that the wall
 // conditions replace (momentum; energy on isothermal walls) ...
 __global__ void k_bcs_adj_mask(int nwall, const int2 *__restrict__ wall, double *__restrict__ y) {
@@ -464,6 +530,24 @@ int launch_visc_adjoint(aerofx_ctx *c, const double *lam, double *Ub) {
   hipLaunchKernelGGL(k_visc_gather, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
                      el->ntrow, c->d_visc_c, c->tab.invcv, 0, Ub, nullptr);
   return check("viscous adjoint launch");
+}
+
+int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, double *dR, const int *done) {
+  if (!c->viscous || !dU) return 0;
+  if (!c->d_visc_c) return set_error(AEROFX_ERR_STATE, "tangent: the viscous contribution buffer is missing");
+  const ElemTables *el = c->elem;
+  hipLaunchKernelGGL(k_visc_tan_elements, dim3((c->T + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->T,
+                     (const int4 *)el->tets, c->tab.X4, V, dU, c->ns, (const int4 *)c->d_visc_pos, c->d_visc_c, done);
+  hipLaunchKernelGGL(k_visc_gather, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
+                     el->ntrow, c->d_visc_c, c->tab.invcv, 1, dR, done);
+  return check("viscous tangent launch");
+}
+
+int launch_bcs_product(aerofx_ctx *c, const double *U, const double *p, double *y, const int *done) {
+  if (!c->viscous || !c->nwall) return 0;
+  hipLaunchKernelGGL(k_bcs_product, dim3((c->nwall + 63) / 64), dim3(64), 0, c->stream, c->nwall, c->d_wall,
+                     c->ns.wall_T, U, p, y, done);
+  return check("wall product launch");
 }
 
 int launch_bcs_adjoint(aerofx_ctx *c, const double *U, const double *psi, double *Ppsi, double *y, bool pre) {

@@ -14,7 +14,9 @@
   vanishes at interior nodes for a linear velocity field;
 * the transposed product (dR/dU)^T lam against central differences, and the
   adjoint solve of the system with the wall rows through <psi, A d> = <rhs, d>;
-* the forward derivative passes and the shape adjoint refuse the viscous terms.
+* the forward state derivative: exact duality with the transposed product
+  (1e-11), its Galerkin part against differences, H2-product Newton;
+* the shape derivatives (forward and adjoint) refuse the viscous terms.
 """
 import numpy as np
 import pytest
@@ -252,7 +254,8 @@ def test_viscous_refusals(gpu):
     op = SpaceOperator(m, p)
     U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
     with pytest.raises(AerofxError):
-        op.compute_derivative_of_residual(U, torch.empty_like(U), dU=torch.zeros_like(U))
+        op.compute_derivative_of_residual(U, torch.empty_like(U),
+                                          dX=torch.zeros((op.num_nodes, 3), dtype=torch.float64, device=gpu))
     with pytest.raises(AerofxError):
         op.compute_transpose_derivative_of_residual(U, torch.ones_like(U), None,
                                                     torch.empty((op.num_nodes, 3), dtype=torch.float64, device=gpu))
@@ -357,4 +360,71 @@ def test_viscous_adjoint_solve(gpu):
     err = np.linalg.norm(y - rhs) / np.linalg.norm(rhs)
     print(f"[viscous] ||A^T psi - rhs|| / ||rhs|| = {err:.2e}", flush=True)
     assert err <= 1e-8
+    op.close()
+
+
+@pytest.mark.parametrize("second", [True, False])
+def test_viscous_tangent(gpu, second):
+    """(dR/dU) dU with the Galerkin term: <lam, (dR/dU) dU> equals
+    <(dR/dU)^T lam, dU> to rounding (two hand-written passes), and the
+    Galerkin part matches central differences of R - R_euler."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _box(6)
+    p = ns_params(0.4, 30.0, second_order=second)
+    p.alpha, p.yaw = 0.05, 0.03
+    U = init_state(p, m.X, "noise", 0.01, seed=21)
+    rng = np.random.default_rng(8)
+    U[:, 1:4] += 0.02 * U[:, :1] * rng.standard_normal((U.shape[0], 3))
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf, second_order=second, alpha=p.alpha, yaw=p.yaw)
+    op, ope = SpaceOperator(m, p), SpaceOperator(m, pe)
+    Ud = torch.from_numpy(U).to(gpu)
+    for _ in range(2):
+        d = rng.standard_normal(U.shape) * (np.abs(U) + 0.1)
+        lam = rng.standard_normal(U.shape)
+        dd, ld = torch.from_numpy(d).to(gpu), torch.from_numpy(lam).to(gpu)
+        jv, jve, gu = torch.empty_like(Ud), torch.empty_like(Ud), torch.empty_like(Ud)
+        op.compute_derivative_of_residual(Ud, jv, dU=dd)
+        ope.compute_derivative_of_residual(Ud, jve, dU=dd)
+        op.compute_transpose_derivative_of_residual(Ud, ld, gu, None)
+        a, b = float(np.sum(lam * jv.cpu().numpy())), float(np.sum(gu.cpu().numpy() * d))
+        print(f"[viscous] <lam, J d> {a:.14e} vs <J^T lam, d> {b:.14e}", flush=True)
+        assert abs(a - b) <= 1e-11 * abs(b)
+        gv = jv.cpu().numpy() - jve.cpu().numpy()
+        H = 1e-5
+        diff = []
+        for h in (H, H / 2):
+            r = []
+            for s in (1.0, -1.0):
+                Q = torch.from_numpy(U + s * h * d).to(gpu)
+                Rv, Re = torch.empty_like(Q), torch.empty_like(Q)
+                op.compute_residual(Q, Rv)
+                ope.compute_residual(Q, Re)
+                r.append((Rv - Re).cpu().numpy())
+            diff.append((r[0] - r[1]) / (2 * h))
+        fd = (4 * diff[1] - diff[0]) / 3
+        err = np.linalg.norm(gv - fd) / np.linalg.norm(fd)
+        print(f"[viscous] Galerkin tangent vs FD {err:.2e}", flush=True)
+        assert err <= 1e-6
+    op.close()
+    ope.close()
+
+
+def test_viscous_newton_exact_products(gpu):
+    """Newton with the exact (H2) products of the Navier-Stokes residual and
+    the wall rows reaches the matrix-free Newton's state."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator, steady_solve
+    m = _channel()
+    p = ns_params(0.3, 100.0)
+    U0 = init_state(p, m.X, "freestream")
+    op = SpaceOperator(m, p)
+    Uf, Uh = torch.from_numpy(U0).to(gpu), torch.from_numpy(U0).to(gpu)
+    of = steady_solve(op, Uf, precision=1, max_steps=300, eps=1e-9, mvp="fd")
+    oh = steady_solve(op, Uh, precision=1, max_steps=300, eps=1e-9, mvp="h2")
+    print(f"[viscous] Newton fd {of['steps']} steps / {of['krylov_its']} its, "
+          f"h2 {oh['steps']} steps / {oh['krylov_its']} its", flush=True)
+    assert of["converged"] and oh["converged"]
+    a, b = Uf.cpu().numpy(), Uh.cpu().numpy()
+    assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
     op.close()
