@@ -984,8 +984,6 @@ int ensure_elem_tables(aerofx_ctx *c) {
 }
 
 int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub, double *Xout) {
-  if (c->viscous && Xout)
-    return set_error(AEROFX_ERR_STATE, "adjoint: the shape derivative of the Navier-Stokes terms is not built");
   int rc = ensure_adjoint(c);
   if (rc) return rc;
   Adjoint *a = c->adj;
@@ -1019,6 +1017,8 @@ int launch_adjoint(aerofx_ctx *c, const double *U, const double *lam, double *Ub
     if (c->T > 0)
       hipLaunchKernelGGL(k_adj_tets, grid_for(c->T), dim3(kAdjBlock), 0, c->stream, c->T, a->tets, a->tedge, t.X4,
                          t.invcv, lam, a->R, a->EX, a->TX);
+    // the Navier-Stokes Galerkin term's coordinate derivative, per element
+    if (c->T > 0 && (rc = launch_visc_adjoint_x(c, lam, a->TX))) return rc;
     if (c->F > 0)
       hipLaunchKernelGGL(k_adj_faces, grid_for(c->F), dim3(kAdjBlock), 0, c->stream, c->F, a->faces, t.fcode,
                          t.fnorm3, t.X4, t.invcv, c->d_V, lam, c->fc, a->FX);
@@ -1149,8 +1149,6 @@ extern "C" int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const d
   if (ctx->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "adjoint_residual: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(ctx, "adjoint_residual")) return rf;
-  if (dRdX_T_lambda)
-    if (int rv = refuse_viscous(ctx, "adjoint_residual: shape derivative")) return rv;
   if (!dRdU_T_lambda && !dRdX_T_lambda) return AEROFX_OK;
   int rc = launch_adjoint(ctx, U, lambda, dRdU_T_lambda, dRdX_T_lambda);
   if (rc) return rc;

@@ -250,7 +250,10 @@ int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *do
 int launch_visc_idtv(aerofx_ctx *c, double **idtv);
 // dR += (d Galerkin / dU) dU / V at the primitive state V; the wall rows of
 // a product y = A p
-int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, double *dR, const int *done);
+int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, const double *dX, double *dR,
+                        const int *done);
+// TX[t][12] += the Galerkin part of the element's coordinate adjoint
+int launch_visc_adjoint_x(aerofx_ctx *c, const double *lam, double *TX);
 int launch_bcs_product(aerofx_ctx *c, const double *U, const double *p, double *y, const int *done);
 // Ub += (d Galerkin / dU)^T lam (the residual's 1/V included), at c->d_V
 int launch_visc_adjoint(aerofx_ctx *c, const double *lam, double *Ub);

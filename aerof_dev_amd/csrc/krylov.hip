@@ -1184,7 +1184,6 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "direct_solve: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(c, "direct_solve")) return rf;
-  if (int rv = refuse_viscous(c, "direct_solve")) return rv;
   if (nrhs < 1 || nrhs > 16) return set_error(AEROFX_ERR_ARG, "direct_solve: 1 <= nrhs <= 16");
   if (o->precision != 0 && o->precision != 1) return set_error(AEROFX_ERR_ARG, "direct_solve: precision 0 or 1");
   int rc = ensure_krylov(c, o->gmres_m);
@@ -1202,12 +1201,35 @@ int aerofx_direct_solve(aerofx_ctx *c, const double *U, int nrhs, const double *
   const bool blocked = nrhs > 1 && c->max_degree <= 64;
   if ((rc = derivative_operators(c, U, o->cfl, blocked ? 0 : o->precision, "direct_solve"))) return rc;
   const int bm = std::min(o->gmres_m, 16);
+  // with the Navier-Stokes wall conditions the system is A x = P rhs with
+  // A = P dR/dU + B: the wall rows of the right-hand sides become the
+  // derivative of the conditions, zero for fixed walls
+  // (applyBCsToDerivativeOfResidual)
+  const bool walls = c->viscous && c->nwall;
+  double *prhs = nullptr;
+  struct FreeRhs {
+    aerofx_ctx *c;
+    double *&p;
+    ~FreeRhs() {
+      if (p) ctx_release(c, p);
+    }
+  } free_rhs{c, prhs};
+  if (walls) {
+    void *p = nullptr;
+    const size_t n = 5 * (size_t)c->N;
+    if ((rc = ctx_alloc(c, sizeof(double) * n * nrhs, &p))) return rc;
+    prhs = (double *)p;
+    for (int q = 0; q < nrhs; ++q)
+      if ((rc = launch_bcs_adjoint(c, U, rhs + q * n, prhs + q * n, nullptr, true))) return rc;
+    rhs = prhs;
+  }
   FgOps ops;
   ops.apply = [&](const double *xv, double *y, int nq) -> int {
     for (int q = 0; q < nq; ++q) {
       const size_t off = (size_t)q * c->ksp->n;
       int r = launch_tangent(c, U, xv + off, nullptr, y + off);
       if (r) return r;
+      if (walls && (r = launch_bcs_product(c, U, xv + off, y + off, nullptr))) return r;
     }
     return 0;
   };

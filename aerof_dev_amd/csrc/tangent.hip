@@ -422,7 +422,7 @@ int launch_tangent(aerofx_ctx *c, const double *U, const double *dU, const doubl
   hipLaunchKernelGGL(k_tan_nodes, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, c->d_V, g->dV, g->S, g->dcv, g->dfn,
                      g->dF, c->fc, dR, nullptr);
   // the Navier-Stokes Galerkin term (state derivative)
-  if ((rc = launch_visc_tangent(c, c->d_V, dU, dR, nullptr))) return rc;
+  if ((rc = launch_visc_tangent(c, c->d_V, dU, dX, dR, nullptr))) return rc;
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_hip_error(e, "tangent launch");
 }
@@ -475,7 +475,7 @@ int launch_h2_apply(aerofx_ctx *c, const double *Q, const double *p, const doubl
   hipLaunchKernelGGL(k_tan_nodes, grid_for(t.N), dim3(kTanBlock), 0, c->stream, t, g->V0, g->dV, g->S, g->dcv,
                      g->dfn, g->dF, c->fc, y, done);
   int rc;
-  if ((rc = launch_visc_tangent(c, g->V0, p, y, done))) return rc;
+  if ((rc = launch_visc_tangent(c, g->V0, p, nullptr, y, done))) return rc;
   if (dt)
     hipLaunchKernelGGL(k_tan_time, grid_for(5LL * t.N), dim3(kTanBlock), 0, c->stream, 5 * (size_t)t.N, p, dt, y,
                        done);
@@ -495,8 +495,6 @@ extern "C" int aerofx_residual_derivative(aerofx_ctx *c, const double *U, const 
   if (c->fc.low_mach)
     return set_error(AEROFX_ERR_STATE, "residual_derivative: not available with low-Mach preconditioning");
   if (int rf = refuse_failsafe_nodes(c, "residual_derivative")) return rf;
-  if (dX)
-    if (int rv = refuse_viscous(c, "residual_derivative: shape derivative")) return rv;
   int rc = launch_tangent(c, U, dU, dX, dR);
   if (rc) return rc;
   hipError_t e = hipMemcpyAsync(c->h_err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream);

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "context.h"
+#include "device_dual.h"
 #include "device_viscous.h"
 #include "host_internal.h"
 
@@ -169,13 +170,139 @@ __global__ __launch_bounds__(kBlock) void k_visc_adj_elements(int T, const int4 
   }
 }
 
+// The shape part of the element's vertex contributions: with the
+// coordinates moving, only the P1 gradients g and the volume change (the
+// states, mean temperature and transport coefficients stay), so
+//   d c_j = dvol (r . g_j) + vol (dr . g_j + r . dg_j),
+// dr from the velocity and temperature gradients along dg.
+template <class T>
+__device__ __forceinline__ T p1_gradients_t(const T x[4][3], T g[4][3]) {
+  T J[3][3];
+#pragma unroll
+  for (int col = 0; col < 3; ++col)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) J[r][col] = x[col + 1][r] - x[0][r];
+  const T det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) + J[1][0] * (J[0][2] * J[2][1] - J[0][1] * J[2][2]) +
+                J[2][0] * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+  const T s = -1.0 / det;
+  g[1][0] = (-J[1][1] * J[2][2] + J[1][2] * J[2][1]) * s;
+  g[1][1] = (J[0][1] * J[2][2] - J[0][2] * J[2][1]) * s;
+  g[1][2] = -(J[0][1] * J[1][2] - J[0][2] * J[1][1]) * s;
+  g[2][0] = -(-J[1][0] * J[2][2] + J[1][2] * J[2][0]) * s;
+  g[2][1] = -(J[0][0] * J[2][2] - J[0][2] * J[2][0]) * s;
+  g[2][2] = (J[0][0] * J[1][2] - J[0][2] * J[1][0]) * s;
+  g[3][0] = -(J[1][0] * J[2][1] - J[1][1] * J[2][0]) * s;
+  g[3][1] = (J[0][0] * J[2][1] - J[0][1] * J[2][0]) * s;
+  g[3][2] = -(J[0][0] * J[1][1] - J[0][1] * J[1][0]) * s;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) g[0][d] = -(g[1][d] + g[2][d] + g[3][d]);
+  return (1.0 / 6.0) * det;
+}
+
+// dc[j][m] for the coordinate change dx[4][3] of the element's vertices
+__device__ __forceinline__ void ns_shape_contrib(const NsElem &e, const double v[4][5], const double x[4][3],
+                                                 const double dx[4][3], double dc[4][5]) {
+  Dual xd[4][3], gd[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xd[k][d] = Dual(x[k][d], dx[k][d]);
+  const Dual vd = p1_gradients_t<Dual>(xd, gd);
+  double g[4][3], dg[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      g[k][d] = gd[k][d].v;
+      dg[k][d] = gd[k][d].d;
+    }
+  const double vol = vd.v, dvol = vd.d;
+  double dgu[3][3], dgT[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      dgu[i][j] = dg[0][j] * v[0][i + 1] + dg[1][j] * v[1][i + 1] + dg[2][j] * v[2][i + 1] + dg[3][j] * v[3][i + 1];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) dgT[j] = dg[0][j] * e.T[0] + dg[1][j] * e.T[1] + dg[2][j] * e.T[2] + dg[3][j] * e.T[3];
+  const double ddiv = dgu[0][0] + dgu[1][1] + dgu[2][2];
+  double dt[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) dt[i][i] = e.lambda * ddiv + 2.0 * e.mu * dgu[i][i];
+  dt[0][1] = dt[1][0] = e.mu * (dgu[1][0] + dgu[0][1]);
+  dt[0][2] = dt[2][0] = e.mu * (dgu[2][0] + dgu[0][2]);
+  dt[1][2] = dt[2][1] = e.mu * (dgu[2][1] + dgu[1][2]);
+  double r[3][5], dr[3][5];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    r[j][0] = dr[j][0] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      r[j][i + 1] = e.tij[i][j];
+      dr[j][i + 1] = dt[i][j];
+    }
+    r[j][4] = e.ucg[0] * e.tij[0][j] + e.ucg[1] * e.tij[1][j] + e.ucg[2] * e.tij[2][j] + e.kappa * e.dTdxj[j];
+    dr[j][4] = e.ucg[0] * dt[0][j] + e.ucg[1] * dt[1][j] + e.ucg[2] * dt[2][j] + e.kappa * dgT[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+      dc[j][m] = dvol * (r[0][m] * g[j][0] + r[1][m] * g[j][1] + r[2][m] * g[j][2]) +
+                 vol * (dr[0][m] * g[j][0] + dr[1][m] * g[j][1] + dr[2][m] * g[j][2] + r[0][m] * dg[j][0] +
+                        r[1][m] * dg[j][1] + r[2][m] * dg[j][2]);
+}
+
+// Adjoint of the shape part: TX[t][3 a + d] += sum_j lam_j / V_j . d c_j /
+// d x_a[d] (added to the element's other geometry terms before the node
+// gather of the adjoint)
+__global__ __launch_bounds__(kBlock) void k_visc_adj_x(int T, const int4 *__restrict__ tets,
+                                                       const double *__restrict__ X4, const double *__restrict__ V,
+                                                       const double *__restrict__ lam,
+                                                       const double *__restrict__ invcv, NsConsts ns,
+                                                       double *__restrict__ TX) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= T) return;
+  double x[4][3], v[4][5], g[4][3];
+  load_element(tets, X4, V, t, x, v);
+  p1_gradients(x, g);
+  NsElem e;
+  ns_element(ns, g, v, e);
+  const int4 nd = tets[t];
+  const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+  double l[4][5];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double ic = invcv[id[j]];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) l[j][m] = lam[5 * (size_t)id[j] + m] * ic;
+  }
+#pragma unroll 1
+  for (int dir = 0; dir < 12; ++dir) {
+    double dx[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dx[k][d] = (3 * k + d == dir) ? 1.0 : 0.0;
+    double dc[4][5];
+    ns_shape_contrib(e, v, x, dx, dc);
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) s += dc[j][m] * l[j][m];
+    TX[12 * (size_t)t + dir] += s;
+  }
+}
+
 // Forward element pass of the state derivative: each element once,
 // dr = sum_k (d r / d U_k) dU_k and its vertex contributions vol (dr . g_j)
 // to C[pos[4 t + j]]; k_visc_gather then adds the node sums (/ V).
 __global__ __launch_bounds__(kBlock) void k_visc_tan_elements(int T, const int4 *__restrict__ tets,
                                                               const double *__restrict__ X4,
                                                               const double *__restrict__ V,
-                                                              const double *__restrict__ dU, NsConsts ns,
+                                                              const double *__restrict__ dU,
+                                                              const double *__restrict__ dX, NsConsts ns,
                                                               const int4 *__restrict__ pos, double *__restrict__ C,
                                                               const int *__restrict__ done) {
   if (done && *done) return;
@@ -188,6 +315,20 @@ __global__ __launch_bounds__(kBlock) void k_visc_tan_elements(int T, const int4 
   ns_element(ns, g, v, e);
   const int4 nd = tets[t];
   const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+  double dc[4][5];
+  if (dX) {
+    double dx[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dx[k][d] = dX[3 * (size_t)id[k] + d];
+    ns_shape_contrib(e, v, x, dx, dc);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) dc[j][m] = 0.0;
+  }
   double dr[3][5];
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -195,6 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_tan_elements(int T, const int4 
     for (int m = 0; m < 5; ++m) dr[j][m] = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
+    if (!dU) break;
 #pragma unroll
     for (int cc = 0; cc < 5; ++cc) {
       const double du = dU[5 * (size_t)id[k] + cc];
@@ -212,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_tan_elements(int T, const int4 
   for (int j = 0; j < 4; ++j) {
     double *o = C + 5 * (size_t)pj[j];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) o[m] = vol * (dr[0][m] * g[j][0] + dr[1][m] * g[j][1] + dr[2][m] * g[j][2]);
+    for (int m = 0; m < 5; ++m) o[m] = vol * (dr[0][m] * g[j][0] + dr[1][m] * g[j][1] + dr[2][m] * g[j][2]) + dc[j][m];
   }
 }
 
@@ -532,15 +674,25 @@ int launch_visc_adjoint(aerofx_ctx *c, const double *lam, double *Ub) {
   return check("viscous adjoint launch");
 }
 
-int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, double *dR, const int *done) {
-  if (!c->viscous || !dU) return 0;
+int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, const double *dX, double *dR,
+                        const int *done) {
+  if (!c->viscous || (!dU && !dX)) return 0;
   if (!c->d_visc_c) return set_error(AEROFX_ERR_STATE, "tangent: the viscous contribution buffer is missing");
   const ElemTables *el = c->elem;
   hipLaunchKernelGGL(k_visc_tan_elements, dim3((c->T + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->T,
-                     (const int4 *)el->tets, c->tab.X4, V, dU, c->ns, (const int4 *)c->d_visc_pos, c->d_visc_c, done);
+                     (const int4 *)el->tets, c->tab.X4, V, dU, dX, c->ns, (const int4 *)c->d_visc_pos, c->d_visc_c,
+                     done);
   hipLaunchKernelGGL(k_visc_gather, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
                      el->ntrow, c->d_visc_c, c->tab.invcv, 1, dR, done);
   return check("viscous tangent launch");
+}
+
+int launch_visc_adjoint_x(aerofx_ctx *c, const double *lam, double *TX) {
+  if (!c->viscous) return 0;
+  const ElemTables *el = c->elem;
+  hipLaunchKernelGGL(k_visc_adj_x, dim3((c->T + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->T,
+                     (const int4 *)el->tets, c->tab.X4, c->d_V, lam, c->tab.invcv, c->ns, TX);
+  return check("viscous shape adjoint launch");
 }
 
 int launch_bcs_product(aerofx_ctx *c, const double *U, const double *p, double *y, const int *done) {

@@ -226,11 +226,11 @@ int aerofx_ctx_set_low_mach(aerofx_ctx *ctx, const aerofx_low_mach *lm);
  * The transposed product aerofx_adjoint_residual(dRdU_T_lambda) includes
  * the Galerkin term's transposed element derivative, and
  * aerofx_adjoint_solve solves with the wall rows (A = P dR/dU + B, A^T psi =
- * (dR/dU)^T P psi + B^T psi); aerofx_residual_derivative(dU) and the exact
- * products (AEROFX_MVP_H2, with the wall rows) include its forward element
- * derivative.  Not with the low-Mach preconditioner; the shape derivatives
- * (dX of aerofx_residual_derivative, dRdX_T_lambda) and aerofx_direct_solve
- * return AEROFX_ERR_STATE. */
+ * (dR/dU)^T P psi + B^T psi); aerofx_residual_derivative (dU and dX), the
+ * exact products (AEROFX_MVP_H2, with the wall rows) and dRdX_T_lambda
+ * include its element derivatives; aerofx_direct_solve solves
+ * (P dR/dU + B) x = P rhs (the wall rows of rhs become the derivative of the
+ * fixed-wall conditions, zero).  Not with the low-Mach preconditioner. */
 #define AEROFX_VISC_CONSTANT 0
 #define AEROFX_VISC_SUTHERLAND 1
 #define AEROFX_VISC_PRANDTL 2

@@ -16,7 +16,8 @@
   adjoint solve of the system with the wall rows through <psi, A d> = <rhs, d>;
 * the forward state derivative: exact duality with the transposed product
   (1e-11), its Galerkin part against differences, H2-product Newton;
-* the shape derivatives (forward and adjoint) refuse the viscous terms.
+* the shape derivative: forward / adjoint duality (1e-11), its Galerkin part
+  against differences over moved meshes; the direct solve with the wall rows.
 """
 import numpy as np
 import pytest
@@ -254,12 +255,6 @@ def test_viscous_refusals(gpu):
     op = SpaceOperator(m, p)
     U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
     with pytest.raises(AerofxError):
-        op.compute_derivative_of_residual(U, torch.empty_like(U),
-                                          dX=torch.zeros((op.num_nodes, 3), dtype=torch.float64, device=gpu))
-    with pytest.raises(AerofxError):
-        op.compute_transpose_derivative_of_residual(U, torch.ones_like(U), None,
-                                                    torch.empty((op.num_nodes, 3), dtype=torch.float64, device=gpu))
-    with pytest.raises(AerofxError):
         op.set_low_mach(LowMachPrec())
     op.close()
 
@@ -427,4 +422,84 @@ def test_viscous_newton_exact_products(gpu):
     assert of["converged"] and oh["converged"]
     a, b = Uf.cpu().numpy(), Uh.cpu().numpy()
     assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
+    op.close()
+
+
+def test_viscous_shape_derivative(gpu):
+    """(dR/dX) dX with the Galerkin term: duality with (dR/dX)^T lam to
+    rounding, and its Galerkin part against central differences of
+    R - R_euler over meshes moved by +-h dX."""
+    import dataclasses
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _box(6)
+    p = ns_params(0.4, 30.0)
+    p.alpha, p.yaw = 0.05, 0.03
+    U = init_state(p, m.X, "noise", 0.01, seed=31)
+    rng = np.random.default_rng(9)
+    U[:, 1:4] += 0.02 * U[:, :1] * rng.standard_normal((U.shape[0], 3))
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf, alpha=p.alpha, yaw=p.yaw)
+    op, ope = SpaceOperator(m, p), SpaceOperator(m, pe)
+    Ud = torch.from_numpy(U).to(gpu)
+    dX = 0.02 * rng.standard_normal(m.X.shape) / 6.0
+    lam = rng.standard_normal(U.shape)
+    dXd, ld = torch.from_numpy(dX).to(gpu), torch.from_numpy(lam).to(gpu)
+    jx, jxe = torch.empty_like(Ud), torch.empty_like(Ud)
+    op.compute_derivative_of_residual(Ud, jx, dX=dXd)
+    ope.compute_derivative_of_residual(Ud, jxe, dX=dXd)
+    gx = torch.empty((op.num_nodes, 3), dtype=torch.float64, device=gpu)
+    op.compute_transpose_derivative_of_residual(Ud, ld, None, gx)
+    a, b = float(np.sum(lam * jx.cpu().numpy())), float(np.sum(gx.cpu().numpy() * dX))
+    print(f"[viscous] <lam, dR/dX dX> {a:.14e} vs <(dR/dX)^T lam, dX> {b:.14e}", flush=True)
+    assert abs(a - b) <= 1e-11 * abs(b)
+    gv = jx.cpu().numpy() - jxe.cpu().numpy()
+
+    def galerkin(X):
+        mm = dataclasses.replace(m, X=np.ascontiguousarray(X))
+        o1, o2 = SpaceOperator(mm, p), SpaceOperator(mm, pe)
+        R1, R2 = torch.empty_like(Ud), torch.empty_like(Ud)
+        o1.compute_residual(Ud, R1)
+        o2.compute_residual(Ud, R2)
+        o1.close()
+        o2.close()
+        return (R1 - R2).cpu().numpy()
+    H = 1e-4
+    d1 = (galerkin(m.X + H * dX) - galerkin(m.X - H * dX)) / (2 * H)
+    d2 = (galerkin(m.X + H / 2 * dX) - galerkin(m.X - H / 2 * dX)) / H
+    fd = (4 * d2 - d1) / 3
+    err = np.linalg.norm(gv - fd) / np.linalg.norm(fd)
+    print(f"[viscous] Galerkin shape derivative vs FD {err:.2e}", flush=True)
+    assert err <= 1e-6
+    op.close()
+    ope.close()
+
+
+def test_viscous_direct_solve(gpu):
+    """(P dR/dU + B) x = P rhs on the converged no-slip channel."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator, direct_solve, steady_solve
+    m = _channel()
+    p = ns_params(0.3, 100.0)
+    U0 = init_state(p, m.X, "freestream")
+    op = SpaceOperator(m, p)
+    Ud = torch.from_numpy(U0).to(gpu)
+    assert steady_solve(op, Ud, precision=1, max_steps=300, eps=1e-8, mvp="fd")["converged"]
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal((2,) + U0.shape)
+    x = torch.zeros((2,) + U0.shape, dtype=torch.float64, device=gpu)
+    st = direct_solve(op, Ud, torch.from_numpy(rhs).to(gpu), x, precision=1, cfl=1e3, eps=1e-10, max_its=600)
+    print(f"[viscous] direct solve: {st}", flush=True)
+    assert st["converged"]
+    wall = op.node_types() == 3
+    for q in range(2):
+        y = torch.empty_like(Ud)
+        op.compute_derivative_of_residual(Ud, y, dU=x[q].contiguous())
+        yv = y.cpu().numpy()
+        xq = x[q].cpu().numpy()
+        yv[wall, 1:4] = xq[wall, 1:4]
+        r = rhs[q].copy()
+        r[wall, 1:4] = 0.0
+        err = np.linalg.norm(yv - r) / np.linalg.norm(r)
+        print(f"[viscous] ||A x - P rhs|| / ||P rhs|| = {err:.2e}", flush=True)
+        assert err <= 1e-8
     op.close()
