@@ -1167,6 +1167,8 @@ extern "C" int aerofx_wall_force(aerofx_ctx *ctx, const double *U, const double 
   if (ctx->F > 0) {
     hipLaunchKernelGGL(k_lift_faces, grid_for(ctx->F), dim3(kAdjBlock), 0, ctx->stream, ctx->F, a->faces, t.fcode,
                        t.fnorm3, t.X4, ctx->d_V, dir[0], dir[1], dir[2], a->Jf, dJdX ? a->FX : nullptr);
+    // the viscous stress on the wall faces (PostFcnNS::computeForce)
+    if ((rc = launch_visc_force(ctx, dir, a->Jf, dJdU || dJdX))) return rc;
     hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, ctx->stream, ctx->F, a->Jf, a->Jf + ctx->F);
   } else {
     hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(256), 0, ctx->stream, 0, a->Jf, a->Jf);  // no faces: J = 0
@@ -1174,6 +1176,7 @@ extern "C" int aerofx_wall_force(aerofx_ctx *ctx, const double *U, const double 
   if (dJdU || dJdX)
     hipLaunchKernelGGL(k_lift_nodes, grid_for(t.N), dim3(kAdjBlock), 0, ctx->stream, t, U, ctx->gas, a->nfslot,
                        a->FX, dir[0], dir[1], dir[2], dJdU, dJdX);
+  if ((dJdU || dJdX) && (rc = launch_visc_force_gather(ctx, dJdU, dJdX))) return rc;
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(a->h_J, a->Jf + ctx->F, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

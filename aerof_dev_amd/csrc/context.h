@@ -172,6 +172,10 @@ struct aerofx_ctx {
   double *d_idtv = nullptr;
   double *d_visc_c = nullptr;  // [4T][5] element contributions of the Galerkin term, in node-slot order
   double *d_adj_tmp = nullptr;  // [N][5] the adjoint with the wall rows masked
+  // viscous wall force: element of each face, per node the wall-face element
+  // slots (4 f + k), per slot the derivatives [4F][5] / [4F][3]
+  int *d_face_elem = nullptr, *d_nfe_row = nullptr, *d_nfe_slot = nullptr;
+  double *d_force_u = nullptr, *d_force_x = nullptr;
   int *d_visc_pos = nullptr;   // [4T] node-slot position of element slot 4 t + j
   // device address of this GPU's static scratch array of
   // aerofx_residual_noalloc (looked up at ctx_create; null if that failed,
@@ -252,6 +256,10 @@ int launch_visc_idtv(aerofx_ctx *c, double **idtv);
 // a product y = A p
 int launch_visc_tangent(aerofx_ctx *c, const double *V, const double *dU, const double *dX, double *dR,
                         const int *done);
+// the viscous wall force: Jf[f] += -(tau n_f) . dir on the wall faces (and
+// the slot derivatives when derivs), then dJdU / dJdX += their node sums
+int launch_visc_force(aerofx_ctx *c, const double dir[3], double *Jf, bool derivs);
+int launch_visc_force_gather(aerofx_ctx *c, double *dJdU, double *dJdX);
 // TX[t][12] += the Galerkin part of the element's coordinate adjoint
 int launch_visc_adjoint_x(aerofx_ctx *c, const double *lam, double *TX);
 int launch_bcs_product(aerofx_ctx *c, const double *U, const double *p, double *y, const int *done);

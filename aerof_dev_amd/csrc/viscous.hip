@@ -358,6 +358,146 @@ __global__ __launch_bounds__(kBlock) void k_visc_tan_elements(int T, const int4 
   }
 }
 
+// Viscous wall force (PostFcnNS::computeViscousForce, PostFcn.C:1578-1612;
+// FaceTria::computeForce, FaceTria.C:28-60): on every wall face f with its
+// element e, F_v = -tau_e n_f (n_f the full outward face normal
+// 0.5 (x2 - x0) x (x1 - x0)), J += F_v . d.  With FU / FX given, the
+// derivatives of J_f with respect to the states [4][5] and coordinates
+// [4][3] of the element's vertices go to the face's slots 4 f + k.
+__global__ __launch_bounds__(kBlock) void k_visc_force(int F, const int *__restrict__ faces,
+                                                       const int *__restrict__ fcode,
+                                                       const double *__restrict__ fnorm,
+                                                       const int *__restrict__ face_elem,
+                                                       const int4 *__restrict__ tets, const double *__restrict__ X4,
+                                                       const double *__restrict__ V, NsConsts ns, double d0, double d1,
+                                                       double d2, double *__restrict__ Jf, double *__restrict__ FU,
+                                                       double *__restrict__ FX) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= F) return;
+  const int t = face_elem[f];
+  if (t < 0 || !is_wall_code(fcode[f])) return;
+  double x[4][3], v[4][5], g[4][3];
+  load_element(tets, X4, V, t, x, v);
+  p1_gradients(x, g);
+  NsElem e;
+  ns_element(ns, g, v, e);
+  const double n[3] = {fnorm[3 * (size_t)f], fnorm[3 * (size_t)f + 1], fnorm[3 * (size_t)f + 2]};
+  const double dd[3] = {d0, d1, d2};
+  // J_f = -sum_ij d_i tau_ij n_j
+  double jf = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) jf += -(e.tij[i][0] * n[0] + e.tij[i][1] * n[1] + e.tij[i][2] * n[2]) * dd[i];
+  Jf[f] += jf;
+  if (!FU) return;
+#pragma unroll 1
+  for (int k = 0; k < 4; ++k) {
+    double vk[5], gk[3];
+    pick_row<5>(v, k, vk);
+    pick_row<3>(g, k, gk);
+#pragma unroll
+    for (int cc = 0; cc < 5; ++cc) {
+      double col[3][5];
+      ns_node_column(ns, e, gk, vk, pick4(e.T, k), cc, col);
+      // col[j][i + 1] = d tau_ij / d U_k[cc]
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s += -col[j][i + 1] * n[j] * dd[i];
+      FU[5 * (4 * (size_t)f + k) + cc] = s;
+    }
+  }
+  // coordinates: through the P1 gradients (dual numbers on the element map)
+  // and through the face normal (the face's vertices)
+  const int4 nd = tets[t];
+  const int id[4] = {nd.x, nd.y, nd.z, nd.w};
+  const int fv[3] = {faces[3 * (size_t)f], faces[3 * (size_t)f + 1], faces[3 * (size_t)f + 2]};
+#pragma unroll 1
+  for (int dir = 0; dir < 12; ++dir) {
+    const int k = dir / 3, dc = dir - 3 * k;
+    Dual xd[4][3], gd[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) xd[a][b] = Dual(x[a][b], (a == k && b == dc) ? 1.0 : 0.0);
+    p1_gradients_t<Dual>(xd, gd);
+    double dgu[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        dgu[i][j] = gd[0][j].d * v[0][i + 1] + gd[1][j].d * v[1][i + 1] + gd[2][j].d * v[2][i + 1] +
+                    gd[3][j].d * v[3][i + 1];
+    const double ddiv = dgu[0][0] + dgu[1][1] + dgu[2][2];
+    double dt[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) dt[i][i] = e.lambda * ddiv + 2.0 * e.mu * dgu[i][i];
+    dt[0][1] = dt[1][0] = e.mu * (dgu[1][0] + dgu[0][1]);
+    dt[0][2] = dt[2][0] = e.mu * (dgu[2][0] + dgu[0][2]);
+    dt[1][2] = dt[2][1] = e.mu * (dgu[2][1] + dgu[1][2]);
+    // the face normal's derivative when the vertex is on the face
+    double dn[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (fv[q] != id[k]) continue;
+      Dual p0[3], p1[3], p2[3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double4 a0 = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)fv[0]);
+        const double4 a1 = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)fv[1]);
+        const double4 a2 = *reinterpret_cast<const double4 *>(X4 + 4 * (size_t)fv[2]);
+        const double c0 = b == 0 ? a0.x : (b == 1 ? a0.y : a0.z);
+        const double c1 = b == 0 ? a1.x : (b == 1 ? a1.y : a1.z);
+        const double c2 = b == 0 ? a2.x : (b == 1 ? a2.y : a2.z);
+        p0[b] = Dual(c0, (q == 0 && b == dc) ? 1.0 : 0.0);
+        p1[b] = Dual(c1, (q == 1 && b == dc) ? 1.0 : 0.0);
+        p2[b] = Dual(c2, (q == 2 && b == dc) ? 1.0 : 0.0);
+      }
+      Dual u[3], w[3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        u[b] = p2[b] - p0[b];
+        w[b] = p1[b] - p0[b];
+      }
+      const Dual c0 = u[1] * w[2] - u[2] * w[1], c1 = u[2] * w[0] - u[0] * w[2], c2 = u[0] * w[1] - u[1] * w[0];
+      dn[0] = 0.5 * c0.d;
+      dn[1] = 0.5 * c1.d;
+      dn[2] = 0.5 * c2.d;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += -(dt[i][j] * n[j] + e.tij[i][j] * dn[j]) * dd[i];
+    FX[3 * (4 * (size_t)f + k) + dc] = s;
+  }
+}
+
+// dJdU[n] += sum over the node's wall-face element slots of FU, and dJdX
+// likewise, in face order
+__global__ __launch_bounds__(kBlock) void k_visc_force_gather(int N, const int *__restrict__ row,
+                                                              const int *__restrict__ slot,
+                                                              const double *__restrict__ FU,
+                                                              const double *__restrict__ FX, double *__restrict__ dJdU,
+                                                              double *__restrict__ dJdX) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double u[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, xs[3] = {0.0, 0.0, 0.0};
+  for (int s = row[n]; s < row[n + 1]; ++s) {
+    const size_t q = slot[s];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) u[k] += FU[5 * q + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xs[k] += FX[3 * q + k];
+  }
+  if (dJdU)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dJdU[5 * (size_t)n + k] += u[k];
+  if (dJdX)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dJdX[3 * (size_t)n + k] += xs[k];
+}
+
 // The wall rows of a product y = A p (H2 products): y_r = p_r on the
 // momentum rows (the isothermal energy row too)
 __global__ void k_bcs_product(int nwall, const int2 *__restrict__ wall, double wall_T, const double *__restrict__ U,
@@ -695,6 +835,22 @@ int launch_visc_adjoint_x(aerofx_ctx *c, const double *lam, double *TX) {
   return check("viscous shape adjoint launch");
 }
 
+int launch_visc_force(aerofx_ctx *c, const double dir[3], double *Jf, bool derivs) {
+  if (!c->viscous || !c->F) return 0;
+  const ElemTables *el = c->elem;
+  hipLaunchKernelGGL(k_visc_force, dim3((c->F + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->F, el->faces,
+                     c->tab.fcode, c->tab.fnorm, c->d_face_elem, (const int4 *)el->tets, c->tab.X4, c->d_V, c->ns,
+                     dir[0], dir[1], dir[2], Jf, derivs ? c->d_force_u : nullptr, derivs ? c->d_force_x : nullptr);
+  return check("viscous force launch");
+}
+
+int launch_visc_force_gather(aerofx_ctx *c, double *dJdU, double *dJdX) {
+  if (!c->viscous || (!dJdU && !dJdX)) return 0;
+  hipLaunchKernelGGL(k_visc_force_gather, dim3((c->N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->N,
+                     c->d_nfe_row, c->d_nfe_slot, c->d_force_u, c->d_force_x, dJdU, dJdX);
+  return check("viscous force gather launch");
+}
+
 int launch_bcs_product(aerofx_ctx *c, const double *U, const double *p, double *y, const int *done) {
   if (!c->viscous || !c->nwall) return 0;
   hipLaunchKernelGGL(k_bcs_product, dim3((c->nwall + 63) / 64), dim3(64), 0, c->stream, c->nwall, c->d_wall,
@@ -827,6 +983,58 @@ int aerofx_ctx_set_viscous(aerofx_ctx *c, const aerofx_viscous *v) {
     hipError_t e = hipMemcpyAsync(c->d_wall, wall.data(), sizeof(int2) * wall.size(), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return set_hip_error(e, "set_viscous upload");
+  }
+  // the element of every face (FaceTria::elemNum) and, for the wall faces,
+  // the slots (4 f + k) of their elements' vertices per node, in face order
+  if (!c->d_face_elem && F > 0) {
+    std::vector<int> nrow(N + 1, 0), nlist(4 * (size_t)c->T);
+    for (size_t k = 0; k < 4 * (size_t)c->T; ++k) nrow[c->tets[k] + 1]++;
+    for (int n = 0; n < N; ++n) nrow[n + 1] += nrow[n];
+    {
+      std::vector<int> fill(nrow.begin(), nrow.end() - 1);
+      for (size_t k = 0; k < 4 * (size_t)c->T; ++k) nlist[fill[c->tets[k]]++] = (int)(k / 4);
+    }
+    std::vector<int> felem(F, -1), frow(N + 1, 0), fslot;
+    for (int f = 0; f < F; ++f) {
+      const int a = c->faces[3 * (size_t)f], b = c->faces[3 * (size_t)f + 1], d = c->faces[3 * (size_t)f + 2];
+      for (int s = nrow[a]; s < nrow[a + 1] && felem[f] < 0; ++s) {
+        const int t = nlist[s];
+        const int *tt = &c->tets[4 * (size_t)t];
+        bool hb = false, hd = false;
+        for (int q = 0; q < 4; ++q) {
+          hb = hb || tt[q] == b;
+          hd = hd || tt[q] == d;
+        }
+        if (hb && hd) felem[f] = t;
+      }
+    }
+    std::vector<std::vector<int>> per(N);
+    for (int f = 0; f < F; ++f) {
+      const int code = c->fcodes[f];
+      const int ac = code < 0 ? -code : code;
+      if (felem[f] < 0 || !(ac == 1 || ac == 2 || ac == 3)) continue;
+      for (int k = 0; k < 4; ++k) per[c->tets[4 * (size_t)felem[f] + k]].push_back(4 * f + k);
+    }
+    for (int n = 0; n < N; ++n) frow[n + 1] = frow[n] + (int)per[n].size();
+    for (int n = 0; n < N; ++n) fslot.insert(fslot.end(), per[n].begin(), per[n].end());
+    if (fslot.empty()) fslot.push_back(0);
+    void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr, *p5 = nullptr;
+    if ((rc = ctx_alloc(c, sizeof(int) * F, &p1)) || (rc = ctx_alloc(c, sizeof(int) * (N + 1), &p2)) ||
+        (rc = ctx_alloc(c, sizeof(int) * fslot.size(), &p3)) ||
+        (rc = ctx_alloc(c, sizeof(double) * 20 * (size_t)F, &p4)) ||
+        (rc = ctx_alloc(c, sizeof(double) * 12 * (size_t)F, &p5)))
+      return rc;
+    hipError_t e = hipMemcpyAsync(p1, felem.data(), sizeof(int) * F, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(p2, frow.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(p3, fslot.data(), sizeof(int) * fslot.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return set_hip_error(e, "set_viscous face tables");
+    c->d_face_elem = (int *)p1;
+    c->d_nfe_row = (int *)p2;
+    c->d_nfe_slot = (int *)p3;
+    c->d_force_u = (double *)p4;
+    c->d_force_x = (double *)p5;
   }
   c->node_type = type;
   c->nwall = (int)wall.size();

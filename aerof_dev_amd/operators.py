@@ -195,7 +195,8 @@ class SpaceOperator:
         check(lib().aerofx_primitive(self._h, _ptr(U, self.n5), _ptr(V, self.n5)), "primitive")
 
     def wall_force(self, U, direction=(0.0, 0.0, 1.0), dJdU=None, dJdX=None) -> float:
-        """Pressure force on the slip walls along direction; optional dJ/dU [N,5], dJ/dX [N,3]."""
+        """Force on the walls along direction (pressure, plus the viscous stress
+        with the Navier-Stokes terms on); optional dJ/dU [N,5], dJ/dX [N,3]."""
         d = (C.c_double * 3)(*direction)
         J = C.c_double()
         check(lib().aerofx_wall_force(self._h, _ptr(U, self.n5), d, C.byref(J), _optr(dJdU, self.n5),

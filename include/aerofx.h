@@ -481,8 +481,10 @@ int aerofx_adjoint_residual(aerofx_ctx *ctx, const double *U, const double *lamb
 int aerofx_residual_derivative(aerofx_ctx *ctx, const double *U, const double *dU,
                                const double *dX, double *dR);
 
-/* Pressure force on the slip-wall faces along dir (lift for dir = e_z):
- * J = sum_{wall faces f} sum_{v in f} p_v (n_f / 3) . dir, with
+/* Force on the wall faces along dir (lift for dir = e_z):
+ * J = sum_{wall faces f} sum_{v in f} p_v (n_f / 3) . dir, plus, with the
+ * Navier-Stokes terms on, the viscous stress -(tau_e n_f) . dir of each wall
+ * face's element (PostFcnNS::computeViscousForce, PostFcn.C:1578-1612), with
  * dJdU (device [N][5], may be NULL) and dJdX (device [N][3], may be NULL).
  * The objective of the shape sensitivity (FluidShapeOptimizationHandler.C:880
  * evaluates the forces through PostOperator). */

@@ -503,3 +503,75 @@ def test_viscous_direct_solve(gpu):
         print(f"[viscous] ||A x - P rhs|| / ||P rhs|| = {err:.2e}", flush=True)
         assert err <= 1e-8
     op.close()
+
+
+def test_viscous_wall_force(gpu):
+    """The wall force with the viscous stress (PostFcnNS::computeForce):
+    its viscous part against tau = mu (grad u + grad u^T) + lambda div u I
+    formed in numpy from the oracle's P1 gradients and transport laws, and
+    dJ/dU, dJ/dX against central differences."""
+    import dataclasses
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator
+    m = _box(4)
+    p = ns_params(0.4, 30.0)
+    p.alpha, p.yaw = 0.05, 0.03
+    U = init_state(p, m.X, "noise", 0.01, seed=41)
+    rng = np.random.default_rng(10)
+    U[:, 1:4] += 0.05 * U[:, :1] * rng.standard_normal((U.shape[0], 3))
+    pe = FlowParams(mach=p.mach, p_inf=p.p_inf, alpha=p.alpha, yaw=p.yaw)
+    direction = (0.3, -0.2, 0.9)
+    op, ope = SpaceOperator(m, p), SpaceOperator(m, pe)
+    Ud = torch.from_numpy(U).to(gpu)
+    gu = torch.empty_like(Ud)
+    gx = torch.empty((op.num_nodes, 3), dtype=torch.float64, device=gpu)
+    J = op.wall_force(Ud, direction, gu, gx)
+    Je = ope.wall_force(Ud, direction)
+    # numpy restatement of the viscous part
+    V = np.empty_like(U)
+    V[:, 0] = U[:, 0]
+    V[:, 1:4] = U[:, 1:4] / U[:, :1]
+    V[:, 4] = (GAM - 1.0) * (U[:, 4] - 0.5 * U[:, 0] * np.sum(V[:, 1:4] ** 2, axis=1))
+    d = np.array(direction)
+    jv = 0.0
+    for f, code in enumerate(m.codes):
+        if abs(code) not in (1, 2, 3):
+            continue
+        a, b, c = m.faces[f]
+        t = next(t for t in range(m.num_tets) if {a, b, c} <= set(m.tets[t].tolist()))
+        nd = m.tets[t]
+        g, _ = O.p1_gradients(m.X[nd])
+        Tcg = np.mean(V[nd, 4] / ((GAM - 1.0) * V[nd, 0]))
+        mu, lam, _ = O.ns_transport(p, Tcg)
+        gradu = V[nd, 1:4].T @ g  # [i][j] = du_i / dx_j
+        tau = mu * (gradu + gradu.T) + lam * np.trace(gradu) * np.eye(3)
+        n = 0.5 * np.cross(m.X[c] - m.X[a], m.X[b] - m.X[a])
+        jv += -(tau @ n) @ d
+    print(f"[viscous] wall force {J:.12e}, pressure part {Je:.12e}, viscous part {J - Je:.6e} vs {jv:.6e}",
+          flush=True)
+    assert abs((J - Je) - jv) <= 1e-10 * abs(jv)
+    # dJ/dU
+    dU = rng.standard_normal(U.shape) * (np.abs(U) + 0.1)
+    h = 1e-6
+    fd = (op.wall_force(torch.from_numpy(U + h * dU).to(gpu), direction) -
+          op.wall_force(torch.from_numpy(U - h * dU).to(gpu), direction)) / (2 * h)
+    a = float(np.sum(gu.cpu().numpy() * dU))
+    print(f"[viscous] dJ/dU . d {a:.12e} vs FD {fd:.12e}", flush=True)
+    assert abs(a - fd) <= 1e-7 * abs(fd)
+    # dJ/dX over moved meshes
+    dX = 0.02 * rng.standard_normal(m.X.shape) / 4.0
+
+    def J_at(X):
+        o = SpaceOperator(dataclasses.replace(m, X=np.ascontiguousarray(X)), p)
+        v = o.wall_force(Ud, direction)
+        o.close()
+        return v
+    H = 1e-5
+    f1 = (J_at(m.X + H * dX) - J_at(m.X - H * dX)) / (2 * H)
+    f2 = (J_at(m.X + H / 2 * dX) - J_at(m.X - H / 2 * dX)) / H
+    fdx = (4 * f2 - f1) / 3
+    ax = float(np.sum(gx.cpu().numpy() * dX))
+    print(f"[viscous] dJ/dX . dX {ax:.12e} vs FD {fdx:.12e}", flush=True)
+    assert abs(ax - fdx) <= 1e-7 * abs(fdx)
+    op.close()
+    ope.close()
