@@ -17,7 +17,9 @@
 * the forward state derivative: exact duality with the transposed product
   (1e-11), its Galerkin part against differences, H2-product Newton;
 * the shape derivative: forward / adjoint duality (1e-11), its Galerkin part
-  against differences over moved meshes; the direct solve with the wall rows.
+  against differences over moved meshes; the direct solve with the wall rows;
+* the wall force with the viscous stress and its derivatives; the adjoint
+  shape gradient of the wall force against re-converged differences.
 """
 import numpy as np
 import pytest
@@ -575,3 +577,58 @@ def test_viscous_wall_force(gpu):
     assert abs(ax - fdx) <= 1e-7 * abs(fdx)
     op.close()
     ope.close()
+
+
+def test_viscous_shape_gradient_vs_fd(gpu):
+    """The whole Navier-Stokes shape sensitivity: dJ/da of the wall force
+    (pressure and viscous stress) with respect to a bump amplitude on the
+    no-slip channel, from the adjoint of the system with the wall rows
+    (dJ/dX + (dR/dX)^T P psi, A^T psi = -dJ/dU), against central differences
+    of J over re-converged solutions on meshes with the bump at +-h."""
+    import torch
+    from aerof_dev_amd.operators import SpaceOperator, adjoint_solve, steady_solve
+
+    def mesh(a):
+        return M.kuhn_box(24, 8, 8, lo=(0.0, 0.0, 0.0), hi=(3.0, 1.0, 1.0), face_codes=M.VISCOUS_CHANNEL,
+                          bumps=((0.05, 1.5, 0.2), (a, 1.2, 0.2)))
+
+    def converge(op, U):
+        r = steady_solve(op, U, precision=0, max_steps=400, eps=1e-9, mvp="fd")
+        assert r["converged"], (r["steps"], r["res"] / r["res0"])
+    # first order: the converged states of the second-order scheme sit near
+    # Van Albada kinks and the Newton iteration stalls at 1e-7 - 1e-6 there
+    p = ns_params(0.3, 100.0, second_order=False)
+    zdir = (0.0, 0.0, 1.0)
+    m = mesh(0.0)
+    op = SpaceOperator(m, p)
+    U = torch.from_numpy(init_state(p, m.X, "freestream")).to(gpu)
+    converge(op, U)
+    dJdU = torch.empty_like(U)
+    dJdX = torch.empty((m.num_nodes, 3), dtype=torch.float64, device=gpu)
+    op.wall_force(U, zdir, dJdU, dJdX)
+    psi = torch.zeros_like(U)
+    st = adjoint_solve(op, U, (-dJdU).contiguous(), psi, precision=1, cfl=1e3, eps=1e-11, max_its=2000)
+    assert st["converged"], st
+    wall = torch.from_numpy(op.node_types() == 3).to(gpu)
+    ppsi = psi.clone()
+    ppsi[wall, 1:4] = 0.0
+    gx = torch.empty_like(dJdX)
+    op.compute_transpose_derivative_of_residual(U, ppsi, None, gx)
+    dXda = mesh(1.0).X - m.X
+    grad = float(np.sum((dJdX + gx).cpu().numpy() * dXda))
+    # the matrix-free Newton stops near 1e-9 (the finite-difference floor),
+    # so the step is 1e-4 (1e-5 of the state error per unit of J / H)
+    H = 1e-4
+    Js = []
+    for s in (1.0, -1.0):
+        mk = mesh(s * H)
+        opk = SpaceOperator(mk, p)
+        Uk = U.clone()
+        converge(opk, Uk)
+        Js.append(opk.wall_force(Uk, zdir))
+        opk.close()
+    fd = (Js[0] - Js[1]) / (2 * H)
+    print(f"[viscous] shape gradient: adjoint {grad:.10e} vs FD {fd:.10e} (rel {abs(grad - fd) / abs(fd):.2e}; "
+          f"adjoint {st['its']} outer / {st['krylov_its']} inner)", flush=True)
+    assert abs(grad - fd) <= 1e-4 * abs(fd)
+    op.close()
