@@ -373,12 +373,31 @@ def viscous_bench(dev, n=NX, steps=20):
         jac["fp64" if precision == 0 else "fp32"] = {
             "feJac_ms": round(fj, 4), "fvJac_ms": round(tj["fvJac"][0] / tj["fvJac"][1], 4),
             "algorithmic_GBs": round(jb / (fj * 1e-3) / 1e9, 1)}
+    # the state derivatives with the Galerkin term: one forward product and
+    # one transposed product (each with its forward residual pass)
+    lam = torch.randn_like(U)
+    dU = torch.randn_like(U)
+    out_t, out_a = torch.empty_like(U), torch.empty_like(U)
+    op.compute_derivative_of_residual(U, out_t, dU=dU)
+    op.compute_transpose_derivative_of_residual(U, lam, out_a, None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        op.compute_derivative_of_residual(U, out_t, dU=dU)
+    torch.cuda.synchronize()
+    tan_ms = (time.perf_counter() - t0) / 3 * 1e3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        op.compute_transpose_derivative_of_residual(U, lam, out_a, None)
+    torch.cuda.synchronize()
+    adj_ms = (time.perf_counter() - t0) / 3 * 1e3
     op.close()
     res = {"workload": f"{n}^3 Kuhn box ({N} nodes, {T} elements, {E} edges), no-slip z- wall, M {mach}, Re 1000, "
                        "Sutherland, second order",
            "residual_ms": round(wall_ms, 4), "finite_volume_ms": round(fv_ms, 4), "feTerm_ms": round(fe_ms, 4),
            "feTerm_algorithmic_GBs": round(fe_bytes / (fe_ms * 1e-3) / 1e9, 1),
-           "feTerm_Melements_per_s": round(T / (fe_ms * 1e-3) / 1e6, 1), "jacobian": jac}
+           "feTerm_Melements_per_s": round(T / (fe_ms * 1e-3) / 1e6, 1), "jacobian": jac,
+           "dRdU_dU_ms": round(tan_ms, 3), "dRdU_T_lambda_ms": round(adj_ms, 3)}
     log(f"[bench] viscous: {res}")
     return res
 
