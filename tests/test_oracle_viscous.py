@@ -296,3 +296,27 @@ def test_viscous_residual_includes_galerkin_before_scaling():
     G = omv.galerkin(V)
     cv = omv.tables()["ctrl_vol"]
     assert np.allclose(Rv, Re + G / cv[:, None], rtol=1e-12, atol=1e-12 * np.abs(Rv).max())
+
+
+def test_viscous_timestep_scaling():
+    """The elemental viscous time-step term (ElemSet::computeTimeStep) is
+    9 mu / rho |T| |grad phi_j|^2 summed over the elements: it scales like
+    the mesh size (|T| ~ h^3, |grad phi|^2 ~ h^-2) and like 1/Re, so the
+    viscous time step cfl V / idtv scales like h^2 Re."""
+    import dataclasses
+    m = _viscous_box(4, jitter=0.2)
+    Tinf = 1.0 / (GAM * (GAM - 1.0) * 0.09)
+
+    def idtv(scale, re):
+        mm = dataclasses.replace(m, X=m.X * scale)
+        pv = ns_params(0.3, re, viscous_cst=1.0)
+        pe = FlowParams(mach=0.3, p_inf=1.0 / (GAM * 0.09))
+        V = _state(mm, pv, lambda x: np.zeros(3), lambda x: Tinf)
+        omv, ome = O.OracleMesh(mm, pv), O.OracleMesh(mm, pe)
+        cv = omv.tables()["ctrl_vol"]
+        dv, de = omv.timestep(V, 1.0), ome.timestep(V, 1.0)
+        return cv / dv - cv / de  # = idtv (the inviscid part cancels)
+    a, b, c = idtv(1.0, 50.0), idtv(2.0, 50.0), idtv(1.0, 100.0)
+    assert np.all(a > 0)
+    assert np.allclose(b, 2.0 * a, rtol=1e-10)
+    assert np.allclose(c, 0.5 * a, rtol=1e-10)
