@@ -249,6 +249,9 @@ __global__ void k_zero_int(int *p, int n, const int *done);
 // conditions (U: the state, for isothermal walls); F's wall rows; the
 // viscous time-step term (idtv = NULL when not used)
 int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, const int *done);
+// the element pass alone into the contribution buffer (the node pass of the
+// residual sums it)
+int launch_visc_elements(aerofx_ctx *c, const double *V, const int *done);
 int launch_visc_jacobian(aerofx_ctx *c, const double *U, int precision);
 int launch_bcs_residual(aerofx_ctx *c, const double *U, double *F, const int *done);
 int launch_visc_idtv(aerofx_ctx *c, double **idtv);

@@ -465,9 +465,13 @@ __global__ __launch_bounds__(kFluxStrip) void k_flux_staged(DevTables t, const d
 // 3. node assembly: lower edges, upper-edge partial sum (formed here from the
 //    edge fluxes for a row of more than fblock upper edges), faces, 1/V
 // ---------------------------------------------------------------------------
+// vrow / vc (or null): the Navier-Stokes Galerkin contributions of the
+// node's elements, one contiguous run per node in element order, added
+// before the 1/V scaling
 __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double *__restrict__ V,
                                                        const double *__restrict__ Fm,
                                                        const double *__restrict__ P, FluxConsts fc, int scale,
+                                                       const int *__restrict__ vrow, const double *__restrict__ vc,
                                                        double *__restrict__ R, const int *__restrict__ done) {
   if (done && *done) return;
   const int nb = (t.N + kBlock - 1) / kBlock;
@@ -514,6 +518,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_nodes(DevTables t, const double
 #pragma unroll
       for (int k = 0; k < 5; ++k) s[k] += fl[k];
     }
+  }
+  if (vc) {
+    double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = vrow[n]; q < vrow[n + 1]; ++q)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) g[k] += vc[5 * (size_t)q + k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) s[k] += g[k];
   }
   const double ic = t.invcv[n];
 #pragma unroll
@@ -587,15 +599,23 @@ int launch_residual_with(aerofx_ctx *c, const ResidualBufs &b, const double *U, 
     else
       launch_flux_edges<false, kFluxBlock>(c, b, done);
   }
+  // the Navier-Stokes Galerkin term (SpaceOperator.C:570-573): its element
+  // pass, summed by the node pass with the fluxes (or, without the
+  // contribution buffer, the per-node kernel after it)
+  const bool fold = c->viscous && c->d_visc_c;
+  if (fold) {
+    int rc = launch_visc_elements(c, b.V, done);
+    if (rc) return rc;
+  }
   {
     ProfScope ps(c, AEROFX_TIMER_FV_NODES);
     hipLaunchKernelGGL(k_flux_nodes, dim3((t.N + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, t, b.V,
-                       b.Fm, b.P, c->fc, scale ? 1 : 0, R, done);
+                       b.Fm, b.P, c->fc, scale ? 1 : 0, fold ? c->elem->ntrow : nullptr,
+                       fold ? c->d_visc_c : nullptr, R, done);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_hip_error(e, "residual launch");
-  // the Navier-Stokes Galerkin term (SpaceOperator.C:570-573)
-  return launch_visc_residual(c, b.V, R, scale, done);
+  return fold ? 0 : launch_visc_residual(c, b.V, R, scale, done);
 }
 
 int launch_residual(aerofx_ctx *c, const double *U, double *R, bool scale, const int *done) {

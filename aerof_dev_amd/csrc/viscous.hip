@@ -756,6 +756,14 @@ int bc_priority(int code) {
 
 }  // namespace
 
+int launch_visc_elements(aerofx_ctx *c, const double *V, const int *done) {
+  const ElemTables *el = c->elem;
+  ProfScope ps(c, AEROFX_TIMER_FE_TERM);
+  hipLaunchKernelGGL(k_visc_elements, dim3((c->T + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->T,
+                     (const int4 *)el->tets, c->tab.X4, V, c->ns, (const int4 *)c->d_visc_pos, c->d_visc_c, done);
+  return check("viscous element launch");
+}
+
 int launch_visc_residual(aerofx_ctx *c, const double *V, double *R, bool scale, const int *done) {
   if (!c->viscous) return 0;
   const ElemTables *el = c->elem;

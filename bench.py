@@ -347,6 +347,8 @@ def viscous_bench(dev, n=NX, steps=20):
     wall_ms = (time.perf_counter() - t0) / steps * 1e3
     tm = op.timers()
     op.profile(False)
+    # feTerm: the element pass; the node sums of the Galerkin term run inside
+    # the finite-volume node pass (fvTermNodes)
     fe_ms = tm["feTerm"][0] / tm["feTerm"][1]
     fv_ms = sum(tm[k][0] / tm[k][1] for k in ("nodalGrad", "fvTerm", "fvTermNodes") if k in tm)
     # algorithmic bytes of the Galerkin pass: per node V, X (72 B), R read and
